@@ -1,0 +1,144 @@
+"""Benchmark: plain PruneJuice pruning pipeline on one MI355X.
+
+Workload (the single-GPU plain configuration): R-MAT scale 22, edge factor 16,
+(a, b, c, d) = (0.57, 0.19, 0.19, 0.05), symmetrised / deduplicated / no
+self-loops (4,194,304 vertices, 128,312,408 directed CSR slots), 32 labels
+drawn from Zipf(s=1.2); template = labelled triangle t0-t1-t2 plus the tail
+t2-t3-t4-t5 with labels [0, 1, 2, 3, 4, 3] (one cycle and one path
+constraint).  One bench step = one whole pipeline solve on the HBM-resident
+graph: state reset, LCC0, LCC to fixed point, CC, LCC, PC, LCC, enumeration.
+
+value = pruning TEPS = CSR slots examined by all steps (counted by the
+kernels) / solve time.  roofline = the dominant kernel's algorithmic bytes per
+launch / its average launch time (HIP events on the context stream) against
+8 TB/s.  cpu_baseline = the oracle (CPU restatement of the reference, faithful
+mode, all host threads) on a bounded sample of the same workload family.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "pruning TEPS + HBM roofline %; batched/served queries/s, p50/p99 vs plain"
+HBM_PEAK_GBS = 8000.0
+
+GRAPH_SEED = 20221
+LABEL_SEED = 7
+TEMPLATE_EDGES = [(0, 1), (1, 2), (2, 0), (2, 3), (3, 4), (4, 5)]
+TEMPLATE_LABELS = [0, 1, 2, 3, 4, 3]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(scale: int):
+    from pattenmatching_sharedmemory_amd import Constraints, Graph
+
+    t0 = time.time()
+    g = Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=GRAPH_SEED).labels_zipf(32, 1.2, seed=LABEL_SEED)
+    p = Graph.pattern(6, TEMPLATE_EDGES, TEMPLATE_LABELS)
+    c = Constraints(p)
+    log(f"[bench] graph scale {scale}: V={g.vertex_count} E={g.edge_count} built in {time.time() - t0:.1f}s; "
+        f"cycles={c.cycles()} paths={c.paths()} tds={len(c.tds())}")
+    return g, p, c
+
+
+def cpu_baseline(scale: int, threads: int) -> dict:
+    import oracle
+
+    g, _, c = workload(scale)
+    o = oracle.Oracle(g, c, threads=threads)
+    t0 = time.time()
+    rows, total_ms = o.run(oracle.FAITHFUL)
+    wall = time.time() - t0
+    slots = sum(r.slots for r in rows)
+    log(f"[bench] cpu baseline scale {scale}: {len(rows)} steps, {total_ms:.0f} ms of step time "
+        f"({wall:.1f}s wall), {slots} slots")
+    return {"value": slots / (total_ms / 1e3), "unit": "TEPS", "cores": threads, "kind": "port",
+            "sample": f"oracle faithful mode (reference loops and data layout, OpenMP), full pipeline on R-MAT scale "
+                      f"{scale} of the same generator, labels and template; time = sum of step times",
+            "steps": len(rows), "seconds": total_ms / 1e3}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--cpu-scale", type=int, default=19)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the pruning path is measured on a single MI355X "
+              "(multi-GPU sharding is a design note, not a measured configuration)", file=sys.stderr)
+        return 2
+
+    from pattenmatching_sharedmemory_amd import Pruner
+
+    g, _, c = workload(args.scale)
+    pr = Pruner(g, c, device=0)
+    for _ in range(args.warmup):
+        rows, _ = pr.run()
+    log(f"[bench] warmup done: {len(rows)} steps, enumeration {rows[-1].enumeration}")
+
+    times, dev = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        rows, dev_ms = pr.run()
+        times.append(time.perf_counter() - t0)
+        dev.append(dev_ms)
+    slots = sum(r.slots for r in rows)
+    sec = sum(times) / len(times)
+    log(f"[bench] {args.steps} solves: {sec * 1e3:.2f} ms/solve wall, {sum(dev) / len(dev):.2f} ms device, "
+        f"{len(rows)} steps, slots {slots}")
+    for r in rows:
+        log(f"    step {r.iteration:3d} {r.step} ve={r.vertex_eliminated} ee={r.edge_eliminated} "
+            f"me={r.match_eliminated} en={r.enumeration} av={r.active_vertices} ae={r.active_edges} "
+            f"slots={r.slots} ms={r.ms:.3f}")
+
+    roofline = None
+    if not args.no_profile and hasattr(pr, "profile"):
+        roofline = pr.profile(HBM_PEAK_GBS)
+
+    result = {
+        "metric": METRIC,
+        "value": slots / sec,
+        "unit": "TEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"plain pipeline, R-MAT scale {args.scale} EF16 (0.57,0.19,0.19,0.05), 32 Zipf(1.2) "
+                               f"labels, triangle+tail template [0,1,2,3,4,3]",
+                   "vertices": g.vertex_count, "directed_slots": g.edge_count, "pipeline_steps": len(rows),
+                   "slots_per_solve": slots, "enumeration": rows[-1].enumeration},
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    pr.close()
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        result["cpu_baseline"] = cpu_baseline(args.cpu_scale, threads)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,191 @@
+/*
+ * prunejuice.h — C ABI of the MI355X (gfx950) PruneJuice pruning library.
+ *
+ * This is the drop-in boundary between host C++ (the step machine, constraint
+ * generation and TOTEM loader) and the HIP kernels.  Plain pointers and sizes
+ * only; every call returns 0 on success and -1 on failure (the reference's
+ * error_t convention, src/totem/totem_comdef.h:31-32), with the message kept
+ * per context and returned by pj_last_error().
+ *
+ * Reference seam replaced: the step classes chosen by the typedef block at
+ * src/patternmatching/multiple_label_cpu.h:34-52, each exposing
+ *   AlgoResults compute(const graph_t&, State*)      (e.g. multiple_label_lcc_cpu.h:23)
+ *   void resetState(State*)                          (multiple_label_cc_cpu.h:28)
+ *   void setConstraintVector(const std::vector<C>&)  (multiple_label_cc_cpu.h:30-31)
+ *   void setConstraintIterator(const size_t&)        (multiple_label_cc_cpu.h:32)
+ * and the host state object MultipleLabelGlobalState
+ * (src/patternmatching/multiple_label/multiple_label_common_cpu.h:23-59).
+ *
+ * Calls into one context must be serialised by the caller.  A context owns
+ * one device CSR, one template and one pruning state on one HIP stream.
+ */
+#ifndef PRUNEJUICE_H
+#define PRUNEJUICE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PJ_MAX_TEMPLATE_VERTICES 64
+
+/* Step identifiers: same numbering as MultipleLabelStep::Step
+ * (src/patternmatching/multiple_label_step.h:16-25). */
+enum pj_step {
+  PJ_STEP_START = 0,
+  PJ_STEP_END = 1,
+  PJ_STEP_LCC0 = 2,
+  PJ_STEP_LCC = 3,
+  PJ_STEP_CC = 4,
+  PJ_STEP_PC = 5,
+  PJ_STEP_TDS = 6,
+  PJ_STEP_ENUMERATION = 7
+};
+
+/* Walk move types: same numbering as Walk::MoveType
+ * (src/patternmatching/utils/walk.h:16-21). */
+enum pj_move {
+  PJ_MOVE_BACK = 0,
+  PJ_MOVE_CHECK = 1,
+  PJ_MOVE_VERTEX_STORE = 2,
+  PJ_MOVE_VERTEX_NO_STORE = 3
+};
+
+/* Per-step counters: AlgoResults (src/patternmatching/algorithms/algo_results.h:11-22). */
+typedef struct pj_algo_results {
+  uint64_t vertex_eliminated;
+  uint64_t edge_eliminated;
+  uint64_t match_eliminated;
+  uint64_t enumeration;
+} pj_algo_results;
+
+/* One template walk (Walk, src/patternmatching/utils/walk.h:14-37): the moves
+ * of a DFS over a template rooted at `root`.  move_back[i] indexes the history
+ * of VERTEX moves (history[0] is the root). */
+typedef struct pj_walk_desc {
+  uint32_t root;
+  uint32_t length;
+  const uint8_t* move;        /* enum pj_move, [length] */
+  const uint32_t* vertex;     /* template vertex of each move, [length] */
+  const uint32_t* move_back;  /* history index for CHECK / MOVE_BACK, [length] */
+} pj_walk_desc;
+
+/* Flattened constraint tables of one template (POD only).
+ *  - local: MultipleLabelConstraintLocal per template vertex
+ *    (multiple_label_constraint_local.h:18-32): origin label, neighbour-label
+ *    multiset (need_label/need_count) and template neighbour set (neighbor_mask).
+ *  - cycles: normalised MultipleLabelConstraintCircular::vertexIndexVector,
+ *    in generation order (multiple_label_generate_constraint_circular.cc:58-86).
+ *  - paths: MultipleLabelConstraintPath::vertexIndexVector + initialLabel
+ *    (multiple_label_constraint_path.cc:9-30).
+ *  - tds: for each template constraint, its walkMap in iteration order
+ *    (multiple_label_constraint_template.h:179-180).
+ *  - enumeration: walkMap.cbegin() of the whole-pattern template
+ *    (multiple_label_enumeration_cpu.tpp:133-135). */
+typedef struct pj_template_desc {
+  uint32_t vertex_count;
+  const uint64_t* vertex_label;   /* [vertex_count] */
+  const uint64_t* neighbor_mask;  /* [vertex_count] */
+  const uint32_t* need_offset;    /* [vertex_count + 1] */
+  const uint64_t* need_label;     /* [need_offset[vertex_count]] */
+  const uint32_t* need_count;     /* [need_offset[vertex_count]] */
+
+  uint32_t cycle_count;
+  const uint32_t* cycle_offset;   /* [cycle_count + 1] */
+  const uint32_t* cycle_vertex;
+
+  uint32_t path_count;
+  const uint32_t* path_offset;    /* [path_count + 1] */
+  const uint32_t* path_vertex;
+  const uint64_t* path_initial_label; /* [path_count] */
+
+  uint32_t tds_count;
+  const uint32_t* tds_walk_offset; /* [tds_count + 1] into walks */
+  uint32_t walk_count;
+  const pj_walk_desc* walks;
+
+  pj_walk_desc enumeration_walk;
+} pj_template_desc;
+
+typedef struct pj_ctx pj_ctx;
+
+/* Creates a context on HIP device `device` with its own stream.
+ * Replaces the OpenMP set-up of MultipleLabelCpu::initialiseTotem
+ * (src/patternmatching/multiple_label_cpu.cc:12-18). */
+int pj_create(pj_ctx** ctx, int device);
+
+/* Copies a CSR graph to HBM: offsets [V+1] (uint64), edges [E] (uint32 ids),
+ * labels [V] (uint64 vertex values).  The caller may free its arrays after the
+ * call.  Replaces the host graph_t consumed by every compute()
+ * (src/totem/totem_graph.h:137-180; loader src/totem/totem_graph.cu:388-409). */
+int pj_upload_graph(pj_ctx* ctx, uint32_t vertex_count, uint64_t edge_count,
+                    const uint64_t* offsets, const uint32_t* edges,
+                    const uint64_t* labels);
+
+/* Same as pj_upload_graph for arrays that already live in device memory of
+ * this context's device.  The context copies them (device to device). */
+int pj_upload_graph_device(pj_ctx* ctx, uint32_t vertex_count, uint64_t edge_count,
+                           const uint64_t* d_offsets, const uint32_t* d_edges,
+                           const uint64_t* d_labels);
+
+/* Installs the constraint tables; replaces the setConstraintVector calls of
+ * MultipleLabelCpu::allocate (src/patternmatching/multiple_label_cpu.cc:84-95).
+ * Also (re)initialises the pruning state as pj_state_reset does. */
+int pj_set_template(pj_ctx* ctx, const pj_template_desc* desc);
+
+/* Pruning state to its initial value: every vertex and edge active, candidate
+ * set {0..P-1}, nothing modified, everything scheduled
+ * (MultipleLabelGlobalState::allocate, multiple_label_common_cpu.tpp:29-66). */
+int pj_state_reset(pj_ctx* ctx);
+
+/* resetState of the NLCC steps: modified flags to 0
+ * (multiple_label_cc_cpu.tpp:72-75, pc_cpu.tpp:77-80, tds_cpu.tpp:185-188). */
+int pj_reset_modified(pj_ctx* ctx);
+
+/* Runs one step.  `constraint_index` selects the cycle / path / TDS
+ * constraint for PJ_STEP_CC / PC / TDS and is ignored otherwise.  The
+ * results are written to *out (may be NULL) and the context's active-vertex
+ * and active-edge counters are updated as the reference's compute() does
+ * (e.g. multiple_label_lcc_cpu.tpp:144-151). */
+int pj_run_step(pj_ctx* ctx, int step, uint32_t constraint_index,
+                pj_algo_results* out);
+
+/* Device time of the last pj_run_step in milliseconds (HIP events on the
+ * context stream). */
+int pj_last_step_ms(pj_ctx* ctx, double* ms);
+
+/* CSR slots the last step examined: every slot a serial scan in the
+ * reference's loop order reads (scans stop at the first hit where the
+ * reference's loop does).  Numerator of the pruning TEPS; no reference
+ * counterpart (the reference times steps only, multiple_label_cpu.cc:137-164). */
+int pj_last_step_slots(pj_ctx* ctx, uint64_t* slots);
+
+/* Active vertex / edge counters: graphActiveVertexCount and
+ * graphActiveEdgeCount (multiple_label_common_cpu.h:43-46), as 64-bit values. */
+int pj_get_counters(pj_ctx* ctx, uint64_t* active_vertices, uint64_t* active_edges);
+
+/* Copies the state back to the host.  Any pointer may be NULL.
+ *   cand      [V] candidate set (bit p = template vertex p) (vertexPatternMatch)
+ *   vflags    [V] bit0 active, bit1 modified, bit2 scheduled
+ *   ea_words  [ceil(E/32)] edge-active bitset, bit e%32 of word e/32 */
+int pj_download_state(pj_ctx* ctx, uint64_t* cand, uint8_t* vflags, uint32_t* ea_words);
+
+/* Overwrites the pruning state from host arrays of the same layout as
+ * pj_download_state (used to start a step from a given state). */
+int pj_upload_state(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags,
+                    const uint32_t* ea_words, uint64_t active_vertices,
+                    uint64_t active_edges);
+
+/* Last error message of the context (never NULL). */
+const char* pj_last_error(pj_ctx* ctx);
+
+/* Releases every device buffer and the stream. */
+void pj_destroy(pj_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRUNEJUICE_H */

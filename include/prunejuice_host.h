@@ -1,0 +1,96 @@
+/*
+ * prunejuice_host.h — C ABI of the host-side library (libpj_host.so): TOTEM
+ * graph loading and writing, synthetic R-MAT graphs, constraint generation
+ * from a template, and the pipeline step machine driven over the device
+ * C ABI of prunejuice.h.
+ *
+ * Reference counterparts:
+ *   pjh_graph_load         graph_initialize + parseVertexFile
+ *                          (src/totem/totem_graph.cu:388-409,
+ *                           src/patternmatching/utils/common_utils.cc:9-59)
+ *   pjh_graph_store_binary graph_store_binary (totem_graph.cu:883-923)
+ *   pjh_constraints_build  the five generators run by MultipleLabelCpu::allocate
+ *                          (src/patternmatching/multiple_label_cpu.cc:54-80)
+ *   pjh_run_pipeline       MultipleLabelCpu::runPatternMatching
+ *                          (multiple_label_cpu.cc:113-185)
+ *   pjh_main               main of patternmatching_binary
+ *                          (src/patternmatching/patternmatching_binary.cc:13-42)
+ * All functions return 0 on success and -1 on failure unless stated.
+ */
+#ifndef PRUNEJUICE_HOST_H
+#define PRUNEJUICE_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "prunejuice.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pjh_graph pjh_graph;
+typedef struct pjh_constraints pjh_constraints;
+
+/* Graphs ------------------------------------------------------------------ */
+int pjh_graph_load(const char* path, const char* vertex_file /* may be NULL */, pjh_graph** out, char* err,
+                   size_t errlen);
+int pjh_graph_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed, pjh_graph** out);
+int pjh_graph_from_csr(uint32_t vertex_count, uint64_t edge_count, const uint64_t* offsets, const uint32_t* edges,
+                       const uint64_t* labels /* may be NULL */, pjh_graph** out);
+int pjh_pattern_from_edges(uint32_t vertex_count, uint32_t edge_count, const uint32_t* src, const uint32_t* dst,
+                           const uint64_t* labels, pjh_graph** out);
+void pjh_graph_labels_uniform(pjh_graph* g, uint32_t n_labels, uint64_t seed);
+void pjh_graph_labels_zipf(pjh_graph* g, uint32_t n_labels, double s, uint64_t seed);
+uint32_t pjh_graph_vertex_count(const pjh_graph* g);
+uint64_t pjh_graph_edge_count(const pjh_graph* g);
+const uint64_t* pjh_graph_offsets(const pjh_graph* g);
+const uint32_t* pjh_graph_edges(const pjh_graph* g);
+const uint64_t* pjh_graph_labels(const pjh_graph* g); /* NULL when not valued */
+int pjh_graph_store_binary(const pjh_graph* g, const char* path, int eid_bytes);
+int pjh_graph_store_text(const pjh_graph* g, const char* path);
+int pjh_graph_store_vertex_file(const pjh_graph* g, const char* path);
+uint64_t pjh_graph_asymmetric_slots(const pjh_graph* g);
+void pjh_graph_free(pjh_graph* g);
+
+/* Constraints ------------------------------------------------------------- */
+int pjh_constraints_build(const pjh_graph* pattern, pjh_constraints** out, char* err, size_t errlen);
+const pj_template_desc* pjh_constraints_desc(const pjh_constraints* c);
+/* Writes the constraint dump (NUL-terminated, truncated to len); returns the
+ * full length. */
+size_t pjh_constraints_dump(const pjh_constraints* c, char* buf, size_t len);
+/* 1 when the reference's per-source edge memo of this cycle / path
+ * constraint can never fire (then its memoised search equals the complete
+ * search the device runs), 0 when it may fire, -1 for other steps. */
+int pjh_constraints_memo_inert(const pjh_constraints* c, int step, uint32_t index);
+void pjh_constraints_free(pjh_constraints* c);
+
+/* Pipeline ---------------------------------------------------------------- */
+typedef struct pjh_row {
+  int32_t iteration;
+  int32_t step;
+  uint64_t vertex_eliminated, edge_eliminated, match_eliminated, enumeration;
+  uint64_t active_vertices, active_edges;
+  uint64_t slots; /* CSR slots examined (pj_last_step_slots) */
+  double ms;      /* device time of the step (HIP events) */
+} pjh_row;
+
+/* Runs the step machine on a context that holds the graph and the
+ * constraints' template (pj_set_template done).  Resets the state first.
+ * With PJH_STOP_BEFORE_ENUMERATION in `flags` the machine stops when it
+ * reaches the enumeration step (the pruning fixed point) without running it.
+ * Returns the number of steps (rows written up to max_rows), -1 on error. */
+#define PJH_STOP_BEFORE_ENUMERATION 1
+int pjh_run_pipeline(pj_ctx* ctx, const pjh_constraints* c, int flags, pjh_row* rows, int max_rows,
+                     double* total_ms);
+
+/* patternmatching_binary: -i graph -b binary-cache -v vertex-data -p pattern-dir
+ * -o output-dir [-d device]; writes results.totem, graph_%04d.totem,
+ * vertex_%04d.totem and log.totem.  Returns the process exit code. */
+int pjh_main(int argc, char** argv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRUNEJUICE_HOST_H */

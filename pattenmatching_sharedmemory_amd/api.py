@@ -1,0 +1,325 @@
+"""Python host API over the native libraries.
+
+Graph        TOTEM graph (CSR) on the host: load / write / R-MAT / labels
+Constraints  every constraint of a template (pattern) graph, generated on the host
+Pruner       one device context: HBM-resident graph, template and pruning state;
+             runs single steps (the reference's step classes) or the whole
+             step machine (MultipleLabelCpu::runPatternMatching,
+             src/patternmatching/multiple_label_cpu.cc:113-185)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import native
+from .native import STEP_CC, STEP_ENUMERATION, STEP_LCC, STEP_LCC0, STEP_PC, STEP_TDS, AlgoResults, Row
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Graph:
+    """A CSR graph owned by the host library (pj::Graph)."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            native.host().pjh_graph_free(h)
+            self._h = C.c_void_p(0)
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    @classmethod
+    def load(cls, path: str, vertex_file: str | None = None) -> "Graph":
+        out = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = native.host().pjh_graph_load(str(path).encode(), vertex_file.encode() if vertex_file else None,
+                                          C.byref(out), err, len(err))
+        if rc:
+            raise ValueError(err.value.decode())
+        return cls(out.value)
+
+    @classmethod
+    def rmat(cls, scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int = 1) -> "Graph":
+        out = C.c_void_p()
+        if native.host().pjh_graph_rmat(scale, edge_factor, a, b, c, seed, C.byref(out)):
+            raise ValueError("bad R-MAT parameters")
+        return cls(out.value)
+
+    @classmethod
+    def from_csr(cls, offsets, edges, labels=None) -> "Graph":
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        ed = np.ascontiguousarray(edges, dtype=np.uint32)
+        lab = None if labels is None else np.ascontiguousarray(labels, dtype=np.uint64)
+        out = C.c_void_p()
+        rc = native.host().pjh_graph_from_csr(len(off) - 1, len(ed), _ptr(off, C.c_uint64), _ptr(ed, C.c_uint32),
+                                              None if lab is None else _ptr(lab, C.c_uint64), C.byref(out))
+        if rc:
+            raise ValueError("bad CSR")
+        return cls(out.value)
+
+    @classmethod
+    def pattern(cls, vertex_count: int, edges, labels) -> "Graph":
+        """Undirected template: each (a, b) stored both ways, rows ascending."""
+        e = np.asarray(edges, dtype=np.uint32).reshape(-1, 2)
+        src, dst = np.ascontiguousarray(e[:, 0]), np.ascontiguousarray(e[:, 1])
+        lab = np.ascontiguousarray(labels, dtype=np.uint64)
+        out = C.c_void_p()
+        rc = native.host().pjh_pattern_from_edges(vertex_count, len(src), _ptr(src, C.c_uint32),
+                                                  _ptr(dst, C.c_uint32), _ptr(lab, C.c_uint64), C.byref(out))
+        if rc:
+            raise ValueError("bad pattern")
+        return cls(out.value)
+
+    def labels_uniform(self, n_labels: int, seed: int) -> "Graph":
+        native.host().pjh_graph_labels_uniform(self._h, n_labels, seed)
+        return self
+
+    def labels_zipf(self, n_labels: int, s: float, seed: int) -> "Graph":
+        native.host().pjh_graph_labels_zipf(self._h, n_labels, s, seed)
+        return self
+
+    @property
+    def vertex_count(self) -> int:
+        return native.host().pjh_graph_vertex_count(self._h)
+
+    @property
+    def edge_count(self) -> int:
+        return native.host().pjh_graph_edge_count(self._h)
+
+    def offsets(self) -> np.ndarray:
+        p = native.host().pjh_graph_offsets(self._h)
+        return np.ctypeslib.as_array(p, shape=(self.vertex_count + 1,)).copy()
+
+    def edges(self) -> np.ndarray:
+        if self.edge_count == 0:
+            return np.zeros(0, dtype=np.uint32)
+        p = native.host().pjh_graph_edges(self._h)
+        return np.ctypeslib.as_array(p, shape=(self.edge_count,)).copy()
+
+    def labels(self) -> np.ndarray | None:
+        p = native.host().pjh_graph_labels(self._h)
+        if not p or self.vertex_count == 0:
+            return None
+        return np.ctypeslib.as_array(p, shape=(self.vertex_count,)).copy()
+
+    def raw_arrays(self):
+        """(offsets, edges, labels) pointers owned by the graph, for C calls."""
+        h = native.host()
+        return h.pjh_graph_offsets(self._h), h.pjh_graph_edges(self._h), h.pjh_graph_labels(self._h)
+
+    def store_binary(self, path: str, eid_bytes: int = 4) -> None:
+        if native.host().pjh_graph_store_binary(self._h, str(path).encode(), eid_bytes):
+            raise OSError(f"cannot write {path}")
+
+    def store_text(self, path: str) -> None:
+        if native.host().pjh_graph_store_text(self._h, str(path).encode()):
+            raise OSError(f"cannot write {path}")
+
+    def store_vertex_file(self, path: str) -> None:
+        if native.host().pjh_graph_store_vertex_file(self._h, str(path).encode()):
+            raise OSError(f"cannot write {path}")
+
+    def asymmetric_slots(self) -> int:
+        return native.host().pjh_graph_asymmetric_slots(self._h)
+
+
+class Constraints:
+    """All constraints of a template, in the reference's generation order."""
+
+    def __init__(self, pattern: Graph):
+        out = C.c_void_p()
+        err = C.create_string_buffer(512)
+        if native.host().pjh_constraints_build(pattern.handle, C.byref(out), err, len(err)):
+            raise ValueError(err.value.decode())
+        self._h = out
+        self.pattern = pattern
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            native.host().pjh_constraints_free(h)
+            self._h = C.c_void_p(0)
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    @property
+    def desc(self):
+        return native.host().pjh_constraints_desc(self._h)
+
+    def dump(self) -> str:
+        n = native.host().pjh_constraints_dump(self._h, None, 0)
+        buf = C.create_string_buffer(n + 1)
+        native.host().pjh_constraints_dump(self._h, buf, n + 1)
+        return buf.value.decode()
+
+    @property
+    def vertex_count(self) -> int:
+        return self.desc.contents.vertex_count
+
+    def cycles(self) -> list[list[int]]:
+        d = self.desc.contents
+        return [[d.cycle_vertex[i] for i in range(d.cycle_offset[c], d.cycle_offset[c + 1])]
+                for c in range(d.cycle_count)]
+
+    def paths(self) -> list[list[int]]:
+        d = self.desc.contents
+        return [[d.path_vertex[i] for i in range(d.path_offset[c], d.path_offset[c + 1])]
+                for c in range(d.path_count)]
+
+    def path_labels(self) -> list[int]:
+        d = self.desc.contents
+        return [d.path_initial_label[c] for c in range(d.path_count)]
+
+    def tds(self) -> list[list[tuple[int, list[tuple[int, int, int]]]]]:
+        """Per TDS constraint: [(root, [(move, vertex, move_back), ...]), ...] in walk order."""
+        d = self.desc.contents
+        out = []
+        for c in range(d.tds_count):
+            out.append([(d.walks[k].root, d.walks[k].moves())
+                        for k in range(d.tds_walk_offset[c], d.tds_walk_offset[c + 1])])
+        return out
+
+    def enumeration_walk(self) -> tuple[int, list[tuple[int, int, int]]]:
+        w = self.desc.contents.enumeration_walk
+        return w.root, w.moves()
+
+    def local(self) -> list[dict]:
+        d = self.desc.contents
+        out = []
+        for p in range(d.vertex_count):
+            need = {d.need_label[i]: d.need_count[i] for i in range(d.need_offset[p], d.need_offset[p + 1])}
+            out.append({"label": d.vertex_label[p], "need": need, "neighbors": d.neighbor_mask[p]})
+        return out
+
+    def memo_inert(self, step: int, index: int) -> int:
+        return native.host().pjh_constraints_memo_inert(self._h, step, index)
+
+
+@dataclass
+class State:
+    cand: np.ndarray      # uint64 [V]
+    vflags: np.ndarray    # uint8 [V]: bit0 active, bit1 modified, bit2 scheduled
+    ea_words: np.ndarray  # uint32 [ceil(E/32)]
+    active_vertices: int
+    active_edges: int
+
+    def edge_active(self, edge_count: int) -> np.ndarray:
+        bits = np.unpackbits(self.ea_words.view(np.uint8), bitorder="little")
+        return bits[:edge_count].astype(bool)
+
+
+class Pruner:
+    """One device context (pj_ctx) with a resident graph and template."""
+
+    def __init__(self, graph: Graph, constraints: Constraints | None = None, device: int = 0):
+        lib = native.hip()
+        ctx = C.c_void_p()
+        if lib.pj_create(C.byref(ctx), device):
+            raise RuntimeError("pj_create failed (no HIP device?)")
+        self._ctx = ctx
+        self.graph = graph
+        off, ed, lab = graph.raw_arrays()
+        if not lab:
+            raise ValueError("graph has no labels")
+        self._check(lib.pj_upload_graph(ctx, graph.vertex_count, graph.edge_count, off, ed, lab))
+        self.constraints = None
+        if constraints is not None:
+            self.set_template(constraints)
+
+    def _check(self, rc: int) -> None:
+        if rc:
+            raise RuntimeError(native.hip().pj_last_error(self._ctx).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_ctx", None) is not None and self._ctx.value:
+            native.hip().pj_destroy(self._ctx)
+            self._ctx = C.c_void_p(0)
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def ctx(self) -> C.c_void_p:
+        return self._ctx
+
+    def set_template(self, constraints: Constraints) -> None:
+        self.constraints = constraints
+        self._check(native.hip().pj_set_template(self._ctx, constraints.desc))
+
+    def reset(self) -> None:
+        self._check(native.hip().pj_state_reset(self._ctx))
+
+    def reset_modified(self) -> None:
+        self._check(native.hip().pj_reset_modified(self._ctx))
+
+    def step(self, step: int, index: int = 0) -> AlgoResults:
+        r = AlgoResults()
+        self._check(native.hip().pj_run_step(self._ctx, step, index, C.byref(r)))
+        return r
+
+    def last_ms(self) -> float:
+        ms = C.c_double()
+        native.hip().pj_last_step_ms(self._ctx, C.byref(ms))
+        return ms.value
+
+    def last_slots(self) -> int:
+        n = C.c_uint64()
+        native.hip().pj_last_step_slots(self._ctx, C.byref(n))
+        return n.value
+
+    def counters(self) -> tuple[int, int]:
+        av, ae = C.c_uint64(), C.c_uint64()
+        native.hip().pj_get_counters(self._ctx, C.byref(av), C.byref(ae))
+        return av.value, ae.value
+
+    def state(self) -> State:
+        V, E = self.graph.vertex_count, self.graph.edge_count
+        cand = np.zeros(V, dtype=np.uint64)
+        vf = np.zeros(V, dtype=np.uint8)
+        ea = np.zeros((E + 31) // 32, dtype=np.uint32)
+        self._check(native.hip().pj_download_state(self._ctx, _ptr(cand, C.c_uint64), _ptr(vf, C.c_uint8),
+                                                   _ptr(ea, C.c_uint32)))
+        av, ae = self.counters()
+        return State(cand, vf, ea, av, ae)
+
+    def load_state(self, st: State) -> None:
+        cand = np.ascontiguousarray(st.cand, dtype=np.uint64)
+        vf = np.ascontiguousarray(st.vflags, dtype=np.uint8)
+        ea = np.ascontiguousarray(st.ea_words, dtype=np.uint32)
+        self._check(native.hip().pj_upload_state(self._ctx, _ptr(cand, C.c_uint64), _ptr(vf, C.c_uint8),
+                                                 _ptr(ea, C.c_uint32), st.active_vertices, st.active_edges))
+
+    def run(self, max_rows: int = 1000, enumerate_matches: bool = True) -> tuple[list[Row], float]:
+        """Whole step machine; returns (rows, total device ms).  Without
+        enumerate_matches it stops at the pruning fixed point."""
+        rows = (Row * max_rows)()
+        total = C.c_double()
+        flags = 0 if enumerate_matches else 1
+        n = native.host().pjh_run_pipeline(self._ctx, self.constraints.handle, flags, rows, max_rows,
+                                           C.byref(total))
+        if n < 0:
+            self._check(1)
+        return list(rows[: min(n, max_rows)]), total.value
+
+
+def run_cli(args: list[str]) -> int:
+    """patternmatching_binary in-process (writes results.totem etc.)."""
+    argv = (C.c_char_p * (len(args) + 1))(b"patternmatching_hip", *[a.encode() for a in args])
+    return native.host().pjh_main(len(args) + 1, argv)
+
+
+__all__ = ["Graph", "Constraints", "Pruner", "State", "run_cli", "STEP_LCC0", "STEP_LCC", "STEP_CC", "STEP_PC",
+           "STEP_TDS", "STEP_ENUMERATION"]

@@ -1,0 +1,1535 @@
+// MI355X (gfx950) kernels of the PruneJuice pruning path and the C ABI of
+// include/prunejuice.h.
+//
+// HBM layout of one context (V vertices, E directed CSR slots, P <= 64
+// template vertices, word = uint32 when P <= 32 else uint64):
+//   offsets  u64[V+1]   edges u32[E]   labels u64[V] (raw vertex values)
+//   code     u8[V]      template label code of each vertex (0xFF: no template vertex has it)
+//   cand     word[V]    candidate set; a vertex is active iff cand != 0
+//   unmatch  word[V]    toUnmatch set of the current step
+//   flags    u8[V]      bit1 modified, bit2 scheduled
+//   ea       u32[ceil(E/32)]  edge-active bitset
+// The reference keeps uint32 flags per vertex and per edge and two 64-bit
+// bitmaps per vertex (multiple_label_common_cpu.h:23-59); activity is folded
+// into the candidate word here because the reference deactivates a vertex
+// exactly when its candidate set becomes empty (lcc0_cpu.tpp:42-45,
+// lcc_cpu.tpp:114-117, cc_cpu.tpp:141-147).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/prunejuice.h"
+
+namespace {
+
+constexpr uint8_t kModified = 2;
+constexpr uint8_t kScheduled = 4;
+constexpr uint8_t kNoCode = 0xFF;
+constexpr int kBlock = 256;
+constexpr int kCounterSlots = 256;  // per-block counters are spread over slots
+constexpr uint32_t kHubDegree = 2048;  // rows above this are split into chunks
+constexpr uint32_t kHubChunk = 2048;
+constexpr int kMaxLabels = 64;
+
+// Counter slots: [slot][0..4] = vertex / edge / match eliminated,
+// enumeration, CSR slots examined (the TEPS numerator: every slot a serial
+// scan in the reference's loop order would read).
+enum { C_VERTEX = 0, C_EDGE = 1, C_MATCH = 2, C_ENUM = 3, C_SLOTS = 4 };
+constexpr int kCounterStride = 8;
+
+// Template tables in device memory.
+struct DevTemplate {
+  uint32_t P;
+  uint32_t K;                       // distinct template labels (codes 0..K-1)
+  uint64_t label_raw[kMaxLabels];   // label value of each code
+  uint64_t code_vertices[kMaxLabels];  // template vertices carrying each code
+  uint8_t pcode[64];                // code of each template vertex
+  uint64_t nbr[64];                 // template neighbours of each vertex
+  uint64_t need[64][8];             // required neighbour count per code, one byte per code
+  // NLCC constraint currently installed for the step kernels.
+  uint32_t nlcc_len;
+  uint32_t nlcc_vertex[64];         // cycle / path vertices, or walk template vertices
+  uint8_t nlcc_move[128];
+  uint32_t nlcc_mb[128];
+  uint32_t nlcc_hidx[128];          // history index written by each VERTEX move
+  uint32_t walk_count;              // walks of the TDS constraint (<= 64)
+  uint32_t walk_root[64];
+  uint32_t walk_off[65];            // into walk_move / walk_vertex / walk_mb / walk_hidx
+  uint8_t walk_move[64 * 128];
+  uint32_t walk_vertex[64 * 128];
+  uint32_t walk_mb[64 * 128];
+  uint32_t walk_hidx[64 * 128];
+  uint32_t path_code;               // code of the path's initial label
+};
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t rl32(uint32_t v, int lane) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), lane));
+}
+__device__ __forceinline__ uint64_t rl64(uint64_t v, int lane) {
+  uint32_t lo = rl32(static_cast<uint32_t>(v), lane);
+  uint32_t hi = rl32(static_cast<uint32_t>(v >> 32), lane);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+template <typename Word>
+__device__ __forceinline__ int popc(Word w) {
+  return sizeof(Word) == 8 ? __popcll(static_cast<unsigned long long>(w)) : __popc(static_cast<unsigned>(w));
+}
+template <typename Word>
+__device__ __forceinline__ int ctz(Word w) {
+  return sizeof(Word) == 8 ? __builtin_ctzll(static_cast<unsigned long long>(w)) : __builtin_ctz(static_cast<unsigned>(w));
+}
+
+__device__ __forceinline__ bool ea_test(const uint32_t* __restrict__ ea, uint64_t e) {
+  return (ea[e >> 5] >> (e & 31)) & 1u;
+}
+
+// Wave sum of a per-lane value, added by lane 0 to a spread counter slot.
+// Must be called with every lane of the wave active.
+__device__ __forceinline__ void wave_add(unsigned long long* counters, int which, uint64_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (lane_id() == 0 && v) {
+    int slot = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) & (kCounterSlots - 1);
+    atomicAdd(&counters[slot * kCounterStride + which], static_cast<unsigned long long>(v));
+  }
+}
+
+__device__ __forceinline__ void set_flag_atomic(uint8_t* flags, uint32_t v, uint8_t bit) {
+  uint32_t* word = reinterpret_cast<uint32_t*>(reinterpret_cast<uintptr_t>(flags + v) & ~uintptr_t(3));
+  atomicOr(word, static_cast<uint32_t>(bit) << (8 * (v & 3)));
+}
+
+// ---------------------------------------------------------------------------
+// Set-up kernels.
+
+template <typename Word>
+__global__ void k_state_init(Word* cand, Word* unmatch, uint8_t* flags, uint32_t V, Word full) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+    cand[v] = full;
+    unmatch[v] = 0;
+    flags[v] = kScheduled;
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t value) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long value) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+__global__ void k_clear_modified(uint8_t* flags, uint32_t V) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
+    flags[v] &= static_cast<uint8_t>(~kModified);
+}
+
+__global__ void k_label_codes(const uint64_t* __restrict__ labels, uint8_t* __restrict__ code, uint32_t V,
+                              const DevTemplate* __restrict__ t) {
+  const uint32_t K = t->K;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t l = labels[v];
+    uint8_t c = kNoCode;
+    for (uint32_t k = 0; k < K; ++k)
+      if (t->label_raw[k] == l) {
+        c = static_cast<uint8_t>(k);
+        break;
+      }
+    code[v] = c;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LCC0 (multiple_label_lcc0_cpu.tpp:25-47): drop every candidate whose label
+// differs from the vertex label; deactivate vertices left without candidates.
+
+template <typename Word>
+__global__ void k_lcc0(Word* __restrict__ cand, const uint8_t* __restrict__ code, uint32_t V,
+                       const DevTemplate* __restrict__ t, unsigned long long* counters) {
+  uint64_t velim = 0, melim = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t span = ((V + stride - 1) / stride) * stride;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < span; v += stride) {
+    if (v < V) {
+      Word c = cand[v];
+      if (c) {
+        uint8_t k = code[v];
+        Word keep = k == kNoCode ? Word(0) : (c & static_cast<Word>(t->code_vertices[k]));
+        if (keep != c) {
+          melim += popc<Word>(c & ~keep);
+          cand[v] = keep;
+          if (!keep) ++velim;
+        }
+      }
+    }
+  }
+  wave_add(counters, C_VERTEX, velim);
+  wave_add(counters, C_MATCH, melim);
+}
+
+// ---------------------------------------------------------------------------
+// Frontier construction.  Vertices are appended to degree bins with one
+// atomic per wave and bin; hubs (degree > kHubDegree) go to no bin, they are
+// served by the static chunk list built at upload.
+// mode 0: active and scheduled (LCC phase 1); mode 1: active (phase 2,
+// NLCC task list); mode 2: active with label code == t->path_code (PC).
+
+constexpr int kBins = 3;
+__constant__ uint32_t c_bin_limit[kBins] = {4, 32, kHubDegree};
+
+template <typename Word>
+__global__ void k_build_bins(const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
+                             const uint8_t* __restrict__ code, const uint64_t* __restrict__ off, uint32_t V,
+                             int mode, const DevTemplate* __restrict__ t, uint32_t* __restrict__ bins,
+                             uint32_t* __restrict__ bin_count) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t span = ((V + stride - 1) / stride) * stride;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < span; v += stride) {
+    int bin = -1;
+    if (v < V && cand[v]) {
+      bool take = true;
+      if (mode == 0) take = flags[v] & kScheduled;
+      if (mode == 2) take = code[v] == t->path_code;
+      if (take) {
+        uint64_t d = off[v + 1] - off[v];
+        if (d <= c_bin_limit[0])
+          bin = 0;
+        else if (d <= c_bin_limit[1])
+          bin = 1;
+        else if (d <= c_bin_limit[2])
+          bin = 2;
+      }
+    }
+    for (int b = 0; b < kBins; ++b) {
+      uint64_t m = __ballot(bin == b);
+      if (!m) continue;
+      uint32_t base = 0;
+      int leader = __builtin_ctzll(m);
+      if (lane_id() == leader) base = atomicAdd(&bin_count[b], static_cast<uint32_t>(__popcll(m)));
+      base = __shfl(base, leader, 64);
+      if (bin == b) {
+        uint32_t rank = __popcll(m & ((1ull << lane_id()) - 1));
+        bins[static_cast<uint64_t>(b) * V + base + rank] = static_cast<uint32_t>(v);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LCC phase 1 (multiple_label_lcc_cpu.tpp:26-102) for one vertex per group of
+// G lanes.  Slots are read G at a time; a slot whose neighbour (active, over
+// an active slot) has no candidate among the template neighbours of v's
+// candidates loses its edge; the others are counted per neighbour label with
+// one ballot per label code into saturating byte counters (cap 127).  Each
+// candidate p of v is then kept only if v's label is p's label and every
+// required neighbour-label count of p is met.
+
+template <typename Word, int KW>
+__device__ __forceinline__ Word lcc_requirements(Word cv, uint8_t vcode, const uint64_t (&cnt)[KW],
+                                                 const DevTemplate* __restrict__ t) {
+  const uint64_t H = 0x8080808080808080ull;
+  Word fail = 0;
+  for (Word m = cv; m; m &= m - 1) {
+    int p = ctz<Word>(m);
+    bool bad = t->pcode[p] != vcode;
+#pragma unroll
+    for (int w = 0; w < KW; ++w) {
+      uint64_t d = (cnt[w] | H) - t->need[p][w];
+      bad |= ((~d) & H) != 0;
+    }
+    if (bad) fail |= Word(1) << p;
+  }
+  return fail;
+}
+
+template <int KW>
+__device__ __forceinline__ void count_labels(uint64_t (&cnt)[KW], bool counted, uint32_t lab, uint64_t gmask,
+                                             uint32_t K) {
+#pragma unroll
+  for (int w = 0; w < KW; ++w) {
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint32_t L = w * 8 + b;
+      if (L >= K) break;
+      uint64_t m = __ballot(counted && lab == L) & gmask;
+      uint32_t c = __popcll(m);
+      if (c) {
+        uint32_t cur = static_cast<uint32_t>(cnt[w] >> (8 * b)) & 0xFFu;
+        cur = min(cur + c, 127u);
+        cnt[w] = (cnt[w] & ~(0xFFull << (8 * b))) | (static_cast<uint64_t>(cur) << (8 * b));
+      }
+    }
+  }
+}
+
+template <typename Word, int G, int KW>
+__global__ void __launch_bounds__(kBlock) k_lcc_phase1(
+    const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code, const Word* __restrict__ cand,
+    Word* __restrict__ unmatch, uint8_t* __restrict__ flags, uint32_t* __restrict__ ea,
+    const DevTemplate* __restrict__ t, unsigned long long* counters) {
+  const int lane = lane_id();
+  const int gl = lane & (G - 1);
+  const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << (lane & ~(G - 1)));
+  const uint32_t n = *list_count;
+  const uint32_t K = t->K;
+  const uint64_t groups_per_grid = (uint64_t)gridDim.x * (blockDim.x / G);
+  const uint64_t span = ((n + groups_per_grid - 1) / groups_per_grid) * groups_per_grid;
+  uint64_t elim_total = 0, slots_total = 0;
+  for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G; g < span; g += groups_per_grid) {
+    if (g < n) {
+      const uint32_t v = list[g];
+      const Word cv = cand[v];
+      Word U = 0;
+      for (Word m = cv; m; m &= m - 1) U |= static_cast<Word>(t->nbr[ctz<Word>(m)]);
+      const uint64_t beg = off[v], end = off[v + 1];
+      uint64_t cnt[KW];
+#pragma unroll
+      for (int w = 0; w < KW; ++w) cnt[w] = 0;
+      uint32_t elim = 0;
+      for (uint64_t base = beg; base < end; base += G) {
+        const uint64_t e = base + gl;
+        bool counted = false;
+        uint32_t lab = kNoCode;
+        if (e < end) {
+          const uint32_t word = ea[e >> 5];
+          if ((word >> (e & 31)) & 1u) {
+            const uint32_t u = edges[e];
+            const Word cu = cand[u];
+            if (cu) {
+              if ((cu & U) == 0) {
+                atomicAnd(&ea[e >> 5], ~(1u << (e & 31)));
+                ++elim;
+              } else {
+                lab = code[u];
+                counted = lab != kNoCode;
+              }
+            }
+          }
+        }
+        count_labels<KW>(cnt, counted, lab, gmask, K);
+      }
+      const bool any_elim = (__ballot(elim != 0) & gmask) != 0;
+      elim_total += elim;
+      if (gl == 0) {
+        slots_total += end - beg;
+        const Word fail = lcc_requirements<Word, KW>(cv, code[v], cnt, t);
+        if (fail) unmatch[v] = fail;
+        if (fail || any_elim) flags[v] |= kModified;
+      }
+    }
+  }
+  wave_add(counters, C_EDGE, elim_total);
+  wave_add(counters, C_SLOTS, slots_total);
+}
+
+// Hub rows: the static chunk list holds (vertex, first slot) per chunk of
+// kHubChunk slots; chunks of a vertex outside the current frontier exit.
+// Label counts and eliminations are summed in hub_count[h][code] and
+// hub_elim[h]; k_lcc_hub_finish evaluates the requirements and clears them.
+
+template <typename Word>
+__global__ void __launch_bounds__(kBlock) k_lcc_hub_chunks(
+    const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
+    const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+    const uint8_t* __restrict__ code, const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
+    uint32_t* __restrict__ ea, uint32_t* __restrict__ hub_count, uint32_t* __restrict__ hub_elim,
+    const DevTemplate* __restrict__ t, unsigned long long* counters) {
+  __shared__ uint32_t s_cnt[kMaxLabels];
+  __shared__ uint32_t s_elim;
+  const uint32_t K = t->K;
+  uint64_t elim_total = 0, slots_total = 0;
+  for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const uint32_t h = chunk_hub[c];
+    const uint32_t v = hubs[h];
+    const Word cv = cand[v];
+    if (!cv || !(flags[v] & kScheduled)) continue;  // block-uniform
+    if (threadIdx.x < kMaxLabels) s_cnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_elim = 0;
+    __syncthreads();
+    Word U = 0;
+    for (Word m = cv; m; m &= m - 1) U |= static_cast<Word>(t->nbr[ctz<Word>(m)]);
+    const uint64_t beg = chunk_start[c];
+    const uint64_t end = min(off[v + 1], beg + kHubChunk);
+    uint32_t elim = 0;
+    for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) {
+      if (!ea_test(ea, e)) continue;
+      const uint32_t u = edges[e];
+      const Word cu = cand[u];
+      if (!cu) continue;
+      if ((cu & U) == 0) {
+        atomicAnd(&ea[e >> 5], ~(1u << (e & 31)));
+        ++elim;
+      } else {
+        uint8_t l = code[u];
+        if (l != kNoCode) atomicAdd(&s_cnt[l], 1u);
+      }
+    }
+    if (elim) atomicAdd(&s_elim, elim);
+    elim_total += elim;
+    __syncthreads();
+    if (threadIdx.x < K && s_cnt[threadIdx.x]) atomicAdd(&hub_count[h * kMaxLabels + threadIdx.x], s_cnt[threadIdx.x]);
+    if (threadIdx.x == 0 && s_elim) atomicAdd(&hub_elim[h], s_elim);
+    if (threadIdx.x == 0) slots_total += end - beg;
+    __syncthreads();
+  }
+  wave_add(counters, C_EDGE, elim_total);
+  wave_add(counters, C_SLOTS, slots_total);
+}
+
+template <typename Word, int KW>
+__global__ void k_lcc_hub_finish(const uint32_t* __restrict__ hubs, uint32_t nhubs, const uint8_t* __restrict__ code,
+                                 const Word* __restrict__ cand, Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
+                                 uint32_t* __restrict__ hub_count, uint32_t* __restrict__ hub_elim,
+                                 const DevTemplate* __restrict__ t) {
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nhubs; h += gridDim.x * blockDim.x) {
+    const uint32_t v = hubs[h];
+    const Word cv = cand[v];
+    uint64_t cnt[KW];
+#pragma unroll
+    for (int w = 0; w < KW; ++w) cnt[w] = 0;
+    for (uint32_t k = 0; k < t->K; ++k) {
+      uint32_t c = min(hub_count[h * kMaxLabels + k], 127u);
+      hub_count[h * kMaxLabels + k] = 0;
+      cnt[k >> 3] |= static_cast<uint64_t>(c) << (8 * (k & 7));
+    }
+    const uint32_t elim = hub_elim[h];
+    hub_elim[h] = 0;
+    if (!cv || !(flags[v] & kScheduled)) continue;
+    const Word fail = lcc_requirements<Word, KW>(cv, code[v], cnt, t);
+    if (fail) unmatch[v] = fail;
+    if (fail || elim) flags[v] |= kModified;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Phase 2, shared by LCC and every NLCC step (lcc_cpu.tpp:107-142,
+// cc_cpu.tpp:134-174): a modified vertex drops its toUnmatch set, is
+// deactivated when nothing is left and is scheduled; any other active vertex
+// is scheduled iff one of its active slots leads to a modified vertex
+// (active or not).  Modified flags are left set.
+
+template <typename Word, int G>
+__global__ void __launch_bounds__(kBlock) k_phase2(
+    const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ edges, Word* __restrict__ cand, Word* __restrict__ unmatch,
+    uint8_t* __restrict__ flags, const uint32_t* __restrict__ ea, unsigned long long* counters) {
+  const int lane = lane_id();
+  const int gl = lane & (G - 1);
+  const int gbase = lane & ~(G - 1);
+  const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
+  const uint32_t n = *list_count;
+  const uint64_t groups_per_grid = (uint64_t)gridDim.x * (blockDim.x / G);
+  const uint64_t span = ((n + groups_per_grid - 1) / groups_per_grid) * groups_per_grid;
+  uint64_t velim = 0, melim = 0, slots = 0;
+  for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G; g < span; g += groups_per_grid) {
+    if (g < n) {
+      const uint32_t v = list[g];
+      const uint8_t f = flags[v];
+      if (f & kModified) {
+        if (gl == 0) {
+          const Word tu = unmatch[v];
+          if (tu) {
+            const Word c = cand[v];
+            const Word nc = c & ~tu;
+            melim += popc<Word>(tu);
+            cand[v] = nc;
+            if (!nc) ++velim;
+            unmatch[v] = 0;
+          }
+          flags[v] = f | kScheduled;
+        }
+      } else {
+        const uint64_t beg = off[v], end = off[v + 1];
+        bool hit = false;
+        uint64_t scanned = end - beg;
+        for (uint64_t base = beg; base < end; base += G) {
+          const uint64_t e = base + gl;
+          bool h = false;
+          if (e < end && ea_test(ea, e)) h = flags[edges[e]] & kModified;
+          const uint64_t m = __ballot(h) & gmask;
+          if (m) {
+            hit = true;
+            scanned = base + (__builtin_ctzll(m) - gbase) + 1 - beg;
+            break;
+          }
+        }
+        if (gl == 0) {
+          flags[v] = hit ? (f | kScheduled) : (f & ~kScheduled);
+          slots += scanned;
+        }
+      }
+    }
+  }
+  wave_add(counters, C_VERTEX, velim);
+  wave_add(counters, C_MATCH, melim);
+  wave_add(counters, C_SLOTS, slots);
+}
+
+// Hub phase 2: a hub that is not modified is scheduled when any chunk finds
+// an active slot to a modified vertex.  Chunks take the minimum offset of
+// such a slot in the row (hub_hit[h], 0xFFFFFFFF when none) and
+// k_phase2_hub_finish applies the result and resets hub_hit.
+template <typename Word>
+__global__ void __launch_bounds__(kBlock) k_phase2_hub_chunks(
+    const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
+    const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+    const Word* __restrict__ cand, const uint8_t* __restrict__ flags, const uint32_t* __restrict__ ea,
+    uint32_t* __restrict__ hub_hit) {
+  __shared__ uint32_t s_min;
+  for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const uint32_t h = chunk_hub[c];
+    const uint32_t v = hubs[h];
+    if (!cand[v] || (flags[v] & kModified)) continue;
+    if (threadIdx.x == 0) s_min = 0xFFFFFFFFu;
+    __syncthreads();
+    const uint64_t row = off[v];
+    const uint64_t beg = chunk_start[c];
+    const uint64_t end = min(off[v + 1], beg + kHubChunk);
+    for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x)
+      if (ea_test(ea, e) && (flags[edges[e]] & kModified)) {
+        atomicMin(&s_min, static_cast<uint32_t>(e - row));
+        break;
+      }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_min != 0xFFFFFFFFu) atomicMin(&hub_hit[h], s_min);
+    __syncthreads();
+  }
+}
+
+template <typename Word>
+__global__ void k_phase2_hub_finish(const uint32_t* __restrict__ hubs, uint32_t nhubs,
+                                    const uint64_t* __restrict__ off, Word* __restrict__ cand,
+                                    Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
+                                    uint32_t* __restrict__ hub_hit, unsigned long long* counters) {
+  uint64_t velim = 0, melim = 0, slots = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t span = ((nhubs + stride - 1) / stride) * stride;
+  for (uint64_t h = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; h < span; h += stride) {
+    if (h >= nhubs) continue;
+    const uint32_t v = hubs[h];
+    const uint32_t hit = hub_hit[h];
+    hub_hit[h] = 0xFFFFFFFFu;
+    const Word c = cand[v];
+    if (!c) continue;
+    const uint8_t f = flags[v];
+    if (f & kModified) {
+      const Word tu = unmatch[v];
+      if (tu) {
+        const Word nc = c & ~tu;
+        melim += popc<Word>(tu);
+        cand[v] = nc;
+        if (!nc) ++velim;
+        unmatch[v] = 0;
+      }
+      flags[v] = f | kScheduled;
+    } else {
+      const bool any = hit != 0xFFFFFFFFu;
+      flags[v] = any ? (f | kScheduled) : (f & ~kScheduled);
+      slots += any ? hit + 1ull : off[v + 1] - off[v];
+    }
+  }
+  wave_add(counters, C_VERTEX, velim);
+  wave_add(counters, C_MATCH, melim);
+  wave_add(counters, C_SLOTS, slots);
+}
+
+// ---------------------------------------------------------------------------
+// NLCC: one wavefront per source vertex runs a depth-first walk whose row
+// scans are done 64 slots at a time (ballot, lowest lane first, so the walk
+// visits neighbours in CSR order).  Level i of the walk keeps its vertex and
+// its resume slot in lane i's registers (walks are at most 64 levels).  The
+// search is complete: it decides whether a walk satisfying the constraint
+// exists (see the NLCC notes in the oracle for when this equals the
+// reference's memoised search).
+
+struct RowScan {
+  uint32_t u;
+  uint64_t next;  // slot after the hit
+  bool found;
+};
+
+// First active slot in [from, end) of a row whose neighbour is active and has
+// template vertex q (q < 0: neighbour must equal `target`; `excl`: neighbour
+// must differ from it).  Adds to `slots` the slots a serial scan reads (up to
+// and including the hit).
+template <typename Word>
+__device__ __forceinline__ RowScan scan_row(uint64_t from, uint64_t end, const uint32_t* __restrict__ edges,
+                                            const uint32_t* __restrict__ ea, const Word* __restrict__ cand, int q,
+                                            uint32_t target, uint32_t excl, uint64_t& slots) {
+  RowScan r{0, end, false};
+  const int lane = lane_id();
+  for (uint64_t base = from; base < end; base += 64) {
+    const uint64_t e = base + lane;
+    bool ok = false;
+    uint32_t u = 0;
+    if (e < end && ea_test(ea, e)) {
+      u = edges[e];
+      if (q < 0) {
+        ok = u == target;
+      } else {
+        Word cu = cand[u];
+        ok = ((cu >> q) & 1) && u != excl;
+      }
+    }
+    uint64_t m = __ballot(ok);
+    if (m) {
+      int first = __builtin_ctzll(m);
+      r.u = rl32(u, first);
+      r.next = base + first + 1;
+      r.found = true;
+      slots += r.next - from;
+      return r;
+    }
+  }
+  if (end > from) slots += end - from;
+  return r;
+}
+
+// Cycle constraint idx[0..L-1] from start position j (cc_cpu.tpp:12-70).
+template <typename Word>
+__device__ bool cycle_walk(uint32_t s, uint32_t j, uint32_t L, const uint64_t* __restrict__ off,
+                           const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
+                           const Word* __restrict__ cand, const DevTemplate* __restrict__ t, uint64_t& slots) {
+  const int lane = lane_id();
+  uint32_t st_v = s;          // lane i: vertex at level i
+  uint64_t st_c = off[s];     // lane i: resume slot at level i
+  int depth = 0;
+  uint32_t v = s;
+  uint64_t cursor = off[s];
+  while (true) {
+    if (depth == static_cast<int>(L) - 1) {
+      RowScan r = scan_row<Word>(off[v], off[v + 1], edges, ea, cand, -1, s, 0xFFFFFFFFu, slots);
+      if (r.found) return true;
+      depth = uniform(depth - 1);
+      v = rl32(st_v, depth);
+      cursor = rl64(st_c, depth);
+      continue;
+    }
+    const int q = static_cast<int>(t->nlcc_vertex[(j + depth + 1) % L]);
+    RowScan r = scan_row<Word>(cursor, off[v + 1], edges, ea, cand, q, 0, 0xFFFFFFFFu, slots);
+    if (r.found) {
+      if (lane == depth) st_c = r.next;
+      depth = uniform(depth + 1);
+      v = r.u;
+      cursor = off[v];
+      if (lane == depth) {
+        st_v = v;
+        st_c = cursor;
+      }
+    } else {
+      if (depth == 0) return false;
+      depth = uniform(depth - 1);
+      v = rl32(st_v, depth);
+      cursor = rl64(st_c, depth);
+    }
+  }
+}
+
+// Path constraint (pc_cpu.tpp:11-75): from position 0 forwards or from the
+// last position backwards over L-1 edges, ending on a vertex other than s.
+template <typename Word>
+__device__ bool path_walk(uint32_t s, bool reverse, uint32_t L, const uint64_t* __restrict__ off,
+                          const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
+                          const Word* __restrict__ cand, const DevTemplate* __restrict__ t, uint64_t& slots) {
+  const int lane = lane_id();
+  uint32_t st_v = s;
+  uint64_t st_c = off[s];
+  int depth = 0;
+  uint32_t v = s;
+  uint64_t cursor = off[s];
+  while (true) {
+    const uint32_t pos = reverse ? (L - 2 - depth) : (depth + 1);
+    const int q = static_cast<int>(t->nlcc_vertex[pos]);
+    const bool last = depth == static_cast<int>(L) - 2;
+    RowScan r = scan_row<Word>(cursor, off[v + 1], edges, ea, cand, q, 0, last ? s : 0xFFFFFFFFu, slots);
+    if (r.found) {
+      if (last) return true;
+      if (lane == depth) st_c = r.next;
+      depth = uniform(depth + 1);
+      v = r.u;
+      cursor = off[v];
+      if (lane == depth) {
+        st_v = v;
+        st_c = cursor;
+      }
+    } else {
+      if (depth == 0) return false;
+      depth = uniform(depth - 1);
+      v = rl32(st_v, depth);
+      cursor = rl64(st_c, depth);
+    }
+  }
+}
+
+// Walk over VERTEX / CHECK / MOVE_BACK moves (tds_cpu.tpp:13-127 and, with
+// counting and injectivity, enumeration_cpu.tpp:38-118).  Lane p keeps the
+// vertex the walk stands on before move p (st_v) and the resume slot of move
+// p (st_c); lane h keeps history entry h.  Walks have at most 64 moves.
+// Returns the number of completions when `count_all`, else 1 on the first
+// completion.  When a search without counting backtracks through a CHECK,
+// the rest of that row is added to `slots`, as a serial scan would read it
+// looking for a second slot to the same vertex.
+template <typename Word>
+__device__ uint64_t template_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ mv,
+                                  const uint32_t* __restrict__ vx, const uint32_t* __restrict__ mb,
+                                  const uint32_t* __restrict__ hidx, bool count_all,
+                                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+                                  const uint32_t* __restrict__ ea, const Word* __restrict__ cand, uint64_t& slots) {
+  const int lane = lane_id();
+  uint32_t st_v = s;
+  uint64_t st_c = 0;
+  uint32_t hist = s;
+  uint64_t total = 0;
+  int pos = 1;
+  uint32_t cur = s;
+  bool resume = false;  // re-entering a VERTEX move after backtracking
+  if (len <= 1) return 1;
+  auto is_vertex = [&](int p) { return mv[p] == PJ_MOVE_VERTEX_STORE || mv[p] == PJ_MOVE_VERTEX_NO_STORE; };
+  // Back to the most recent VERTEX move; false when none is left.
+  auto backtrack = [&]() -> bool {
+    pos = uniform(pos - 1);
+    while (pos >= 1 && !is_vertex(pos)) {
+      if (!count_all && mv[pos] == PJ_MOVE_CHECK) {
+        const uint32_t at = rl32(st_v, pos);
+        slots += off[at + 1] - rl64(st_c, pos);
+      }
+      pos = uniform(pos - 1);
+    }
+    if (pos < 1) return false;
+    cur = rl32(st_v, pos);
+    resume = true;
+    return true;
+  };
+  while (true) {
+    if (pos == static_cast<int>(len)) {
+      ++total;
+      if (!count_all) return total;
+      if (!backtrack()) return total;
+      continue;
+    }
+    const uint8_t m = mv[pos];
+    if (m == PJ_MOVE_VERTEX_STORE || m == PJ_MOVE_VERTEX_NO_STORE) {
+      const uint64_t from = resume ? rl64(st_c, pos) : off[cur];
+      resume = false;
+      const int q = static_cast<int>(vx[pos]);
+      const uint32_t h = hidx[pos];
+      const uint64_t end = off[cur + 1];
+      RowScan r{0, end, false};
+      for (uint64_t base = from; base < end; base += 64) {
+        const uint64_t e = base + lane;
+        bool ok = false;
+        uint32_t u = 0;
+        if (e < end && ea_test(ea, e)) {
+          u = edges[e];
+          ok = (cand[u] >> q) & 1;
+        }
+        if (count_all)  // injective embeddings; every lane active here
+          for (uint32_t k = 0; k < h; ++k) ok = ok && rl32(hist, static_cast<int>(k)) != u;
+        uint64_t bm = __ballot(ok);
+        if (bm) {
+          int first = __builtin_ctzll(bm);
+          r.u = rl32(u, first);
+          r.next = base + first + 1;
+          r.found = true;
+          break;
+        }
+      }
+      slots += (r.found ? r.next : end) - from;
+      if (r.found) {
+        if (lane == pos) st_c = r.next;
+        if (lane == static_cast<int>(h)) hist = r.u;
+        cur = r.u;
+        pos = uniform(pos + 1);
+        if (lane == pos) st_v = cur;
+        continue;
+      }
+    } else if (m == PJ_MOVE_CHECK) {
+      const uint32_t target = rl32(hist, static_cast<int>(mb[pos]));
+      RowScan r = scan_row<Word>(off[cur], off[cur + 1], edges, ea, cand, -1, target, 0xFFFFFFFFu, slots);
+      if (r.found) {
+        if (lane == pos) st_c = r.next;
+        pos = uniform(pos + 1);
+        if (lane == pos) st_v = cur;
+        continue;
+      }
+    } else {  // MOVE_BACK
+      cur = rl32(hist, static_cast<int>(mb[pos]));
+      pos = uniform(pos + 1);
+      if (lane == pos) st_v = cur;
+      continue;
+    }
+    if (!backtrack()) return total;
+  }
+}
+
+// One wave per source in the task list (active vertices); failing
+// (source, template vertex) pairs go to toUnmatch and mark the source
+// modified (cc_cpu.tpp:94-129, pc_cpu.tpp:95-147, tds_cpu.tpp:153-188).
+// kind 0: cycle, 1: path, 2: TDS walks, 3: enumeration (counts only).
+template <typename Word>
+__global__ void __launch_bounds__(kBlock) k_nlcc(int kind, const uint32_t* __restrict__ list,
+                                                 const uint32_t* __restrict__ list_count,
+                                                 const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+                                                 const uint32_t* __restrict__ ea, const Word* __restrict__ cand,
+                                                 Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
+                                                 const DevTemplate* __restrict__ t, unsigned long long* counters) {
+  const uint32_t n = *list_count;
+  const uint32_t waves = gridDim.x * (blockDim.x / 64);
+  uint64_t enumerated = 0, slots = 0;
+  for (uint32_t w = uniform(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)); w < n; w += waves) {
+    const uint32_t s = list[w];
+    const Word cs = cand[s];
+    Word fail = 0;
+    const uint32_t L = t->nlcc_len;
+    if (kind == 0) {
+      for (uint32_t j = 0; j < L; ++j) {
+        const uint32_t q = t->nlcc_vertex[j];
+        if (!((cs >> q) & 1)) continue;
+        if (!cycle_walk<Word>(s, j, L, off, edges, ea, cand, t, slots)) fail |= Word(1) << q;
+      }
+    } else if (kind == 1) {
+      const uint32_t q0 = t->nlcc_vertex[0], q1 = t->nlcc_vertex[L - 1];
+      if (((cs >> q0) & 1) && !path_walk<Word>(s, false, L, off, edges, ea, cand, t, slots)) fail |= Word(1) << q0;
+      if (((cs >> q1) & 1) && !path_walk<Word>(s, true, L, off, edges, ea, cand, t, slots)) fail |= Word(1) << q1;
+    } else if (kind == 2) {
+      for (uint32_t k = 0; k < t->walk_count; ++k) {
+        const uint32_t x = t->walk_root[k];
+        if (!((cs >> x) & 1)) continue;
+        const uint32_t o = t->walk_off[k];
+        const uint32_t len = t->walk_off[k + 1] - o;
+        if (!template_walk<Word>(s, len, t->walk_move + o, t->walk_vertex + o, t->walk_mb + o, t->walk_hidx + o,
+                                 false, off, edges, ea, cand, slots))
+          fail |= Word(1) << x;
+      }
+    } else {
+      const uint32_t x = t->walk_root[0];
+      if ((cs >> x) & 1) {
+        const uint32_t len = t->walk_off[1];
+        enumerated += template_walk<Word>(s, len, t->walk_move, t->walk_vertex, t->walk_mb, t->walk_hidx, true, off,
+                                          edges, ea, cand, slots);
+      }
+    }
+    if (fail && lane_id() == 0) {
+      if (sizeof(Word) == 8)
+        atomicOr(reinterpret_cast<unsigned long long*>(&unmatch[s]), static_cast<unsigned long long>(fail));
+      else
+        atomicOr(reinterpret_cast<unsigned int*>(&unmatch[s]), static_cast<unsigned int>(fail));
+      set_flag_atomic(flags, s, kModified);
+    }
+  }
+  if (lane_id() == 0) {
+    const int slot = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) & (kCounterSlots - 1);
+    if (enumerated) atomicAdd(&counters[slot * kCounterStride + C_ENUM], static_cast<unsigned long long>(enumerated));
+    if (slots) atomicAdd(&counters[slot * kCounterStride + C_SLOTS], static_cast<unsigned long long>(slots));
+  }
+}
+
+template <typename Word>
+__global__ void k_cand_to_u64(const Word* __restrict__ cand, uint64_t* __restrict__ out, const uint8_t* __restrict__ flags,
+                              uint8_t* __restrict__ vflags, uint32_t V) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t c = static_cast<uint64_t>(cand[v]);
+    out[v] = c;
+    vflags[v] = static_cast<uint8_t>((flags[v] & (kModified | kScheduled)) | (c ? 1 : 0));
+  }
+}
+
+template <typename Word>
+__global__ void k_u64_to_cand(const uint64_t* __restrict__ in, const uint8_t* __restrict__ vflags, Word* __restrict__ cand,
+                              Word* __restrict__ unmatch, uint8_t* __restrict__ flags, uint32_t V) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint8_t vf = vflags[v];
+    cand[v] = (vf & 1) ? static_cast<Word>(in[v]) : Word(0);
+    unmatch[v] = 0;
+    flags[v] = vf & (kModified | kScheduled);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context and C ABI.
+
+struct pj_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  int cus = 256;
+
+  uint32_t V = 0;
+  uint64_t E = 0;
+  uint64_t* d_off = nullptr;
+  uint32_t* d_edges = nullptr;
+  uint64_t* d_labels = nullptr;
+  uint8_t* d_code = nullptr;
+  void* d_cand = nullptr;
+  void* d_unmatch = nullptr;
+  void* d_u64 = nullptr;   // staging for downloads / uploads
+  uint8_t* d_flags = nullptr;
+  uint8_t* d_vflags = nullptr;
+  uint32_t* d_ea = nullptr;
+  uint64_t ea_words = 0;
+  uint32_t* d_bins = nullptr;        // kBins * V
+  uint32_t* d_bin_count = nullptr;   // kBins
+  unsigned long long* d_counters = nullptr;  // kCounterSlots * kCounterStride
+  // hubs
+  uint32_t nhubs = 0, nchunks = 0;
+  uint32_t* d_hubs = nullptr;
+  uint32_t* d_chunk_hub = nullptr;
+  uint64_t* d_chunk_start = nullptr;
+  uint32_t* d_hub_count = nullptr;
+  uint32_t* d_hub_elim = nullptr;
+  uint32_t* d_nhubs = nullptr;       // device copy of nhubs (NLCC task count)
+  uint32_t* d_hub_hit = nullptr;     // phase-2 first-hit offset per hub
+
+  DevTemplate* d_t = nullptr;
+  DevTemplate h_t{};
+  bool have_template = false;
+  int word_bytes = 8;
+  int kw = 1;
+  // host copy of the constraint tables
+  std::vector<std::vector<uint32_t>> cycles, paths;
+  std::vector<uint64_t> path_label;
+  struct HostWalk {
+    uint32_t root;
+    std::vector<uint8_t> move;
+    std::vector<uint32_t> vertex, mb;
+  };
+  std::vector<std::vector<HostWalk>> tds;
+  HostWalk enumeration;
+
+  uint64_t active_vertices = 0, active_edges = 0;
+  float last_ms = 0.f;
+  uint64_t last_slots = 0;
+  std::vector<unsigned long long> h_counters;
+};
+
+namespace {
+
+#define PJ_HIP(call)                                                             \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);              \
+      return -1;                                                                 \
+    }                                                                            \
+  } while (0)
+
+int grid_for(const pj_ctx* ctx, uint64_t items, int per_block = kBlock) {
+  uint64_t g = (items + per_block - 1) / per_block;
+  uint64_t cap = static_cast<uint64_t>(ctx->cus) * 16;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return static_cast<int>(g);
+}
+
+template <typename T>
+int dev_alloc(pj_ctx* ctx, T** p, uint64_t count) {
+  if (*p) {
+    hipFree(*p);
+    *p = nullptr;
+  }
+  if (count == 0) count = 1;
+  PJ_HIP(hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)));
+  return 0;
+}
+
+void free_graph(pj_ctx* ctx) {
+  void* bufs[] = {ctx->d_off, ctx->d_edges, ctx->d_labels, ctx->d_code, ctx->d_cand, ctx->d_unmatch, ctx->d_u64,
+                  ctx->d_flags, ctx->d_vflags, ctx->d_ea, ctx->d_bins, ctx->d_bin_count, ctx->d_hubs,
+                  ctx->d_chunk_hub, ctx->d_chunk_start, ctx->d_hub_count, ctx->d_hub_elim, ctx->d_nhubs, ctx->d_hub_hit};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  ctx->d_off = nullptr;
+  ctx->d_edges = nullptr;
+  ctx->d_labels = nullptr;
+  ctx->d_code = nullptr;
+  ctx->d_cand = ctx->d_unmatch = ctx->d_u64 = nullptr;
+  ctx->d_flags = ctx->d_vflags = nullptr;
+  ctx->d_ea = nullptr;
+  ctx->d_bins = ctx->d_bin_count = nullptr;
+  ctx->d_hubs = ctx->d_chunk_hub = nullptr;
+  ctx->d_chunk_start = nullptr;
+  ctx->d_hub_count = ctx->d_hub_elim = nullptr;
+  ctx->d_nhubs = nullptr;
+  ctx->d_hub_hit = nullptr;
+  ctx->V = 0;
+  ctx->E = 0;
+}
+
+// Builds the static hub chunk list from host offsets.
+int setup_hubs(pj_ctx* ctx, const uint64_t* h_off) {
+  std::vector<uint32_t> hubs, chunk_hub;
+  std::vector<uint64_t> chunk_start;
+  for (uint32_t v = 0; v < ctx->V; ++v) {
+    uint64_t d = h_off[v + 1] - h_off[v];
+    if (d <= kHubDegree) continue;
+    uint32_t h = static_cast<uint32_t>(hubs.size());
+    hubs.push_back(v);
+    for (uint64_t s = h_off[v]; s < h_off[v + 1]; s += kHubChunk) {
+      chunk_hub.push_back(h);
+      chunk_start.push_back(s);
+    }
+  }
+  ctx->nhubs = static_cast<uint32_t>(hubs.size());
+  ctx->nchunks = static_cast<uint32_t>(chunk_hub.size());
+  if (dev_alloc(ctx, &ctx->d_hubs, hubs.size()) || dev_alloc(ctx, &ctx->d_chunk_hub, chunk_hub.size()) ||
+      dev_alloc(ctx, &ctx->d_chunk_start, chunk_start.size()) ||
+      dev_alloc(ctx, &ctx->d_hub_count, static_cast<uint64_t>(hubs.size()) * kMaxLabels) ||
+      dev_alloc(ctx, &ctx->d_hub_elim, hubs.size()) || dev_alloc(ctx, &ctx->d_nhubs, 1) ||
+      dev_alloc(ctx, &ctx->d_hub_hit, hubs.size()))
+    return -1;
+  PJ_HIP(hipMemcpyAsync(ctx->d_nhubs, &ctx->nhubs, 4, hipMemcpyHostToDevice, ctx->stream));
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  if (!hubs.empty()) {
+    PJ_HIP(hipMemcpyAsync(ctx->d_hubs, hubs.data(), hubs.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    PJ_HIP(hipMemcpyAsync(ctx->d_chunk_hub, chunk_hub.data(), chunk_hub.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    PJ_HIP(hipMemcpyAsync(ctx->d_chunk_start, chunk_start.data(), chunk_start.size() * 8, hipMemcpyHostToDevice,
+                          ctx->stream));
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, hubs.size() * kMaxLabels)), dim3(kBlock), 0, ctx->stream,
+                       ctx->d_hub_count, static_cast<uint64_t>(hubs.size()) * kMaxLabels, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, hubs.size())), dim3(kBlock), 0, ctx->stream, ctx->d_hub_elim,
+                       static_cast<uint64_t>(hubs.size()), 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, hubs.size())), dim3(kBlock), 0, ctx->stream, ctx->d_hub_hit,
+                       static_cast<uint64_t>(hubs.size()), 0xFFFFFFFFu);
+    PJ_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return 0;
+}
+
+int alloc_state(pj_ctx* ctx) {
+  const uint64_t V = ctx->V;
+  if (dev_alloc(ctx, &ctx->d_code, V) || dev_alloc(ctx, reinterpret_cast<uint64_t**>(&ctx->d_cand), V) ||
+      dev_alloc(ctx, reinterpret_cast<uint64_t**>(&ctx->d_unmatch), V) ||
+      dev_alloc(ctx, reinterpret_cast<uint64_t**>(&ctx->d_u64), V) || dev_alloc(ctx, &ctx->d_flags, V + 4) ||
+      dev_alloc(ctx, &ctx->d_vflags, V + 4) || dev_alloc(ctx, &ctx->d_ea, ctx->ea_words) ||
+      dev_alloc(ctx, &ctx->d_bins, V * kBins) || dev_alloc(ctx, &ctx->d_bin_count, 4) ||
+      dev_alloc(ctx, &ctx->d_counters, kCounterSlots * kCounterStride))
+    return -1;
+  return 0;
+}
+
+template <typename Word>
+int state_reset_t(pj_ctx* ctx) {
+  const Word full = ctx->h_t.P >= 8 * sizeof(Word) ? static_cast<Word>(~Word(0))
+                                                   : static_cast<Word>((Word(1) << ctx->h_t.P) - 1);
+  hipLaunchKernelGGL(k_state_init<Word>, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream,
+                     static_cast<Word*>(ctx->d_cand), static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->V, full);
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, ctx->ea_words)), dim3(kBlock), 0, ctx->stream, ctx->d_ea,
+                     ctx->ea_words, 0xFFFFFFFFu);
+  PJ_HIP(hipGetLastError());
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->active_vertices = ctx->V;
+  ctx->active_edges = ctx->E;
+  return 0;
+}
+
+int zero_counters(pj_ctx* ctx) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(4), dim3(kBlock), 0, ctx->stream, ctx->d_counters,
+                     static_cast<uint64_t>(kCounterSlots * kCounterStride), 0ull);
+  return 0;
+}
+
+int zero_bins(pj_ctx* ctx) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bin_count, static_cast<uint64_t>(4), 0u);
+  return 0;
+}
+
+template <typename Word>
+void launch_bins(pj_ctx* ctx, int mode) {
+  zero_bins(ctx);
+  hipLaunchKernelGGL(k_build_bins<Word>, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream,
+                     static_cast<const Word*>(ctx->d_cand), ctx->d_flags, ctx->d_code, ctx->d_off, ctx->V, mode,
+                     ctx->d_t, ctx->d_bins, ctx->d_bin_count);
+}
+
+// Phase 2 over every active vertex (bins + hubs).
+template <typename Word>
+void launch_phase2(pj_ctx* ctx) {
+  launch_bins<Word>(ctx, 1);
+  Word* cand = static_cast<Word*>(ctx->d_cand);
+  Word* unm = static_cast<Word*>(ctx->d_unmatch);
+  const uint64_t V = ctx->V;
+  const int g = ctx->cus * 8;
+  hipLaunchKernelGGL((k_phase2<Word, 4>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins, ctx->d_bin_count + 0,
+                     ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, ctx->d_ea, ctx->d_counters);
+  hipLaunchKernelGGL((k_phase2<Word, 16>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
+                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, ctx->d_ea,
+                     ctx->d_counters);
+  hipLaunchKernelGGL((k_phase2<Word, 64>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
+                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, ctx->d_ea,
+                     ctx->d_counters);
+  if (ctx->nhubs) {
+    hipLaunchKernelGGL(k_phase2_hub_chunks<Word>, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
+                       ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
+                       ctx->d_edges, cand, ctx->d_flags, ctx->d_ea, ctx->d_hub_hit);
+    hipLaunchKernelGGL(k_phase2_hub_finish<Word>, dim3(grid_for(ctx, ctx->nhubs)), dim3(kBlock), 0, ctx->stream,
+                       ctx->d_hubs, ctx->nhubs, ctx->d_off, cand, unm, ctx->d_flags, ctx->d_hub_hit, ctx->d_counters);
+  }
+}
+
+template <typename Word, int KW>
+void launch_lcc_phase1(pj_ctx* ctx) {
+  launch_bins<Word>(ctx, 0);
+  const Word* cand = static_cast<const Word*>(ctx->d_cand);
+  Word* unm = static_cast<Word*>(ctx->d_unmatch);
+  const uint64_t V = ctx->V;
+  const int g = ctx->cus * 8;
+  hipLaunchKernelGGL((k_lcc_phase1<Word, 4, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
+                     ctx->d_bin_count + 0, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, ctx->d_ea,
+                     ctx->d_t, ctx->d_counters);
+  hipLaunchKernelGGL((k_lcc_phase1<Word, 16, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
+                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, ctx->d_ea,
+                     ctx->d_t, ctx->d_counters);
+  hipLaunchKernelGGL((k_lcc_phase1<Word, 64, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
+                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, ctx->d_ea,
+                     ctx->d_t, ctx->d_counters);
+  if (ctx->nhubs) {
+    hipLaunchKernelGGL(k_lcc_hub_chunks<Word>, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
+                       ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
+                       ctx->d_edges, ctx->d_code, cand, ctx->d_flags, ctx->d_ea, ctx->d_hub_count, ctx->d_hub_elim,
+                       ctx->d_t, ctx->d_counters);
+    hipLaunchKernelGGL((k_lcc_hub_finish<Word, KW>), dim3(grid_for(ctx, ctx->nhubs)), dim3(kBlock), 0, ctx->stream,
+                       ctx->d_hubs, ctx->nhubs, ctx->d_code, cand, unm, ctx->d_flags, ctx->d_hub_count, ctx->d_hub_elim,
+                       ctx->d_t);
+  }
+}
+
+template <typename Word>
+void launch_lcc(pj_ctx* ctx) {
+  switch (ctx->kw) {
+    case 1:
+      launch_lcc_phase1<Word, 1>(ctx);
+      break;
+    case 2:
+      launch_lcc_phase1<Word, 2>(ctx);
+      break;
+    case 4:
+      launch_lcc_phase1<Word, 4>(ctx);
+      break;
+    default:
+      launch_lcc_phase1<Word, 8>(ctx);
+      break;
+  }
+  launch_phase2<Word>(ctx);
+}
+
+template <typename Word>
+void launch_nlcc(pj_ctx* ctx, int kind) {
+  launch_bins<Word>(ctx, kind == 1 ? 2 : 1);
+  const Word* cand = static_cast<const Word*>(ctx->d_cand);
+  Word* unm = static_cast<Word*>(ctx->d_unmatch);
+  const uint64_t V = ctx->V;
+  const int g = ctx->cus * 8;
+  // Every bin and the hubs hold active vertices; hubs are run from their list.
+  for (int b = 0; b < kBins; ++b)
+    hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, kind, ctx->d_bins + b * V,
+                       ctx->d_bin_count + b, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t,
+                       ctx->d_counters);
+}
+
+// Hubs are not in the bins; the NLCC task list for them is the hub list with
+// a count.  Hubs that are inactive (or of the wrong label for PC) are skipped
+// by the kernels' candidate tests, so the whole list is passed.
+template <typename Word>
+void launch_nlcc_hubs(pj_ctx* ctx, int kind, uint32_t* d_hub_n) {
+  if (!ctx->nhubs) return;
+  const Word* cand = static_cast<const Word*>(ctx->d_cand);
+  Word* unm = static_cast<Word*>(ctx->d_unmatch);
+  hipLaunchKernelGGL(k_nlcc<Word>, dim3(std::min<uint32_t>(ctx->nhubs, ctx->cus * 8)), dim3(64), 0, ctx->stream, kind,
+                     ctx->d_hubs, d_hub_n, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t,
+                     ctx->d_counters);
+}
+
+int install_nlcc(pj_ctx* ctx, int step, uint32_t index) {
+  DevTemplate& t = ctx->h_t;
+  if (step == PJ_STEP_CC) {
+    if (index >= ctx->cycles.size()) {
+      ctx->err = "cycle constraint index out of range";
+      return -1;
+    }
+    const auto& c = ctx->cycles[index];
+    t.nlcc_len = static_cast<uint32_t>(c.size());
+    for (size_t i = 0; i < c.size(); ++i) t.nlcc_vertex[i] = c[i];
+  } else if (step == PJ_STEP_PC) {
+    if (index >= ctx->paths.size()) {
+      ctx->err = "path constraint index out of range";
+      return -1;
+    }
+    const auto& p = ctx->paths[index];
+    t.nlcc_len = static_cast<uint32_t>(p.size());
+    for (size_t i = 0; i < p.size(); ++i) t.nlcc_vertex[i] = p[i];
+    t.path_code = kNoCode;
+    for (uint32_t k = 0; k < t.K; ++k)
+      if (t.label_raw[k] == ctx->path_label[index]) t.path_code = k;
+  } else {
+    const std::vector<pj_ctx::HostWalk>* walks;
+    std::vector<pj_ctx::HostWalk> single;
+    if (step == PJ_STEP_TDS) {
+      if (index >= ctx->tds.size()) {
+        ctx->err = "TDS constraint index out of range";
+        return -1;
+      }
+      walks = &ctx->tds[index];
+    } else {
+      single.push_back(ctx->enumeration);
+      walks = &single;
+    }
+    if (walks->size() > 64) {
+      ctx->err = "more than 64 walks in a TDS constraint";
+      return -1;
+    }
+    t.walk_count = static_cast<uint32_t>(walks->size());
+    uint32_t o = 0;
+    for (size_t k = 0; k < walks->size(); ++k) {
+      const auto& w = (*walks)[k];
+      if (w.move.size() > 64) {
+        ctx->err = "walks of more than 64 moves are not supported on the device";
+        return -1;
+      }
+      t.walk_root[k] = w.root;
+      t.walk_off[k] = o;
+      uint32_t h = 0;
+      for (size_t i = 0; i < w.move.size(); ++i) {
+        t.walk_move[o + i] = w.move[i];
+        t.walk_vertex[o + i] = w.vertex[i];
+        t.walk_mb[o + i] = w.mb[i];
+        const bool vtx = w.move[i] == PJ_MOVE_VERTEX_STORE || w.move[i] == PJ_MOVE_VERTEX_NO_STORE;
+        t.walk_hidx[o + i] = vtx ? h : 0;
+        if (vtx) ++h;
+      }
+      o += static_cast<uint32_t>(w.move.size());
+    }
+    t.walk_off[walks->size()] = o;
+  }
+  PJ_HIP(hipMemcpyAsync(ctx->d_t, &t, sizeof(DevTemplate), hipMemcpyHostToDevice, ctx->stream));
+  return 0;
+}
+
+template <typename Word>
+int run_step_t(pj_ctx* ctx, int step, uint32_t index) {
+  zero_counters(ctx);
+  switch (step) {
+    case PJ_STEP_LCC0:
+      hipLaunchKernelGGL(k_lcc0<Word>, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream,
+                         static_cast<Word*>(ctx->d_cand), ctx->d_code, ctx->V, ctx->d_t, ctx->d_counters);
+      break;
+    case PJ_STEP_LCC:
+      launch_lcc<Word>(ctx);
+      break;
+    case PJ_STEP_CC:
+    case PJ_STEP_PC:
+    case PJ_STEP_TDS:
+    case PJ_STEP_ENUMERATION: {
+      const int kind = step == PJ_STEP_CC ? 0 : step == PJ_STEP_PC ? 1 : step == PJ_STEP_TDS ? 2 : 3;
+      launch_nlcc<Word>(ctx, kind);
+      launch_nlcc_hubs<Word>(ctx, kind, ctx->d_nhubs);
+      if (step != PJ_STEP_ENUMERATION) launch_phase2<Word>(ctx);
+      break;
+    }
+    default:
+      ctx->err = "unknown step";
+      return -1;
+  }
+  PJ_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename Word>
+int download_t(pj_ctx* ctx, uint64_t* cand, uint8_t* vflags, uint32_t* ea_words) {
+  hipLaunchKernelGGL(k_cand_to_u64<Word>, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream,
+                     static_cast<const Word*>(ctx->d_cand), static_cast<uint64_t*>(ctx->d_u64), ctx->d_flags,
+                     ctx->d_vflags, ctx->V);
+  PJ_HIP(hipGetLastError());
+  if (cand) PJ_HIP(hipMemcpyAsync(cand, ctx->d_u64, ctx->V * 8ull, hipMemcpyDeviceToHost, ctx->stream));
+  if (vflags) PJ_HIP(hipMemcpyAsync(vflags, ctx->d_vflags, ctx->V, hipMemcpyDeviceToHost, ctx->stream));
+  if (ea_words) PJ_HIP(hipMemcpyAsync(ea_words, ctx->d_ea, ctx->ea_words * 4, hipMemcpyDeviceToHost, ctx->stream));
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+template <typename Word>
+int upload_state_t(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags, const uint32_t* ea_words) {
+  PJ_HIP(hipMemcpyAsync(ctx->d_u64, cand, ctx->V * 8ull, hipMemcpyHostToDevice, ctx->stream));
+  PJ_HIP(hipMemcpyAsync(ctx->d_vflags, vflags, ctx->V, hipMemcpyHostToDevice, ctx->stream));
+  PJ_HIP(hipMemcpyAsync(ctx->d_ea, ea_words, ctx->ea_words * 4, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_u64_to_cand<Word>, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream,
+                     static_cast<const uint64_t*>(ctx->d_u64), ctx->d_vflags, static_cast<Word*>(ctx->d_cand),
+                     static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->V);
+  PJ_HIP(hipGetLastError());
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pj_create(pj_ctx** out, int device) {
+  if (!out) return -1;
+  *out = nullptr;
+  auto* ctx = new pj_ctx();
+  ctx->device = device;
+  ctx->h_counters.resize(kCounterSlots * kCounterStride);
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&ctx->d_t), sizeof(DevTemplate));
+  if (e != hipSuccess) {
+    fprintf(stderr, "pj_create: %s\n", hipGetErrorString(e));
+    delete ctx;
+    return -1;
+  }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    ctx->cus = cus;
+  *out = ctx;
+  return 0;
+}
+
+static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* offsets, const uint32_t* edges,
+                         const uint64_t* labels, hipMemcpyKind kind) {
+  PJ_HIP(hipSetDevice(ctx->device));
+  free_graph(ctx);
+  ctx->V = V;
+  ctx->E = E;
+  ctx->ea_words = (E + 31) / 32;
+  if (dev_alloc(ctx, &ctx->d_off, V + 1ull) || dev_alloc(ctx, &ctx->d_edges, E) || dev_alloc(ctx, &ctx->d_labels, V) ||
+      alloc_state(ctx))
+    return -1;
+  PJ_HIP(hipMemcpyAsync(ctx->d_off, offsets, (V + 1ull) * 8, kind, ctx->stream));
+  if (E) PJ_HIP(hipMemcpyAsync(ctx->d_edges, edges, E * 4, kind, ctx->stream));
+  if (V) PJ_HIP(hipMemcpyAsync(ctx->d_labels, labels, V * 8ull, kind, ctx->stream));
+  std::vector<uint64_t> h_off;
+  const uint64_t* host_off = offsets;
+  if (kind == hipMemcpyDeviceToDevice) {
+    h_off.resize(V + 1ull);
+    PJ_HIP(hipMemcpyAsync(h_off.data(), offsets, (V + 1ull) * 8, hipMemcpyDeviceToHost, ctx->stream));
+    PJ_HIP(hipStreamSynchronize(ctx->stream));
+    host_off = h_off.data();
+  }
+  if (host_off[V] != E) {
+    ctx->err = "offsets[V] != edge_count";
+    return -1;
+  }
+  if (setup_hubs(ctx, host_off)) return -1;
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->have_template = false;
+  ctx->active_vertices = V;
+  ctx->active_edges = E;
+  return 0;
+}
+
+int pj_upload_graph(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* offsets, const uint32_t* edges,
+                    const uint64_t* labels) {
+  if (!ctx || !offsets || (E && !edges) || (V && !labels)) return -1;
+  return upload_common(ctx, V, E, offsets, edges, labels, hipMemcpyHostToDevice);
+}
+
+int pj_upload_graph_device(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* offsets, const uint32_t* edges,
+                           const uint64_t* labels) {
+  if (!ctx || !offsets || (E && !edges) || (V && !labels)) return -1;
+  return upload_common(ctx, V, E, offsets, edges, labels, hipMemcpyDeviceToDevice);
+}
+
+int pj_set_template(pj_ctx* ctx, const pj_template_desc* d) {
+  if (!ctx) return -1;
+  if (!d || d->vertex_count == 0 || d->vertex_count > 64) {
+    ctx->err = "template must have 1..64 vertices";
+    return -1;
+  }
+  if (!ctx->d_off) {
+    ctx->err = "no graph uploaded";
+    return -1;
+  }
+  PJ_HIP(hipSetDevice(ctx->device));
+  DevTemplate& t = ctx->h_t;
+  std::memset(&t, 0, sizeof(t));
+  const uint32_t P = d->vertex_count;
+  t.P = P;
+  // Dense codes for the template's distinct labels, in vertex order.
+  for (uint32_t p = 0; p < P; ++p) {
+    uint32_t k = 0;
+    while (k < t.K && t.label_raw[k] != d->vertex_label[p]) ++k;
+    if (k == t.K) t.label_raw[t.K++] = d->vertex_label[p];
+    t.pcode[p] = static_cast<uint8_t>(k);
+    t.code_vertices[k] |= 1ull << p;
+    t.nbr[p] = d->neighbor_mask[p];
+  }
+  for (uint32_t p = 0; p < P; ++p) {
+    for (uint32_t i = d->need_offset[p]; i < d->need_offset[p + 1]; ++i) {
+      uint32_t k = 0;
+      while (k < t.K && t.label_raw[k] != d->need_label[i]) ++k;
+      if (k == t.K) {
+        ctx->err = "neighbour label not carried by any template vertex";
+        return -1;
+      }
+      uint32_t need = std::min<uint32_t>(d->need_count[i], 127u);
+      t.need[p][k >> 3] |= static_cast<uint64_t>(need) << (8 * (k & 7));
+    }
+  }
+  ctx->kw = t.K <= 8 ? 1 : t.K <= 16 ? 2 : t.K <= 32 ? 4 : 8;
+  ctx->word_bytes = P <= 32 ? 4 : 8;
+  ctx->cycles.clear();
+  for (uint32_t c = 0; c < d->cycle_count; ++c)
+    ctx->cycles.emplace_back(d->cycle_vertex + d->cycle_offset[c], d->cycle_vertex + d->cycle_offset[c + 1]);
+  ctx->paths.clear();
+  ctx->path_label.clear();
+  for (uint32_t c = 0; c < d->path_count; ++c) {
+    ctx->paths.emplace_back(d->path_vertex + d->path_offset[c], d->path_vertex + d->path_offset[c + 1]);
+    ctx->path_label.push_back(d->path_initial_label[c]);
+  }
+  auto copy_walk = [](const pj_walk_desc& w) {
+    pj_ctx::HostWalk h;
+    h.root = w.root;
+    h.move.assign(w.move, w.move + w.length);
+    h.vertex.assign(w.vertex, w.vertex + w.length);
+    h.mb.assign(w.move_back, w.move_back + w.length);
+    return h;
+  };
+  ctx->tds.clear();
+  for (uint32_t c = 0; c < d->tds_count; ++c) {
+    std::vector<pj_ctx::HostWalk> ws;
+    for (uint32_t k = d->tds_walk_offset[c]; k < d->tds_walk_offset[c + 1]; ++k) ws.push_back(copy_walk(d->walks[k]));
+    ctx->tds.push_back(std::move(ws));
+  }
+  ctx->enumeration = copy_walk(d->enumeration_walk);
+  PJ_HIP(hipMemcpyAsync(ctx->d_t, &t, sizeof(DevTemplate), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_label_codes, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream, ctx->d_labels,
+                     ctx->d_code, ctx->V, ctx->d_t);
+  PJ_HIP(hipGetLastError());
+  ctx->have_template = true;
+  return pj_state_reset(ctx);
+}
+
+int pj_state_reset(pj_ctx* ctx) {
+  if (!ctx) return -1;
+  if (!ctx->have_template) {
+    ctx->err = "no template installed";
+    return -1;
+  }
+  PJ_HIP(hipSetDevice(ctx->device));
+  return ctx->word_bytes == 4 ? state_reset_t<uint32_t>(ctx) : state_reset_t<uint64_t>(ctx);
+}
+
+int pj_reset_modified(pj_ctx* ctx) {
+  if (!ctx) return -1;
+  if (!ctx->have_template) {
+    ctx->err = "no template installed";
+    return -1;
+  }
+  hipLaunchKernelGGL(k_clear_modified, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream, ctx->d_flags,
+                     ctx->V);
+  PJ_HIP(hipGetLastError());
+  return 0;
+}
+
+int pj_run_step(pj_ctx* ctx, int step, uint32_t index, pj_algo_results* out) {
+  if (!ctx) return -1;
+  if (!ctx->have_template) {
+    ctx->err = "no template installed";
+    return -1;
+  }
+  PJ_HIP(hipSetDevice(ctx->device));
+  if (step == PJ_STEP_CC || step == PJ_STEP_PC || step == PJ_STEP_TDS || step == PJ_STEP_ENUMERATION)
+    if (install_nlcc(ctx, step, index)) return -1;
+  PJ_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  int rc = ctx->word_bytes == 4 ? run_step_t<uint32_t>(ctx, step, index) : run_step_t<uint64_t>(ctx, step, index);
+  if (rc) return rc;
+  PJ_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  PJ_HIP(hipMemcpyAsync(ctx->h_counters.data(), ctx->d_counters, kCounterSlots * kCounterStride * 8, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  PJ_HIP(hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
+  pj_algo_results r{0, 0, 0, 0};
+  ctx->last_slots = 0;
+  for (int s = 0; s < kCounterSlots; ++s) {
+    r.vertex_eliminated += ctx->h_counters[s * kCounterStride + C_VERTEX];
+    r.edge_eliminated += ctx->h_counters[s * kCounterStride + C_EDGE];
+    r.match_eliminated += ctx->h_counters[s * kCounterStride + C_MATCH];
+    r.enumeration += ctx->h_counters[s * kCounterStride + C_ENUM];
+    ctx->last_slots += ctx->h_counters[s * kCounterStride + C_SLOTS];
+  }
+  ctx->active_vertices -= r.vertex_eliminated;
+  ctx->active_edges -= r.edge_eliminated;
+  if (out) *out = r;
+  return 0;
+}
+
+int pj_last_step_ms(pj_ctx* ctx, double* ms) {
+  if (!ctx || !ms) return -1;
+  *ms = ctx->last_ms;
+  return 0;
+}
+
+int pj_last_step_slots(pj_ctx* ctx, uint64_t* slots) {
+  if (!ctx || !slots) return -1;
+  *slots = ctx->last_slots;
+  return 0;
+}
+
+int pj_get_counters(pj_ctx* ctx, uint64_t* av, uint64_t* ae) {
+  if (!ctx) return -1;
+  if (av) *av = ctx->active_vertices;
+  if (ae) *ae = ctx->active_edges;
+  return 0;
+}
+
+int pj_download_state(pj_ctx* ctx, uint64_t* cand, uint8_t* vflags, uint32_t* ea_words) {
+  if (!ctx) return -1;
+  if (!ctx->have_template) {
+    ctx->err = "no template installed";
+    return -1;
+  }
+  PJ_HIP(hipSetDevice(ctx->device));
+  return ctx->word_bytes == 4 ? download_t<uint32_t>(ctx, cand, vflags, ea_words)
+                              : download_t<uint64_t>(ctx, cand, vflags, ea_words);
+}
+
+int pj_upload_state(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags, const uint32_t* ea_words,
+                    uint64_t active_vertices, uint64_t active_edges) {
+  if (!ctx || !cand || !vflags || !ea_words) return -1;
+  if (!ctx->have_template) {
+    ctx->err = "no template installed";
+    return -1;
+  }
+  PJ_HIP(hipSetDevice(ctx->device));
+  int rc = ctx->word_bytes == 4 ? upload_state_t<uint32_t>(ctx, cand, vflags, ea_words)
+                                : upload_state_t<uint64_t>(ctx, cand, vflags, ea_words);
+  if (rc) return rc;
+  ctx->active_vertices = active_vertices;
+  ctx->active_edges = active_edges;
+  return 0;
+}
+
+const char* pj_last_error(pj_ctx* ctx) {
+  if (!ctx) return "null context";
+  return ctx->err.c_str();
+}
+
+void pj_destroy(pj_ctx* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  free_graph(ctx);
+  if (ctx->d_t) hipFree(ctx->d_t);
+  if (ctx->d_counters) hipFree(ctx->d_counters);
+  if (ctx->ev0) hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+}  // extern "C"
