@@ -59,7 +59,7 @@ def cpu_baseline(scale: int, threads: int) -> dict:
     g, _, c = workload(scale)
     o = oracle.Oracle(g, c, threads=threads)
     t0 = time.time()
-    rows, total_ms = o.run(oracle.FAITHFUL)
+    rows, total_ms = o.run(oracle.FAITHFUL, enumerate_matches=False)
     wall = time.time() - t0
     slots = sum(r.slots for r in rows)
     log(f"[bench] cpu baseline scale {scale}: {len(rows)} steps, {total_ms:.0f} ms of step time "
@@ -90,15 +90,17 @@ def main() -> int:
     g, _, c = workload(args.scale)
     pr = Pruner(g, c, device=0)
     for _ in range(args.warmup):
-        rows, _ = pr.run()
-    log(f"[bench] warmup done: {len(rows)} steps, enumeration {rows[-1].enumeration}")
+        t0 = time.perf_counter()
+        rows, _ = pr.run(enumerate_matches=False)
+        log(f"[bench] warmup solve: {len(rows)} steps in {time.perf_counter() - t0:.3f}s")
 
     times, dev = [], []
-    for _ in range(args.steps):
+    for k in range(args.steps):
         t0 = time.perf_counter()
-        rows, dev_ms = pr.run()
+        rows, dev_ms = pr.run(enumerate_matches=False)
         times.append(time.perf_counter() - t0)
         dev.append(dev_ms)
+        log(f"[bench] solve {k}: {times[-1] * 1e3:.2f} ms wall, {dev_ms:.2f} ms device")
     slots = sum(r.slots for r in rows)
     sec = sum(times) / len(times)
     log(f"[bench] {args.steps} solves: {sec * 1e3:.2f} ms/solve wall, {sum(dev) / len(dev):.2f} ms device, "
@@ -128,7 +130,8 @@ def main() -> int:
         "config": {"workload": f"plain pipeline, R-MAT scale {args.scale} EF16 (0.57,0.19,0.19,0.05), 32 Zipf(1.2) "
                                f"labels, triangle+tail template [0,1,2,3,4,3]",
                    "vertices": g.vertex_count, "directed_slots": g.edge_count, "pipeline_steps": len(rows),
-                   "slots_per_solve": slots, "enumeration": rows[-1].enumeration},
+                   "slots_per_solve": slots,
+                   "solve": "LCC0, LCC to fixed point, CC, LCC, PC, LCC (pruning fixed point; no enumeration)"},
         "roofline": roofline,
         "cpu_baseline": None,
     }

@@ -22,7 +22,7 @@ class OracleRow(C.Structure):
 
     def key(self) -> tuple:
         return (self.iteration, self.step, self.vertex_eliminated, self.edge_eliminated, self.match_eliminated,
-                self.enumeration, self.active_vertices, self.active_edges, self.slots)
+                self.enumeration, self.active_vertices, self.active_edges)
 
 
 class _Results(C.Structure):

@@ -64,6 +64,7 @@ struct DevTemplate {
   uint32_t walk_mb[64 * 128];
   uint32_t walk_hidx[64 * 128];
   uint32_t path_code;               // code of the path's initial label
+  uint32_t rows_sorted;             // every CSR row ascending (binary search allowed)
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -126,6 +127,18 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t value) {
 __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long value) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = value;
+}
+
+// Sets *unsorted when some row is not in ascending order.
+__global__ void k_rows_sorted(const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges, uint32_t V,
+                              uint32_t* unsorted) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t e = off[v] + 1; e < off[v + 1]; ++e)
+      if (edges[e] < edges[e - 1]) {
+        atomicOr(unsorted, 1u);
+        break;
+      }
+  }
 }
 
 __global__ void k_clear_modified(uint8_t* flags, uint32_t V) {
@@ -595,9 +608,71 @@ __device__ __forceinline__ RowScan scan_row(uint64_t from, uint64_t end, const u
   return r;
 }
 
-// Cycle constraint idx[0..L-1] from start position j (cc_cpu.tpp:12-70).
+// Whether row(a) holds an active slot to b, by binary search (rows sorted
+// ascending); `probes` counts the slots read.
+__device__ __forceinline__ bool active_slot(uint32_t a, uint32_t b, const uint64_t* __restrict__ off,
+                                            const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
+                                            uint64_t& probes) {
+  uint64_t lo = off[a], hi = off[a + 1];
+  const uint64_t end = hi;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    ++probes;
+    if (edges[mid] < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  for (; lo < end; ++lo) {
+    ++probes;
+    if (edges[lo] != b) break;
+    if (ea_test(ea, lo)) return true;
+  }
+  return false;
+}
+
+// Last two hops of a cycle on sorted rows: is there a u with an active slot
+// v->u, template vertex q among u's candidates, and an active slot u->s?
+// The shorter of row(v) and row(s) is read 64 slots at a time, each lane
+// binary-searching the other side, so a low-degree source next to a hub
+// never reads the hub's row.
 template <typename Word>
-__device__ bool cycle_walk(uint32_t s, uint32_t j, uint32_t L, const uint64_t* __restrict__ off,
+__device__ bool close_cycle(uint32_t v, uint32_t s, int q, const uint64_t* __restrict__ off,
+                            const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
+                            const Word* __restrict__ cand, uint64_t& slots) {
+  const int lane = lane_id();
+  const uint64_t vb = off[v], ve = off[v + 1], sb = off[s], se = off[s + 1];
+  const bool from_v = (ve - vb) <= (se - sb);
+  const uint64_t b = from_v ? vb : sb, end = from_v ? ve : se;
+  uint64_t probes = 0, read = 0;
+  bool found = false;
+  for (uint64_t base = b; base < end; base += 64) {
+    const uint64_t e = base + lane;
+    bool ok = false;
+    if (e < end) {
+      const uint32_t u = edges[e];
+      if ((cand[u] >> q) & 1) {
+        if (from_v)
+          ok = ea_test(ea, e) && active_slot(u, s, off, edges, ea, probes);
+        else
+          ok = active_slot(v, u, off, edges, ea, probes) && active_slot(u, s, off, edges, ea, probes);
+      }
+    }
+    read += min<uint64_t>(64, end - base);
+    if (__ballot(ok)) {
+      found = true;
+      break;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) probes += __shfl_xor(probes, o, 64);
+  slots += read + probes;
+  return found;
+}
+
+// Cycle constraint idx[0..L-1] from start position j (cc_cpu.tpp:12-70).
+// With sorted rows the last two hops are decided by close_cycle.
+template <typename Word>
+__device__ bool cycle_walk(uint32_t s, uint32_t j, uint32_t L, bool sorted, const uint64_t* __restrict__ off,
                            const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
                            const Word* __restrict__ cand, const DevTemplate* __restrict__ t, uint64_t& slots) {
   const int lane = lane_id();
@@ -607,6 +682,14 @@ __device__ bool cycle_walk(uint32_t s, uint32_t j, uint32_t L, const uint64_t* _
   uint32_t v = s;
   uint64_t cursor = off[s];
   while (true) {
+    if (sorted && depth == static_cast<int>(L) - 2) {
+      const int q = static_cast<int>(t->nlcc_vertex[(j + L - 1) % L]);
+      if (close_cycle<Word>(v, s, q, off, edges, ea, cand, slots)) return true;
+      depth = uniform(depth - 1);
+      v = rl32(st_v, depth);
+      cursor = rl64(st_c, depth);
+      continue;
+    }
     if (depth == static_cast<int>(L) - 1) {
       RowScan r = scan_row<Word>(off[v], off[v + 1], edges, ea, cand, -1, s, 0xFFFFFFFFu, slots);
       if (r.found) return true;
@@ -795,7 +878,7 @@ __global__ void __launch_bounds__(kBlock) k_nlcc(int kind, const uint32_t* __res
       for (uint32_t j = 0; j < L; ++j) {
         const uint32_t q = t->nlcc_vertex[j];
         if (!((cs >> q) & 1)) continue;
-        if (!cycle_walk<Word>(s, j, L, off, edges, ea, cand, t, slots)) fail |= Word(1) << q;
+        if (!cycle_walk<Word>(s, j, L, t->rows_sorted != 0, off, edges, ea, cand, t, slots)) fail |= Word(1) << q;
       }
     } else if (kind == 1) {
       const uint32_t q0 = t->nlcc_vertex[0], q1 = t->nlcc_vertex[L - 1];
@@ -880,6 +963,8 @@ struct pj_ctx {
   uint8_t* d_vflags = nullptr;
   uint32_t* d_ea = nullptr;
   uint64_t ea_words = 0;
+  uint32_t ea_tail = 0;  // host staging of the last edge-active word
+  bool rows_sorted = false;
   uint32_t* d_bins = nullptr;        // kBins * V
   uint32_t* d_bin_count = nullptr;   // kBins
   unsigned long long* d_counters = nullptr;  // kCounterSlots * kCounterStride
@@ -1028,6 +1113,10 @@ int state_reset_t(pj_ctx* ctx) {
                      static_cast<Word*>(ctx->d_cand), static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->V, full);
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, ctx->ea_words)), dim3(kBlock), 0, ctx->stream, ctx->d_ea,
                      ctx->ea_words, 0xFFFFFFFFu);
+  if (ctx->E & 31) {  // bits past the last slot stay 0
+    ctx->ea_tail = (1u << (ctx->E & 31)) - 1u;
+    PJ_HIP(hipMemcpyAsync(ctx->d_ea + ctx->ea_words - 1, &ctx->ea_tail, 4, hipMemcpyHostToDevice, ctx->stream));
+  }
   PJ_HIP(hipGetLastError());
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   ctx->active_vertices = ctx->V;
@@ -1154,6 +1243,7 @@ void launch_nlcc_hubs(pj_ctx* ctx, int kind, uint32_t* d_hub_n) {
 
 int install_nlcc(pj_ctx* ctx, int step, uint32_t index) {
   DevTemplate& t = ctx->h_t;
+  t.rows_sorted = ctx->rows_sorted ? 1u : 0u;
   if (step == PJ_STEP_CC) {
     if (index >= ctx->cycles.size()) {
       ctx->err = "cycle constraint index out of range";
@@ -1325,6 +1415,15 @@ static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* of
     return -1;
   }
   if (setup_hubs(ctx, host_off)) return -1;
+  {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bin_count, static_cast<uint64_t>(4), 0u);
+    hipLaunchKernelGGL(k_rows_sorted, dim3(grid_for(ctx, V)), dim3(kBlock), 0, ctx->stream, ctx->d_off, ctx->d_edges,
+                       V, ctx->d_bin_count);
+    uint32_t unsorted = 0;
+    PJ_HIP(hipMemcpyAsync(&unsorted, ctx->d_bin_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PJ_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->rows_sorted = unsorted == 0;
+  }
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   ctx->have_template = false;
   ctx->active_vertices = V;

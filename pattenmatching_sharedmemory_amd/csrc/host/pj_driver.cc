@@ -4,6 +4,7 @@
 #include <sys/stat.h>
 
 #include <cstdarg>
+#include <cstdlib>
 #include <chrono>
 #include <cstdio>
 #include <ctime>
@@ -143,6 +144,7 @@ int run_step_machine(MultipleLabelDeviceState* state, const ConstraintSet& cs,
   bool finished = false;
   int iteration = 0;
   double total = 0.0;
+  const bool trace = std::getenv("PJ_TRACE") != nullptr;  // per-step progress on stderr
   machine.getNextStep(results, &step, &name);
   while (!finished && iteration < 1000) {
     if (stop_before_enumeration && step == MultipleLabelStep::E_ENUMERATION) break;
@@ -189,6 +191,10 @@ int run_step_machine(MultipleLabelDeviceState* state, const ConstraintSet& cs,
     pj_last_step_ms(state->ctx, &dev);
     pj_last_step_slots(state->ctx, &slots);
     total += dev;
+    if (trace)
+      std::fprintf(stderr, "[pj] step %d %s %.3f ms device, %llu slots, active %llu\n", iteration,
+                   MultipleLabelStep::name(step), dev, static_cast<unsigned long long>(slots),
+                   static_cast<unsigned long long>(state->graphActiveVertexCount));
     if (on_step)
       on_step(StepRecord{iteration, step, results, state->graphActiveVertexCount, state->graphActiveEdgeCount, slots,
                          wall, dev});

@@ -62,9 +62,11 @@ class Row(C.Structure):
                 ("ms", C.c_double)]
 
     def key(self) -> tuple:
-        """Everything but the time."""
+        """The step's results (what the reference's results.totem row holds,
+        times excluded).  Examined-slot counts are not part of it: they
+        measure how much a given algorithm read, not what it decided."""
         return (self.iteration, self.step, self.vertex_eliminated, self.edge_eliminated, self.match_eliminated,
-                self.enumeration, self.active_vertices, self.active_edges, self.slots)
+                self.enumeration, self.active_vertices, self.active_edges)
 
 
 _hip = None

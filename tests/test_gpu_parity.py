@@ -110,6 +110,8 @@ def _random_state(rng, V, E, P):
     vf |= (rng.random(V) < 0.3).astype(np.uint8) << 1
     vf |= (rng.random(V) < 0.6).astype(np.uint8) << 2
     ea = rng.integers(0, 1 << 32, (E + 31) // 32, dtype=np.uint64).astype(np.uint32)
+    if E % 32:
+        ea[-1] &= np.uint32((1 << (E % 32)) - 1)  # no bits past the last slot
     return cand, vf, ea
 
 
