@@ -53,6 +53,42 @@ def workload(scale: int):
     return g, p, c
 
 
+def algorithmic_bytes(k: dict, vertices: int) -> float:
+    """Bytes a kernel must move, counted once per access at natural width with
+    no cache credit (W = candidate word bytes): offsets 8, ids 4, labels 1,
+    flags 1, edge-active 1/8 per slot read or written.
+      phase 1 (LCC): items*(9.25 + 2W) + slots*(5.25 + W) + 0.125*eliminated
+      phase 2:       items*(3W + 0.5) + 4.25*slots
+      NLCC walks:    8*items + (4.25 + W)*slots
+      bins:          vertices*(W + 9) + 4*listed"""
+    W = k["word_bytes"]
+    name = k["name"]
+    if name.startswith("lcc_phase1"):
+        return k["items"] * (9.25 + 2 * W) + k["slots"] * (5.25 + W) + 0.125 * k["edges_eliminated"]
+    if name.startswith("phase2"):
+        return k["items"] * (3 * W + 0.5) + 4.25 * k["slots"]
+    if name.startswith("nlcc"):
+        return 8 * k["items"] + (4.25 + W) * k["slots"]
+    if name == "build_bins":
+        return k["launches"] * vertices * (W + 9)
+    if name == "lcc0":
+        return k["launches"] * vertices * (2 * W + 1.125)
+    return 0.0
+
+
+def roofline_from(stats: list[dict], vertices: int) -> dict | None:
+    if not stats:
+        return None
+    dom = max(stats, key=lambda k: k["ms"])
+    nbytes = algorithmic_bytes(dom, vertices)
+    per_launch = nbytes / dom["launches"]
+    avg_ms = dom["ms"] / dom["launches"]
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom["name"],
+            "bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches": dom["launches"]}
+
+
 def cpu_baseline(scale: int, threads: int) -> dict:
     import oracle
 
@@ -94,6 +130,10 @@ def main() -> int:
         rows, _ = pr.run(enumerate_matches=False)
         log(f"[bench] warmup solve: {len(rows)} steps in {time.perf_counter() - t0:.3f}s")
 
+    # Kernel launches of the timed solves are bracketed with HIP events on
+    # the context stream (pj_set_kernel_timing).
+    if not args.no_profile:
+        pr.kernel_timing(True)
     times, dev = [], []
     for k in range(args.steps):
         t0 = time.perf_counter()
@@ -111,8 +151,17 @@ def main() -> int:
             f"slots={r.slots} ms={r.ms:.3f}")
 
     roofline = None
-    if not args.no_profile and hasattr(pr, "profile"):
-        roofline = pr.profile(HBM_PEAK_GBS)
+    kstats = []
+    if not args.no_profile:
+        kstats = pr.kernel_stats()
+        pr.kernel_timing(False)
+        total_k = sum(k["ms"] for k in kstats)
+        for k in sorted(kstats, key=lambda k: -k["ms"]):
+            gbs = algorithmic_bytes(k, g.vertex_count) / (k["ms"] * 1e-3) / 1e9 if k["ms"] else 0.0
+            log(f"    kernel {k['name']:24s} launches {k['launches']:5d} ms {k['ms']:9.3f} "
+                f"({100 * k['ms'] / max(total_k, 1e-9):5.1f}%) items {k['items']:11d} slots {k['slots']:12d} "
+                f"algorithmic {gbs:8.1f} GB/s")
+        roofline = roofline_from(kstats, g.vertex_count)
 
     result = {
         "metric": METRIC,
@@ -134,6 +183,9 @@ def main() -> int:
                    "solve": "LCC0, LCC to fixed point, CC, LCC, PC, LCC (pruning fixed point; no enumeration)"},
         "roofline": roofline,
         "cpu_baseline": None,
+        "kernels": [{"name": k["name"], "launches": k["launches"], "ms": round(k["ms"], 4),
+                     "algorithmic_GBs": round(algorithmic_bytes(k, g.vertex_count) / (k["ms"] * 1e-3) / 1e9, 1)
+                     if k["ms"] else 0.0} for k in sorted(kstats, key=lambda k: -k["ms"])],
     }
     pr.close()
     if not args.no_cpu:

@@ -162,6 +162,24 @@ int pj_last_step_ms(pj_ctx* ctx, double* ms);
  * counterpart (the reference times steps only, multiple_label_cpu.cc:137-164). */
 int pj_last_step_slots(pj_ctx* ctx, uint64_t* slots);
 
+/* Per-kernel timing.  pj_set_kernel_timing(ctx, 1) brackets every kernel
+ * launch of later steps with HIP events on the context stream and keeps
+ * per-launch counters; pj_kernel_stats returns the totals per kernel kind
+ * (number of kinds written, at most max; out may be NULL to count).
+ * Enabling or disabling clears the totals.  No reference counterpart (the
+ * reference times whole steps only). */
+typedef struct pj_kernel_stat {
+  char name[32];
+  uint64_t launches;
+  double ms;                 /* summed launch time */
+  uint64_t items;            /* vertices (or hub chunks) the launches were given */
+  uint64_t slots;            /* CSR slots examined */
+  uint64_t edges_eliminated; /* edge-active bits cleared */
+  uint32_t word_bytes;       /* candidate word width (4 or 8) */
+} pj_kernel_stat;
+int pj_set_kernel_timing(pj_ctx* ctx, int enable);
+int pj_kernel_stats(pj_ctx* ctx, pj_kernel_stat* out, int max);
+
 /* Active vertex / edge counters: graphActiveVertexCount and
  * graphActiveEdgeCount (multiple_label_common_cpu.h:43-46), as 64-bit values. */
 int pj_get_counters(pj_ctx* ctx, uint64_t* active_vertices, uint64_t* active_edges);

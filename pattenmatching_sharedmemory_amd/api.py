@@ -275,6 +275,17 @@ class Pruner:
         native.hip().pj_last_step_ms(self._ctx, C.byref(ms))
         return ms.value
 
+    def kernel_timing(self, enable: bool) -> None:
+        """Per-kernel HIP-event timing of later steps (clears the totals)."""
+        self._check(native.hip().pj_set_kernel_timing(self._ctx, 1 if enable else 0))
+
+    def kernel_stats(self) -> list[dict]:
+        n = native.hip().pj_kernel_stats(self._ctx, None, 0)
+        arr = (native.KernelStat * max(n, 1))()
+        n = native.hip().pj_kernel_stats(self._ctx, arr, n)
+        return [{"name": a.name.decode(), "launches": a.launches, "ms": a.ms, "items": a.items, "slots": a.slots,
+                 "edges_eliminated": a.edges_eliminated, "word_bytes": a.word_bytes} for a in arr[:n]]
+
     def last_slots(self) -> int:
         n = C.c_uint64()
         native.hip().pj_last_step_slots(self._ctx, C.byref(n))

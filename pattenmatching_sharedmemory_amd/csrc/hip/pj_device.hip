@@ -38,8 +38,20 @@ constexpr int kMaxLabels = 64;
 // Counter slots: [slot][0..4] = vertex / edge / match eliminated,
 // enumeration, CSR slots examined (the TEPS numerator: every slot a serial
 // scan in the reference's loop order would read).
-enum { C_VERTEX = 0, C_EDGE = 1, C_MATCH = 2, C_ENUM = 3, C_SLOTS = 4 };
+enum { C_VERTEX = 0, C_EDGE = 1, C_MATCH = 2, C_ENUM = 3, C_SLOTS = 4, C_ITEMS = 5 };
 constexpr int kCounterStride = 8;
+constexpr size_t kCounterBlock = static_cast<size_t>(kCounterSlots) * kCounterStride;
+constexpr int kMaxLaunches = 32;  // timed launches per step
+
+// Kernel kinds reported by pj_kernel_stats.
+enum KernelId {
+  K_LCC0, K_BINS, K_LCC_P1_G4, K_LCC_P1_G16, K_LCC_P1_G64, K_LCC_HUB, K_LCC_HUB_FINISH,
+  K_P2_G4, K_P2_G16, K_P2_G64, K_P2_HUB, K_P2_HUB_FINISH, K_NLCC, K_NLCC_HUB, K_COUNT
+};
+const char* const kKernelNames[K_COUNT] = {
+    "lcc0", "build_bins", "lcc_phase1_g4", "lcc_phase1_g16", "lcc_phase1_g64", "lcc_phase1_hub_chunks",
+    "lcc_phase1_hub_finish", "phase2_g4", "phase2_g16", "phase2_g64", "phase2_hub_chunks", "phase2_hub_finish",
+    "nlcc_walk", "nlcc_walk_hubs"};
 
 // Template tables in device memory.
 struct DevTemplate {
@@ -129,15 +141,39 @@ __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
     p[i] = value;
 }
 
-// Sets *unsorted when some row is not in ascending order.
-__global__ void k_rows_sorted(const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges, uint32_t V,
-                              uint32_t* unsorted) {
+// Row-order check: k_row_starts marks the first slot of every row in a
+// zeroed bitset, k_rows_sorted sets *unsorted when a slot that starts no row
+// holds a smaller id than the slot before it.
+__global__ void k_row_starts(const uint64_t* __restrict__ off, uint32_t V, uint64_t E, uint32_t* __restrict__ bits) {
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
-    for (uint64_t e = off[v] + 1; e < off[v + 1]; ++e)
-      if (edges[e] < edges[e - 1]) {
-        atomicOr(unsorted, 1u);
-        break;
-      }
+    const uint64_t e = off[v];
+    if (e < E) atomicOr(&bits[e >> 5], 1u << (e & 31));
+  }
+}
+
+__global__ void k_rows_sorted(const uint32_t* __restrict__ edges, uint64_t E, const uint32_t* __restrict__ bits,
+                              uint32_t* unsorted) {
+  for (uint64_t e = 1 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x)
+    if (!((bits[e >> 5] >> (e & 31)) & 1u) && edges[e] < edges[e - 1]) atomicOr(unsorted, 1u);
+}
+
+// Sums the kCounterSlots slots of counter block blockIdx.x.
+__global__ void k_sum_counters(const unsigned long long* __restrict__ counters, unsigned long long* __restrict__ sums) {
+  __shared__ unsigned long long part[kBlock / 64][kCounterStride];
+  const unsigned long long* blk = counters + static_cast<size_t>(blockIdx.x) * kCounterBlock;
+  unsigned long long acc[kCounterStride] = {};
+  for (int s = threadIdx.x; s < kCounterSlots; s += blockDim.x)
+    for (int k = 0; k < kCounterStride; ++k) acc[k] += blk[s * kCounterStride + k];
+  for (int k = 0; k < kCounterStride; ++k) {
+    unsigned long long x = acc[k];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6][k] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < kCounterStride) {
+    unsigned long long x = 0;
+    for (int w = 0; w < kBlock / 64; ++w) x += part[w][threadIdx.x];
+    sums[static_cast<size_t>(blockIdx.x) * kCounterStride + threadIdx.x] = x;
   }
 }
 
@@ -199,40 +235,71 @@ __global__ void k_lcc0(Word* __restrict__ cand, const uint8_t* __restrict__ code
 constexpr int kBins = 3;
 __constant__ uint32_t c_bin_limit[kBins] = {4, 32, kHubDegree};
 
+constexpr int kBinIters = 16;                 // vertices per thread
+constexpr int kBinVerts = kBlock * kBinIters;  // vertices per block
+
 template <typename Word>
-__global__ void k_build_bins(const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
-                             const uint8_t* __restrict__ code, const uint64_t* __restrict__ off, uint32_t V,
-                             int mode, const DevTemplate* __restrict__ t, uint32_t* __restrict__ bins,
-                             uint32_t* __restrict__ bin_count) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t span = ((V + stride - 1) / stride) * stride;
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < span; v += stride) {
-    int bin = -1;
-    if (v < V && cand[v]) {
-      bool take = true;
-      if (mode == 0) take = flags[v] & kScheduled;
-      if (mode == 2) take = code[v] == t->path_code;
-      if (take) {
-        uint64_t d = off[v + 1] - off[v];
-        if (d <= c_bin_limit[0])
-          bin = 0;
-        else if (d <= c_bin_limit[1])
-          bin = 1;
-        else if (d <= c_bin_limit[2])
-          bin = 2;
-      }
+__device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __restrict__ cand,
+                                      const uint8_t* __restrict__ flags, const uint8_t* __restrict__ code,
+                                      const uint64_t* __restrict__ off, int mode, uint32_t path_code) {
+  if (v >= V || !cand[v]) return -1;
+  if (mode == 0 && !(flags[v] & kScheduled)) return -1;
+  if (mode == 2 && code[v] != path_code) return -1;
+  const uint64_t d = off[v + 1] - off[v];
+  if (d <= c_bin_limit[0]) return 0;
+  if (d <= c_bin_limit[1]) return 1;
+  if (d <= c_bin_limit[2]) return 2;
+  return -1;
+}
+
+// Each block compacts kBinVerts consecutive vertices: pass 1 counts per bin
+// with ballots, one global atomic per block and bin reserves space, pass 2
+// writes the vertex ids (each wave's run stays contiguous in its bin).
+template <typename Word>
+__global__ void __launch_bounds__(kBlock) k_build_bins(
+    const Word* __restrict__ cand, const uint8_t* __restrict__ flags, const uint8_t* __restrict__ code,
+    const uint64_t* __restrict__ off, uint32_t V, int mode, const DevTemplate* __restrict__ t,
+    uint32_t* __restrict__ bins, uint32_t* __restrict__ bin_count) {
+  __shared__ uint32_t s_wave[kBlock / 64][kBins];
+  __shared__ uint32_t s_base[kBlock / 64][kBins];
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const uint32_t path_code = mode == 2 ? t->path_code : 0u;
+  const uint64_t v0 = static_cast<uint64_t>(blockIdx.x) * kBinVerts + threadIdx.x;
+  uint32_t cnt[kBins] = {0, 0, 0};
+#pragma unroll 4
+  for (int i = 0; i < kBinIters; ++i) {
+    const int b = bin_of<Word>(v0 + static_cast<uint64_t>(i) * kBlock, V, cand, flags, code, off, mode, path_code);
+#pragma unroll
+    for (int k = 0; k < kBins; ++k) cnt[k] += __popcll(__ballot(b == k));
+  }
+  if (lane == 0)
+    for (int k = 0; k < kBins; ++k) s_wave[wave][k] = cnt[k];
+  __syncthreads();
+  if (threadIdx.x < kBins) {
+    const int k = threadIdx.x;
+    uint32_t tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) tot += s_wave[w][k];
+    uint32_t base = tot ? atomicAdd(&bin_count[k], tot) : 0u;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      s_base[w][k] = base;
+      base += s_wave[w][k];
     }
-    for (int b = 0; b < kBins; ++b) {
-      uint64_t m = __ballot(bin == b);
-      if (!m) continue;
-      uint32_t base = 0;
-      int leader = __builtin_ctzll(m);
-      if (lane_id() == leader) base = atomicAdd(&bin_count[b], static_cast<uint32_t>(__popcll(m)));
-      base = __shfl(base, leader, 64);
-      if (bin == b) {
-        uint32_t rank = __popcll(m & ((1ull << lane_id()) - 1));
-        bins[static_cast<uint64_t>(b) * V + base + rank] = static_cast<uint32_t>(v);
-      }
+  }
+  __syncthreads();
+  uint32_t pos[kBins];
+#pragma unroll
+  for (int k = 0; k < kBins; ++k) pos[k] = s_base[wave][k];
+  const uint64_t below = (1ull << lane) - 1;
+#pragma unroll 4
+  for (int i = 0; i < kBinIters; ++i) {
+    const uint64_t v = v0 + static_cast<uint64_t>(i) * kBlock;
+    const int b = bin_of<Word>(v, V, cand, flags, code, off, mode, path_code);
+#pragma unroll
+    for (int k = 0; k < kBins; ++k) {
+      const uint64_t m = __ballot(b == k);
+      if (b == k) bins[static_cast<uint64_t>(k) * V + pos[k] + __popcll(m & below)] = static_cast<uint32_t>(v);
+      pos[k] += __popcll(m);
     }
   }
 }
@@ -295,6 +362,7 @@ __global__ void __launch_bounds__(kBlock) k_lcc_phase1(
   const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << (lane & ~(G - 1)));
   const uint32_t n = *list_count;
   const uint32_t K = t->K;
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_ITEMS] = n;
   const uint64_t groups_per_grid = (uint64_t)gridDim.x * (blockDim.x / G);
   const uint64_t span = ((n + groups_per_grid - 1) / groups_per_grid) * groups_per_grid;
   uint64_t elim_total = 0, slots_total = 0;
@@ -360,7 +428,7 @@ __global__ void __launch_bounds__(kBlock) k_lcc_hub_chunks(
   __shared__ uint32_t s_cnt[kMaxLabels];
   __shared__ uint32_t s_elim;
   const uint32_t K = t->K;
-  uint64_t elim_total = 0, slots_total = 0;
+  uint64_t elim_total = 0, slots_total = 0, items = 0;
   for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const uint32_t h = chunk_hub[c];
     const uint32_t v = hubs[h];
@@ -392,11 +460,15 @@ __global__ void __launch_bounds__(kBlock) k_lcc_hub_chunks(
     __syncthreads();
     if (threadIdx.x < K && s_cnt[threadIdx.x]) atomicAdd(&hub_count[h * kMaxLabels + threadIdx.x], s_cnt[threadIdx.x]);
     if (threadIdx.x == 0 && s_elim) atomicAdd(&hub_elim[h], s_elim);
-    if (threadIdx.x == 0) slots_total += end - beg;
+    if (threadIdx.x == 0) {
+      slots_total += end - beg;
+      ++items;
+    }
     __syncthreads();
   }
   wave_add(counters, C_EDGE, elim_total);
   wave_add(counters, C_SLOTS, slots_total);
+  wave_add(counters, C_ITEMS, items);
 }
 
 template <typename Word, int KW>
@@ -441,6 +513,7 @@ __global__ void __launch_bounds__(kBlock) k_phase2(
   const int gbase = lane & ~(G - 1);
   const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
   const uint32_t n = *list_count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_ITEMS] = n;
   const uint64_t groups_per_grid = (uint64_t)gridDim.x * (blockDim.x / G);
   const uint64_t span = ((n + groups_per_grid - 1) / groups_per_grid) * groups_per_grid;
   uint64_t velim = 0, melim = 0, slots = 0;
@@ -867,6 +940,7 @@ __global__ void __launch_bounds__(kBlock) k_nlcc(int kind, const uint32_t* __res
                                                  Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
                                                  const DevTemplate* __restrict__ t, unsigned long long* counters) {
   const uint32_t n = *list_count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_ITEMS] = n;
   const uint32_t waves = gridDim.x * (blockDim.x / 64);
   uint64_t enumerated = 0, slots = 0;
   for (uint32_t w = uniform(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)); w < n; w += waves) {
@@ -995,9 +1069,17 @@ struct pj_ctx {
   HostWalk enumeration;
 
   uint64_t active_vertices = 0, active_edges = 0;
+  // kernel timing (pj_set_kernel_timing)
+  bool timing = false, cur_timed = false;
+  int nlaunch = 0;
+  int launch_kid[kMaxLaunches] = {};
+  hipEvent_t lev[2 * kMaxLaunches] = {};
+  uint64_t k_launches[K_COUNT] = {}, k_items[K_COUNT] = {}, k_slots[K_COUNT] = {}, k_elim[K_COUNT] = {};
+  double k_ms[K_COUNT] = {};
   float last_ms = 0.f;
   uint64_t last_slots = 0;
-  std::vector<unsigned long long> h_counters;
+  unsigned long long* h_sums = nullptr;   // pinned: per counter block, kCounterStride sums
+  unsigned long long* d_sums = nullptr;
 };
 
 namespace {
@@ -1100,7 +1182,7 @@ int alloc_state(pj_ctx* ctx) {
       dev_alloc(ctx, reinterpret_cast<uint64_t**>(&ctx->d_u64), V) || dev_alloc(ctx, &ctx->d_flags, V + 4) ||
       dev_alloc(ctx, &ctx->d_vflags, V + 4) || dev_alloc(ctx, &ctx->d_ea, ctx->ea_words) ||
       dev_alloc(ctx, &ctx->d_bins, V * kBins) || dev_alloc(ctx, &ctx->d_bin_count, 4) ||
-      dev_alloc(ctx, &ctx->d_counters, kCounterSlots * kCounterStride))
+      dev_alloc(ctx, &ctx->d_counters, (kMaxLaunches + 1) * kCounterBlock))
     return -1;
   return 0;
 }
@@ -1124,9 +1206,32 @@ int state_reset_t(pj_ctx* ctx) {
   return 0;
 }
 
+// Counter block of the next launch.  Without kernel timing every launch
+// adds into block 0; with it each launch gets its own block and a pair of
+// HIP events so that pj_kernel_stats can attribute time and work per kernel.
+unsigned long long* begin_launch(pj_ctx* ctx, int kid) {
+  if (!ctx->timing || ctx->nlaunch >= kMaxLaunches) {
+    ctx->cur_timed = false;
+    return ctx->d_counters;
+  }
+  const int n = ctx->nlaunch;
+  ctx->launch_kid[n] = kid;
+  hipEventRecord(ctx->lev[2 * n], ctx->stream);
+  ctx->cur_timed = true;
+  return ctx->d_counters + static_cast<size_t>(n + 1) * kCounterBlock;
+}
+
+void end_launch(pj_ctx* ctx) {
+  if (!ctx->cur_timed) return;
+  hipEventRecord(ctx->lev[2 * ctx->nlaunch + 1], ctx->stream);
+  ++ctx->nlaunch;
+  ctx->cur_timed = false;
+}
+
 int zero_counters(pj_ctx* ctx) {
-  hipLaunchKernelGGL(k_fill_u64, dim3(4), dim3(kBlock), 0, ctx->stream, ctx->d_counters,
-                     static_cast<uint64_t>(kCounterSlots * kCounterStride), 0ull);
+  const uint64_t n = ctx->timing ? static_cast<uint64_t>(kMaxLaunches + 1) * kCounterBlock : kCounterBlock;
+  hipLaunchKernelGGL(k_fill_u64, dim3(ctx->timing ? 64 : 4), dim3(kBlock), 0, ctx->stream, ctx->d_counters, n, 0ull);
+  ctx->nlaunch = 0;
   return 0;
 }
 
@@ -1138,9 +1243,12 @@ int zero_bins(pj_ctx* ctx) {
 template <typename Word>
 void launch_bins(pj_ctx* ctx, int mode) {
   zero_bins(ctx);
-  hipLaunchKernelGGL(k_build_bins<Word>, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream,
+  begin_launch(ctx, K_BINS);
+  const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(ctx->V) + kBinVerts - 1) / kBinVerts);
+  hipLaunchKernelGGL(k_build_bins<Word>, dim3(std::max<uint32_t>(blocks, 1)), dim3(kBlock), 0, ctx->stream,
                      static_cast<const Word*>(ctx->d_cand), ctx->d_flags, ctx->d_code, ctx->d_off, ctx->V, mode,
                      ctx->d_t, ctx->d_bins, ctx->d_bin_count);
+  end_launch(ctx);
 }
 
 // Phase 2 over every active vertex (bins + hubs).
@@ -1151,20 +1259,28 @@ void launch_phase2(pj_ctx* ctx) {
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   const uint64_t V = ctx->V;
   const int g = ctx->cus * 8;
+  unsigned long long* c = begin_launch(ctx, K_P2_G4);
   hipLaunchKernelGGL((k_phase2<Word, 4>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins, ctx->d_bin_count + 0,
-                     ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, ctx->d_ea, ctx->d_counters);
+                     ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, ctx->d_ea, c);
+  end_launch(ctx);
+  c = begin_launch(ctx, K_P2_G16);
   hipLaunchKernelGGL((k_phase2<Word, 16>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
-                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, ctx->d_ea,
-                     ctx->d_counters);
+                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, ctx->d_ea, c);
+  end_launch(ctx);
+  c = begin_launch(ctx, K_P2_G64);
   hipLaunchKernelGGL((k_phase2<Word, 64>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
-                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, ctx->d_ea,
-                     ctx->d_counters);
+                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, ctx->d_ea, c);
+  end_launch(ctx);
   if (ctx->nhubs) {
+    begin_launch(ctx, K_P2_HUB);
     hipLaunchKernelGGL(k_phase2_hub_chunks<Word>, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
                        ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
                        ctx->d_edges, cand, ctx->d_flags, ctx->d_ea, ctx->d_hub_hit);
+    end_launch(ctx);
+    c = begin_launch(ctx, K_P2_HUB_FINISH);
     hipLaunchKernelGGL(k_phase2_hub_finish<Word>, dim3(grid_for(ctx, ctx->nhubs)), dim3(kBlock), 0, ctx->stream,
-                       ctx->d_hubs, ctx->nhubs, ctx->d_off, cand, unm, ctx->d_flags, ctx->d_hub_hit, ctx->d_counters);
+                       ctx->d_hubs, ctx->nhubs, ctx->d_off, cand, unm, ctx->d_flags, ctx->d_hub_hit, c);
+    end_launch(ctx);
   }
 }
 
@@ -1175,23 +1291,33 @@ void launch_lcc_phase1(pj_ctx* ctx) {
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   const uint64_t V = ctx->V;
   const int g = ctx->cus * 8;
+  unsigned long long* c = begin_launch(ctx, K_LCC_P1_G4);
   hipLaunchKernelGGL((k_lcc_phase1<Word, 4, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
                      ctx->d_bin_count + 0, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, ctx->d_ea,
-                     ctx->d_t, ctx->d_counters);
+                     ctx->d_t, c);
+  end_launch(ctx);
+  c = begin_launch(ctx, K_LCC_P1_G16);
   hipLaunchKernelGGL((k_lcc_phase1<Word, 16, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
                      ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, ctx->d_ea,
-                     ctx->d_t, ctx->d_counters);
+                     ctx->d_t, c);
+  end_launch(ctx);
+  c = begin_launch(ctx, K_LCC_P1_G64);
   hipLaunchKernelGGL((k_lcc_phase1<Word, 64, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
                      ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, ctx->d_ea,
-                     ctx->d_t, ctx->d_counters);
+                     ctx->d_t, c);
+  end_launch(ctx);
   if (ctx->nhubs) {
+    c = begin_launch(ctx, K_LCC_HUB);
     hipLaunchKernelGGL(k_lcc_hub_chunks<Word>, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
                        ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
                        ctx->d_edges, ctx->d_code, cand, ctx->d_flags, ctx->d_ea, ctx->d_hub_count, ctx->d_hub_elim,
-                       ctx->d_t, ctx->d_counters);
+                       ctx->d_t, c);
+    end_launch(ctx);
+    begin_launch(ctx, K_LCC_HUB_FINISH);
     hipLaunchKernelGGL((k_lcc_hub_finish<Word, KW>), dim3(grid_for(ctx, ctx->nhubs)), dim3(kBlock), 0, ctx->stream,
                        ctx->d_hubs, ctx->nhubs, ctx->d_code, cand, unm, ctx->d_flags, ctx->d_hub_count, ctx->d_hub_elim,
                        ctx->d_t);
+    end_launch(ctx);
   }
 }
 
@@ -1221,11 +1347,13 @@ void launch_nlcc(pj_ctx* ctx, int kind) {
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   const uint64_t V = ctx->V;
   const int g = ctx->cus * 8;
-  // Every bin and the hubs hold active vertices; hubs are run from their list.
-  for (int b = 0; b < kBins; ++b)
+  // Every bin holds active vertices; hubs are run from their own list.
+  for (int b = 0; b < kBins; ++b) {
+    unsigned long long* c = begin_launch(ctx, K_NLCC);
     hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, kind, ctx->d_bins + b * V,
-                       ctx->d_bin_count + b, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t,
-                       ctx->d_counters);
+                       ctx->d_bin_count + b, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t, c);
+    end_launch(ctx);
+  }
 }
 
 // Hubs are not in the bins; the NLCC task list for them is the hub list with
@@ -1236,9 +1364,10 @@ void launch_nlcc_hubs(pj_ctx* ctx, int kind, uint32_t* d_hub_n) {
   if (!ctx->nhubs) return;
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
+  unsigned long long* c = begin_launch(ctx, K_NLCC_HUB);
   hipLaunchKernelGGL(k_nlcc<Word>, dim3(std::min<uint32_t>(ctx->nhubs, ctx->cus * 8)), dim3(64), 0, ctx->stream, kind,
-                     ctx->d_hubs, d_hub_n, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t,
-                     ctx->d_counters);
+                     ctx->d_hubs, d_hub_n, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t, c);
+  end_launch(ctx);
 }
 
 int install_nlcc(pj_ctx* ctx, int step, uint32_t index) {
@@ -1311,10 +1440,13 @@ template <typename Word>
 int run_step_t(pj_ctx* ctx, int step, uint32_t index) {
   zero_counters(ctx);
   switch (step) {
-    case PJ_STEP_LCC0:
+    case PJ_STEP_LCC0: {
+      unsigned long long* c = begin_launch(ctx, K_LCC0);
       hipLaunchKernelGGL(k_lcc0<Word>, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream,
-                         static_cast<Word*>(ctx->d_cand), ctx->d_code, ctx->V, ctx->d_t, ctx->d_counters);
+                         static_cast<Word*>(ctx->d_cand), ctx->d_code, ctx->V, ctx->d_t, c);
+      end_launch(ctx);
       break;
+    }
     case PJ_STEP_LCC:
       launch_lcc<Word>(ctx);
       break;
@@ -1371,12 +1503,16 @@ int pj_create(pj_ctx** out, int device) {
   *out = nullptr;
   auto* ctx = new pj_ctx();
   ctx->device = device;
-  ctx->h_counters.resize(kCounterSlots * kCounterStride);
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&ctx->d_t), sizeof(DevTemplate));
+  if (e == hipSuccess)
+    e = hipHostMalloc(reinterpret_cast<void**>(&ctx->h_sums), (kMaxLaunches + 1) * kCounterStride * 8,
+                      hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipMalloc(reinterpret_cast<void**>(&ctx->d_sums), (kMaxLaunches + 1) * kCounterStride * 8);
   if (e != hipSuccess) {
     fprintf(stderr, "pj_create: %s\n", hipGetErrorString(e));
     delete ctx;
@@ -1416,9 +1552,14 @@ static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* of
   }
   if (setup_hubs(ctx, host_off)) return -1;
   {
+    // The edge-active bitset is free until a template is installed.
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bin_count, static_cast<uint64_t>(4), 0u);
-    hipLaunchKernelGGL(k_rows_sorted, dim3(grid_for(ctx, V)), dim3(kBlock), 0, ctx->stream, ctx->d_off, ctx->d_edges,
-                       V, ctx->d_bin_count);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, ctx->ea_words)), dim3(kBlock), 0, ctx->stream, ctx->d_ea,
+                       ctx->ea_words, 0u);
+    hipLaunchKernelGGL(k_row_starts, dim3(grid_for(ctx, V)), dim3(kBlock), 0, ctx->stream, ctx->d_off, V, E,
+                       ctx->d_ea);
+    hipLaunchKernelGGL(k_rows_sorted, dim3(grid_for(ctx, E)), dim3(kBlock), 0, ctx->stream, ctx->d_edges, E,
+                       ctx->d_ea, ctx->d_bin_count);
     uint32_t unsorted = 0;
     PJ_HIP(hipMemcpyAsync(&unsorted, ctx->d_bin_count, 4, hipMemcpyDeviceToHost, ctx->stream));
     PJ_HIP(hipStreamSynchronize(ctx->stream));
@@ -1548,23 +1689,69 @@ int pj_run_step(pj_ctx* ctx, int step, uint32_t index, pj_algo_results* out) {
   int rc = ctx->word_bytes == 4 ? run_step_t<uint32_t>(ctx, step, index) : run_step_t<uint64_t>(ctx, step, index);
   if (rc) return rc;
   PJ_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-  PJ_HIP(hipMemcpyAsync(ctx->h_counters.data(), ctx->d_counters, kCounterSlots * kCounterStride * 8, hipMemcpyDeviceToHost,
-                        ctx->stream));
+  const size_t blocks = ctx->timing ? static_cast<size_t>(ctx->nlaunch) + 1 : 1;
+  hipLaunchKernelGGL(k_sum_counters, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, ctx->stream,
+                     ctx->d_counters, ctx->d_sums);
+  PJ_HIP(hipGetLastError());
+  PJ_HIP(hipMemcpyAsync(ctx->h_sums, ctx->d_sums, blocks * kCounterStride * 8, hipMemcpyDeviceToHost, ctx->stream));
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   PJ_HIP(hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
   pj_algo_results r{0, 0, 0, 0};
   ctx->last_slots = 0;
-  for (int s = 0; s < kCounterSlots; ++s) {
-    r.vertex_eliminated += ctx->h_counters[s * kCounterStride + C_VERTEX];
-    r.edge_eliminated += ctx->h_counters[s * kCounterStride + C_EDGE];
-    r.match_eliminated += ctx->h_counters[s * kCounterStride + C_MATCH];
-    r.enumeration += ctx->h_counters[s * kCounterStride + C_ENUM];
-    ctx->last_slots += ctx->h_counters[s * kCounterStride + C_SLOTS];
+  for (size_t b = 0; b < blocks; ++b) {
+    const unsigned long long* sum = ctx->h_sums + b * kCounterStride;
+    r.vertex_eliminated += sum[C_VERTEX];
+    r.edge_eliminated += sum[C_EDGE];
+    r.match_eliminated += sum[C_MATCH];
+    r.enumeration += sum[C_ENUM];
+    ctx->last_slots += sum[C_SLOTS];
+    if (b == 0) continue;
+    const int kid = ctx->launch_kid[b - 1];
+    float ms = 0.f;
+    PJ_HIP(hipEventElapsedTime(&ms, ctx->lev[2 * (b - 1)], ctx->lev[2 * (b - 1) + 1]));
+    ctx->k_launches[kid] += 1;
+    ctx->k_ms[kid] += ms;
+    ctx->k_items[kid] += sum[C_ITEMS];
+    ctx->k_slots[kid] += sum[C_SLOTS];
+    ctx->k_elim[kid] += sum[C_EDGE];
   }
   ctx->active_vertices -= r.vertex_eliminated;
   ctx->active_edges -= r.edge_eliminated;
   if (out) *out = r;
   return 0;
+}
+
+int pj_set_kernel_timing(pj_ctx* ctx, int enable) {
+  if (!ctx) return -1;
+  if (enable && !ctx->lev[0])
+    for (int i = 0; i < 2 * kMaxLaunches; ++i) PJ_HIP(hipEventCreate(&ctx->lev[i]));
+  ctx->timing = enable != 0;
+  for (int k = 0; k < K_COUNT; ++k) {
+    ctx->k_launches[k] = ctx->k_items[k] = ctx->k_slots[k] = ctx->k_elim[k] = 0;
+    ctx->k_ms[k] = 0.0;
+  }
+  return 0;
+}
+
+int pj_kernel_stats(pj_ctx* ctx, pj_kernel_stat* out, int max) {
+  if (!ctx) return -1;
+  int n = 0;
+  for (int k = 0; k < K_COUNT; ++k) {
+    if (!ctx->k_launches[k]) continue;
+    if (out && n < max) {
+      pj_kernel_stat& st = out[n];
+      std::memset(&st, 0, sizeof(st));
+      std::strncpy(st.name, kKernelNames[k], sizeof(st.name) - 1);
+      st.launches = ctx->k_launches[k];
+      st.ms = ctx->k_ms[k];
+      st.items = ctx->k_items[k];
+      st.slots = ctx->k_slots[k];
+      st.edges_eliminated = ctx->k_elim[k];
+      st.word_bytes = static_cast<uint32_t>(ctx->word_bytes);
+    }
+    ++n;
+  }
+  return n;
 }
 
 int pj_last_step_ms(pj_ctx* ctx, double* ms) {
@@ -1625,8 +1812,12 @@ void pj_destroy(pj_ctx* ctx) {
   free_graph(ctx);
   if (ctx->d_t) hipFree(ctx->d_t);
   if (ctx->d_counters) hipFree(ctx->d_counters);
+  if (ctx->d_sums) hipFree(ctx->d_sums);
+  if (ctx->h_sums) hipHostFree(ctx->h_sums);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  for (int i = 0; i < 2 * kMaxLaunches; ++i)
+    if (ctx->lev[i]) hipEventDestroy(ctx->lev[i]);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
 }

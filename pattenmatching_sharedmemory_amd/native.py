@@ -54,6 +54,11 @@ class TemplateDesc(C.Structure):
     ]
 
 
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("ms", C.c_double), ("items", C.c_uint64),
+                ("slots", C.c_uint64), ("edges_eliminated", C.c_uint64), ("word_bytes", C.c_uint32)]
+
+
 class Row(C.Structure):
     """pjh_row: one step of the pipeline."""
     _fields_ = [("iteration", C.c_int32), ("step", C.c_int32), ("vertex_eliminated", C.c_uint64),
@@ -85,6 +90,8 @@ HIP_SIGNATURES = {
     "pj_run_step": (C.c_int, [P, C.c_int, C.c_uint32, C.POINTER(AlgoResults)]),
     "pj_last_step_ms": (C.c_int, [P, C.POINTER(C.c_double)]),
     "pj_last_step_slots": (C.c_int, [P, u64p]),
+    "pj_set_kernel_timing": (C.c_int, [P, C.c_int]),
+    "pj_kernel_stats": (C.c_int, [P, C.POINTER(KernelStat), C.c_int]),
     "pj_get_counters": (C.c_int, [P, u64p, u64p]),
     "pj_download_state": (C.c_int, [P, u64p, u8p, u32p]),
     "pj_upload_state": (C.c_int, [P, u64p, u8p, u32p, C.c_uint64, C.c_uint64]),

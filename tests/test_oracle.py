@@ -1,0 +1,210 @@
+"""Pins the oracle against what the mathematics fixes: brute-force
+embeddings and candidate sets, the LCC0 closed form, the greatest fixed point
+of the local constraint, brute-force walk existence for the cycle and path
+steps, and agreement of its faithful (reference) and complete modes where
+the reference's memo is inert."""
+import numpy as np
+import pytest
+
+from bruteforce import csr_rows, embeddings, ideal_candidates, lcc_greatest_fixed_point, walk_exists
+from conftest import FIXTURES, fixture_paths
+from pattenmatching_sharedmemory_amd.native import STEP_CC, STEP_ENUMERATION, STEP_LCC, STEP_LCC0, STEP_PC, STEP_TDS
+
+KNOWN_ENUMERATION = {"test_1": 1, "test_2": 2, "test_3": 2, "test_4": 1}
+
+
+def fixture(pj, name):
+    f = fixture_paths(name)
+    return pj.Graph.load(f["graph"], f["vertex"]), pj.Graph.load(f["pattern"], f["pattern_vertex"])
+
+
+def rows_of(g):
+    return csr_rows(g.offsets(), [int(x) for x in g.edges()])
+
+
+def cand_sets(cand, vflags):
+    return {v: {b for b in range(64) if (int(cand[v]) >> b) & 1} for v in range(len(cand)) if vflags[v] & 1}
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fixture_known_answers(pj, orc, name, mode):
+    g, p = fixture(pj, name)
+    c = pj.Constraints(p)
+    o = orc.Oracle(g, c, threads=1)
+    rows, _ = o.run(mode)
+    cand, vf, *_ = o.state()
+    g_rows, p_rows = rows_of(g), rows_of(p)
+    g_lab, p_lab = [int(x) for x in g.labels()], [int(x) for x in p.labels()]
+    emb = embeddings(g_rows, g_lab, p_rows, p_lab)
+    assert len(emb) == KNOWN_ENUMERATION[name]
+    assert rows[-1].step == STEP_ENUMERATION and rows[-1].enumeration == len(emb)
+    ideal = ideal_candidates(g_rows, g_lab, p_rows, p_lab)
+    got = cand_sets(cand, vf)
+    for v, s in ideal.items():  # pruning never removes a true match
+        assert s <= got.get(v, set()), (name, v)
+    if name != "test_4":  # the pipeline reaches the exact solution on these
+        assert got == ideal
+
+
+def test_step_machine_sequence(pj, orc):
+    g, p = fixture(pj, "test_4")
+    c = pj.Constraints(p)
+    rows, _ = orc.Oracle(g, c).run(orc.COMPLETE)
+    steps = [r.step for r in rows]
+    assert steps[0] == STEP_LCC0 and steps[1] == STEP_LCC and steps[-1] == STEP_ENUMERATION
+    n = {s: steps.count(s) for s in (STEP_CC, STEP_PC, STEP_TDS)}
+    assert n == {STEP_CC: len(c.cycles()), STEP_PC: len(c.paths()), STEP_TDS: len(c.tds())}
+    for i, r in enumerate(rows[:-1]):
+        if r.step in (STEP_CC, STEP_PC, STEP_TDS):
+            assert steps[i + 1] == STEP_LCC
+        if r.step == STEP_LCC and steps[i + 1] != STEP_LCC:
+            assert not (r.vertex_eliminated or r.edge_eliminated or r.match_eliminated)
+    assert [s for s in steps if s in (STEP_CC, STEP_PC, STEP_TDS)] == sorted(
+        [s for s in steps if s in (STEP_CC, STEP_PC, STEP_TDS)])
+
+
+def test_lcc0_closed_form(pj, orc):
+    g = pj.Graph.rmat(11, 8, seed=4).labels_uniform(6, seed=5)
+    p = pj.Graph.pattern(5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0)], [0, 1, 1, 3, 9])
+    c = pj.Constraints(p)
+    o = orc.Oracle(g, c)
+    ve, ee, me, en = o.step(STEP_LCC0)
+    lab = g.labels()
+    plab = [0, 1, 1, 3, 9]
+    assert ee == 0 and en == 0
+    assert me == sum(5 - plab.count(int(l)) for l in lab)
+    assert ve == sum(1 for l in lab if int(l) not in plab)
+    cand, vf, *_ = o.state()
+    for v in range(g.vertex_count):
+        assert int(cand[v]) == sum(1 << q for q in range(5) if plab[q] == int(lab[v]))
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_lcc_reaches_the_greatest_fixed_point(pj, orc, seed):
+    g = pj.Graph.rmat(9, 6, seed=seed).labels_uniform(4, seed=seed + 10)
+    p = pj.Graph.pattern(5, [(0, 1), (1, 2), (2, 0), (2, 3), (3, 4)], [0, 1, 2, 3, 1])
+    c = pj.Constraints(p)
+    o = orc.Oracle(g, c, threads=2)
+    o.step(STEP_LCC0)
+    while any(o.step(STEP_LCC)):
+        pass
+    cand, vf, ea, *_ = o.state()
+    got = cand_sets(cand, vf)
+    gfp = lcc_greatest_fixed_point(rows_of(g), [int(x) for x in g.labels()], rows_of(p),
+                                   [int(x) for x in p.labels()], order_seed=seed)
+    assert got == {v: s for v, s in enumerate(gfp) if s}
+    # edge activity is symmetric between active vertices at the fixed point
+    off, ed = g.offsets(), g.edges()
+    active = {v for v in got}
+    bits = np.unpackbits(ea.view(np.uint8), bitorder="little")
+    slot = {(v, int(ed[e])): bool(bits[e]) for v in range(g.vertex_count) for e in range(off[v], off[v + 1])}
+    for (v, u), a in slot.items():
+        if v in active and u in active:
+            assert a == slot[(u, v)]
+
+
+def _one_nlcc_step(pj, orc, g, p, step, index, mode):
+    c = pj.Constraints(p)
+    o = orc.Oracle(g, c, threads=4)
+    o.step(STEP_LCC0)
+    while any(o.step(STEP_LCC)):
+        pass
+    before = o.state()
+    o.reset_modified()
+    o.step(step, index, mode)
+    return c, before, o.state()
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_cycle_step_is_walk_existence(pj, orc, seed):
+    g = pj.Graph.rmat(10, 8, seed=seed).labels_uniform(4, seed=seed)
+    p = pj.Graph.pattern(4, [(0, 1), (1, 2), (2, 3), (3, 0)], [0, 1, 2, 3])
+    c, before, after = _one_nlcc_step(pj, orc, g, p, STEP_CC, 0, orc.COMPLETE)
+    cand0, vf0, ea0, *_ = before
+    bits = np.unpackbits(ea0.view(np.uint8), bitorder="little")
+    off = g.offsets()
+    rows = rows_of(g)
+    cyc = c.cycles()[0]
+    L = len(cyc)
+    active_slot = lambda w, i: bool(bits[off[w] + i])
+    cand = [int(x) if vf0[v] & 1 else 0 for v, x in enumerate(cand0)]
+    expect = list(cand)
+    for v in range(g.vertex_count):
+        for j in range(L):
+            q = cyc[j]
+            if not (cand[v] >> q) & 1:
+                continue
+            pos = [cyc[(j + i) % L] for i in range(L)]
+            if not walk_exists(rows, active_slot, cand, v, pos, closing_target=v):
+                expect[v] &= ~(1 << q)
+    cand1, vf1, *_ = after
+    assert [int(x) if vf1[v] & 1 else 0 for v, x in enumerate(cand1)] == expect
+
+
+@pytest.mark.parametrize("seed", [7, 8])
+def test_path_step_is_walk_existence(pj, orc, seed):
+    g = pj.Graph.rmat(10, 8, seed=seed).labels_uniform(5, seed=seed)
+    p = pj.Graph.pattern(6, [(0, 1), (1, 2), (2, 0), (2, 3), (3, 4), (4, 5)], [0, 1, 2, 3, 4, 3])
+    c, before, after = _one_nlcc_step(pj, orc, g, p, STEP_PC, 0, orc.COMPLETE)
+    cand0, vf0, ea0, *_ = before
+    bits = np.unpackbits(ea0.view(np.uint8), bitorder="little")
+    off = g.offsets()
+    rows = rows_of(g)
+    path = c.paths()[0]
+    active_slot = lambda w, i: bool(bits[off[w] + i])
+    cand = [int(x) if vf0[v] & 1 else 0 for v, x in enumerate(cand0)]
+    expect = list(cand)
+    lab = g.labels()
+    for v in range(g.vertex_count):
+        if int(lab[v]) != c.path_labels()[0]:
+            continue
+        for pos in (path, path[::-1]):
+            if (cand[v] >> pos[0]) & 1 and not walk_exists(rows, active_slot, cand, v, pos, end_excl=v):
+                expect[v] &= ~(1 << pos[0])
+    cand1, vf1, *_ = after
+    assert [int(x) if vf1[v] & 1 else 0 for v, x in enumerate(cand1)] == expect
+
+
+def _faithful_equals_complete(pj, orc, g, p, threads):
+    c = pj.Constraints(p)
+    a, _ = orc.Oracle(g, c, threads=1).run(orc.FAITHFUL)
+    b, _ = orc.Oracle(g, c, threads=threads).run(orc.FAITHFUL)
+    d, _ = orc.Oracle(g, c, threads=threads).run(orc.COMPLETE)
+    assert [r.key() for r in a] == [r.key() for r in b] == [r.key() for r in d]
+
+
+def test_memo_inert_cycle_faithful_equals_complete(pj, orc):
+    g = pj.Graph.rmat(13, 16, seed=1).labels_uniform(8, seed=2)
+    _faithful_equals_complete(pj, orc, g, pj.Graph.pattern(4, [(0, 1), (1, 2), (2, 3), (3, 0)], [0, 1, 2, 3]), 8)
+
+
+def test_memo_inert_path_faithful_equals_complete(pj, orc):
+    g = pj.Graph.rmat(12, 16, seed=20221).labels_zipf(32, 1.2, seed=7)
+    p = pj.Graph.pattern(6, [(0, 1), (1, 2), (2, 0), (2, 3), (3, 4), (4, 5)], [7, 8, 9, 10, 11, 10])
+    _faithful_equals_complete(pj, orc, g, p, 8)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixtures_faithful_equals_complete(pj, orc, name):
+    g, p = fixture(pj, name)
+    _faithful_equals_complete(pj, orc, g, p, 4)
+
+
+def test_oracle_detects_a_planted_mistake(pj, orc):
+    """A candidate set corrupted on the CPU changes the oracle's answers:
+    the comparison the GPU tests rely on is sensitive to single bits."""
+    g, p = fixture(pj, "test_2")
+    c = pj.Constraints(p)
+    o = orc.Oracle(g, c)
+    rows, _ = o.run()
+    cand, vf, ea, av, ae = o.state()
+    o2 = orc.Oracle(g, c)
+    o2.step(STEP_LCC0)
+    cand2, vf2, ea2, av2, ae2 = o2.state()
+    cand2[3] = 0  # drop vertex 3, part of one of the two matched triangles
+    vf2[3] = vf2[3] & 0xFE
+    o2.load_state(cand2, vf2, ea2, av2 - 1, ae2)
+    while any(o2.step(STEP_LCC)):
+        pass
+    assert o2.step(STEP_ENUMERATION)[3] != rows[-1].enumeration
