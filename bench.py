@@ -6,13 +6,21 @@ self-loops (4,194,304 vertices, 128,312,408 directed CSR slots), 32 labels
 drawn from Zipf(s=1.2); template = labelled triangle t0-t1-t2 plus the tail
 t2-t3-t4-t5 with labels [0, 1, 2, 3, 4, 3] (one cycle and one path
 constraint).  One bench step = one whole pipeline solve on the HBM-resident
-graph: state reset, LCC0, LCC to fixed point, CC, LCC, PC, LCC, enumeration.
+graph: state reset, LCC0, LCC to fixed point, CC, LCC, PC, LCC to the
+pruning fixed point (enumeration is not part of the timed solve).
 
-value = pruning TEPS = CSR slots examined by all steps (counted by the
-kernels) / solve time.  roofline = the dominant kernel's algorithmic bytes per
-launch / its average launch time (HIP events on the context stream) against
-8 TB/s.  cpu_baseline = the oracle (CPU restatement of the reference, faithful
-mode, all host threads) on a bounded sample of the same workload family.
+value = pruning TEPS = CSR slots a serial scan in the reference's loop order
+reads over all steps (counted by the kernels; rows of vertices skipped as
+unchanged count as the reference would read them) / solve time.  roofline =
+the dominant kernel's algorithmic bytes per launch / its average launch time
+(HIP events on the context stream) against 8 TB/s; traffic = HBM bytes per
+launch of that kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+summary of the same workload (profiles/), when there is one.  cpu_baseline =
+the oracle (CPU restatement of the reference, faithful mode, all host
+threads) on a bounded sample of the same workload family.
+
+--copy-calibration MIB runs only a device-to-device copy of MIB MiB three
+times (the plain copy kernel the PMC byte counters are scaled by).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -89,6 +97,34 @@ def roofline_from(stats: list[dict], vertices: int) -> dict | None:
             "bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches": dom["launches"]}
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "pruning_rmat22_hbm_traffic.md"
+
+
+def measured_traffic(kernel: str) -> float | None:
+    """HBM bytes per launch of `kernel` from the committed PMC summary."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic:" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
+def copy_calibration(mib: int) -> int:
+    import torch
+
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+    return 0
+
+
 def cpu_baseline(scale: int, threads: int) -> dict:
     import oracle
 
@@ -115,11 +151,14 @@ def main() -> int:
     ap.add_argument("--cpu-scale", type=int, default=19)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the pruning path is measured on a single MI355X "
               "(multi-GPU sharding is a design note, not a measured configuration)", file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
 
     from pattenmatching_sharedmemory_amd import Pruner
 
@@ -162,6 +201,8 @@ def main() -> int:
                 f"({100 * k['ms'] / max(total_k, 1e-9):5.1f}%) items {k['items']:11d} slots {k['slots']:12d} "
                 f"algorithmic {gbs:8.1f} GB/s")
         roofline = roofline_from(kstats, g.vertex_count)
+        if roofline and args.scale == 22:
+            roofline["traffic"] = measured_traffic(roofline["kernel"])
 
     result = {
         "metric": METRIC,

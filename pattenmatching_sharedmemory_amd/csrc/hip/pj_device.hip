@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -38,7 +39,12 @@ constexpr int kMaxLabels = 64;
 // Counter slots: [slot][0..4] = vertex / edge / match eliminated,
 // enumeration, CSR slots examined (the TEPS numerator: every slot a serial
 // scan in the reference's loop order would read).
-enum { C_VERTEX = 0, C_EDGE = 1, C_MATCH = 2, C_ENUM = 3, C_SLOTS = 4, C_ITEMS = 5 };
+// C_SLOTS: slots examined here that a serial scan in the reference's loop
+// order also reads; C_SKIP: slots such a scan reads that this launch does not
+// examine (rows of clean vertices, which are skipped; hub rows, which another
+// launch reads); C_XSLOTS: slots examined here that the reference's count
+// does not include (dirty pushes, hub chunk reads).
+enum { C_VERTEX = 0, C_EDGE = 1, C_MATCH = 2, C_ENUM = 3, C_SLOTS = 4, C_ITEMS = 5, C_SKIP = 6, C_XSLOTS = 7 };
 constexpr int kCounterStride = 8;
 constexpr size_t kCounterBlock = static_cast<size_t>(kCounterSlots) * kCounterStride;
 constexpr int kMaxLaunches = 32;  // timed launches per step
@@ -157,6 +163,36 @@ __global__ void k_rows_sorted(const uint32_t* __restrict__ edges, uint64_t E, co
     if (!((bits[e >> 5] >> (e & 31)) & 1u) && edges[e] < edges[e - 1]) atomicOr(unsorted, 1u);
 }
 
+// Symmetry check on sorted rows, one wave per vertex u: every slot u->v must
+// have a reverse slot v->u (binary search of u in row v); *asym is set
+// otherwise.
+__global__ void __launch_bounds__(kBlock) k_rows_symmetric(const uint64_t* __restrict__ off,
+                                                           const uint32_t* __restrict__ edges, uint32_t V,
+                                                           uint32_t* asym) {
+  const int lane = __lane_id();
+  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  bool bad = false;
+  for (uint64_t u = wave; u < V; u += waves) {
+    const uint64_t beg = off[u], end = off[u + 1];
+    for (uint64_t e = beg + lane; e < end; e += 64) {
+      const uint32_t v = edges[e];
+      if (v >= V) {
+        bad = true;
+        continue;
+      }
+      uint64_t lo = off[v], hi = off[v + 1];
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (edges[mid] < u) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo == off[v + 1] || edges[lo] != u) bad = true;
+    }
+  }
+  if (__ballot(bad) && lane == 0) atomicOr(asym, 1u);
+}
+
 // Sums the kCounterSlots slots of counter block blockIdx.x.
 __global__ void k_sum_counters(const unsigned long long* __restrict__ counters, unsigned long long* __restrict__ sums) {
   __shared__ unsigned long long part[kBlock / 64][kCounterStride];
@@ -238,12 +274,20 @@ __constant__ uint32_t c_bin_limit[kBins] = {4, 32, kHubDegree};
 constexpr int kBinIters = 16;                 // vertices per thread
 constexpr int kBinVerts = kBlock * kBinIters;  // vertices per block
 
+// mode 0 (LCC phase 1 frontier): active, scheduled and, when `dirty` is
+// given, dirty.  A scheduled vertex whose candidate set and whose neighbours'
+// candidate sets are unchanged since its last evaluation would remove nothing
+// and eliminate no slot: its previous removals were applied, its eliminated
+// slots stay eliminated and the surviving slots and label counts are the
+// same.  Skipping it leaves every result, flag and counter unchanged.
 template <typename Word>
 __device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __restrict__ cand,
                                       const uint8_t* __restrict__ flags, const uint8_t* __restrict__ code,
-                                      const uint64_t* __restrict__ off, int mode, uint32_t path_code) {
+                                      const uint8_t* __restrict__ dirty, const uint64_t* __restrict__ off, int mode,
+                                      uint32_t path_code) {
   if (v >= V || !cand[v]) return -1;
   if (mode == 0 && !(flags[v] & kScheduled)) return -1;
+  if (mode == 0 && dirty && !dirty[v]) return -2;  // clean: skipped
   if (mode == 2 && code[v] != path_code) return -1;
   const uint64_t d = off[v + 1] - off[v];
   if (d <= c_bin_limit[0]) return 0;
@@ -258,8 +302,9 @@ __device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __rest
 template <typename Word>
 __global__ void __launch_bounds__(kBlock) k_build_bins(
     const Word* __restrict__ cand, const uint8_t* __restrict__ flags, const uint8_t* __restrict__ code,
-    const uint64_t* __restrict__ off, uint32_t V, int mode, const DevTemplate* __restrict__ t,
-    uint32_t* __restrict__ bins, uint32_t* __restrict__ bin_count) {
+    const uint8_t* __restrict__ dirty, const uint64_t* __restrict__ off, uint32_t V, int mode,
+    const DevTemplate* __restrict__ t, uint32_t* __restrict__ bins, uint32_t* __restrict__ bin_count,
+    unsigned long long* counters) {
   __shared__ uint32_t s_wave[kBlock / 64][kBins];
   __shared__ uint32_t s_base[kBlock / 64][kBins];
   const int lane = lane_id();
@@ -267,12 +312,16 @@ __global__ void __launch_bounds__(kBlock) k_build_bins(
   const uint32_t path_code = mode == 2 ? t->path_code : 0u;
   const uint64_t v0 = static_cast<uint64_t>(blockIdx.x) * kBinVerts + threadIdx.x;
   uint32_t cnt[kBins] = {0, 0, 0};
+  uint64_t skipped = 0;
 #pragma unroll 4
   for (int i = 0; i < kBinIters; ++i) {
-    const int b = bin_of<Word>(v0 + static_cast<uint64_t>(i) * kBlock, V, cand, flags, code, off, mode, path_code);
+    const uint64_t v = v0 + static_cast<uint64_t>(i) * kBlock;
+    const int b = bin_of<Word>(v, V, cand, flags, code, dirty, off, mode, path_code);
+    if (b == -2) skipped += off[v + 1] - off[v];
 #pragma unroll
     for (int k = 0; k < kBins; ++k) cnt[k] += __popcll(__ballot(b == k));
   }
+  if (dirty) wave_add(counters, C_SKIP, skipped);
   if (lane == 0)
     for (int k = 0; k < kBins; ++k) s_wave[wave][k] = cnt[k];
   __syncthreads();
@@ -294,7 +343,7 @@ __global__ void __launch_bounds__(kBlock) k_build_bins(
 #pragma unroll 4
   for (int i = 0; i < kBinIters; ++i) {
     const uint64_t v = v0 + static_cast<uint64_t>(i) * kBlock;
-    const int b = bin_of<Word>(v, V, cand, flags, code, off, mode, path_code);
+    const int b = bin_of<Word>(v, V, cand, flags, code, dirty, off, mode, path_code);
 #pragma unroll
     for (int k = 0; k < kBins; ++k) {
       const uint64_t m = __ballot(b == k);
@@ -355,7 +404,7 @@ template <typename Word, int G, int KW>
 __global__ void __launch_bounds__(kBlock) k_lcc_phase1(
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, const uint64_t* __restrict__ off,
     const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code, const Word* __restrict__ cand,
-    Word* __restrict__ unmatch, uint8_t* __restrict__ flags, uint32_t* __restrict__ ea,
+    Word* __restrict__ unmatch, uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty, uint32_t* __restrict__ ea,
     const DevTemplate* __restrict__ t, unsigned long long* counters) {
   const int lane = lane_id();
   const int gl = lane & (G - 1);
@@ -406,6 +455,7 @@ __global__ void __launch_bounds__(kBlock) k_lcc_phase1(
         const Word fail = lcc_requirements<Word, KW>(cv, code[v], cnt, t);
         if (fail) unmatch[v] = fail;
         if (fail || any_elim) flags[v] |= kModified;
+        if (dirty) dirty[v] = 0;
       }
     }
   }
@@ -423,8 +473,8 @@ __global__ void __launch_bounds__(kBlock) k_lcc_hub_chunks(
     const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
     const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
     const uint8_t* __restrict__ code, const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
-    uint32_t* __restrict__ ea, uint32_t* __restrict__ hub_count, uint32_t* __restrict__ hub_elim,
-    const DevTemplate* __restrict__ t, unsigned long long* counters) {
+    const uint8_t* __restrict__ dirty, uint32_t* __restrict__ ea, uint32_t* __restrict__ hub_count,
+    uint32_t* __restrict__ hub_elim, const DevTemplate* __restrict__ t, unsigned long long* counters) {
   __shared__ uint32_t s_cnt[kMaxLabels];
   __shared__ uint32_t s_elim;
   const uint32_t K = t->K;
@@ -433,7 +483,7 @@ __global__ void __launch_bounds__(kBlock) k_lcc_hub_chunks(
     const uint32_t h = chunk_hub[c];
     const uint32_t v = hubs[h];
     const Word cv = cand[v];
-    if (!cv || !(flags[v] & kScheduled)) continue;  // block-uniform
+    if (!cv || !(flags[v] & kScheduled) || (dirty && !dirty[v])) continue;  // block-uniform
     if (threadIdx.x < kMaxLabels) s_cnt[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_elim = 0;
     __syncthreads();
@@ -474,8 +524,8 @@ __global__ void __launch_bounds__(kBlock) k_lcc_hub_chunks(
 template <typename Word, int KW>
 __global__ void k_lcc_hub_finish(const uint32_t* __restrict__ hubs, uint32_t nhubs, const uint8_t* __restrict__ code,
                                  const Word* __restrict__ cand, Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
-                                 uint32_t* __restrict__ hub_count, uint32_t* __restrict__ hub_elim,
-                                 const DevTemplate* __restrict__ t) {
+                                 uint8_t* __restrict__ dirty, uint32_t* __restrict__ hub_count,
+                                 uint32_t* __restrict__ hub_elim, const DevTemplate* __restrict__ t) {
   for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nhubs; h += gridDim.x * blockDim.x) {
     const uint32_t v = hubs[h];
     const Word cv = cand[v];
@@ -489,10 +539,11 @@ __global__ void k_lcc_hub_finish(const uint32_t* __restrict__ hubs, uint32_t nhu
     }
     const uint32_t elim = hub_elim[h];
     hub_elim[h] = 0;
-    if (!cv || !(flags[v] & kScheduled)) continue;
+    if (!cv || !(flags[v] & kScheduled) || (dirty && !dirty[v])) continue;
     const Word fail = lcc_requirements<Word, KW>(cv, code[v], cnt, t);
     if (fail) unmatch[v] = fail;
     if (fail || elim) flags[v] |= kModified;
+    if (dirty) dirty[v] = 0;
   }
 }
 
@@ -507,7 +558,8 @@ template <typename Word, int G>
 __global__ void __launch_bounds__(kBlock) k_phase2(
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, const uint64_t* __restrict__ off,
     const uint32_t* __restrict__ edges, Word* __restrict__ cand, Word* __restrict__ unmatch,
-    uint8_t* __restrict__ flags, const uint32_t* __restrict__ ea, unsigned long long* counters) {
+    uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty, const uint32_t* __restrict__ ea,
+    unsigned long long* counters) {
   const int lane = lane_id();
   const int gl = lane & (G - 1);
   const int gbase = lane & ~(G - 1);
@@ -516,14 +568,14 @@ __global__ void __launch_bounds__(kBlock) k_phase2(
   if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_ITEMS] = n;
   const uint64_t groups_per_grid = (uint64_t)gridDim.x * (blockDim.x / G);
   const uint64_t span = ((n + groups_per_grid - 1) / groups_per_grid) * groups_per_grid;
-  uint64_t velim = 0, melim = 0, slots = 0;
+  uint64_t velim = 0, melim = 0, slots = 0, xslots = 0;
   for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G; g < span; g += groups_per_grid) {
     if (g < n) {
       const uint32_t v = list[g];
       const uint8_t f = flags[v];
       if (f & kModified) {
+        const Word tu = unmatch[v];  // group-uniform
         if (gl == 0) {
-          const Word tu = unmatch[v];
           if (tu) {
             const Word c = cand[v];
             const Word nc = c & ~tu;
@@ -533,6 +585,14 @@ __global__ void __launch_bounds__(kBlock) k_phase2(
             unmatch[v] = 0;
           }
           flags[v] = f | kScheduled;
+        }
+        if (tu && dirty) {  // v's candidates change: v and every vertex reading them re-evaluate
+          const uint64_t beg = off[v], end = off[v + 1];
+          if (gl == 0) {
+            dirty[v] = 1;
+            xslots += end - beg;
+          }
+          for (uint64_t e = beg + gl; e < end; e += G) dirty[edges[e]] = 1;
         }
       } else {
         const uint64_t beg = off[v], end = off[v + 1];
@@ -559,6 +619,7 @@ __global__ void __launch_bounds__(kBlock) k_phase2(
   wave_add(counters, C_VERTEX, velim);
   wave_add(counters, C_MATCH, melim);
   wave_add(counters, C_SLOTS, slots);
+  wave_add(counters, C_XSLOTS, xslots);
 }
 
 // Hub phase 2: a hub that is not modified is scheduled when any chunk finds
@@ -569,34 +630,51 @@ template <typename Word>
 __global__ void __launch_bounds__(kBlock) k_phase2_hub_chunks(
     const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
     const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
-    const Word* __restrict__ cand, const uint8_t* __restrict__ flags, const uint32_t* __restrict__ ea,
-    uint32_t* __restrict__ hub_hit) {
+    const Word* __restrict__ cand, const Word* __restrict__ unmatch, const uint8_t* __restrict__ flags,
+    uint8_t* __restrict__ dirty, const uint32_t* __restrict__ ea, uint32_t* __restrict__ hub_hit,
+    unsigned long long* counters) {
   __shared__ uint32_t s_min;
+  uint64_t examined = 0;  // slots read here; the reference-order count is taken in k_phase2_hub_finish
   for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const uint32_t h = chunk_hub[c];
     const uint32_t v = hubs[h];
-    if (!cand[v] || (flags[v] & kModified)) continue;
+    if (!cand[v]) continue;
+    if (flags[v] & kModified) {
+      if (dirty && unmatch[v]) {  // the hub's candidates change: dirty its neighbours
+        const uint64_t beg = chunk_start[c];
+        const uint64_t end = min(off[v + 1], beg + kHubChunk);
+        for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) {
+          dirty[edges[e]] = 1;
+          ++examined;
+        }
+      }
+      continue;
+    }
     if (threadIdx.x == 0) s_min = 0xFFFFFFFFu;
     __syncthreads();
     const uint64_t row = off[v];
     const uint64_t beg = chunk_start[c];
     const uint64_t end = min(off[v + 1], beg + kHubChunk);
-    for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x)
+    for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) {
+      ++examined;
       if (ea_test(ea, e) && (flags[edges[e]] & kModified)) {
         atomicMin(&s_min, static_cast<uint32_t>(e - row));
         break;
       }
+    }
     __syncthreads();
     if (threadIdx.x == 0 && s_min != 0xFFFFFFFFu) atomicMin(&hub_hit[h], s_min);
     __syncthreads();
   }
+  wave_add(counters, C_XSLOTS, examined);
 }
 
 template <typename Word>
 __global__ void k_phase2_hub_finish(const uint32_t* __restrict__ hubs, uint32_t nhubs,
                                     const uint64_t* __restrict__ off, Word* __restrict__ cand,
                                     Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
-                                    uint32_t* __restrict__ hub_hit, unsigned long long* counters) {
+                                    uint8_t* __restrict__ dirty, uint32_t* __restrict__ hub_hit,
+                                    unsigned long long* counters) {
   uint64_t velim = 0, melim = 0, slots = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t span = ((nhubs + stride - 1) / stride) * stride;
@@ -616,6 +694,7 @@ __global__ void k_phase2_hub_finish(const uint32_t* __restrict__ hubs, uint32_t 
         cand[v] = nc;
         if (!nc) ++velim;
         unmatch[v] = 0;
+        if (dirty) dirty[v] = 1;  // k_phase2_hub_chunks dirtied the neighbours
       }
       flags[v] = f | kScheduled;
     } else {
@@ -626,7 +705,7 @@ __global__ void k_phase2_hub_finish(const uint32_t* __restrict__ hubs, uint32_t 
   }
   wave_add(counters, C_VERTEX, velim);
   wave_add(counters, C_MATCH, melim);
-  wave_add(counters, C_SLOTS, slots);
+  wave_add(counters, C_SKIP, slots);  // reference-order slots of the hub scans (read by k_phase2_hub_chunks)
 }
 
 // ---------------------------------------------------------------------------
@@ -1035,10 +1114,14 @@ struct pj_ctx {
   void* d_u64 = nullptr;   // staging for downloads / uploads
   uint8_t* d_flags = nullptr;
   uint8_t* d_vflags = nullptr;
+  uint8_t* d_dirty = nullptr;  // per vertex: candidates of v or of a neighbour changed since v's last evaluation
   uint32_t* d_ea = nullptr;
   uint64_t ea_words = 0;
   uint32_t ea_tail = 0;  // host staging of the last edge-active word
   bool rows_sorted = false;
+  bool symmetric = false;   // every slot u->v has a reverse slot v->u
+  bool skip_clean = false;  // phase 1 skips clean scheduled vertices (rows sorted and symmetric)
+  bool dirty_enabled = true;
   uint32_t* d_bins = nullptr;        // kBins * V
   uint32_t* d_bin_count = nullptr;   // kBins
   unsigned long long* d_counters = nullptr;  // kCounterSlots * kCounterStride
@@ -1114,7 +1197,7 @@ int dev_alloc(pj_ctx* ctx, T** p, uint64_t count) {
 
 void free_graph(pj_ctx* ctx) {
   void* bufs[] = {ctx->d_off, ctx->d_edges, ctx->d_labels, ctx->d_code, ctx->d_cand, ctx->d_unmatch, ctx->d_u64,
-                  ctx->d_flags, ctx->d_vflags, ctx->d_ea, ctx->d_bins, ctx->d_bin_count, ctx->d_hubs,
+                  ctx->d_flags, ctx->d_vflags, ctx->d_dirty, ctx->d_ea, ctx->d_bins, ctx->d_bin_count, ctx->d_hubs,
                   ctx->d_chunk_hub, ctx->d_chunk_start, ctx->d_hub_count, ctx->d_hub_elim, ctx->d_nhubs, ctx->d_hub_hit};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -1123,7 +1206,7 @@ void free_graph(pj_ctx* ctx) {
   ctx->d_labels = nullptr;
   ctx->d_code = nullptr;
   ctx->d_cand = ctx->d_unmatch = ctx->d_u64 = nullptr;
-  ctx->d_flags = ctx->d_vflags = nullptr;
+  ctx->d_flags = ctx->d_vflags = ctx->d_dirty = nullptr;
   ctx->d_ea = nullptr;
   ctx->d_bins = ctx->d_bin_count = nullptr;
   ctx->d_hubs = ctx->d_chunk_hub = nullptr;
@@ -1180,11 +1263,19 @@ int alloc_state(pj_ctx* ctx) {
   if (dev_alloc(ctx, &ctx->d_code, V) || dev_alloc(ctx, reinterpret_cast<uint64_t**>(&ctx->d_cand), V) ||
       dev_alloc(ctx, reinterpret_cast<uint64_t**>(&ctx->d_unmatch), V) ||
       dev_alloc(ctx, reinterpret_cast<uint64_t**>(&ctx->d_u64), V) || dev_alloc(ctx, &ctx->d_flags, V + 4) ||
-      dev_alloc(ctx, &ctx->d_vflags, V + 4) || dev_alloc(ctx, &ctx->d_ea, ctx->ea_words) ||
+      dev_alloc(ctx, &ctx->d_vflags, V + 4) || dev_alloc(ctx, &ctx->d_dirty, V + 4) ||
+      dev_alloc(ctx, &ctx->d_ea, ctx->ea_words) ||
       dev_alloc(ctx, &ctx->d_bins, V * kBins) || dev_alloc(ctx, &ctx->d_bin_count, 4) ||
       dev_alloc(ctx, &ctx->d_counters, (kMaxLaunches + 1) * kCounterBlock))
     return -1;
   return 0;
+}
+
+// Marks every vertex for re-evaluation (the candidate sets were replaced).
+void mark_all_dirty(pj_ctx* ctx) {
+  if (!ctx->skip_clean) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, (ctx->V + 4) / 4)), dim3(kBlock), 0, ctx->stream,
+                     reinterpret_cast<uint32_t*>(ctx->d_dirty), static_cast<uint64_t>((ctx->V + 4) / 4), 0x01010101u);
 }
 
 template <typename Word>
@@ -1199,6 +1290,7 @@ int state_reset_t(pj_ctx* ctx) {
     ctx->ea_tail = (1u << (ctx->E & 31)) - 1u;
     PJ_HIP(hipMemcpyAsync(ctx->d_ea + ctx->ea_words - 1, &ctx->ea_tail, 4, hipMemcpyHostToDevice, ctx->stream));
   }
+  mark_all_dirty(ctx);
   PJ_HIP(hipGetLastError());
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   ctx->active_vertices = ctx->V;
@@ -1243,11 +1335,12 @@ int zero_bins(pj_ctx* ctx) {
 template <typename Word>
 void launch_bins(pj_ctx* ctx, int mode) {
   zero_bins(ctx);
-  begin_launch(ctx, K_BINS);
+  unsigned long long* c = begin_launch(ctx, K_BINS);
   const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(ctx->V) + kBinVerts - 1) / kBinVerts);
   hipLaunchKernelGGL(k_build_bins<Word>, dim3(std::max<uint32_t>(blocks, 1)), dim3(kBlock), 0, ctx->stream,
-                     static_cast<const Word*>(ctx->d_cand), ctx->d_flags, ctx->d_code, ctx->d_off, ctx->V, mode,
-                     ctx->d_t, ctx->d_bins, ctx->d_bin_count);
+                     static_cast<const Word*>(ctx->d_cand), ctx->d_flags, ctx->d_code,
+                     mode == 0 && ctx->skip_clean ? ctx->d_dirty : nullptr, ctx->d_off, ctx->V, mode, ctx->d_t,
+                     ctx->d_bins, ctx->d_bin_count, c);
   end_launch(ctx);
 }
 
@@ -1255,31 +1348,32 @@ void launch_bins(pj_ctx* ctx, int mode) {
 template <typename Word>
 void launch_phase2(pj_ctx* ctx) {
   launch_bins<Word>(ctx, 1);
+  uint8_t* dirty = ctx->skip_clean ? ctx->d_dirty : nullptr;
   Word* cand = static_cast<Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   const uint64_t V = ctx->V;
   const int g = ctx->cus * 8;
   unsigned long long* c = begin_launch(ctx, K_P2_G4);
   hipLaunchKernelGGL((k_phase2<Word, 4>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins, ctx->d_bin_count + 0,
-                     ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, ctx->d_ea, c);
+                     ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, dirty, ctx->d_ea, c);
   end_launch(ctx);
   c = begin_launch(ctx, K_P2_G16);
   hipLaunchKernelGGL((k_phase2<Word, 16>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
-                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, ctx->d_ea, c);
+                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, dirty, ctx->d_ea, c);
   end_launch(ctx);
   c = begin_launch(ctx, K_P2_G64);
   hipLaunchKernelGGL((k_phase2<Word, 64>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
-                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, ctx->d_ea, c);
+                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, dirty, ctx->d_ea, c);
   end_launch(ctx);
   if (ctx->nhubs) {
-    begin_launch(ctx, K_P2_HUB);
+    c = begin_launch(ctx, K_P2_HUB);
     hipLaunchKernelGGL(k_phase2_hub_chunks<Word>, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
                        ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
-                       ctx->d_edges, cand, ctx->d_flags, ctx->d_ea, ctx->d_hub_hit);
+                       ctx->d_edges, cand, unm, ctx->d_flags, dirty, ctx->d_ea, ctx->d_hub_hit, c);
     end_launch(ctx);
     c = begin_launch(ctx, K_P2_HUB_FINISH);
     hipLaunchKernelGGL(k_phase2_hub_finish<Word>, dim3(grid_for(ctx, ctx->nhubs)), dim3(kBlock), 0, ctx->stream,
-                       ctx->d_hubs, ctx->nhubs, ctx->d_off, cand, unm, ctx->d_flags, ctx->d_hub_hit, c);
+                       ctx->d_hubs, ctx->nhubs, ctx->d_off, cand, unm, ctx->d_flags, dirty, ctx->d_hub_hit, c);
     end_launch(ctx);
   }
 }
@@ -1287,36 +1381,37 @@ void launch_phase2(pj_ctx* ctx) {
 template <typename Word, int KW>
 void launch_lcc_phase1(pj_ctx* ctx) {
   launch_bins<Word>(ctx, 0);
+  uint8_t* dirty = ctx->skip_clean ? ctx->d_dirty : nullptr;
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   const uint64_t V = ctx->V;
   const int g = ctx->cus * 8;
   unsigned long long* c = begin_launch(ctx, K_LCC_P1_G4);
   hipLaunchKernelGGL((k_lcc_phase1<Word, 4, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
-                     ctx->d_bin_count + 0, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, ctx->d_ea,
-                     ctx->d_t, c);
+                     ctx->d_bin_count + 0, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, dirty,
+                     ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
   c = begin_launch(ctx, K_LCC_P1_G16);
   hipLaunchKernelGGL((k_lcc_phase1<Word, 16, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
-                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, ctx->d_ea,
-                     ctx->d_t, c);
+                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, dirty,
+                     ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
   c = begin_launch(ctx, K_LCC_P1_G64);
   hipLaunchKernelGGL((k_lcc_phase1<Word, 64, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
-                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, ctx->d_ea,
-                     ctx->d_t, c);
+                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, dirty,
+                     ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
   if (ctx->nhubs) {
     c = begin_launch(ctx, K_LCC_HUB);
     hipLaunchKernelGGL(k_lcc_hub_chunks<Word>, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
                        ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
-                       ctx->d_edges, ctx->d_code, cand, ctx->d_flags, ctx->d_ea, ctx->d_hub_count, ctx->d_hub_elim,
-                       ctx->d_t, c);
+                       ctx->d_edges, ctx->d_code, cand, ctx->d_flags, dirty, ctx->d_ea, ctx->d_hub_count,
+                       ctx->d_hub_elim, ctx->d_t, c);
     end_launch(ctx);
     begin_launch(ctx, K_LCC_HUB_FINISH);
     hipLaunchKernelGGL((k_lcc_hub_finish<Word, KW>), dim3(grid_for(ctx, ctx->nhubs)), dim3(kBlock), 0, ctx->stream,
-                       ctx->d_hubs, ctx->nhubs, ctx->d_code, cand, unm, ctx->d_flags, ctx->d_hub_count, ctx->d_hub_elim,
-                       ctx->d_t);
+                       ctx->d_hubs, ctx->nhubs, ctx->d_code, cand, unm, ctx->d_flags, dirty, ctx->d_hub_count,
+                       ctx->d_hub_elim, ctx->d_t);
     end_launch(ctx);
   }
 }
@@ -1445,6 +1540,7 @@ int run_step_t(pj_ctx* ctx, int step, uint32_t index) {
       hipLaunchKernelGGL(k_lcc0<Word>, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream,
                          static_cast<Word*>(ctx->d_cand), ctx->d_code, ctx->V, ctx->d_t, c);
       end_launch(ctx);
+      mark_all_dirty(ctx);  // every candidate set was rewritten
       break;
     }
     case PJ_STEP_LCC:
@@ -1489,6 +1585,7 @@ int upload_state_t(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags, con
   hipLaunchKernelGGL(k_u64_to_cand<Word>, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream,
                      static_cast<const uint64_t*>(ctx->d_u64), ctx->d_vflags, static_cast<Word*>(ctx->d_cand),
                      static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->V);
+  mark_all_dirty(ctx);
   PJ_HIP(hipGetLastError());
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
@@ -1521,6 +1618,9 @@ int pj_create(pj_ctx** out, int device) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
     ctx->cus = cus;
+  // PJ_SKIP_CLEAN=0 evaluates every scheduled vertex in LCC phase 1 (the
+  // results are the same either way; the switch exists for comparisons).
+  if (const char* sc = getenv("PJ_SKIP_CLEAN")) ctx->dirty_enabled = strcmp(sc, "0") != 0;
   *out = ctx;
   return 0;
 }
@@ -1564,6 +1664,17 @@ static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* of
     PJ_HIP(hipMemcpyAsync(&unsorted, ctx->d_bin_count, 4, hipMemcpyDeviceToHost, ctx->stream));
     PJ_HIP(hipStreamSynchronize(ctx->stream));
     ctx->rows_sorted = unsorted == 0;
+    ctx->symmetric = false;
+    if (ctx->rows_sorted) {
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bin_count, static_cast<uint64_t>(4), 0u);
+      hipLaunchKernelGGL(k_rows_symmetric, dim3(grid_for(ctx, static_cast<uint64_t>(V) * 64)), dim3(kBlock), 0,
+                         ctx->stream, ctx->d_off, ctx->d_edges, V, ctx->d_bin_count);
+      uint32_t asym = 0;
+      PJ_HIP(hipMemcpyAsync(&asym, ctx->d_bin_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+      PJ_HIP(hipStreamSynchronize(ctx->stream));
+      ctx->symmetric = asym == 0;
+    }
+    ctx->skip_clean = ctx->rows_sorted && ctx->symmetric && ctx->dirty_enabled;
   }
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   ctx->have_template = false;
@@ -1704,7 +1815,7 @@ int pj_run_step(pj_ctx* ctx, int step, uint32_t index, pj_algo_results* out) {
     r.edge_eliminated += sum[C_EDGE];
     r.match_eliminated += sum[C_MATCH];
     r.enumeration += sum[C_ENUM];
-    ctx->last_slots += sum[C_SLOTS];
+    ctx->last_slots += sum[C_SLOTS] + sum[C_SKIP];
     if (b == 0) continue;
     const int kid = ctx->launch_kid[b - 1];
     float ms = 0.f;
@@ -1712,7 +1823,7 @@ int pj_run_step(pj_ctx* ctx, int step, uint32_t index, pj_algo_results* out) {
     ctx->k_launches[kid] += 1;
     ctx->k_ms[kid] += ms;
     ctx->k_items[kid] += sum[C_ITEMS];
-    ctx->k_slots[kid] += sum[C_SLOTS];
+    ctx->k_slots[kid] += sum[C_SLOTS] + sum[C_XSLOTS];
     ctx->k_elim[kid] += sum[C_EDGE];
   }
   ctx->active_vertices -= r.vertex_eliminated;

@@ -149,3 +149,39 @@ def test_single_steps_from_random_states(pj, orc, seed):
         assert np.array_equal(st.vflags, ovf), label
         assert np.array_equal(st.ea_words, oea), label
     pr.close()
+
+
+@pytest.mark.parametrize("case", ["C2-shape s16 prune", "TDS s10", "hub"])
+def test_clean_vertex_skipping_changes_nothing(pj, orc, case, monkeypatch):
+    """Skipping clean scheduled vertices in LCC phase 1 leaves the step trace
+    (including the reference-order slot counts), the candidate sets, flags and
+    edge-active bits identical to evaluating every scheduled vertex."""
+    if case == "TDS s10":
+        edges = [(0, 1), (1, 2), (2, 3), (3, 0), (1, 4), (4, 5), (5, 2), (0, 2), (5, 6), (6, 7)]
+        g = pj.Graph.rmat(10, 8, seed=5).labels_zipf(8, 1.0, seed=6)
+        p = pj.Graph.pattern(8, edges, [0, 1, 2, 3, 4, 5, 6, 7])
+    elif case == "hub":
+        g = pj.Graph.rmat(14, 32, seed=3).labels_uniform(4, seed=4)  # R-MAT rows above 2048 slots
+        p = pj.Graph.pattern(4, [(0, 1), (1, 2), (2, 0), (2, 3)], [0, 1, 2, 3])
+    else:
+        g = pj.Graph.rmat(16, 16, seed=20221).labels_zipf(32, 1.2, seed=7)
+        p = pj.Graph.pattern(6, TRIANGLE_TAIL, [0, 1, 2, 3, 4, 3])
+    c = pj.Constraints(p)
+    runs = []
+    for skip in ("0", "1"):
+        monkeypatch.setenv("PJ_SKIP_CLEAN", skip)
+        pr = pj.Pruner(g, c)
+        pr.kernel_timing(True)
+        rows, _ = pr.run(enumerate_matches=False)
+        examined = sum(k["slots"] for k in pr.kernel_stats() if k["name"].startswith("lcc_"))
+        runs.append((rows, pr.state(), examined))
+        pr.close()
+    (r0, s0, x0), (r1, s1, x1) = runs
+    print(f"[{case}] steps={len(r0)} lcc slots examined: all {x0}, skipping clean {x1}", flush=True)
+    assert [(r.key(), r.slots) for r in r0] == [(r.key(), r.slots) for r in r1]
+    assert np.array_equal(s0.cand, s1.cand) and np.array_equal(s0.vflags, s1.vflags)
+    assert np.array_equal(s0.ea_words, s1.ea_words)
+    assert x1 <= x0
+    o = orc.Oracle(g, c, threads=8)
+    orows, _ = o.run(orc.COMPLETE, enumerate_matches=False)
+    assert [r.key() for r in r1] == [r.key() for r in orows]
