@@ -175,7 +175,7 @@ typedef struct pj_kernel_stat {
   uint64_t items;            /* vertices (or hub chunks) the launches were given */
   uint64_t slots;            /* CSR slots examined */
   uint64_t edges_eliminated; /* edge-active bits cleared */
-  uint32_t word_bytes;       /* candidate word width (4 or 8) */
+  uint32_t word_bytes;       /* candidate word width (1, 2, 4 or 8 bytes) */
 } pj_kernel_stat;
 int pj_set_kernel_timing(pj_ctx* ctx, int enable);
 int pj_kernel_stats(pj_ctx* ctx, pj_kernel_stat* out, int max);

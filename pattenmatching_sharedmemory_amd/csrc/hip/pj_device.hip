@@ -35,10 +35,11 @@ constexpr int kCounterSlots = 256;  // per-block counters are spread over slots
 constexpr uint32_t kHubDegree = 2048;  // rows above this are split into chunks
 constexpr uint32_t kHubChunk = 2048;
 constexpr int kMaxLabels = 64;
+static_assert(kHubChunk % kBlock == 0, "hub chunks are split evenly over a block");
 
-// Counter slots: [slot][0..4] = vertex / edge / match eliminated,
-// enumeration, CSR slots examined (the TEPS numerator: every slot a serial
-// scan in the reference's loop order would read).
+// Counter slots: [slot][counter], summed per launch.  C_VERTEX / C_EDGE /
+// C_MATCH: eliminated vertices, edges and matches; C_ENUM: enumerated
+// embeddings; C_ITEMS: vertices (or hub chunks) a launch was given;
 // C_SLOTS: slots examined here that a serial scan in the reference's loop
 // order also reads; C_SKIP: slots such a scan reads that this launch does not
 // examine (rows of clean vertices, which are skipped; hub rows, which another
@@ -52,12 +53,12 @@ constexpr int kMaxLaunches = 32;  // timed launches per step
 // Kernel kinds reported by pj_kernel_stats.
 enum KernelId {
   K_LCC0, K_BINS, K_LCC_P1_G4, K_LCC_P1_G16, K_LCC_P1_G64, K_LCC_HUB, K_LCC_HUB_FINISH,
-  K_P2_G4, K_P2_G16, K_P2_G64, K_P2_HUB, K_P2_HUB_FINISH, K_NLCC, K_NLCC_HUB, K_COUNT
+  K_P2_G4, K_P2_G16, K_P2_G64, K_P2_HUB, K_P2_HUB_FINISH, K_NLCC, K_NLCC_HUB, K_HUB_PUSH, K_COUNT
 };
 const char* const kKernelNames[K_COUNT] = {
     "lcc0", "build_bins", "lcc_phase1_g4", "lcc_phase1_g16", "lcc_phase1_g64", "lcc_phase1_hub_chunks",
     "lcc_phase1_hub_finish", "phase2_g4", "phase2_g16", "phase2_g64", "phase2_hub_chunks", "phase2_hub_finish",
-    "nlcc_walk", "nlcc_walk_hubs"};
+    "nlcc_walk", "nlcc_walk_hubs", "hub_rescan_push"};
 
 // Template tables in device memory.
 struct DevTemplate {
@@ -120,6 +121,24 @@ __device__ __forceinline__ void wave_add(unsigned long long* counters, int which
   }
 }
 
+// Clears the edge-active bits of the eliminated slots among the G
+// consecutive slots [sb, sb + G) of a lane group (lane gl holds slot sb + gl;
+// `el`: that slot is eliminated) with one atomic per touched word.  Every
+// lane of the group must reach it.
+template <int G>
+__device__ __forceinline__ void clear_slots(uint32_t* __restrict__ ea, uint64_t sb, bool el, int gl, int gbase) {
+  const uint64_t rel = (__ballot(el) >> gbase) & (G == 64 ? ~0ull : ((1ull << G) - 1));
+  if (!rel) return;
+  const int sh = static_cast<int>(sb & 31);
+  const uint64_t lo = rel << sh;
+  const uint64_t hi = sh ? rel >> (64 - sh) : 0ull;
+  const uint32_t m = gl == 0 ? static_cast<uint32_t>(lo)
+                   : gl == 1 ? static_cast<uint32_t>(lo >> 32)
+                   : gl == 2 ? static_cast<uint32_t>(hi)
+                             : 0u;
+  if (m) atomicAnd(&ea[(sb >> 5) + gl], ~m);
+}
+
 __device__ __forceinline__ void set_flag_atomic(uint8_t* flags, uint32_t v, uint8_t bit) {
   uint32_t* word = reinterpret_cast<uint32_t*>(reinterpret_cast<uintptr_t>(flags + v) & ~uintptr_t(3));
   atomicOr(word, static_cast<uint32_t>(bit) << (8 * (v & 3)));
@@ -148,8 +167,9 @@ __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
 }
 
 // Row-order check: k_row_starts marks the first slot of every row in a
-// zeroed bitset, k_rows_sorted sets *unsorted when a slot that starts no row
-// holds a smaller id than the slot before it.
+// zeroed bitset, k_rows_sorted sets bit 0 of *unsorted when a slot that starts
+// no row holds a smaller id than the slot before it, bit 1 when it holds the
+// same id.
 __global__ void k_row_starts(const uint64_t* __restrict__ off, uint32_t V, uint64_t E, uint32_t* __restrict__ bits) {
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t e = off[v];
@@ -160,37 +180,62 @@ __global__ void k_row_starts(const uint64_t* __restrict__ off, uint32_t V, uint6
 __global__ void k_rows_sorted(const uint32_t* __restrict__ edges, uint64_t E, const uint32_t* __restrict__ bits,
                               uint32_t* unsorted) {
   for (uint64_t e = 1 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x)
-    if (!((bits[e >> 5] >> (e & 31)) & 1u) && edges[e] < edges[e - 1]) atomicOr(unsorted, 1u);
+    if (!((bits[e >> 5] >> (e & 31)) & 1u)) {
+      if (edges[e] < edges[e - 1]) atomicOr(unsorted, 1u);
+      else if (edges[e] == edges[e - 1]) atomicOr(unsorted, 2u);  // duplicate slot
+    }
 }
 
-// Symmetry check on sorted rows, one wave per vertex u: every slot u->v must
-// have a reverse slot v->u (binary search of u in row v); *asym is set
-// otherwise.
+// Symmetry check on rows sorted strictly ascending, one wave per vertex u.
+// Order vertices by (degree, id).  Every slot u->v with v before u is looked
+// up in the shorter row(v) by binary search (flag[0] set when u is missing);
+// flag[1] / flag[2] count the slots u->v with v before / after u.  When all
+// lookups succeed and the counts agree, the graph is symmetric: reversing the
+// "v before u" slots maps them injectively (rows hold no duplicates) onto
+// "after" slots, and equal counts make that map onto.
 __global__ void __launch_bounds__(kBlock) k_rows_symmetric(const uint64_t* __restrict__ off,
                                                            const uint32_t* __restrict__ edges, uint32_t V,
-                                                           uint32_t* asym) {
+                                                           unsigned long long* flag) {
   const int lane = __lane_id();
   const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
   const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   bool bad = false;
+  uint64_t before = 0, after = 0;
   for (uint64_t u = wave; u < V; u += waves) {
     const uint64_t beg = off[u], end = off[u + 1];
+    const uint64_t du = end - beg;
     for (uint64_t e = beg + lane; e < end; e += 64) {
       const uint32_t v = edges[e];
       if (v >= V) {
         bad = true;
         continue;
       }
+      if (v == u) continue;
       uint64_t lo = off[v], hi = off[v + 1];
-      while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (edges[mid] < u) lo = mid + 1;
-        else hi = mid;
+      const uint64_t dv = hi - lo;
+      if (dv < du || (dv == du && v < u)) {
+        ++before;
+        const uint64_t last = hi;
+        while (lo < hi) {
+          const uint64_t mid = (lo + hi) >> 1;
+          if (edges[mid] < u) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo == last || edges[lo] != u) bad = true;
+      } else {
+        ++after;
       }
-      if (lo == off[v + 1] || edges[lo] != u) bad = true;
     }
   }
-  if (__ballot(bad) && lane == 0) atomicOr(asym, 1u);
+  for (int o = 32; o > 0; o >>= 1) {
+    before += __shfl_xor(before, o, 64);
+    after += __shfl_xor(after, o, 64);
+  }
+  if (__ballot(bad) && lane == 0) atomicOr(&flag[0], 1ull);
+  if (lane == 0) {
+    if (before) atomicAdd(&flag[1], before);
+    if (after) atomicAdd(&flag[2], after);
+  }
 }
 
 // Sums the kCounterSlots slots of counter block blockIdx.x.
@@ -265,8 +310,9 @@ __global__ void k_lcc0(Word* __restrict__ cand, const uint8_t* __restrict__ code
 // Frontier construction.  Vertices are appended to degree bins with one
 // atomic per wave and bin; hubs (degree > kHubDegree) go to no bin, they are
 // served by the static chunk list built at upload.
-// mode 0: active and scheduled (LCC phase 1); mode 1: active (phase 2,
-// NLCC task list); mode 2: active with label code == t->path_code (PC).
+// mode 0: active and scheduled (LCC phase 1); mode 1: active (NLCC task
+// list); mode 2: active with label code == t->path_code (PC); mode 3: the
+// phase 2 frontier (see bin_of).
 
 constexpr int kBins = 3;
 __constant__ uint32_t c_bin_limit[kBins] = {4, 32, kHubDegree};
@@ -280,6 +326,13 @@ constexpr int kBinVerts = kBlock * kBinIters;  // vertices per block
 // and eliminate no slot: its previous removals were applied, its eliminated
 // slots stay eliminated and the surviving slots and label counts are the
 // same.  Skipping it leaves every result, flag and counter unchanged.
+//
+// mode 3 (phase 2 frontier): active, and modified or, when `dirty` (then the
+// rescan marks) is given, marked for a rescan.  An unmodified vertex keeps
+// the result of its last scan until a neighbour turns modified (the only
+// change that can move its first hit: its own edge-active bits change only
+// when it is evaluated, and a vertex that loses a slot turns modified), so
+// its scheduled flag stands and last_scan[v] holds the slots the scan reads.
 template <typename Word>
 __device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __restrict__ cand,
                                       const uint8_t* __restrict__ flags, const uint8_t* __restrict__ code,
@@ -288,6 +341,7 @@ __device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __rest
   if (v >= V || !cand[v]) return -1;
   if (mode == 0 && !(flags[v] & kScheduled)) return -1;
   if (mode == 0 && dirty && !dirty[v]) return -2;  // clean: skipped
+  if (mode == 3 && dirty && !dirty[v] && !(flags[v] & kModified)) return -2;  // scan result stands
   if (mode == 2 && code[v] != path_code) return -1;
   const uint64_t d = off[v + 1] - off[v];
   if (d <= c_bin_limit[0]) return 0;
@@ -304,7 +358,7 @@ __global__ void __launch_bounds__(kBlock) k_build_bins(
     const Word* __restrict__ cand, const uint8_t* __restrict__ flags, const uint8_t* __restrict__ code,
     const uint8_t* __restrict__ dirty, const uint64_t* __restrict__ off, uint32_t V, int mode,
     const DevTemplate* __restrict__ t, uint32_t* __restrict__ bins, uint32_t* __restrict__ bin_count,
-    unsigned long long* counters) {
+    const uint32_t* __restrict__ last_scan, unsigned long long* counters) {
   __shared__ uint32_t s_wave[kBlock / 64][kBins];
   __shared__ uint32_t s_base[kBlock / 64][kBins];
   const int lane = lane_id();
@@ -317,7 +371,7 @@ __global__ void __launch_bounds__(kBlock) k_build_bins(
   for (int i = 0; i < kBinIters; ++i) {
     const uint64_t v = v0 + static_cast<uint64_t>(i) * kBlock;
     const int b = bin_of<Word>(v, V, cand, flags, code, dirty, off, mode, path_code);
-    if (b == -2) skipped += off[v + 1] - off[v];
+    if (b == -2) skipped += mode == 0 ? off[v + 1] - off[v] : last_scan[v];
 #pragma unroll
     for (int k = 0; k < kBins; ++k) cnt[k] += __popcll(__ballot(b == k));
   }
@@ -404,17 +458,19 @@ template <typename Word, int G, int KW>
 __global__ void __launch_bounds__(kBlock) k_lcc_phase1(
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, const uint64_t* __restrict__ off,
     const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code, const Word* __restrict__ cand,
-    Word* __restrict__ unmatch, uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty, uint32_t* __restrict__ ea,
-    const DevTemplate* __restrict__ t, unsigned long long* counters) {
+    Word* __restrict__ unmatch, uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty,
+    uint8_t* __restrict__ rescan, uint32_t* __restrict__ ea, const DevTemplate* __restrict__ t,
+    unsigned long long* counters) {
   const int lane = lane_id();
   const int gl = lane & (G - 1);
-  const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << (lane & ~(G - 1)));
+  const int gbase = lane & ~(G - 1);
+  const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
   const uint32_t n = *list_count;
   const uint32_t K = t->K;
   if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_ITEMS] = n;
   const uint64_t groups_per_grid = (uint64_t)gridDim.x * (blockDim.x / G);
   const uint64_t span = ((n + groups_per_grid - 1) / groups_per_grid) * groups_per_grid;
-  uint64_t elim_total = 0, slots_total = 0;
+  uint64_t elim_total = 0, slots_total = 0, xslots = 0;
   for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G; g < span; g += groups_per_grid) {
     if (g < n) {
       const uint32_t v = list[g];
@@ -426,41 +482,65 @@ __global__ void __launch_bounds__(kBlock) k_lcc_phase1(
 #pragma unroll
       for (int w = 0; w < KW; ++w) cnt[w] = 0;
       uint32_t elim = 0;
-      for (uint64_t base = beg; base < end; base += G) {
-        const uint64_t e = base + gl;
-        bool counted = false;
-        uint32_t lab = kNoCode;
-        if (e < end) {
-          const uint32_t word = ea[e >> 5];
-          if ((word >> (e & 31)) & 1u) {
-            const uint32_t u = edges[e];
-            const Word cu = cand[u];
-            if (cu) {
-              if ((cu & U) == 0) {
-                atomicAnd(&ea[e >> 5], ~(1u << (e & 31)));
-                ++elim;
-              } else {
-                lab = code[u];
-                counted = lab != kNoCode;
-              }
+      // kUnroll slices of G slots per pass: the edge-active words and ids of
+      // all slices are loaded first, then all neighbour gathers, so each pass
+      // waits on two rounds of memory latency.
+      constexpr int kUnroll = G == 64 ? 4 : G == 16 ? 2 : 1;
+      for (uint64_t base = beg; base < end; base += static_cast<uint64_t>(G) * kUnroll) {
+        uint32_t word[kUnroll], u[kUnroll];
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k) {
+          const uint64_t e = base + k * G + gl;
+          word[k] = e < end ? ea[e >> 5] : 0u;
+          u[k] = e < end ? edges[e] : 0u;
+        }
+        Word cu[kUnroll];
+        uint8_t lb[kUnroll];
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k) {
+          const uint64_t e = base + k * G + gl;
+          const bool act = (word[k] >> (e & 31)) & 1u;
+          cu[k] = act ? cand[u[k]] : Word(0);
+          lb[k] = act ? code[u[k]] : kNoCode;
+        }
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k) {
+          bool counted = false, el = false;
+          uint32_t lab = kNoCode;
+          if (cu[k]) {
+            if ((cu[k] & U) == 0) {
+              el = true;
+              ++elim;
+            } else {
+              lab = lb[k];
+              counted = lab != kNoCode;
             }
           }
+          clear_slots<G>(ea, base + k * G, el, gl, gbase);
+          count_labels<KW>(cnt, counted, lab, gmask, K);
         }
-        count_labels<KW>(cnt, counted, lab, gmask, K);
       }
       const bool any_elim = (__ballot(elim != 0) & gmask) != 0;
       elim_total += elim;
+      // every lane of the group holds the same counts, so each can evaluate
+      const Word fail = lcc_requirements<Word, KW>(cv, code[v], cnt, t);
+      const uint8_t f = flags[v];
       if (gl == 0) {
         slots_total += end - beg;
-        const Word fail = lcc_requirements<Word, KW>(cv, code[v], cnt, t);
         if (fail) unmatch[v] = fail;
-        if (fail || any_elim) flags[v] |= kModified;
+        if (fail || any_elim) flags[v] = f | kModified;
         if (dirty) dirty[v] = 0;
+      }
+      if (rescan && (fail || any_elim) && !(f & kModified)) {
+        // v turns modified: the phase 2 scans of its neighbours can change
+        for (uint64_t e = beg + gl; e < end; e += G) rescan[edges[e]] = 1;
+        if (gl == 0) xslots += end - beg;
       }
     }
   }
   wave_add(counters, C_EDGE, elim_total);
   wave_add(counters, C_SLOTS, slots_total);
+  wave_add(counters, C_XSLOTS, xslots);
 }
 
 // Hub rows: the static chunk list holds (vertex, first slot) per chunk of
@@ -475,46 +555,65 @@ __global__ void __launch_bounds__(kBlock) k_lcc_hub_chunks(
     const uint8_t* __restrict__ code, const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
     const uint8_t* __restrict__ dirty, uint32_t* __restrict__ ea, uint32_t* __restrict__ hub_count,
     uint32_t* __restrict__ hub_elim, const DevTemplate* __restrict__ t, unsigned long long* counters) {
-  __shared__ uint32_t s_cnt[kMaxLabels];
-  __shared__ uint32_t s_elim;
-  const uint32_t K = t->K;
+  // Each thread owns slots threadIdx.x + k * kBlock of the chunk (k < kPer);
+  // ids and edge-active words of all of them are loaded before the gathers.
+  constexpr int kPer = kHubChunk / kBlock;
+  const int lane = lane_id();
   uint64_t elim_total = 0, slots_total = 0, items = 0;
   for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const uint32_t h = chunk_hub[c];
     const uint32_t v = hubs[h];
     const Word cv = cand[v];
     if (!cv || !(flags[v] & kScheduled) || (dirty && !dirty[v])) continue;  // block-uniform
-    if (threadIdx.x < kMaxLabels) s_cnt[threadIdx.x] = 0;
-    if (threadIdx.x == 0) s_elim = 0;
-    __syncthreads();
     Word U = 0;
     for (Word m = cv; m; m &= m - 1) U |= static_cast<Word>(t->nbr[ctz<Word>(m)]);
     const uint64_t beg = chunk_start[c];
     const uint64_t end = min(off[v + 1], beg + kHubChunk);
-    uint32_t elim = 0;
-    for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) {
-      if (!ea_test(ea, e)) continue;
-      const uint32_t u = edges[e];
-      const Word cu = cand[u];
-      if (!cu) continue;
-      if ((cu & U) == 0) {
-        atomicAnd(&ea[e >> 5], ~(1u << (e & 31)));
-        ++elim;
-      } else {
-        uint8_t l = code[u];
-        if (l != kNoCode) atomicAdd(&s_cnt[l], 1u);
+    uint32_t word[kPer], u[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const uint64_t e = beg + threadIdx.x + k * kBlock;
+      word[k] = e < end ? ea[e >> 5] : 0u;
+      u[k] = e < end ? edges[e] : 0u;
+    }
+    Word cu[kPer];
+    uint8_t lb[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const uint64_t e = beg + threadIdx.x + k * kBlock;
+      const bool act = (word[k] >> (e & 31)) & 1u;
+      cu[k] = act ? cand[u[k]] : Word(0);
+      lb[k] = act ? code[u[k]] : kNoCode;
+    }
+    uint32_t elim = 0, mine = 0;  // lane L counts label code L
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      bool counted = false, el = false;
+      if (cu[k]) {
+        if ((cu[k] & U) == 0) {
+          el = true;
+          ++elim;
+        } else {
+          counted = lb[k] != kNoCode;
+        }
+      }
+      clear_slots<64>(ea, beg + k * kBlock + (threadIdx.x & ~63u), el, lane, 0);
+      // one ballot per distinct label code among the counted slots
+      for (uint64_t m = __ballot(counted); m; m = __ballot(counted)) {
+        const uint32_t L = rl32(lb[k], __builtin_ctzll(m));
+        const uint64_t same = __ballot(counted && lb[k] == L);
+        if (lane == static_cast<int>(L)) mine += __popcll(same);
+        counted = counted && lb[k] != L;
       }
     }
-    if (elim) atomicAdd(&s_elim, elim);
-    elim_total += elim;
-    __syncthreads();
-    if (threadIdx.x < K && s_cnt[threadIdx.x]) atomicAdd(&hub_count[h * kMaxLabels + threadIdx.x], s_cnt[threadIdx.x]);
-    if (threadIdx.x == 0 && s_elim) atomicAdd(&hub_elim[h], s_elim);
+    if (mine) atomicAdd(&hub_count[h * kMaxLabels + lane], mine);
+    for (int o = 32; o > 0; o >>= 1) elim += __shfl_xor(elim, o, 64);
+    if (lane == 0 && elim) atomicAdd(&hub_elim[h], elim);
+    elim_total += lane == 0 ? elim : 0u;
     if (threadIdx.x == 0) {
       slots_total += end - beg;
       ++items;
     }
-    __syncthreads();
   }
   wave_add(counters, C_EDGE, elim_total);
   wave_add(counters, C_SLOTS, slots_total);
@@ -525,7 +624,8 @@ template <typename Word, int KW>
 __global__ void k_lcc_hub_finish(const uint32_t* __restrict__ hubs, uint32_t nhubs, const uint8_t* __restrict__ code,
                                  const Word* __restrict__ cand, Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
                                  uint8_t* __restrict__ dirty, uint32_t* __restrict__ hub_count,
-                                 uint32_t* __restrict__ hub_elim, const DevTemplate* __restrict__ t) {
+                                 uint32_t* __restrict__ hub_elim, uint32_t* __restrict__ hub_new,
+                                 const DevTemplate* __restrict__ t) {
   for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nhubs; h += gridDim.x * blockDim.x) {
     const uint32_t v = hubs[h];
     const Word cv = cand[v];
@@ -539,12 +639,36 @@ __global__ void k_lcc_hub_finish(const uint32_t* __restrict__ hubs, uint32_t nhu
     }
     const uint32_t elim = hub_elim[h];
     hub_elim[h] = 0;
-    if (!cv || !(flags[v] & kScheduled) || (dirty && !dirty[v])) continue;
+    hub_new[h] = 0;
+    const uint8_t f = flags[v];
+    if (!cv || !(f & kScheduled) || (dirty && !dirty[v])) continue;
     const Word fail = lcc_requirements<Word, KW>(cv, code[v], cnt, t);
     if (fail) unmatch[v] = fail;
-    if (fail || elim) flags[v] |= kModified;
+    if (fail || elim) {
+      flags[v] = f | kModified;
+      hub_new[h] = !(f & kModified);  // k_hub_rescan_push marks the neighbours
+    }
     if (dirty) dirty[v] = 0;
   }
+}
+
+// Hubs that turned modified in the last phase 1 mark their neighbours for a
+// phase 2 rescan, one block per chunk of their rows.
+__global__ void __launch_bounds__(kBlock) k_hub_rescan_push(
+    const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
+    const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+    const uint32_t* __restrict__ hub_new, uint8_t* __restrict__ rescan, unsigned long long* counters) {
+  uint64_t pushed = 0;
+  for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const uint32_t h = chunk_hub[c];
+    if (!hub_new[h]) continue;
+    const uint32_t v = hubs[h];
+    const uint64_t beg = chunk_start[c];
+    const uint64_t end = min(off[v + 1], beg + kHubChunk);
+    for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) rescan[edges[e]] = 1;
+    if (threadIdx.x == 0) pushed += end - beg;
+  }
+  wave_add(counters, C_XSLOTS, pushed);
 }
 
 // ---------------------------------------------------------------------------
@@ -558,8 +682,8 @@ template <typename Word, int G>
 __global__ void __launch_bounds__(kBlock) k_phase2(
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, const uint64_t* __restrict__ off,
     const uint32_t* __restrict__ edges, Word* __restrict__ cand, Word* __restrict__ unmatch,
-    uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty, const uint32_t* __restrict__ ea,
-    unsigned long long* counters) {
+    uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty, uint8_t* __restrict__ rescan,
+    uint32_t* __restrict__ last_scan, const uint32_t* __restrict__ ea, unsigned long long* counters) {
   const int lane = lane_id();
   const int gl = lane & (G - 1);
   const int gbase = lane & ~(G - 1);
@@ -598,20 +722,39 @@ __global__ void __launch_bounds__(kBlock) k_phase2(
         const uint64_t beg = off[v], end = off[v + 1];
         bool hit = false;
         uint64_t scanned = end - beg;
-        for (uint64_t base = beg; base < end; base += G) {
-          const uint64_t e = base + gl;
-          bool h = false;
-          if (e < end && ea_test(ea, e)) h = flags[edges[e]] & kModified;
-          const uint64_t m = __ballot(h) & gmask;
-          if (m) {
-            hit = true;
-            scanned = base + (__builtin_ctzll(m) - gbase) + 1 - beg;
-            break;
+        // kUnroll slices of G slots per pass, loads batched as in phase 1;
+        // the first hit in slot order ends the scan.
+        constexpr int kUnroll = G == 64 ? 4 : G == 16 ? 2 : 1;
+        for (uint64_t base = beg; base < end && !hit; base += static_cast<uint64_t>(G) * kUnroll) {
+          uint32_t word[kUnroll], u[kUnroll];
+#pragma unroll
+          for (int k = 0; k < kUnroll; ++k) {
+            const uint64_t e = base + k * G + gl;
+            word[k] = e < end ? ea[e >> 5] : 0u;
+            u[k] = e < end ? edges[e] : 0u;
+          }
+          uint8_t fl[kUnroll];
+#pragma unroll
+          for (int k = 0; k < kUnroll; ++k) {
+            const uint64_t e = base + k * G + gl;
+            fl[k] = ((word[k] >> (e & 31)) & 1u) ? flags[u[k]] : uint8_t(0);
+          }
+#pragma unroll
+          for (int k = 0; k < kUnroll; ++k) {
+            const uint64_t m = __ballot((fl[k] & kModified) != 0) & gmask;
+            if (m && !hit) {
+              hit = true;
+              scanned = base + k * G + (__builtin_ctzll(m) - gbase) + 1 - beg;
+            }
           }
         }
         if (gl == 0) {
           flags[v] = hit ? (f | kScheduled) : (f & ~kScheduled);
           slots += scanned;
+          if (rescan) {
+            last_scan[v] = static_cast<uint32_t>(scanned);
+            rescan[v] = 0;
+          }
         }
       }
     }
@@ -631,9 +774,8 @@ __global__ void __launch_bounds__(kBlock) k_phase2_hub_chunks(
     const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
     const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
     const Word* __restrict__ cand, const Word* __restrict__ unmatch, const uint8_t* __restrict__ flags,
-    uint8_t* __restrict__ dirty, const uint32_t* __restrict__ ea, uint32_t* __restrict__ hub_hit,
-    unsigned long long* counters) {
-  __shared__ uint32_t s_min;
+    uint8_t* __restrict__ dirty, const uint8_t* __restrict__ rescan, const uint32_t* __restrict__ ea,
+    uint32_t* __restrict__ hub_hit, unsigned long long* counters) {
   uint64_t examined = 0;  // slots read here; the reference-order count is taken in k_phase2_hub_finish
   for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const uint32_t h = chunk_hub[c];
@@ -643,28 +785,36 @@ __global__ void __launch_bounds__(kBlock) k_phase2_hub_chunks(
       if (dirty && unmatch[v]) {  // the hub's candidates change: dirty its neighbours
         const uint64_t beg = chunk_start[c];
         const uint64_t end = min(off[v + 1], beg + kHubChunk);
-        for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) {
-          dirty[edges[e]] = 1;
-          ++examined;
-        }
+        for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) dirty[edges[e]] = 1;
+        if (threadIdx.x == 0) examined += end - beg;
       }
       continue;
     }
-    if (threadIdx.x == 0) s_min = 0xFFFFFFFFu;
-    __syncthreads();
+    if (rescan && !rescan[v]) continue;  // scan result stands (counted in k_build_bins)
     const uint64_t row = off[v];
     const uint64_t beg = chunk_start[c];
     const uint64_t end = min(off[v + 1], beg + kHubChunk);
-    for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) {
-      ++examined;
-      if (ea_test(ea, e) && (flags[edges[e]] & kModified)) {
-        atomicMin(&s_min, static_cast<uint32_t>(e - row));
-        break;
-      }
+    constexpr int kPer = kHubChunk / kBlock;
+    uint32_t word[kPer], u[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const uint64_t e = beg + threadIdx.x + k * kBlock;
+      word[k] = e < end ? ea[e >> 5] : 0u;
+      u[k] = e < end ? edges[e] : 0u;
     }
-    __syncthreads();
-    if (threadIdx.x == 0 && s_min != 0xFFFFFFFFu) atomicMin(&hub_hit[h], s_min);
-    __syncthreads();
+    uint8_t fl[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const uint64_t e = beg + threadIdx.x + k * kBlock;
+      fl[k] = ((word[k] >> (e & 31)) & 1u) ? flags[u[k]] : uint8_t(0);
+    }
+    uint32_t first = 0xFFFFFFFFu;  // this thread's lowest hit (row offset)
+#pragma unroll
+    for (int k = kPer - 1; k >= 0; --k)
+      if (fl[k] & kModified) first = static_cast<uint32_t>(beg + threadIdx.x + k * kBlock - row);
+    for (int o = 32; o > 0; o >>= 1) first = min(first, static_cast<uint32_t>(__shfl_xor(first, o, 64)));
+    if (lane_id() == 0 && first != 0xFFFFFFFFu) atomicMin(&hub_hit[h], first);
+    if (threadIdx.x == 0) examined += end - beg;
   }
   wave_add(counters, C_XSLOTS, examined);
 }
@@ -673,7 +823,8 @@ template <typename Word>
 __global__ void k_phase2_hub_finish(const uint32_t* __restrict__ hubs, uint32_t nhubs,
                                     const uint64_t* __restrict__ off, Word* __restrict__ cand,
                                     Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
-                                    uint8_t* __restrict__ dirty, uint32_t* __restrict__ hub_hit,
+                                    uint8_t* __restrict__ dirty, uint8_t* __restrict__ rescan,
+                                    uint32_t* __restrict__ last_scan, uint32_t* __restrict__ hub_hit,
                                     unsigned long long* counters) {
   uint64_t velim = 0, melim = 0, slots = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -697,10 +848,15 @@ __global__ void k_phase2_hub_finish(const uint32_t* __restrict__ hubs, uint32_t 
         if (dirty) dirty[v] = 1;  // k_phase2_hub_chunks dirtied the neighbours
       }
       flags[v] = f | kScheduled;
-    } else {
+    } else if (!rescan || rescan[v]) {
       const bool any = hit != 0xFFFFFFFFu;
+      const uint64_t scanned = any ? hit + 1ull : off[v + 1] - off[v];
       flags[v] = any ? (f | kScheduled) : (f & ~kScheduled);
-      slots += any ? hit + 1ull : off[v + 1] - off[v];
+      slots += scanned;
+      if (rescan) {
+        last_scan[v] = static_cast<uint32_t>(scanned);
+        rescan[v] = 0;
+      }
     }
   }
   wave_add(counters, C_VERTEX, velim);
@@ -1055,11 +1211,8 @@ __global__ void __launch_bounds__(kBlock) k_nlcc(int kind, const uint32_t* __res
                                           edges, ea, cand, slots);
       }
     }
-    if (fail && lane_id() == 0) {
-      if (sizeof(Word) == 8)
-        atomicOr(reinterpret_cast<unsigned long long*>(&unmatch[s]), static_cast<unsigned long long>(fail));
-      else
-        atomicOr(reinterpret_cast<unsigned int*>(&unmatch[s]), static_cast<unsigned int>(fail));
+    if (fail && lane_id() == 0) {  // s is in one task list once: this wave is its only writer
+      unmatch[s] = static_cast<Word>(unmatch[s] | fail);
       set_flag_atomic(flags, s, kModified);
     }
   }
@@ -1115,6 +1268,9 @@ struct pj_ctx {
   uint8_t* d_flags = nullptr;
   uint8_t* d_vflags = nullptr;
   uint8_t* d_dirty = nullptr;  // per vertex: candidates of v or of a neighbour changed since v's last evaluation
+  uint8_t* d_rescan = nullptr;       // per vertex: a neighbour turned modified since v's last phase 2 scan
+  uint32_t* d_last_scan = nullptr;   // per vertex: slots its last phase 2 scan read
+  uint32_t* d_hub_new = nullptr;     // per hub: turned modified in the last phase 1
   uint32_t* d_ea = nullptr;
   uint64_t ea_words = 0;
   uint32_t ea_tail = 0;  // host staging of the last edge-active word
@@ -1122,6 +1278,7 @@ struct pj_ctx {
   bool symmetric = false;   // every slot u->v has a reverse slot v->u
   bool skip_clean = false;  // phase 1 skips clean scheduled vertices (rows sorted and symmetric)
   bool dirty_enabled = true;
+  bool rescan_all = false;  // every rescan mark is set (phase 1 need not push)
   uint32_t* d_bins = nullptr;        // kBins * V
   uint32_t* d_bin_count = nullptr;   // kBins
   unsigned long long* d_counters = nullptr;  // kCounterSlots * kCounterStride
@@ -1197,8 +1354,10 @@ int dev_alloc(pj_ctx* ctx, T** p, uint64_t count) {
 
 void free_graph(pj_ctx* ctx) {
   void* bufs[] = {ctx->d_off, ctx->d_edges, ctx->d_labels, ctx->d_code, ctx->d_cand, ctx->d_unmatch, ctx->d_u64,
-                  ctx->d_flags, ctx->d_vflags, ctx->d_dirty, ctx->d_ea, ctx->d_bins, ctx->d_bin_count, ctx->d_hubs,
-                  ctx->d_chunk_hub, ctx->d_chunk_start, ctx->d_hub_count, ctx->d_hub_elim, ctx->d_nhubs, ctx->d_hub_hit};
+                  ctx->d_flags, ctx->d_vflags, ctx->d_dirty, ctx->d_rescan, ctx->d_ea, ctx->d_bins, ctx->d_bin_count,
+                  ctx->d_hubs,
+                  ctx->d_chunk_hub, ctx->d_chunk_start, ctx->d_hub_count, ctx->d_hub_elim, ctx->d_nhubs, ctx->d_hub_hit,
+                  ctx->d_last_scan, ctx->d_hub_new};
   for (void* b : bufs)
     if (b) hipFree(b);
   ctx->d_off = nullptr;
@@ -1206,7 +1365,7 @@ void free_graph(pj_ctx* ctx) {
   ctx->d_labels = nullptr;
   ctx->d_code = nullptr;
   ctx->d_cand = ctx->d_unmatch = ctx->d_u64 = nullptr;
-  ctx->d_flags = ctx->d_vflags = ctx->d_dirty = nullptr;
+  ctx->d_flags = ctx->d_vflags = ctx->d_dirty = ctx->d_rescan = nullptr;
   ctx->d_ea = nullptr;
   ctx->d_bins = ctx->d_bin_count = nullptr;
   ctx->d_hubs = ctx->d_chunk_hub = nullptr;
@@ -1214,6 +1373,7 @@ void free_graph(pj_ctx* ctx) {
   ctx->d_hub_count = ctx->d_hub_elim = nullptr;
   ctx->d_nhubs = nullptr;
   ctx->d_hub_hit = nullptr;
+  ctx->d_last_scan = ctx->d_hub_new = nullptr;
   ctx->V = 0;
   ctx->E = 0;
 }
@@ -1238,7 +1398,7 @@ int setup_hubs(pj_ctx* ctx, const uint64_t* h_off) {
       dev_alloc(ctx, &ctx->d_chunk_start, chunk_start.size()) ||
       dev_alloc(ctx, &ctx->d_hub_count, static_cast<uint64_t>(hubs.size()) * kMaxLabels) ||
       dev_alloc(ctx, &ctx->d_hub_elim, hubs.size()) || dev_alloc(ctx, &ctx->d_nhubs, 1) ||
-      dev_alloc(ctx, &ctx->d_hub_hit, hubs.size()))
+      dev_alloc(ctx, &ctx->d_hub_hit, hubs.size()) || dev_alloc(ctx, &ctx->d_hub_new, hubs.size()))
     return -1;
   PJ_HIP(hipMemcpyAsync(ctx->d_nhubs, &ctx->nhubs, 4, hipMemcpyHostToDevice, ctx->stream));
   PJ_HIP(hipStreamSynchronize(ctx->stream));
@@ -1253,6 +1413,8 @@ int setup_hubs(pj_ctx* ctx, const uint64_t* h_off) {
                        static_cast<uint64_t>(hubs.size()), 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, hubs.size())), dim3(kBlock), 0, ctx->stream, ctx->d_hub_hit,
                        static_cast<uint64_t>(hubs.size()), 0xFFFFFFFFu);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, hubs.size())), dim3(kBlock), 0, ctx->stream, ctx->d_hub_new,
+                       static_cast<uint64_t>(hubs.size()), 0u);
     PJ_HIP(hipStreamSynchronize(ctx->stream));
   }
   return 0;
@@ -1264,6 +1426,7 @@ int alloc_state(pj_ctx* ctx) {
       dev_alloc(ctx, reinterpret_cast<uint64_t**>(&ctx->d_unmatch), V) ||
       dev_alloc(ctx, reinterpret_cast<uint64_t**>(&ctx->d_u64), V) || dev_alloc(ctx, &ctx->d_flags, V + 4) ||
       dev_alloc(ctx, &ctx->d_vflags, V + 4) || dev_alloc(ctx, &ctx->d_dirty, V + 4) ||
+      dev_alloc(ctx, &ctx->d_rescan, V + 4) || dev_alloc(ctx, &ctx->d_last_scan, V) ||
       dev_alloc(ctx, &ctx->d_ea, ctx->ea_words) ||
       dev_alloc(ctx, &ctx->d_bins, V * kBins) || dev_alloc(ctx, &ctx->d_bin_count, 4) ||
       dev_alloc(ctx, &ctx->d_counters, (kMaxLaunches + 1) * kCounterBlock))
@@ -1271,11 +1434,21 @@ int alloc_state(pj_ctx* ctx) {
   return 0;
 }
 
+// Marks every vertex for a phase 2 rescan (modified flags were cleared or
+// set outside phase 1).
+void mark_all_rescan(pj_ctx* ctx) {
+  if (!ctx->skip_clean) return;
+  ctx->rescan_all = true;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, (ctx->V + 4) / 4)), dim3(kBlock), 0, ctx->stream,
+                     reinterpret_cast<uint32_t*>(ctx->d_rescan), static_cast<uint64_t>((ctx->V + 4) / 4), 0x01010101u);
+}
+
 // Marks every vertex for re-evaluation (the candidate sets were replaced).
 void mark_all_dirty(pj_ctx* ctx) {
   if (!ctx->skip_clean) return;
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, (ctx->V + 4) / 4)), dim3(kBlock), 0, ctx->stream,
                      reinterpret_cast<uint32_t*>(ctx->d_dirty), static_cast<uint64_t>((ctx->V + 4) / 4), 0x01010101u);
+  mark_all_rescan(ctx);
 }
 
 template <typename Word>
@@ -1337,51 +1510,58 @@ void launch_bins(pj_ctx* ctx, int mode) {
   zero_bins(ctx);
   unsigned long long* c = begin_launch(ctx, K_BINS);
   const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(ctx->V) + kBinVerts - 1) / kBinVerts);
+  const uint8_t* marks = !ctx->skip_clean ? nullptr : mode == 0 ? ctx->d_dirty : mode == 3 ? ctx->d_rescan : nullptr;
   hipLaunchKernelGGL(k_build_bins<Word>, dim3(std::max<uint32_t>(blocks, 1)), dim3(kBlock), 0, ctx->stream,
-                     static_cast<const Word*>(ctx->d_cand), ctx->d_flags, ctx->d_code,
-                     mode == 0 && ctx->skip_clean ? ctx->d_dirty : nullptr, ctx->d_off, ctx->V, mode, ctx->d_t,
-                     ctx->d_bins, ctx->d_bin_count, c);
+                     static_cast<const Word*>(ctx->d_cand), ctx->d_flags, ctx->d_code, marks, ctx->d_off, ctx->V, mode,
+                     ctx->d_t, ctx->d_bins, ctx->d_bin_count, ctx->d_last_scan, c);
   end_launch(ctx);
 }
 
 // Phase 2 over every active vertex (bins + hubs).
 template <typename Word>
 void launch_phase2(pj_ctx* ctx) {
-  launch_bins<Word>(ctx, 1);
+  launch_bins<Word>(ctx, 3);
   uint8_t* dirty = ctx->skip_clean ? ctx->d_dirty : nullptr;
+  uint8_t* rescan = ctx->skip_clean ? ctx->d_rescan : nullptr;
+  uint32_t* last = ctx->d_last_scan;
   Word* cand = static_cast<Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   const uint64_t V = ctx->V;
   const int g = ctx->cus * 8;
   unsigned long long* c = begin_launch(ctx, K_P2_G4);
   hipLaunchKernelGGL((k_phase2<Word, 4>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins, ctx->d_bin_count + 0,
-                     ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, dirty, ctx->d_ea, c);
+                     ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, dirty, rescan, last, ctx->d_ea, c);
   end_launch(ctx);
   c = begin_launch(ctx, K_P2_G16);
   hipLaunchKernelGGL((k_phase2<Word, 16>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
-                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, dirty, ctx->d_ea, c);
+                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, dirty, rescan, last,
+                     ctx->d_ea, c);
   end_launch(ctx);
   c = begin_launch(ctx, K_P2_G64);
   hipLaunchKernelGGL((k_phase2<Word, 64>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
-                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, dirty, ctx->d_ea, c);
+                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, dirty, rescan, last,
+                     ctx->d_ea, c);
   end_launch(ctx);
   if (ctx->nhubs) {
     c = begin_launch(ctx, K_P2_HUB);
     hipLaunchKernelGGL(k_phase2_hub_chunks<Word>, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
                        ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
-                       ctx->d_edges, cand, unm, ctx->d_flags, dirty, ctx->d_ea, ctx->d_hub_hit, c);
+                       ctx->d_edges, cand, unm, ctx->d_flags, dirty, rescan, ctx->d_ea, ctx->d_hub_hit, c);
     end_launch(ctx);
     c = begin_launch(ctx, K_P2_HUB_FINISH);
     hipLaunchKernelGGL(k_phase2_hub_finish<Word>, dim3(grid_for(ctx, ctx->nhubs)), dim3(kBlock), 0, ctx->stream,
-                       ctx->d_hubs, ctx->nhubs, ctx->d_off, cand, unm, ctx->d_flags, dirty, ctx->d_hub_hit, c);
+                       ctx->d_hubs, ctx->nhubs, ctx->d_off, cand, unm, ctx->d_flags, dirty, rescan, last,
+                       ctx->d_hub_hit, c);
     end_launch(ctx);
   }
+  ctx->rescan_all = false;  // the scans cleared the marks of the vertices they read
 }
 
 template <typename Word, int KW>
 void launch_lcc_phase1(pj_ctx* ctx) {
   launch_bins<Word>(ctx, 0);
   uint8_t* dirty = ctx->skip_clean ? ctx->d_dirty : nullptr;
+  uint8_t* rescan = ctx->skip_clean && !ctx->rescan_all ? ctx->d_rescan : nullptr;  // push target
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   const uint64_t V = ctx->V;
@@ -1389,17 +1569,17 @@ void launch_lcc_phase1(pj_ctx* ctx) {
   unsigned long long* c = begin_launch(ctx, K_LCC_P1_G4);
   hipLaunchKernelGGL((k_lcc_phase1<Word, 4, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
                      ctx->d_bin_count + 0, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, dirty,
-                     ctx->d_ea, ctx->d_t, c);
+                     rescan, ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
   c = begin_launch(ctx, K_LCC_P1_G16);
   hipLaunchKernelGGL((k_lcc_phase1<Word, 16, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
                      ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, dirty,
-                     ctx->d_ea, ctx->d_t, c);
+                     rescan, ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
   c = begin_launch(ctx, K_LCC_P1_G64);
   hipLaunchKernelGGL((k_lcc_phase1<Word, 64, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
                      ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, dirty,
-                     ctx->d_ea, ctx->d_t, c);
+                     rescan, ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
   if (ctx->nhubs) {
     c = begin_launch(ctx, K_LCC_HUB);
@@ -1411,8 +1591,15 @@ void launch_lcc_phase1(pj_ctx* ctx) {
     begin_launch(ctx, K_LCC_HUB_FINISH);
     hipLaunchKernelGGL((k_lcc_hub_finish<Word, KW>), dim3(grid_for(ctx, ctx->nhubs)), dim3(kBlock), 0, ctx->stream,
                        ctx->d_hubs, ctx->nhubs, ctx->d_code, cand, unm, ctx->d_flags, dirty, ctx->d_hub_count,
-                       ctx->d_hub_elim, ctx->d_t);
+                       ctx->d_hub_elim, ctx->d_hub_new, ctx->d_t);
     end_launch(ctx);
+    if (rescan) {
+      c = begin_launch(ctx, K_HUB_PUSH);
+      hipLaunchKernelGGL(k_hub_rescan_push, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
+                         ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
+                         ctx->d_edges, ctx->d_hub_new, rescan, c);
+      end_launch(ctx);
+    }
   }
 }
 
@@ -1553,7 +1740,10 @@ int run_step_t(pj_ctx* ctx, int step, uint32_t index) {
       const int kind = step == PJ_STEP_CC ? 0 : step == PJ_STEP_PC ? 1 : step == PJ_STEP_TDS ? 2 : 3;
       launch_nlcc<Word>(ctx, kind);
       launch_nlcc_hubs<Word>(ctx, kind, ctx->d_nhubs);
-      if (step != PJ_STEP_ENUMERATION) launch_phase2<Word>(ctx);
+      if (step != PJ_STEP_ENUMERATION) {
+        mark_all_rescan(ctx);  // sources that failed turned modified
+        launch_phase2<Word>(ctx);
+      }
       break;
     }
     default:
@@ -1589,6 +1779,23 @@ int upload_state_t(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags, con
   PJ_HIP(hipGetLastError());
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
+}
+
+// Candidate words are the narrowest unsigned type holding one bit per
+// template vertex (1, 2, 4 or 8 bytes), so the per-slot gathers of
+// neighbour candidates touch as few cache lines as possible.
+template <typename F>
+int with_word(const pj_ctx* ctx, F&& f) {
+  switch (ctx->word_bytes) {
+    case 1:
+      return f(uint8_t{});
+    case 2:
+      return f(uint16_t{});
+    case 4:
+      return f(uint32_t{});
+    default:
+      return f(uint64_t{});
+  }
 }
 
 }  // namespace
@@ -1663,16 +1870,17 @@ static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* of
     uint32_t unsorted = 0;
     PJ_HIP(hipMemcpyAsync(&unsorted, ctx->d_bin_count, 4, hipMemcpyDeviceToHost, ctx->stream));
     PJ_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->rows_sorted = unsorted == 0;
+    ctx->rows_sorted = (unsorted & 1u) == 0;
     ctx->symmetric = false;
-    if (ctx->rows_sorted) {
-      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bin_count, static_cast<uint64_t>(4), 0u);
+    if (unsorted == 0) {
+      unsigned long long* f = ctx->d_sums;  // three counters; d_sums is free outside pj_run_step
+      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, f, static_cast<uint64_t>(3), 0ull);
       hipLaunchKernelGGL(k_rows_symmetric, dim3(grid_for(ctx, static_cast<uint64_t>(V) * 64)), dim3(kBlock), 0,
-                         ctx->stream, ctx->d_off, ctx->d_edges, V, ctx->d_bin_count);
-      uint32_t asym = 0;
-      PJ_HIP(hipMemcpyAsync(&asym, ctx->d_bin_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+                         ctx->stream, ctx->d_off, ctx->d_edges, V, f);
+      unsigned long long h[3] = {1, 0, 0};
+      PJ_HIP(hipMemcpyAsync(h, f, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
       PJ_HIP(hipStreamSynchronize(ctx->stream));
-      ctx->symmetric = asym == 0;
+      ctx->symmetric = h[0] == 0 && h[1] == h[2];
     }
     ctx->skip_clean = ctx->rows_sorted && ctx->symmetric && ctx->dirty_enabled;
   }
@@ -1732,7 +1940,7 @@ int pj_set_template(pj_ctx* ctx, const pj_template_desc* d) {
     }
   }
   ctx->kw = t.K <= 8 ? 1 : t.K <= 16 ? 2 : t.K <= 32 ? 4 : 8;
-  ctx->word_bytes = P <= 32 ? 4 : 8;
+  ctx->word_bytes = P <= 8 ? 1 : P <= 16 ? 2 : P <= 32 ? 4 : 8;
   ctx->cycles.clear();
   for (uint32_t c = 0; c < d->cycle_count; ++c)
     ctx->cycles.emplace_back(d->cycle_vertex + d->cycle_offset[c], d->cycle_vertex + d->cycle_offset[c + 1]);
@@ -1772,7 +1980,7 @@ int pj_state_reset(pj_ctx* ctx) {
     return -1;
   }
   PJ_HIP(hipSetDevice(ctx->device));
-  return ctx->word_bytes == 4 ? state_reset_t<uint32_t>(ctx) : state_reset_t<uint64_t>(ctx);
+  return with_word(ctx, [&](auto w) { return state_reset_t<decltype(w)>(ctx); });
 }
 
 int pj_reset_modified(pj_ctx* ctx) {
@@ -1783,6 +1991,7 @@ int pj_reset_modified(pj_ctx* ctx) {
   }
   hipLaunchKernelGGL(k_clear_modified, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream, ctx->d_flags,
                      ctx->V);
+  mark_all_rescan(ctx);
   PJ_HIP(hipGetLastError());
   return 0;
 }
@@ -1797,7 +2006,7 @@ int pj_run_step(pj_ctx* ctx, int step, uint32_t index, pj_algo_results* out) {
   if (step == PJ_STEP_CC || step == PJ_STEP_PC || step == PJ_STEP_TDS || step == PJ_STEP_ENUMERATION)
     if (install_nlcc(ctx, step, index)) return -1;
   PJ_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-  int rc = ctx->word_bytes == 4 ? run_step_t<uint32_t>(ctx, step, index) : run_step_t<uint64_t>(ctx, step, index);
+  int rc = with_word(ctx, [&](auto w) { return run_step_t<decltype(w)>(ctx, step, index); });
   if (rc) return rc;
   PJ_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   const size_t blocks = ctx->timing ? static_cast<size_t>(ctx->nlaunch) + 1 : 1;
@@ -1891,8 +2100,7 @@ int pj_download_state(pj_ctx* ctx, uint64_t* cand, uint8_t* vflags, uint32_t* ea
     return -1;
   }
   PJ_HIP(hipSetDevice(ctx->device));
-  return ctx->word_bytes == 4 ? download_t<uint32_t>(ctx, cand, vflags, ea_words)
-                              : download_t<uint64_t>(ctx, cand, vflags, ea_words);
+  return with_word(ctx, [&](auto w) { return download_t<decltype(w)>(ctx, cand, vflags, ea_words); });
 }
 
 int pj_upload_state(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags, const uint32_t* ea_words,
@@ -1903,8 +2111,7 @@ int pj_upload_state(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags, co
     return -1;
   }
   PJ_HIP(hipSetDevice(ctx->device));
-  int rc = ctx->word_bytes == 4 ? upload_state_t<uint32_t>(ctx, cand, vflags, ea_words)
-                                : upload_state_t<uint64_t>(ctx, cand, vflags, ea_words);
+  int rc = with_word(ctx, [&](auto w) { return upload_state_t<decltype(w)>(ctx, cand, vflags, ea_words); });
   if (rc) return rc;
   ctx->active_vertices = active_vertices;
   ctx->active_edges = active_edges;
