@@ -11,7 +11,8 @@ import numpy as np
 LIB = Path(__file__).resolve().parent / "libpj_oracle.so"
 
 FAITHFUL = 0   # reference DFS with memo + alreadyMatched shortcut
-COMPLETE = 1   # complete search (what the device computes)
+COMPLETE = 1   # complete search (no memo, no shortcut)
+FAITHFUL_NO_SHORTCUT = 2  # memos on, every walk runs (bounds every schedule of FAITHFUL)
 
 
 class OracleRow(C.Structure):
@@ -51,6 +52,7 @@ def lib() -> C.CDLL:
             "or_run_pipeline": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(OracleRow), C.c_int,
                                           C.POINTER(C.c_double)]),
             "or_last_slots": (C.c_uint64, [P]),
+            "or_last_marks": (None, [P, u64p]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(l, name)
@@ -94,6 +96,12 @@ class Oracle:
         if lib().or_run_step(self._s, step, index, mode, C.byref(r)):
             raise ValueError("bad step")
         return (r.vertex_eliminated, r.edge_eliminated, r.match_eliminated, r.enumeration)
+
+    def last_marks(self):
+        """alreadyMatched as the last NLCC step's phase 1 left it."""
+        out = np.zeros(self.graph.vertex_count, dtype=np.uint64)
+        lib().or_last_marks(self._s, out.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return out
 
     def last_slots(self) -> int:
         return lib().or_last_slots(self._s)

@@ -12,19 +12,25 @@
 // utils/bitmap_fixed.h:18-70), per-thread std::unordered_map label counters
 // (multiple_label_lcc_cpu.tpp:19-21).
 //
-// Two modes for the non-local steps (CC, PC, TDS):
+// Three modes for the non-local steps (CC, PC, TDS):
 //   mode 0 "faithful": the reference's recursive DFS with its per-source
 //          edge memo (which stores template ids but tests positions,
-//          cc_cpu.tpp:53-61) and the cross-source alreadyMatched shortcut
-//          (cc_cpu.tpp:102-108).  With more than one thread the shortcut
-//          makes results depend on scheduling, exactly as in the reference.
+//          cc_cpu.tpp:53-61), the TDS hypothesis memo, and the cross-source
+//          alreadyMatched shortcut (cc_cpu.tpp:102-108).  With more than one
+//          thread the shortcut can make results depend on scheduling,
+//          exactly as in the reference.
 //   mode 1 "complete": the same walks without memo and without the
 //          shortcut, i.e. "does a walk satisfying the constraint exist".
-//          This is what the HIP kernels compute.  The two modes agree
-//          whenever the memo never fires on a needed branch (memo-inert
-//          constraints: pjh_constraints_memo_inert of the host library
-//          decides it for cycle and path constraints) — tests pin that
-//          agreement.
+//          The two modes agree whenever the memo never fires on a needed
+//          branch (memo-inert constraints: pjh_constraints_memo_inert of the
+//          host library decides it for cycle and path constraints) — tests
+//          pin that agreement.
+//   mode 2 "faithful without the shortcut": memos on, every walk runs.  Its
+//          failures F0 and the alreadyMatched marks its successful walks
+//          leave (or_last_marks) M0 bound every schedule's failures F of
+//          mode 0: F0 minus M0 is contained in F, which is contained in F0
+//          (a walk's own outcome does not depend on the marks; the shortcut
+//          only skips walks whose root some successful walk marked).
 #include <omp.h>
 
 #include <algorithm>
@@ -79,6 +85,7 @@ struct or_state {
   Tables t;
   std::vector<uint32_t> active, modified, scheduled, edge_active;
   std::vector<uint64_t> match, already, unmatch;
+  std::vector<uint64_t> last_marks;  // alreadyMatched at the end of the last NLCC phase 1
   uint64_t active_v = 0, active_e = 0;
   uint64_t slots = 0;  // slots examined by the last step
   int threads = 1;
@@ -350,7 +357,8 @@ bool tds_walk(or_state* s, const Walk& w, TdsMemo* memo, bool faithful, std::vec
 // Phase 1 of the three NLCC steps (cc_cpu.tpp:94-129, pc_cpu.tpp:95-147,
 // tds_cpu.tpp:153-188), then phase 2.
 pj_algo_results nlcc(or_state* s, int step, uint32_t index, int mode) {
-  const bool faithful = mode == 0;
+  const bool faithful = mode == 0 || mode == 2;
+  const bool shortcut = mode == 0;
   uint64_t slots = 0;
 #pragma omp parallel for num_threads(s->threads) reduction(+ : slots) schedule(guided)
   for (int64_t vi = 0; vi < static_cast<int64_t>(s->V); ++vi) {
@@ -364,7 +372,7 @@ pj_algo_results nlcc(or_state* s, int step, uint32_t index, int mode) {
       for (size_t j = 0; j < idx.size(); ++j) {
         const uint32_t q = idx[j];
         if (!(s->match[v] >> q & 1)) continue;
-        if (faithful && test_atomic(&s->already[v], q)) continue;
+        if (shortcut && test_atomic(&s->already[v], q)) continue;
         memo.clear();
         if (!cc_walk(s, idx, &memo, faithful, v, v, j, idx.size())) {
           s->unmatch[v] |= 1ull << q;
@@ -379,7 +387,7 @@ pj_algo_results nlcc(or_state* s, int step, uint32_t index, int mode) {
       for (int r = 0; r < 2; ++r) {
         const uint32_t q = ends[r];
         if (!(s->match[v] >> q & 1)) continue;
-        if (faithful && test_atomic(&s->already[v], q)) continue;
+        if (shortcut && test_atomic(&s->already[v], q)) continue;
         memo.clear();
         if (!pc_walk(s, idx, r == 1, &memo, faithful, v, v, idx.size() - 1)) {
           s->unmatch[v] |= 1ull << q;
@@ -392,7 +400,7 @@ pj_algo_results nlcc(or_state* s, int step, uint32_t index, int mode) {
       std::vector<uint32_t> hist, hyp;
       for (const auto& w : walks) {
         if (!(s->match[v] >> w.root & 1)) continue;
-        if (faithful && test_atomic(&s->already[v], w.root)) continue;
+        if (shortcut && test_atomic(&s->already[v], w.root)) continue;
         memo.hyp.clear();
         memo.edge.clear();
         hist.assign(1, v);
@@ -407,6 +415,7 @@ pj_algo_results nlcc(or_state* s, int step, uint32_t index, int mode) {
     slots += tl_slots - before;
   }
   s->slots += slots;
+  s->last_marks = s->already;
   size_t velim = 0, melim = 0;
   phase2(s, true, &velim, &melim);
   s->active_v -= velim;
@@ -540,6 +549,12 @@ void or_reset(or_state* s) {
 }
 
 uint64_t or_last_slots(const or_state* s) { return s->slots; }
+
+// alreadyMatched as the last NLCC step's phase 1 left it (before phase 2
+// clears it); zeros before any NLCC step.
+void or_last_marks(const or_state* s, uint64_t* out) {
+  for (uint32_t v = 0; v < s->V; ++v) out[v] = v < s->last_marks.size() ? s->last_marks[v] : 0;
+}
 
 void or_reset_modified(or_state* s) { std::fill(s->modified.begin(), s->modified.end(), 0u); }
 

@@ -82,6 +82,10 @@ struct DevTemplate {
   uint32_t walk_vertex[64 * 128];
   uint32_t walk_mb[64 * 128];
   uint32_t walk_hidx[64 * 128];
+  uint8_t walk_test[64 * 128];      // memoised walks: template id / position tested in the edge memo
+  uint8_t walk_mark[64 * 128];      // memoised walks: mark alreadyMatched on success
+  uint32_t nlcc_memo;               // run the installed NLCC step as memoised walks (walk_* hold the plans)
+  uint32_t walk_hyp;                // memoised walks use the hypothesis memo (TDS)
   uint32_t path_code;               // code of the path's initial label
   uint32_t rows_sorted;             // every CSR row ascending (binary search allowed)
 };
@@ -1163,6 +1167,289 @@ __device__ uint64_t template_walk(uint32_t s, uint32_t len, const uint8_t* __res
   }
 }
 
+// ---------------------------------------------------------------------------
+// Memoised walks: the reference's own search (cc_cpu.tpp:12-70,
+// pc_cpu.tpp:11-75, tds_cpu.tpp:13-127).  Its per-source memos can prune
+// branches a complete search would take, so when they can fire (always for
+// TDS; for cycles and paths when the host analysis says so) the device runs
+// the same search.  Each wave owns a region of an open-addressing table in
+// global memory (64-bit key, 64-bit value, 32-bit epoch per entry); an entry
+// is live while its epoch equals the wave's epoch, so a new walk (epoch + 1)
+// starts from an empty table.  Keys carry a 2-bit kind in bits 62-63:
+//   1: undirected edge {a, b}          -> template-id mask (edge memo)
+//   2: (hypothesis node, vertex)       -> child node (hash-consed trie)
+//   3: (vertex, template id, node)     -> present (hypothesis memo)
+// A hypothesis (the STOREd vertices so far, in order) is a node of a trie
+// whose nodes are hash-consed, so equal hypotheses have equal node ids.
+// Vertex ids are < 2^30 and nodes < 2^26 (checked on the host / capacity).
+
+constexpr uint8_t kMoveClose = 4;  // last hop of a cycle: an active slot back to the source
+constexpr uint8_t kMoveLast = 5;   // last hop of a path: a candidate other than the source
+constexpr int kMemoWaves = 2048;            // waves of a memoised launch (first pass)
+constexpr int kMemoLog2 = 14;               // entries per wave, first pass
+constexpr int kMemoWaves2 = 32;             // second pass, for sources whose walks outgrew the first
+constexpr int kMemoLog2b = 22;
+
+struct MemoArgs {
+  uint64_t* keys;
+  uint64_t* vals;
+  uint32_t* epochs;
+  uint32_t* wave_epoch;  // per wave, persists across launches
+  uint32_t log2;         // entries per wave = 1 << log2
+  void* already;         // alreadyMatched (Word[V])
+  uint32_t* ovf_list;    // sources whose table overflowed
+  uint32_t* ovf_count;
+};
+
+struct MemoTable {
+  uint64_t* keys;
+  uint64_t* vals;
+  uint32_t* epochs;
+  uint32_t mask;
+  uint32_t limit;
+  uint32_t epoch;
+  uint32_t used;
+  uint32_t next_node;
+  bool overflow;
+};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ull;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+template <typename T>
+__device__ __forceinline__ T wg_load(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T>
+__device__ __forceinline__ void wg_store(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Wave-cooperative lookup (64 consecutive probe slots per round).  Returns
+// the slot holding `key`, or the free slot where it belongs; *found says
+// which.  Every lane gets the same answer.
+__device__ uint32_t memo_probe(const MemoTable& m, uint64_t key, bool* found) {
+  const int lane = lane_id();
+  const uint32_t h = static_cast<uint32_t>(mix64(key)) & m.mask;
+  for (uint32_t i = 0; i <= m.mask; i += 64) {
+    const uint32_t slot = (h + i + lane) & m.mask;
+    const bool live = wg_load(&m.epochs[slot]) == m.epoch;
+    const bool hit = live && wg_load(&m.keys[slot]) == key;
+    const uint64_t stop = __ballot(hit || !live);
+    if (stop) {
+      const int l = __builtin_ctzll(stop);
+      *found = (__ballot(hit) >> l) & 1;
+      return rl32(slot, l);
+    }
+  }
+  *found = false;
+  return 0;  // not reached: the table never fills past its limit
+}
+
+__device__ void memo_put(MemoTable& m, uint32_t slot, uint64_t key, uint64_t val) {
+  if (lane_id() == 0) {
+    wg_store(&m.vals[slot], val);
+    wg_store(&m.keys[slot], key);
+    wg_store(&m.epochs[slot], m.epoch);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  if (++m.used > m.limit) m.overflow = true;
+}
+
+__device__ __forceinline__ uint64_t edge_key(uint32_t a, uint32_t b) {
+  const uint64_t lo = min(a, b), hi = max(a, b);
+  return (1ull << 62) | (lo << 30) | hi;
+}
+
+// Edge memo test: was template id / position `bit` inserted for edge {a, b}?
+__device__ bool memo_edge_has(const MemoTable& m, uint32_t a, uint32_t b, uint32_t bit) {
+  bool found;
+  const uint32_t slot = memo_probe(m, edge_key(a, b), &found);
+  return found && ((wg_load(&m.vals[slot]) >> bit) & 1);
+}
+
+__device__ void memo_edge_add(MemoTable& m, uint32_t a, uint32_t b, uint32_t bit) {
+  bool found;
+  const uint64_t key = edge_key(a, b);
+  const uint32_t slot = memo_probe(m, key, &found);
+  if (found) {
+    const uint64_t v = wg_load(&m.vals[slot]) | (1ull << bit);
+    if (lane_id() == 0) wg_store(&m.vals[slot], v);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  } else {
+    memo_put(m, slot, key, 1ull << bit);
+  }
+}
+
+__device__ __forceinline__ uint64_t hyp_key(uint32_t u, uint32_t y, uint32_t node) {
+  return (3ull << 62) | (static_cast<uint64_t>(u) << 32) | (static_cast<uint64_t>(y) << 26) | node;
+}
+
+__device__ bool memo_hyp_has(const MemoTable& m, uint32_t u, uint32_t y, uint32_t node) {
+  bool found;
+  memo_probe(m, hyp_key(u, y, node), &found);
+  return found;
+}
+
+__device__ void memo_hyp_add(MemoTable& m, uint32_t u, uint32_t y, uint32_t node) {
+  bool found;
+  const uint64_t key = hyp_key(u, y, node);
+  const uint32_t slot = memo_probe(m, key, &found);
+  if (!found) memo_put(m, slot, key, 0);
+}
+
+// The hypothesis node reached from `node` by storing vertex u.
+__device__ uint32_t memo_trie_child(MemoTable& m, uint32_t node, uint32_t u) {
+  bool found;
+  const uint64_t key = (2ull << 62) | (static_cast<uint64_t>(node) << 30) | u;
+  const uint32_t slot = memo_probe(m, key, &found);
+  if (found) return static_cast<uint32_t>(wg_load(&m.vals[slot]));
+  const uint32_t child = ++m.next_node;
+  memo_put(m, slot, key, child);
+  return child;
+}
+
+template <typename Word>
+__device__ __forceinline__ void atomic_or_word(Word* p, Word bits) {
+  if (sizeof(Word) == 8) {
+    atomicOr(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(bits));
+  } else if (sizeof(Word) == 4) {
+    atomicOr(reinterpret_cast<unsigned int*>(p), static_cast<unsigned int>(bits));
+  } else {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    atomicOr(reinterpret_cast<unsigned int*>(a & ~uintptr_t(3)), static_cast<unsigned int>(bits) << (8 * (a & 3)));
+  }
+}
+
+// One memoised search (a plan of moves, as template_walk) from source s.
+// Move p: VERTEX moves test walk_test[p] and insert mv vertex vx[p] in the
+// edge memo, and with `hyp` consult / fill the hypothesis memo; kMoveClose
+// and kMoveLast end the walk.  On success every move with mk[p] marks the
+// vertex it reached in alreadyMatched with vx[p].  Returns 1 (found), 0 (no
+// walk) or -1 (the wave's table overflowed; the caller retries the source
+// with a larger table).
+template <typename Word>
+__device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ mv, const uint32_t* __restrict__ vx,
+                         const uint8_t* __restrict__ tb, const uint8_t* __restrict__ mk,
+                         const uint32_t* __restrict__ mb, const uint32_t* __restrict__ hidx, bool hyp,
+                         const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+                         const uint32_t* __restrict__ ea, const Word* __restrict__ cand, Word* already,
+                         MemoTable& m, uint64_t& slots) {
+  const int lane = lane_id();
+  uint32_t st_v = s;   // lane p: vertex before move p
+  uint64_t st_c = 0;   // lane p: resume slot of move p
+  uint32_t st_n = 0;   // lane p: hypothesis node before move p
+  uint32_t hist = s;   // lane h: history entry h
+  int pos = 1;
+  uint32_t cur = s, node = 0;
+  bool resume = false;
+  if (len <= 1) return 1;
+  auto searching = [&](int p) { return mv[p] == PJ_MOVE_VERTEX_STORE || mv[p] == PJ_MOVE_VERTEX_NO_STORE; };
+  auto backtrack = [&]() -> bool {
+    pos = uniform(pos - 1);
+    while (pos >= 1 && !searching(pos)) pos = uniform(pos - 1);
+    if (pos < 1) return false;
+    cur = rl32(st_v, pos);
+    node = rl32(st_n, pos);
+    resume = true;
+    return true;
+  };
+  while (true) {
+    if (pos == static_cast<int>(len)) {
+      const uint32_t after = __shfl(st_v, (lane + 1) & 63, 64);
+      if (lane >= 1 && lane < static_cast<int>(len) && mk[lane])
+        atomic_or_word<Word>(&already[lane + 1 < static_cast<int>(len) ? after : cur], Word(1) << vx[lane]);
+      return 1;
+    }
+    const uint8_t mo = mv[pos];
+    if (mo == PJ_MOVE_VERTEX_STORE || mo == PJ_MOVE_VERTEX_NO_STORE) {
+      const uint64_t from = resume ? rl64(st_c, pos) : off[cur];
+      resume = false;
+      const uint32_t y = vx[pos], xt = tb[pos];
+      const uint64_t end = off[cur + 1];
+      bool found = false;
+      uint32_t u = 0;
+      uint64_t base = from, next = end;
+      while (base < end) {
+        const uint64_t e = base + lane;
+        bool ok = false;
+        uint32_t w = 0;
+        if (e < end && ea_test(ea, e)) {
+          w = edges[e];
+          ok = (cand[w] >> y) & 1;
+        }
+        const uint64_t bm = __ballot(ok);
+        if (!bm) {
+          base += 64;
+          continue;
+        }
+        const int first = __builtin_ctzll(bm);
+        u = rl32(w, first);
+        next = base + first + 1;
+        if (memo_edge_has(m, cur, u, xt) || (hyp && memo_hyp_has(m, u, y, node))) {
+          base = next;
+          continue;
+        }
+        found = true;
+        break;
+      }
+      slots += (found ? next : end) - from;
+      if (found) {
+        const uint32_t node2 = (hyp && mo == PJ_MOVE_VERTEX_STORE) ? memo_trie_child(m, node, u) : node;
+        if (hyp) memo_hyp_add(m, u, y, node2);
+        memo_edge_add(m, cur, u, y);
+        if (m.overflow) return -1;
+        if (lane == pos) st_c = next;
+        if (lane == static_cast<int>(hidx[pos])) hist = u;
+        cur = u;
+        node = node2;
+        pos = uniform(pos + 1);
+        if (lane == pos) {
+          st_v = cur;
+          st_n = node;
+        }
+        continue;
+      }
+    } else if (mo == PJ_MOVE_CHECK || mo == kMoveClose) {
+      const uint32_t target = mo == kMoveClose ? s : rl32(hist, static_cast<int>(mb[pos]));
+      RowScan r = scan_row<Word>(off[cur], off[cur + 1], edges, ea, cand, -1, target, 0xFFFFFFFFu, slots);
+      if (r.found) {
+        pos = uniform(pos + 1);
+        if (lane == pos) {
+          st_v = cur;
+          st_n = node;
+        }
+        continue;
+      }
+    } else if (mo == kMoveLast) {
+      RowScan r = scan_row<Word>(off[cur], off[cur + 1], edges, ea, cand, static_cast<int>(vx[pos]), 0, s, slots);
+      if (r.found) {
+        cur = r.u;
+        pos = uniform(pos + 1);
+        continue;
+      }
+    } else {  // MOVE_BACK
+      cur = rl32(hist, static_cast<int>(mb[pos]));
+      pos = uniform(pos + 1);
+      if (lane == pos) {
+        st_v = cur;
+        st_n = node;
+      }
+      continue;
+    }
+    if (!backtrack()) return 0;
+  }
+}
+
 // One wave per source in the task list (active vertices); failing
 // (source, template vertex) pairs go to toUnmatch and mark the source
 // modified (cc_cpu.tpp:94-129, pc_cpu.tpp:95-147, tds_cpu.tpp:153-188).
@@ -1173,17 +1460,57 @@ __global__ void __launch_bounds__(kBlock) k_nlcc(int kind, const uint32_t* __res
                                                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
                                                  const uint32_t* __restrict__ ea, const Word* __restrict__ cand,
                                                  Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
-                                                 const DevTemplate* __restrict__ t, unsigned long long* counters) {
+                                                 const DevTemplate* __restrict__ t, MemoArgs ma,
+                                                 unsigned long long* counters) {
   const uint32_t n = *list_count;
   if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_ITEMS] = n;
   const uint32_t waves = gridDim.x * (blockDim.x / 64);
+  const uint32_t wave_id = uniform(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6));
+  const bool memo = t->nlcc_memo != 0 && kind != 3;
+  MemoTable m{};
+  Word* already = static_cast<Word*>(ma.already);
+  if (memo) {
+    const uint64_t base = static_cast<uint64_t>(wave_id) << ma.log2;
+    m.keys = ma.keys + base;
+    m.vals = ma.vals + base;
+    m.epochs = ma.epochs + base;
+    m.mask = (1u << ma.log2) - 1;
+    m.limit = (3u << ma.log2) / 4;
+    m.epoch = ma.wave_epoch[wave_id];
+  }
   uint64_t enumerated = 0, slots = 0;
-  for (uint32_t w = uniform(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)); w < n; w += waves) {
+  for (uint32_t w = wave_id; w < n; w += waves) {
     const uint32_t s = list[w];
     const Word cs = cand[s];
     Word fail = 0;
     const uint32_t L = t->nlcc_len;
-    if (kind == 0) {
+    if (memo) {
+      // Plans in walk order; a root already matched by another source's
+      // walk is taken as matched (cc_cpu.tpp:102-108, pc_cpu.tpp:104-142,
+      // tds_cpu.tpp:160-188).
+      bool overflow = false;
+      for (uint32_t k = 0; k < t->walk_count && !overflow; ++k) {
+        const uint32_t x = t->walk_root[k];
+        if (!((cs >> x) & 1)) continue;
+        const Word done = __hip_atomic_load(&already[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((done >> x) & 1) continue;
+        const uint32_t o = t->walk_off[k];
+        const uint32_t len = t->walk_off[k + 1] - o;
+        m.epoch = uniform(static_cast<int>(m.epoch + 1));
+        m.used = 0;
+        m.next_node = 0;
+        m.overflow = false;
+        const int r = memo_walk<Word>(s, len, t->walk_move + o, t->walk_vertex + o, t->walk_test + o,
+                                      t->walk_mark + o, t->walk_mb + o, t->walk_hidx + o, t->walk_hyp != 0, off,
+                                      edges, ea, cand, already, m, slots);
+        if (r < 0) overflow = true;
+        else if (r == 0) fail |= Word(1) << x;
+      }
+      if (overflow) {  // retried with a larger table; nothing of s is written here
+        if (lane_id() == 0) ma.ovf_list[atomicAdd(ma.ovf_count, 1u)] = s;
+        fail = 0;
+      }
+    } else if (kind == 0) {
       for (uint32_t j = 0; j < L; ++j) {
         const uint32_t q = t->nlcc_vertex[j];
         if (!((cs >> q) & 1)) continue;
@@ -1216,6 +1543,7 @@ __global__ void __launch_bounds__(kBlock) k_nlcc(int kind, const uint32_t* __res
       set_flag_atomic(flags, s, kModified);
     }
   }
+  if (memo && lane_id() == 0) ma.wave_epoch[wave_id] = m.epoch;
   if (lane_id() == 0) {
     const int slot = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) & (kCounterSlots - 1);
     if (enumerated) atomicAdd(&counters[slot * kCounterStride + C_ENUM], static_cast<unsigned long long>(enumerated));
@@ -1279,6 +1607,11 @@ struct pj_ctx {
   bool skip_clean = false;  // phase 1 skips clean scheduled vertices (rows sorted and symmetric)
   bool dirty_enabled = true;
   bool rescan_all = false;  // every rescan mark is set (phase 1 need not push)
+  bool memo_walks = false;  // run every cycle / path step as memoised walks (PJ_MEMO_WALKS=1)
+  // memoised walks: per-wave tables (two passes), alreadyMatched, overflow list
+  MemoArgs memo1{}, memo2{};
+  void* d_already = nullptr;  // Word[V]
+  uint32_t* d_ovf = nullptr;  // [0] first-pass overflows, [1] second-pass overflows, then the list
   uint32_t* d_bins = nullptr;        // kBins * V
   uint32_t* d_bin_count = nullptr;   // kBins
   unsigned long long* d_counters = nullptr;  // kCounterSlots * kCounterStride
@@ -1357,7 +1690,9 @@ void free_graph(pj_ctx* ctx) {
                   ctx->d_flags, ctx->d_vflags, ctx->d_dirty, ctx->d_rescan, ctx->d_ea, ctx->d_bins, ctx->d_bin_count,
                   ctx->d_hubs,
                   ctx->d_chunk_hub, ctx->d_chunk_start, ctx->d_hub_count, ctx->d_hub_elim, ctx->d_nhubs, ctx->d_hub_hit,
-                  ctx->d_last_scan, ctx->d_hub_new};
+                  ctx->d_last_scan, ctx->d_hub_new, ctx->memo1.keys, ctx->memo1.vals, ctx->memo1.epochs,
+                  ctx->memo1.wave_epoch, ctx->memo2.keys, ctx->memo2.vals, ctx->memo2.epochs,
+                  ctx->memo2.wave_epoch, ctx->d_already, ctx->d_ovf};
   for (void* b : bufs)
     if (b) hipFree(b);
   ctx->d_off = nullptr;
@@ -1374,6 +1709,10 @@ void free_graph(pj_ctx* ctx) {
   ctx->d_nhubs = nullptr;
   ctx->d_hub_hit = nullptr;
   ctx->d_last_scan = ctx->d_hub_new = nullptr;
+  ctx->memo1 = MemoArgs{};
+  ctx->memo2 = MemoArgs{};
+  ctx->d_already = nullptr;
+  ctx->d_ovf = nullptr;
   ctx->V = 0;
   ctx->E = 0;
 }
@@ -1622,20 +1961,85 @@ void launch_lcc(pj_ctx* ctx) {
   launch_phase2<Word>(ctx);
 }
 
+// Memoised walks: tables are allocated on first use (the second pass only
+// when a source outgrows the first); alreadyMatched and the overflow
+// counters are cleared for every step.
+int alloc_memo(pj_ctx* ctx, MemoArgs* m, int waves, int log2) {
+  if (m->keys) return 0;
+  const uint64_t n = static_cast<uint64_t>(waves) << log2;
+  if (dev_alloc(ctx, &m->keys, n) || dev_alloc(ctx, &m->vals, n) || dev_alloc(ctx, &m->epochs, n) ||
+      dev_alloc(ctx, &m->wave_epoch, waves))
+    return -1;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, n)), dim3(kBlock), 0, ctx->stream, m->epochs, n, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, ctx->stream, m->wave_epoch, static_cast<uint64_t>(waves),
+                     0u);
+  m->log2 = static_cast<uint32_t>(log2);
+  m->already = ctx->d_already;
+  return 0;
+}
+
+int prepare_memo(pj_ctx* ctx) {
+  const uint64_t V = ctx->V;
+  if (!ctx->d_already) {
+    uint64_t* a = nullptr;
+    if (dev_alloc(ctx, &a, V + 1) || dev_alloc(ctx, &ctx->d_ovf, 2 * V + 2)) return -1;
+    ctx->d_already = a;
+  }
+  if (alloc_memo(ctx, &ctx->memo1, kMemoWaves, kMemoLog2)) return -1;
+  ctx->memo1.ovf_count = ctx->d_ovf;
+  ctx->memo1.ovf_list = ctx->d_ovf + 2;
+  const uint64_t words = (V * ctx->word_bytes + 3) / 4;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, words)), dim3(kBlock), 0, ctx->stream,
+                     static_cast<uint32_t*>(ctx->d_already), words, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_ovf, static_cast<uint64_t>(2), 0u);
+  PJ_HIP(hipGetLastError());
+  return 0;
+}
+
 template <typename Word>
-void launch_nlcc(pj_ctx* ctx, int kind) {
+int launch_nlcc(pj_ctx* ctx, int kind) {
   launch_bins<Word>(ctx, kind == 1 ? 2 : 1);
+  const bool memo = ctx->h_t.nlcc_memo && kind != 3;
+  if (memo && prepare_memo(ctx)) return -1;
+  const MemoArgs ma = memo ? ctx->memo1 : MemoArgs{};
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   const uint64_t V = ctx->V;
-  const int g = ctx->cus * 8;
+  const int g = memo ? kMemoWaves / (kBlock / 64) : ctx->cus * 8;
   // Every bin holds active vertices; hubs are run from their own list.
   for (int b = 0; b < kBins; ++b) {
     unsigned long long* c = begin_launch(ctx, K_NLCC);
     hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, kind, ctx->d_bins + b * V,
-                       ctx->d_bin_count + b, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t, c);
+                       ctx->d_bin_count + b, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t,
+                       ma, c);
     end_launch(ctx);
   }
+  return 0;
+}
+
+// Sources whose walks outgrew the first-pass tables run again, one wave
+// each, with tables of 2^kMemoLog2b entries.
+template <typename Word>
+int launch_nlcc_retry(pj_ctx* ctx, int kind) {
+  uint32_t n[2] = {0, 0};
+  PJ_HIP(hipMemcpyAsync(n, ctx->d_ovf, 8, hipMemcpyDeviceToHost, ctx->stream));
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  if (!n[0]) return 0;
+  if (alloc_memo(ctx, &ctx->memo2, kMemoWaves2, kMemoLog2b)) return -1;
+  ctx->memo2.ovf_count = ctx->d_ovf + 1;
+  ctx->memo2.ovf_list = ctx->d_ovf + 2 + ctx->V;
+  unsigned long long* c = begin_launch(ctx, K_NLCC);
+  hipLaunchKernelGGL(k_nlcc<Word>, dim3(kMemoWaves2), dim3(64), 0, ctx->stream, kind, ctx->d_ovf + 2, ctx->d_ovf,
+                     ctx->d_off, ctx->d_edges, ctx->d_ea, static_cast<const Word*>(ctx->d_cand),
+                     static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->d_t, ctx->memo2, c);
+  end_launch(ctx);
+  PJ_HIP(hipMemcpyAsync(n, ctx->d_ovf, 8, hipMemcpyDeviceToHost, ctx->stream));
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  if (n[1]) {
+    ctx->err = "memoised walk of a source exceeds 2^22 memo entries";
+    return -1;
+  }
+  return 0;
 }
 
 // Hubs are not in the bins; the NLCC task list for them is the hub list with
@@ -1646,34 +2050,132 @@ void launch_nlcc_hubs(pj_ctx* ctx, int kind, uint32_t* d_hub_n) {
   if (!ctx->nhubs) return;
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
+  const bool memo = ctx->h_t.nlcc_memo && kind != 3;
+  const MemoArgs ma = memo ? ctx->memo1 : MemoArgs{};
+  const uint32_t g = std::min<uint32_t>(ctx->nhubs, memo ? kMemoWaves : ctx->cus * 8);
   unsigned long long* c = begin_launch(ctx, K_NLCC_HUB);
-  hipLaunchKernelGGL(k_nlcc<Word>, dim3(std::min<uint32_t>(ctx->nhubs, ctx->cus * 8)), dim3(64), 0, ctx->stream, kind,
-                     ctx->d_hubs, d_hub_n, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t, c);
+  hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(64), 0, ctx->stream, kind, ctx->d_hubs, d_hub_n, ctx->d_off,
+                     ctx->d_edges, ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
   end_launch(ctx);
+}
+
+// Whether the reference's edge memo can fire on a search whose memo levels
+// are `tr` (position -> next position): it inserts template ids and tests
+// positions (cc_cpu.tpp:53-61, pc_cpu.tpp:59-67), so a test hits only when a
+// template id equal to the tested position was inserted for the same
+// undirected edge, which needs matching label codes at both ends.
+bool memo_may_fire(const std::vector<uint32_t>& idx, const uint8_t* pcode,
+                   const std::vector<std::pair<size_t, size_t>>& tr) {
+  for (const auto& cur : tr)
+    for (const auto& prev : tr) {
+      if (idx[prev.second] != cur.first) continue;
+      const uint8_t lc = pcode[idx[cur.first]], lu = pcode[idx[cur.second]];
+      const uint8_t pf = pcode[idx[prev.first]], pt = pcode[idx[prev.second]];
+      if ((pf == lc && pt == lu) || (pf == lu && pt == lc)) return true;
+    }
+  return false;
+}
+
+// Appends one memoised-walk plan to the walk tables.
+struct PlanMove {
+  uint8_t move;
+  uint32_t vertex, mb, hidx;
+  uint8_t test, mark;
+};
+
+int add_plan(pj_ctx* ctx, uint32_t root, const std::vector<PlanMove>& mv, uint32_t* o) {
+  DevTemplate& t = ctx->h_t;
+  if (t.walk_count >= 64 || mv.size() > 64 || *o + mv.size() > 64 * 128) {
+    ctx->err = "memoised walk plan exceeds the device tables (64 walks of up to 64 moves)";
+    return -1;
+  }
+  const uint32_t k = t.walk_count++;
+  t.walk_root[k] = root;
+  t.walk_off[k] = *o;
+  for (size_t i = 0; i < mv.size(); ++i) {
+    t.walk_move[*o + i] = mv[i].move;
+    t.walk_vertex[*o + i] = mv[i].vertex;
+    t.walk_mb[*o + i] = mv[i].mb;
+    t.walk_hidx[*o + i] = mv[i].hidx;
+    t.walk_test[*o + i] = mv[i].test;
+    t.walk_mark[*o + i] = mv[i].mark;
+  }
+  *o += static_cast<uint32_t>(mv.size());
+  t.walk_off[k + 1] = *o;
+  return 0;
 }
 
 int install_nlcc(pj_ctx* ctx, int step, uint32_t index) {
   DevTemplate& t = ctx->h_t;
   t.rows_sorted = ctx->rows_sorted ? 1u : 0u;
+  t.nlcc_memo = 0;
+  t.walk_hyp = 0;
+  const bool force_memo = ctx->memo_walks;
   if (step == PJ_STEP_CC) {
     if (index >= ctx->cycles.size()) {
       ctx->err = "cycle constraint index out of range";
       return -1;
     }
     const auto& c = ctx->cycles[index];
-    t.nlcc_len = static_cast<uint32_t>(c.size());
-    for (size_t i = 0; i < c.size(); ++i) t.nlcc_vertex[i] = c[i];
+    const size_t L = c.size();
+    t.nlcc_len = static_cast<uint32_t>(L);
+    for (size_t i = 0; i < L; ++i) t.nlcc_vertex[i] = c[i];
+    bool fire = false;
+    for (size_t j = 0; j < L && !fire; ++j) {
+      std::vector<std::pair<size_t, size_t>> tr;
+      for (size_t d = 0; d + 1 < L; ++d) tr.emplace_back((j + d) % L, (j + d + 1) % L);
+      fire = memo_may_fire(c, t.pcode, tr);
+    }
+    if (fire || force_memo) {
+      // start j: hops to idx[(j+p)%L] testing position (j+p-1)%L, marking the
+      // intermediates (cc_cpu.tpp:24-68), then a slot back to the source.
+      t.nlcc_memo = 1;
+      t.walk_count = 0;
+      uint32_t o = 0;
+      for (size_t j = 0; j < L; ++j) {
+        std::vector<PlanMove> mv;
+        mv.push_back({PJ_MOVE_VERTEX_NO_STORE, c[j], 0, 0, 0, 0});
+        for (size_t p = 1; p < L; ++p)
+          mv.push_back({PJ_MOVE_VERTEX_NO_STORE, c[(j + p) % L], 0, static_cast<uint32_t>(p),
+                        static_cast<uint8_t>((j + p - 1) % L), 1});
+        mv.push_back({kMoveClose, c[j], 0, 0, 0, 0});
+        if (add_plan(ctx, c[j], mv, &o)) return -1;
+      }
+    }
   } else if (step == PJ_STEP_PC) {
     if (index >= ctx->paths.size()) {
       ctx->err = "path constraint index out of range";
       return -1;
     }
     const auto& p = ctx->paths[index];
-    t.nlcc_len = static_cast<uint32_t>(p.size());
-    for (size_t i = 0; i < p.size(); ++i) t.nlcc_vertex[i] = p[i];
+    const size_t L = p.size();
+    t.nlcc_len = static_cast<uint32_t>(L);
+    for (size_t i = 0; i < L; ++i) t.nlcc_vertex[i] = p[i];
     t.path_code = kNoCode;
     for (uint32_t k = 0; k < t.K; ++k)
       if (t.label_raw[k] == ctx->path_label[index]) t.path_code = k;
+    std::vector<std::pair<size_t, size_t>> fw, bw;
+    for (size_t d = 0; d + 2 < L; ++d) fw.emplace_back(d, d + 1);
+    for (size_t d = 0; d + 2 < L; ++d) bw.emplace_back(L - 1 - d, L - 2 - d);
+    if (memo_may_fire(p, t.pcode, fw) || memo_may_fire(p, t.pcode, bw) || force_memo) {
+      // forward from idx[0] and backward from idx[L-1]; hop h tests position
+      // h-1 (forward) or L-h (backward); the last hop needs a candidate other
+      // than the source and marks it (pc_cpu.tpp:11-75)
+      t.nlcc_memo = 1;
+      t.walk_count = 0;
+      uint32_t o = 0;
+      for (int r = 0; r < 2; ++r) {
+        std::vector<PlanMove> mv;
+        mv.push_back({PJ_MOVE_VERTEX_NO_STORE, r ? p[L - 1] : p[0], 0, 0, 0, 0});
+        for (size_t h = 1; h + 1 < L; ++h) {
+          const size_t np = r ? L - 1 - h : h;
+          const size_t cp = r ? L - h : h - 1;
+          mv.push_back({PJ_MOVE_VERTEX_NO_STORE, p[np], 0, static_cast<uint32_t>(h), static_cast<uint8_t>(cp), 0});
+        }
+        mv.push_back({kMoveLast, r ? p[0] : p[L - 1], 0, 0, 0, 1});
+        if (add_plan(ctx, r ? p[L - 1] : p[0], mv, &o)) return -1;
+      }
+    }
   } else {
     const std::vector<pj_ctx::HostWalk>* walks;
     std::vector<pj_ctx::HostWalk> single;
@@ -1708,11 +2210,23 @@ int install_nlcc(pj_ctx* ctx, int step, uint32_t index) {
         t.walk_mb[o + i] = w.mb[i];
         const bool vtx = w.move[i] == PJ_MOVE_VERTEX_STORE || w.move[i] == PJ_MOVE_VERTEX_NO_STORE;
         t.walk_hidx[o + i] = vtx ? h : 0;
+        // TDS memos: the edge memo tests the previous move's template vertex
+        // and every VERTEX move marks its vertex (tds_cpu.tpp:31-82)
+        t.walk_test[o + i] = static_cast<uint8_t>(i ? w.vertex[i - 1] : 0);
+        t.walk_mark[o + i] = vtx && i ? 1 : 0;
         if (vtx) ++h;
       }
       o += static_cast<uint32_t>(w.move.size());
     }
     t.walk_off[walks->size()] = o;
+    if (step == PJ_STEP_TDS) {
+      t.nlcc_memo = 1;
+      t.walk_hyp = 1;
+    }
+  }
+  if (t.nlcc_memo && ctx->V >= (1u << 30)) {
+    ctx->err = "memoised walks need vertex ids below 2^30";
+    return -1;
   }
   PJ_HIP(hipMemcpyAsync(ctx->d_t, &t, sizeof(DevTemplate), hipMemcpyHostToDevice, ctx->stream));
   return 0;
@@ -1738,8 +2252,9 @@ int run_step_t(pj_ctx* ctx, int step, uint32_t index) {
     case PJ_STEP_TDS:
     case PJ_STEP_ENUMERATION: {
       const int kind = step == PJ_STEP_CC ? 0 : step == PJ_STEP_PC ? 1 : step == PJ_STEP_TDS ? 2 : 3;
-      launch_nlcc<Word>(ctx, kind);
+      if (launch_nlcc<Word>(ctx, kind)) return -1;
       launch_nlcc_hubs<Word>(ctx, kind, ctx->d_nhubs);
+      if (ctx->h_t.nlcc_memo && kind != 3 && launch_nlcc_retry<Word>(ctx, kind)) return -1;
       if (step != PJ_STEP_ENUMERATION) {
         mark_all_rescan(ctx);  // sources that failed turned modified
         launch_phase2<Word>(ctx);
@@ -1828,6 +2343,9 @@ int pj_create(pj_ctx** out, int device) {
   // PJ_SKIP_CLEAN=0 evaluates every scheduled vertex in LCC phase 1 (the
   // results are the same either way; the switch exists for comparisons).
   if (const char* sc = getenv("PJ_SKIP_CLEAN")) ctx->dirty_enabled = strcmp(sc, "0") != 0;
+  // PJ_MEMO_WALKS=1 runs cycle and path steps as the reference's memoised
+  // search even where the complete search is known to give the same result.
+  if (const char* mw = getenv("PJ_MEMO_WALKS")) ctx->memo_walks = strcmp(mw, "0") != 0;
   *out = ctx;
   return 0;
 }

@@ -208,3 +208,37 @@ def test_oracle_detects_a_planted_mistake(pj, orc):
     while any(o2.step(STEP_LCC)):
         pass
     assert o2.step(STEP_ENUMERATION)[3] != rows[-1].enumeration
+
+
+TDS_EDGES = [(0, 1), (1, 2), (2, 3), (3, 0), (1, 4), (4, 5), (5, 2), (0, 2), (5, 6), (6, 7)]
+
+
+def test_faithful_outcomes_lie_between_the_no_shortcut_bounds(pj, orc):
+    """On a graph where the reference's TDS result depends on scheduling, the
+    faithful mode at any thread count fails at least the no-shortcut
+    failures that no successful walk marks, and at most the no-shortcut
+    failures (the bound the GPU tests use for such graphs)."""
+    g = pj.Graph.rmat(12, 8, seed=5).labels_zipf(8, 1.0, seed=6)
+    c = pj.Constraints(pj.Graph.pattern(8, TDS_EDGES, list(range(8))))
+    o = orc.Oracle(g, c, threads=4)
+    o.step(STEP_LCC0, 0, orc.FAITHFUL)
+    while any(o.step(STEP_LCC, 0, orc.FAITHFUL)):
+        pass
+    o.reset_modified()
+    before = o.state()
+    cand = before[0]
+    nos = orc.Oracle(g, c, threads=4)
+    nos.load_state(*before)
+    nos.step(STEP_TDS, 0, orc.FAITHFUL_NO_SHORTCUT)
+    f0 = cand & ~nos.state()[0]
+    m0 = nos.last_marks()
+    assert np.any(f0 & m0)  # the shortcut can matter on this graph
+    outcomes = set()
+    for threads in (1, 2, 4, 8, 16, 32, 32):
+        x = orc.Oracle(g, c, threads=threads)
+        x.load_state(*before)
+        x.step(STEP_TDS, 0, orc.FAITHFUL)
+        f = cand & ~x.state()[0]
+        assert np.all((f & ~f0) == 0) and np.all((f0 & ~m0 & ~f) == 0), threads
+        outcomes.add(f.tobytes())
+    print(f"{len(outcomes)} distinct outcomes over the thread counts")
