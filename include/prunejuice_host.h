@@ -81,6 +81,10 @@ typedef struct pjh_row {
  * reaches the enumeration step (the pruning fixed point) without running it.
  * Returns the number of steps (rows written up to max_rows), -1 on error. */
 #define PJH_STOP_BEFORE_ENUMERATION 1
+/* With PJH_STOP_BEFORE_TDS it stops when the first TDS step is due (used to
+ * measure the other phases on graphs where the memoised TDS search of the
+ * reference does not finish in reasonable time). */
+#define PJH_STOP_BEFORE_TDS 2
 int pjh_run_pipeline(pj_ctx* ctx, const pjh_constraints* c, int flags, pjh_row* rows, int max_rows,
                      double* total_ms);
 

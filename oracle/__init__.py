@@ -124,10 +124,11 @@ class Oracle:
                            vflags.ctypes.data_as(C.POINTER(C.c_uint8)), ea_words.ctypes.data_as(C.POINTER(C.c_uint32)),
                            av, ae)
 
-    def run(self, mode: int = COMPLETE, max_rows: int = 1000, enumerate_matches: bool = True):
+    def run(self, mode: int = COMPLETE, max_rows: int = 1000, enumerate_matches: bool = True, run_tds: bool = True):
         rows = (OracleRow * max_rows)()
         total = C.c_double()
-        n = lib().or_run_pipeline(self._s, mode, 0 if enumerate_matches else 1, rows, max_rows, C.byref(total))
+        flags = (0 if enumerate_matches else 1) | (0 if run_tds else 2)
+        n = lib().or_run_pipeline(self._s, mode, flags, rows, max_rows, C.byref(total))
         if n < 0:
             raise ValueError("oracle pipeline failed")
         return list(rows[: min(n, max_rows)]), total.value

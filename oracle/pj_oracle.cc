@@ -627,8 +627,9 @@ void or_set_state(or_state* s, const uint64_t* cand, const uint8_t* vflags, cons
 // PC, else TDS (always constraint 0: the reference never advances the TDS
 // iterator, tds_cpu.tpp:135-244) -> LCC ... -> ENUMERATION -> END, at most
 // 1000 steps.  Writes one row per step; returns the number of rows.
-int or_run_pipeline(or_state* s, int mode, int stop_before_enumeration, or_row* rows, int max_rows,
-                    double* total_ms) {
+// flags: bit 0 stops before the enumeration step, bit 1 before the first TDS
+// step (as PJH_STOP_BEFORE_ENUMERATION / PJH_STOP_BEFORE_TDS).
+int or_run_pipeline(or_state* s, int mode, int flags, or_row* rows, int max_rows, double* total_ms) {
   or_reset(s);
   int n_cc = static_cast<int>(s->t.cycles.size()), n_pc = static_cast<int>(s->t.paths.size()),
       n_tds = static_cast<int>(s->t.tds.size());
@@ -638,7 +639,8 @@ int or_run_pipeline(or_state* s, int mode, int stop_before_enumeration, or_row* 
   double total = 0.0;
   for (int it = 0; it < 1000; ++it) {
     uint32_t index = 0;
-    if (stop_before_enumeration && step == PJ_STEP_ENUMERATION) break;
+    if ((flags & 1) && step == PJ_STEP_ENUMERATION) break;
+    if ((flags & 2) && step == PJ_STEP_TDS) break;
     if (step == PJ_STEP_CC || step == PJ_STEP_PC || step == PJ_STEP_TDS) or_reset_modified(s);
     if (step == PJ_STEP_CC) index = cc;
     if (step == PJ_STEP_PC) index = pc;

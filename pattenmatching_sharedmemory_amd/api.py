@@ -313,12 +313,14 @@ class Pruner:
         self._check(native.hip().pj_upload_state(self._ctx, _ptr(cand, C.c_uint64), _ptr(vf, C.c_uint8),
                                                  _ptr(ea, C.c_uint32), st.active_vertices, st.active_edges))
 
-    def run(self, max_rows: int = 1000, enumerate_matches: bool = True) -> tuple[list[Row], float]:
+    def run(self, max_rows: int = 1000, enumerate_matches: bool = True,
+            run_tds: bool = True) -> tuple[list[Row], float]:
         """Whole step machine; returns (rows, total device ms).  Without
-        enumerate_matches it stops at the pruning fixed point."""
+        enumerate_matches it stops at the pruning fixed point; without run_tds
+        it stops when the first TDS step is due."""
         rows = (Row * max_rows)()
         total = C.c_double()
-        flags = 0 if enumerate_matches else 1
+        flags = (0 if enumerate_matches else 1) | (0 if run_tds else 2)
         n = native.host().pjh_run_pipeline(self._ctx, self.constraints.handle, flags, rows, max_rows,
                                            C.byref(total))
         if n < 0:

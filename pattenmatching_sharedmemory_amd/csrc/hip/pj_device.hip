@@ -15,6 +15,7 @@
 // exactly when its candidate set becomes empty (lcc0_cpu.tpp:42-45,
 // lcc_cpu.tpp:114-117, cc_cpu.tpp:141-147).
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstdio>
@@ -53,12 +54,12 @@ constexpr int kMaxLaunches = 32;  // timed launches per step
 // Kernel kinds reported by pj_kernel_stats.
 enum KernelId {
   K_LCC0, K_BINS, K_LCC_P1_G4, K_LCC_P1_G16, K_LCC_P1_G64, K_LCC_HUB, K_LCC_HUB_FINISH,
-  K_P2_G4, K_P2_G16, K_P2_G64, K_P2_HUB, K_P2_HUB_FINISH, K_NLCC, K_NLCC_HUB, K_HUB_PUSH, K_COUNT
+  K_P2_G4, K_P2_G16, K_P2_G64, K_P2_HUB, K_P2_HUB_FINISH, K_NLCC, K_NLCC_HUB, K_HUB_PUSH, K_VIEW, K_COUNT
 };
 const char* const kKernelNames[K_COUNT] = {
     "lcc0", "build_bins", "lcc_phase1_g4", "lcc_phase1_g16", "lcc_phase1_g64", "lcc_phase1_hub_chunks",
     "lcc_phase1_hub_finish", "phase2_g4", "phase2_g16", "phase2_g64", "phase2_hub_chunks", "phase2_hub_finish",
-    "nlcc_walk", "nlcc_walk_hubs", "hub_rescan_push"};
+    "nlcc_walk", "nlcc_walk_hubs", "hub_rescan_push", "active_view"};
 
 // Template tables in device memory.
 struct DevTemplate {
@@ -113,6 +114,12 @@ __device__ __forceinline__ int ctz(Word w) {
 
 __device__ __forceinline__ bool ea_test(const uint32_t* __restrict__ ea, uint64_t e) {
   return (ea[e >> 5] >> (e & 31)) & 1u;
+}
+
+// Slot test for walks that may run on the active view (no bitset: every
+// slot of the view is active).
+__device__ __forceinline__ bool walk_slot(const uint32_t* __restrict__ ea, uint64_t e) {
+  return !ea || ea_test(ea, e);
 }
 
 // Wave sum of a per-lane value, added by lane 0 to a spread counter slot.
@@ -869,305 +876,6 @@ __global__ void k_phase2_hub_finish(const uint32_t* __restrict__ hubs, uint32_t 
 }
 
 // ---------------------------------------------------------------------------
-// NLCC: one wavefront per source vertex runs a depth-first walk whose row
-// scans are done 64 slots at a time (ballot, lowest lane first, so the walk
-// visits neighbours in CSR order).  Level i of the walk keeps its vertex and
-// its resume slot in lane i's registers (walks are at most 64 levels).  The
-// search is complete: it decides whether a walk satisfying the constraint
-// exists (see the NLCC notes in the oracle for when this equals the
-// reference's memoised search).
-
-struct RowScan {
-  uint32_t u;
-  uint64_t next;  // slot after the hit
-  bool found;
-};
-
-// First active slot in [from, end) of a row whose neighbour is active and has
-// template vertex q (q < 0: neighbour must equal `target`; `excl`: neighbour
-// must differ from it).  Adds to `slots` the slots a serial scan reads (up to
-// and including the hit).
-template <typename Word>
-__device__ __forceinline__ RowScan scan_row(uint64_t from, uint64_t end, const uint32_t* __restrict__ edges,
-                                            const uint32_t* __restrict__ ea, const Word* __restrict__ cand, int q,
-                                            uint32_t target, uint32_t excl, uint64_t& slots) {
-  RowScan r{0, end, false};
-  const int lane = lane_id();
-  for (uint64_t base = from; base < end; base += 64) {
-    const uint64_t e = base + lane;
-    bool ok = false;
-    uint32_t u = 0;
-    if (e < end && ea_test(ea, e)) {
-      u = edges[e];
-      if (q < 0) {
-        ok = u == target;
-      } else {
-        Word cu = cand[u];
-        ok = ((cu >> q) & 1) && u != excl;
-      }
-    }
-    uint64_t m = __ballot(ok);
-    if (m) {
-      int first = __builtin_ctzll(m);
-      r.u = rl32(u, first);
-      r.next = base + first + 1;
-      r.found = true;
-      slots += r.next - from;
-      return r;
-    }
-  }
-  if (end > from) slots += end - from;
-  return r;
-}
-
-// Whether row(a) holds an active slot to b, by binary search (rows sorted
-// ascending); `probes` counts the slots read.
-__device__ __forceinline__ bool active_slot(uint32_t a, uint32_t b, const uint64_t* __restrict__ off,
-                                            const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
-                                            uint64_t& probes) {
-  uint64_t lo = off[a], hi = off[a + 1];
-  const uint64_t end = hi;
-  while (lo < hi) {
-    const uint64_t mid = (lo + hi) >> 1;
-    ++probes;
-    if (edges[mid] < b)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  for (; lo < end; ++lo) {
-    ++probes;
-    if (edges[lo] != b) break;
-    if (ea_test(ea, lo)) return true;
-  }
-  return false;
-}
-
-// Last two hops of a cycle on sorted rows: is there a u with an active slot
-// v->u, template vertex q among u's candidates, and an active slot u->s?
-// The shorter of row(v) and row(s) is read 64 slots at a time, each lane
-// binary-searching the other side, so a low-degree source next to a hub
-// never reads the hub's row.
-template <typename Word>
-__device__ bool close_cycle(uint32_t v, uint32_t s, int q, const uint64_t* __restrict__ off,
-                            const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
-                            const Word* __restrict__ cand, uint64_t& slots) {
-  const int lane = lane_id();
-  const uint64_t vb = off[v], ve = off[v + 1], sb = off[s], se = off[s + 1];
-  const bool from_v = (ve - vb) <= (se - sb);
-  const uint64_t b = from_v ? vb : sb, end = from_v ? ve : se;
-  uint64_t probes = 0, read = 0;
-  bool found = false;
-  for (uint64_t base = b; base < end; base += 64) {
-    const uint64_t e = base + lane;
-    bool ok = false;
-    if (e < end) {
-      const uint32_t u = edges[e];
-      if ((cand[u] >> q) & 1) {
-        if (from_v)
-          ok = ea_test(ea, e) && active_slot(u, s, off, edges, ea, probes);
-        else
-          ok = active_slot(v, u, off, edges, ea, probes) && active_slot(u, s, off, edges, ea, probes);
-      }
-    }
-    read += min<uint64_t>(64, end - base);
-    if (__ballot(ok)) {
-      found = true;
-      break;
-    }
-  }
-  for (int o = 32; o > 0; o >>= 1) probes += __shfl_xor(probes, o, 64);
-  slots += read + probes;
-  return found;
-}
-
-// Cycle constraint idx[0..L-1] from start position j (cc_cpu.tpp:12-70).
-// With sorted rows the last two hops are decided by close_cycle.
-template <typename Word>
-__device__ bool cycle_walk(uint32_t s, uint32_t j, uint32_t L, bool sorted, const uint64_t* __restrict__ off,
-                           const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
-                           const Word* __restrict__ cand, const DevTemplate* __restrict__ t, uint64_t& slots) {
-  const int lane = lane_id();
-  uint32_t st_v = s;          // lane i: vertex at level i
-  uint64_t st_c = off[s];     // lane i: resume slot at level i
-  int depth = 0;
-  uint32_t v = s;
-  uint64_t cursor = off[s];
-  while (true) {
-    if (sorted && depth == static_cast<int>(L) - 2) {
-      const int q = static_cast<int>(t->nlcc_vertex[(j + L - 1) % L]);
-      if (close_cycle<Word>(v, s, q, off, edges, ea, cand, slots)) return true;
-      depth = uniform(depth - 1);
-      v = rl32(st_v, depth);
-      cursor = rl64(st_c, depth);
-      continue;
-    }
-    if (depth == static_cast<int>(L) - 1) {
-      RowScan r = scan_row<Word>(off[v], off[v + 1], edges, ea, cand, -1, s, 0xFFFFFFFFu, slots);
-      if (r.found) return true;
-      depth = uniform(depth - 1);
-      v = rl32(st_v, depth);
-      cursor = rl64(st_c, depth);
-      continue;
-    }
-    const int q = static_cast<int>(t->nlcc_vertex[(j + depth + 1) % L]);
-    RowScan r = scan_row<Word>(cursor, off[v + 1], edges, ea, cand, q, 0, 0xFFFFFFFFu, slots);
-    if (r.found) {
-      if (lane == depth) st_c = r.next;
-      depth = uniform(depth + 1);
-      v = r.u;
-      cursor = off[v];
-      if (lane == depth) {
-        st_v = v;
-        st_c = cursor;
-      }
-    } else {
-      if (depth == 0) return false;
-      depth = uniform(depth - 1);
-      v = rl32(st_v, depth);
-      cursor = rl64(st_c, depth);
-    }
-  }
-}
-
-// Path constraint (pc_cpu.tpp:11-75): from position 0 forwards or from the
-// last position backwards over L-1 edges, ending on a vertex other than s.
-template <typename Word>
-__device__ bool path_walk(uint32_t s, bool reverse, uint32_t L, const uint64_t* __restrict__ off,
-                          const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
-                          const Word* __restrict__ cand, const DevTemplate* __restrict__ t, uint64_t& slots) {
-  const int lane = lane_id();
-  uint32_t st_v = s;
-  uint64_t st_c = off[s];
-  int depth = 0;
-  uint32_t v = s;
-  uint64_t cursor = off[s];
-  while (true) {
-    const uint32_t pos = reverse ? (L - 2 - depth) : (depth + 1);
-    const int q = static_cast<int>(t->nlcc_vertex[pos]);
-    const bool last = depth == static_cast<int>(L) - 2;
-    RowScan r = scan_row<Word>(cursor, off[v + 1], edges, ea, cand, q, 0, last ? s : 0xFFFFFFFFu, slots);
-    if (r.found) {
-      if (last) return true;
-      if (lane == depth) st_c = r.next;
-      depth = uniform(depth + 1);
-      v = r.u;
-      cursor = off[v];
-      if (lane == depth) {
-        st_v = v;
-        st_c = cursor;
-      }
-    } else {
-      if (depth == 0) return false;
-      depth = uniform(depth - 1);
-      v = rl32(st_v, depth);
-      cursor = rl64(st_c, depth);
-    }
-  }
-}
-
-// Walk over VERTEX / CHECK / MOVE_BACK moves (tds_cpu.tpp:13-127 and, with
-// counting and injectivity, enumeration_cpu.tpp:38-118).  Lane p keeps the
-// vertex the walk stands on before move p (st_v) and the resume slot of move
-// p (st_c); lane h keeps history entry h.  Walks have at most 64 moves.
-// Returns the number of completions when `count_all`, else 1 on the first
-// completion.  When a search without counting backtracks through a CHECK,
-// the rest of that row is added to `slots`, as a serial scan would read it
-// looking for a second slot to the same vertex.
-template <typename Word>
-__device__ uint64_t template_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ mv,
-                                  const uint32_t* __restrict__ vx, const uint32_t* __restrict__ mb,
-                                  const uint32_t* __restrict__ hidx, bool count_all,
-                                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
-                                  const uint32_t* __restrict__ ea, const Word* __restrict__ cand, uint64_t& slots) {
-  const int lane = lane_id();
-  uint32_t st_v = s;
-  uint64_t st_c = 0;
-  uint32_t hist = s;
-  uint64_t total = 0;
-  int pos = 1;
-  uint32_t cur = s;
-  bool resume = false;  // re-entering a VERTEX move after backtracking
-  if (len <= 1) return 1;
-  auto is_vertex = [&](int p) { return mv[p] == PJ_MOVE_VERTEX_STORE || mv[p] == PJ_MOVE_VERTEX_NO_STORE; };
-  // Back to the most recent VERTEX move; false when none is left.
-  auto backtrack = [&]() -> bool {
-    pos = uniform(pos - 1);
-    while (pos >= 1 && !is_vertex(pos)) {
-      if (!count_all && mv[pos] == PJ_MOVE_CHECK) {
-        const uint32_t at = rl32(st_v, pos);
-        slots += off[at + 1] - rl64(st_c, pos);
-      }
-      pos = uniform(pos - 1);
-    }
-    if (pos < 1) return false;
-    cur = rl32(st_v, pos);
-    resume = true;
-    return true;
-  };
-  while (true) {
-    if (pos == static_cast<int>(len)) {
-      ++total;
-      if (!count_all) return total;
-      if (!backtrack()) return total;
-      continue;
-    }
-    const uint8_t m = mv[pos];
-    if (m == PJ_MOVE_VERTEX_STORE || m == PJ_MOVE_VERTEX_NO_STORE) {
-      const uint64_t from = resume ? rl64(st_c, pos) : off[cur];
-      resume = false;
-      const int q = static_cast<int>(vx[pos]);
-      const uint32_t h = hidx[pos];
-      const uint64_t end = off[cur + 1];
-      RowScan r{0, end, false};
-      for (uint64_t base = from; base < end; base += 64) {
-        const uint64_t e = base + lane;
-        bool ok = false;
-        uint32_t u = 0;
-        if (e < end && ea_test(ea, e)) {
-          u = edges[e];
-          ok = (cand[u] >> q) & 1;
-        }
-        if (count_all)  // injective embeddings; every lane active here
-          for (uint32_t k = 0; k < h; ++k) ok = ok && rl32(hist, static_cast<int>(k)) != u;
-        uint64_t bm = __ballot(ok);
-        if (bm) {
-          int first = __builtin_ctzll(bm);
-          r.u = rl32(u, first);
-          r.next = base + first + 1;
-          r.found = true;
-          break;
-        }
-      }
-      slots += (r.found ? r.next : end) - from;
-      if (r.found) {
-        if (lane == pos) st_c = r.next;
-        if (lane == static_cast<int>(h)) hist = r.u;
-        cur = r.u;
-        pos = uniform(pos + 1);
-        if (lane == pos) st_v = cur;
-        continue;
-      }
-    } else if (m == PJ_MOVE_CHECK) {
-      const uint32_t target = rl32(hist, static_cast<int>(mb[pos]));
-      RowScan r = scan_row<Word>(off[cur], off[cur + 1], edges, ea, cand, -1, target, 0xFFFFFFFFu, slots);
-      if (r.found) {
-        if (lane == pos) st_c = r.next;
-        pos = uniform(pos + 1);
-        if (lane == pos) st_v = cur;
-        continue;
-      }
-    } else {  // MOVE_BACK
-      cur = rl32(hist, static_cast<int>(mb[pos]));
-      pos = uniform(pos + 1);
-      if (lane == pos) st_v = cur;
-      continue;
-    }
-    if (!backtrack()) return total;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Memoised walks: the reference's own search (cc_cpu.tpp:12-70,
 // pc_cpu.tpp:11-75, tds_cpu.tpp:13-127).  Its per-source memos can prune
 // branches a complete search would take, so when they can fire (always for
@@ -1187,8 +895,11 @@ constexpr uint8_t kMoveClose = 4;  // last hop of a cycle: an active slot back t
 constexpr uint8_t kMoveLast = 5;   // last hop of a path: a candidate other than the source
 constexpr int kMemoWaves = 2048;            // waves of a memoised launch (first pass)
 constexpr int kMemoLog2 = 14;               // entries per wave, first pass
-constexpr int kMemoWaves2 = 32;             // second pass, for sources whose walks outgrew the first
-constexpr int kMemoLog2b = 22;
+// Later passes, for sources whose walks outgrew the previous tables: fewer
+// waves with larger tables (the last one holds 2^29 entries, 10.7 GB).
+constexpr int kRetryPasses = 3;
+constexpr int kRetryWaves[kRetryPasses] = {32, 4, 1};
+constexpr int kRetryLog2[kRetryPasses] = {22, 26, 29};
 
 struct MemoArgs {
   uint64_t* keys;
@@ -1314,8 +1025,31 @@ __device__ uint32_t memo_trie_child(MemoTable& m, uint32_t node, uint32_t u) {
   const uint32_t slot = memo_probe(m, key, &found);
   if (found) return static_cast<uint32_t>(wg_load(&m.vals[slot]));
   const uint32_t child = ++m.next_node;
+  if (child >= (1u << 26) - 1) m.overflow = true;  // node ids are 26 bits in hypothesis keys
   memo_put(m, slot, key, child);
   return child;
+}
+
+// Failed-state memo of the complete search: a cycle or path walk has no
+// visited set, so whether it can be finished from vertex v after d hops
+// depends on (v, d) only (the source and start are fixed for the walk).
+// States are remembered while the table has room; the memo only saves work.
+__device__ __forceinline__ uint64_t state_key(uint32_t v, int depth) {
+  return (1ull << 62) | (static_cast<uint64_t>(depth) << 32) | v;
+}
+
+__device__ bool state_failed(const MemoTable& m, uint32_t v, int depth) {
+  bool found;
+  memo_probe(m, state_key(v, depth), &found);
+  return found;
+}
+
+__device__ void state_fail(MemoTable& m, uint32_t v, int depth) {
+  if (m.used >= m.limit) return;
+  bool found;
+  const uint64_t key = state_key(v, depth);
+  const uint32_t slot = memo_probe(m, key, &found);
+  if (!found) memo_put(m, slot, key, 0);
 }
 
 template <typename Word>
@@ -1327,6 +1061,326 @@ __device__ __forceinline__ void atomic_or_word(Word* p, Word bits) {
   } else {
     const uintptr_t a = reinterpret_cast<uintptr_t>(p);
     atomicOr(reinterpret_cast<unsigned int*>(a & ~uintptr_t(3)), static_cast<unsigned int>(bits) << (8 * (a & 3)));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// NLCC: one wavefront per source vertex runs a depth-first walk whose row
+// scans are done 64 slots at a time (ballot, lowest lane first, so the walk
+// visits neighbours in CSR order).  Level i of the walk keeps its vertex and
+// its resume slot in lane i's registers (walks are at most 64 levels).  The
+// search is complete: it decides whether a walk satisfying the constraint
+// exists (see the NLCC notes in the oracle for when this equals the
+// reference's memoised search).
+
+struct RowScan {
+  uint32_t u;
+  uint64_t next;  // slot after the hit
+  bool found;
+};
+
+// First active slot in [from, end) of a row whose neighbour is active and has
+// template vertex q (q < 0: neighbour must equal `target`; `excl`: neighbour
+// must differ from it).  Adds to `slots` the slots a serial scan reads (up to
+// and including the hit).
+template <typename Word>
+__device__ __forceinline__ RowScan scan_row(uint64_t from, uint64_t end, const uint32_t* __restrict__ edges,
+                                            const uint32_t* __restrict__ ea, const Word* __restrict__ cand, int q,
+                                            uint32_t target, uint32_t excl, uint64_t& slots) {
+  RowScan r{0, end, false};
+  const int lane = lane_id();
+  for (uint64_t base = from; base < end; base += 64) {
+    const uint64_t e = base + lane;
+    bool ok = false;
+    uint32_t u = 0;
+    if (e < end && walk_slot(ea, e)) {
+      u = edges[e];
+      if (q < 0) {
+        ok = u == target;
+      } else {
+        Word cu = cand[u];
+        ok = ((cu >> q) & 1) && u != excl;
+      }
+    }
+    uint64_t m = __ballot(ok);
+    if (m) {
+      int first = __builtin_ctzll(m);
+      r.u = rl32(u, first);
+      r.next = base + first + 1;
+      r.found = true;
+      slots += r.next - from;
+      return r;
+    }
+  }
+  if (end > from) slots += end - from;
+  return r;
+}
+
+// Whether row(a) holds an active slot to b, by binary search (rows sorted
+// ascending); `probes` counts the slots read.
+__device__ __forceinline__ bool active_slot(uint32_t a, uint32_t b, const uint64_t* __restrict__ off,
+                                            const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
+                                            uint64_t& probes) {
+  uint64_t lo = off[a], hi = off[a + 1];
+  const uint64_t end = hi;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    ++probes;
+    if (edges[mid] < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  for (; lo < end; ++lo) {
+    ++probes;
+    if (edges[lo] != b) break;
+    if (ea_test(ea, lo)) return true;
+  }
+  return false;
+}
+
+// Last two hops of a cycle on sorted rows: is there a u with an active slot
+// v->u, template vertex q among u's candidates, and an active slot u->s?
+// The shorter of row(v) and row(s) is read 64 slots at a time, each lane
+// binary-searching the other side, so a low-degree source next to a hub
+// never reads the hub's row.
+template <typename Word>
+__device__ bool close_cycle(uint32_t v, uint32_t s, int q, const uint64_t* __restrict__ off,
+                            const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
+                            const Word* __restrict__ cand, uint64_t& slots) {
+  const int lane = lane_id();
+  const uint64_t vb = off[v], ve = off[v + 1], sb = off[s], se = off[s + 1];
+  const bool from_v = (ve - vb) <= (se - sb);
+  const uint64_t b = from_v ? vb : sb, end = from_v ? ve : se;
+  uint64_t probes = 0, read = 0;
+  bool found = false;
+  for (uint64_t base = b; base < end; base += 64) {
+    const uint64_t e = base + lane;
+    bool ok = false;
+    if (e < end) {
+      const uint32_t u = edges[e];
+      if ((cand[u] >> q) & 1) {
+        if (from_v)
+          ok = ea_test(ea, e) && active_slot(u, s, off, edges, ea, probes);
+        else
+          ok = active_slot(v, u, off, edges, ea, probes) && active_slot(u, s, off, edges, ea, probes);
+      }
+    }
+    read += min<uint64_t>(64, end - base);
+    if (__ballot(ok)) {
+      found = true;
+      break;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) probes += __shfl_xor(probes, o, 64);
+  slots += read + probes;
+  return found;
+}
+
+// Cycle constraint idx[0..L-1] from start position j (cc_cpu.tpp:12-70).
+// With sorted rows the last two hops are decided by close_cycle.
+template <typename Word>
+__device__ bool cycle_walk(uint32_t s, uint32_t j, uint32_t L, bool sorted, const uint64_t* __restrict__ off,
+                           const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
+                           const Word* __restrict__ cand, const DevTemplate* __restrict__ t, MemoTable* fm,
+                           uint64_t& slots) {
+  const int lane = lane_id();
+  uint32_t st_v = s;          // lane i: vertex at level i
+  uint64_t st_c = off[s];     // lane i: resume slot at level i
+  int depth = 0;
+  uint32_t v = s;
+  uint64_t cursor = off[s];
+  if (fm) {  // a fresh failed-state memo for this walk
+    fm->epoch = static_cast<uint32_t>(uniform(static_cast<int>(fm->epoch + 1)));
+    fm->used = 0;
+  }
+  while (true) {
+    if (sorted && depth == static_cast<int>(L) - 2) {
+      const int q = static_cast<int>(t->nlcc_vertex[(j + L - 1) % L]);
+      if (close_cycle<Word>(v, s, q, off, edges, ea, cand, slots)) return true;
+      if (fm && depth >= 2) state_fail(*fm, v, depth);
+      depth = uniform(depth - 1);
+      v = rl32(st_v, depth);
+      cursor = rl64(st_c, depth);
+      continue;
+    }
+    if (depth == static_cast<int>(L) - 1) {
+      RowScan r = scan_row<Word>(off[v], off[v + 1], edges, ea, cand, -1, s, 0xFFFFFFFFu, slots);
+      if (r.found) return true;
+      depth = uniform(depth - 1);
+      v = rl32(st_v, depth);
+      cursor = rl64(st_c, depth);
+      continue;
+    }
+    const int q = static_cast<int>(t->nlcc_vertex[(j + depth + 1) % L]);
+    RowScan r = scan_row<Word>(cursor, off[v + 1], edges, ea, cand, q, 0, 0xFFFFFFFFu, slots);
+    if (r.found) {
+      if (fm && depth + 1 >= 2 && depth + 1 <= static_cast<int>(L) - 2 && state_failed(*fm, r.u, depth + 1)) {
+        cursor = r.next;  // known dead end: next neighbour
+        continue;
+      }
+      if (lane == depth) st_c = r.next;
+      depth = uniform(depth + 1);
+      v = r.u;
+      cursor = off[v];
+      if (lane == depth) {
+        st_v = v;
+        st_c = cursor;
+      }
+    } else {
+      if (depth == 0) return false;
+      if (fm && depth >= 2) state_fail(*fm, v, depth);
+      depth = uniform(depth - 1);
+      v = rl32(st_v, depth);
+      cursor = rl64(st_c, depth);
+    }
+  }
+}
+
+// Path constraint (pc_cpu.tpp:11-75): from position 0 forwards or from the
+// last position backwards over L-1 edges, ending on a vertex other than s.
+template <typename Word>
+__device__ bool path_walk(uint32_t s, bool reverse, uint32_t L, const uint64_t* __restrict__ off,
+                          const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
+                          const Word* __restrict__ cand, const DevTemplate* __restrict__ t, MemoTable* fm,
+                          uint64_t& slots) {
+  const int lane = lane_id();
+  uint32_t st_v = s;
+  uint64_t st_c = off[s];
+  int depth = 0;
+  uint32_t v = s;
+  uint64_t cursor = off[s];
+  if (fm) {
+    fm->epoch = static_cast<uint32_t>(uniform(static_cast<int>(fm->epoch + 1)));
+    fm->used = 0;
+  }
+  while (true) {
+    const uint32_t pos = reverse ? (L - 2 - depth) : (depth + 1);
+    const int q = static_cast<int>(t->nlcc_vertex[pos]);
+    const bool last = depth == static_cast<int>(L) - 2;
+    RowScan r = scan_row<Word>(cursor, off[v + 1], edges, ea, cand, q, 0, last ? s : 0xFFFFFFFFu, slots);
+    if (r.found) {
+      if (last) return true;
+      if (fm && depth + 1 >= 2 && state_failed(*fm, r.u, depth + 1)) {
+        cursor = r.next;
+        continue;
+      }
+      if (lane == depth) st_c = r.next;
+      depth = uniform(depth + 1);
+      v = r.u;
+      cursor = off[v];
+      if (lane == depth) {
+        st_v = v;
+        st_c = cursor;
+      }
+    } else {
+      if (depth == 0) return false;
+      if (fm && depth >= 2) state_fail(*fm, v, depth);
+      depth = uniform(depth - 1);
+      v = rl32(st_v, depth);
+      cursor = rl64(st_c, depth);
+    }
+  }
+}
+
+// Walk over VERTEX / CHECK / MOVE_BACK moves (tds_cpu.tpp:13-127 and, with
+// counting and injectivity, enumeration_cpu.tpp:38-118).  Lane p keeps the
+// vertex the walk stands on before move p (st_v) and the resume slot of move
+// p (st_c); lane h keeps history entry h.  Walks have at most 64 moves.
+// Returns the number of completions when `count_all`, else 1 on the first
+// completion.  When a search without counting backtracks through a CHECK,
+// the rest of that row is added to `slots`, as a serial scan would read it
+// looking for a second slot to the same vertex.
+template <typename Word>
+__device__ uint64_t template_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ mv,
+                                  const uint32_t* __restrict__ vx, const uint32_t* __restrict__ mb,
+                                  const uint32_t* __restrict__ hidx, bool count_all,
+                                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+                                  const uint32_t* __restrict__ ea, const Word* __restrict__ cand, uint64_t& slots) {
+  const int lane = lane_id();
+  uint32_t st_v = s;
+  uint64_t st_c = 0;
+  uint32_t hist = s;
+  uint64_t total = 0;
+  int pos = 1;
+  uint32_t cur = s;
+  bool resume = false;  // re-entering a VERTEX move after backtracking
+  if (len <= 1) return 1;
+  auto is_vertex = [&](int p) { return mv[p] == PJ_MOVE_VERTEX_STORE || mv[p] == PJ_MOVE_VERTEX_NO_STORE; };
+  // Back to the most recent VERTEX move; false when none is left.
+  auto backtrack = [&]() -> bool {
+    pos = uniform(pos - 1);
+    while (pos >= 1 && !is_vertex(pos)) {
+      if (!count_all && mv[pos] == PJ_MOVE_CHECK) {
+        const uint32_t at = rl32(st_v, pos);
+        slots += off[at + 1] - rl64(st_c, pos);
+      }
+      pos = uniform(pos - 1);
+    }
+    if (pos < 1) return false;
+    cur = rl32(st_v, pos);
+    resume = true;
+    return true;
+  };
+  while (true) {
+    if (pos == static_cast<int>(len)) {
+      ++total;
+      if (!count_all) return total;
+      if (!backtrack()) return total;
+      continue;
+    }
+    const uint8_t m = mv[pos];
+    if (m == PJ_MOVE_VERTEX_STORE || m == PJ_MOVE_VERTEX_NO_STORE) {
+      const uint64_t from = resume ? rl64(st_c, pos) : off[cur];
+      resume = false;
+      const int q = static_cast<int>(vx[pos]);
+      const uint32_t h = hidx[pos];
+      const uint64_t end = off[cur + 1];
+      RowScan r{0, end, false};
+      for (uint64_t base = from; base < end; base += 64) {
+        const uint64_t e = base + lane;
+        bool ok = false;
+        uint32_t u = 0;
+        if (e < end && ea_test(ea, e)) {
+          u = edges[e];
+          ok = (cand[u] >> q) & 1;
+        }
+        if (count_all)  // injective embeddings; every lane active here
+          for (uint32_t k = 0; k < h; ++k) ok = ok && rl32(hist, static_cast<int>(k)) != u;
+        uint64_t bm = __ballot(ok);
+        if (bm) {
+          int first = __builtin_ctzll(bm);
+          r.u = rl32(u, first);
+          r.next = base + first + 1;
+          r.found = true;
+          break;
+        }
+      }
+      slots += (r.found ? r.next : end) - from;
+      if (r.found) {
+        if (lane == pos) st_c = r.next;
+        if (lane == static_cast<int>(h)) hist = r.u;
+        cur = r.u;
+        pos = uniform(pos + 1);
+        if (lane == pos) st_v = cur;
+        continue;
+      }
+    } else if (m == PJ_MOVE_CHECK) {
+      const uint32_t target = rl32(hist, static_cast<int>(mb[pos]));
+      RowScan r = scan_row<Word>(off[cur], off[cur + 1], edges, ea, cand, -1, target, 0xFFFFFFFFu, slots);
+      if (r.found) {
+        if (lane == pos) st_c = r.next;
+        pos = uniform(pos + 1);
+        if (lane == pos) st_v = cur;
+        continue;
+      }
+    } else {  // MOVE_BACK
+      cur = rl32(hist, static_cast<int>(mb[pos]));
+      pos = uniform(pos + 1);
+      if (lane == pos) st_v = cur;
+      continue;
+    }
+    if (!backtrack()) return total;
   }
 }
 
@@ -1383,7 +1437,7 @@ __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ m
         const uint64_t e = base + lane;
         bool ok = false;
         uint32_t w = 0;
-        if (e < end && ea_test(ea, e)) {
+        if (e < end && walk_slot(ea, e)) {
           w = edges[e];
           ok = (cand[w] >> y) & 1;
         }
@@ -1450,6 +1504,104 @@ __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ m
   }
 }
 
+// ---------------------------------------------------------------------------
+// Active view for the memoised walks: per active vertex, in row order, the
+// slots with the edge-active bit set whose neighbour is active.  Those walks
+// only follow such slots and only check slots to vertices they have reached
+// (active), so on the view they make the same moves in the same order while
+// skipping the dead slots of rows they rescan (at large scales the active
+// vertices are a few per cent of the graph, and hub rows are rescanned at
+// every backtrack).  The memo keys are vertex pairs, unaffected by the copy.
+
+template <typename Word, int G>
+__global__ void __launch_bounds__(kBlock) k_view_count(
+    const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea, const Word* __restrict__ cand,
+    uint64_t* __restrict__ counts) {
+  const int lane = lane_id();
+  const int gl = lane & (G - 1);
+  const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << (lane & ~(G - 1)));
+  const uint32_t n = *list_count;
+  const uint64_t groups_per_grid = (uint64_t)gridDim.x * (blockDim.x / G);
+  const uint64_t span = ((n + groups_per_grid - 1) / groups_per_grid) * groups_per_grid;
+  for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G; g < span; g += groups_per_grid) {
+    if (g >= n) continue;  // group-uniform
+    const uint32_t v = list[g];
+    const uint64_t beg = off[v], end = off[v + 1];
+    uint64_t kept = 0;
+    for (uint64_t base = beg; base < end; base += G) {
+      const uint64_t e = base + gl;
+      const bool keep = e < end && ea_test(ea, e) && cand[edges[e]] != 0;
+      kept += __popcll(__ballot(keep) & gmask);
+    }
+    if (gl == 0) counts[v] = kept;
+  }
+}
+
+template <typename Word, int G>
+__global__ void __launch_bounds__(kBlock) k_view_fill(
+    const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea, const Word* __restrict__ cand,
+    const uint64_t* __restrict__ voff, uint32_t* __restrict__ vedges) {
+  const int lane = lane_id();
+  const int gl = lane & (G - 1);
+  const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << (lane & ~(G - 1)));
+  const uint64_t below = ((1ull << lane) - 1) & gmask;
+  const uint32_t n = *list_count;
+  const uint64_t groups_per_grid = (uint64_t)gridDim.x * (blockDim.x / G);
+  const uint64_t span = ((n + groups_per_grid - 1) / groups_per_grid) * groups_per_grid;
+  for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G; g < span; g += groups_per_grid) {
+    if (g >= n) continue;
+    const uint32_t v = list[g];
+    const uint64_t beg = off[v], end = off[v + 1];
+    uint64_t out = voff[v];
+    for (uint64_t base = beg; base < end; base += G) {
+      const uint64_t e = base + gl;
+      uint32_t u = 0;
+      bool keep = false;
+      if (e < end && ea_test(ea, e)) {
+        u = edges[e];
+        keep = cand[u] != 0;
+      }
+      const uint64_t m = __ballot(keep) & gmask;
+      if (keep) vedges[out + __popcll(m & below)] = u;
+      out += __popcll(m);
+    }
+  }
+}
+
+// Hub rows: one wave per hub walks the row in order (count, then fill).
+template <typename Word>
+__global__ void __launch_bounds__(kBlock) k_view_hubs(const uint32_t* __restrict__ hubs, uint32_t nhubs,
+                                                      const uint64_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ edges,
+                                                      const uint32_t* __restrict__ ea, const Word* __restrict__ cand,
+                                                      uint64_t* __restrict__ counts, const uint64_t* __restrict__ voff,
+                                                      uint32_t* __restrict__ vedges) {
+  const int lane = lane_id();
+  const uint64_t below = (1ull << lane) - 1;
+  const uint32_t waves = gridDim.x * (blockDim.x / 64);
+  for (uint32_t h = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); h < nhubs; h += waves) {
+    const uint32_t v = hubs[h];
+    if (!cand[v]) continue;  // wave-uniform
+    const uint64_t beg = off[v], end = off[v + 1];
+    uint64_t out = voff ? voff[v] : 0;
+    for (uint64_t base = beg; base < end; base += 64) {
+      const uint64_t e = base + lane;
+      uint32_t u = 0;
+      bool keep = false;
+      if (e < end && ea_test(ea, e)) {
+        u = edges[e];
+        keep = cand[u] != 0;
+      }
+      const uint64_t m = __ballot(keep);
+      if (vedges && keep) vedges[out + __popcll(m & below)] = u;
+      out += __popcll(m);
+    }
+    if (!voff && lane == 0) counts[v] = out;
+  }
+}
+
 // One wave per source in the task list (active vertices); failing
 // (source, template vertex) pairs go to toUnmatch and mark the source
 // modified (cc_cpu.tpp:94-129, pc_cpu.tpp:95-147, tds_cpu.tpp:153-188).
@@ -1469,7 +1621,9 @@ __global__ void __launch_bounds__(kBlock) k_nlcc(int kind, const uint32_t* __res
   const bool memo = t->nlcc_memo != 0 && kind != 3;
   MemoTable m{};
   Word* already = static_cast<Word*>(ma.already);
-  if (memo) {
+  const bool tables = ma.keys != nullptr;
+  MemoTable* fm = tables && !memo && kind <= 1 ? &m : nullptr;  // failed-state memo of long walks
+  if (tables) {
     const uint64_t base = static_cast<uint64_t>(wave_id) << ma.log2;
     m.keys = ma.keys + base;
     m.vals = ma.vals + base;
@@ -1514,12 +1668,14 @@ __global__ void __launch_bounds__(kBlock) k_nlcc(int kind, const uint32_t* __res
       for (uint32_t j = 0; j < L; ++j) {
         const uint32_t q = t->nlcc_vertex[j];
         if (!((cs >> q) & 1)) continue;
-        if (!cycle_walk<Word>(s, j, L, t->rows_sorted != 0, off, edges, ea, cand, t, slots)) fail |= Word(1) << q;
+        if (!cycle_walk<Word>(s, j, L, t->rows_sorted != 0, off, edges, ea, cand, t, fm, slots)) fail |= Word(1) << q;
       }
     } else if (kind == 1) {
       const uint32_t q0 = t->nlcc_vertex[0], q1 = t->nlcc_vertex[L - 1];
-      if (((cs >> q0) & 1) && !path_walk<Word>(s, false, L, off, edges, ea, cand, t, slots)) fail |= Word(1) << q0;
-      if (((cs >> q1) & 1) && !path_walk<Word>(s, true, L, off, edges, ea, cand, t, slots)) fail |= Word(1) << q1;
+      if (((cs >> q0) & 1) && !path_walk<Word>(s, false, L, off, edges, ea, cand, t, fm, slots))
+        fail |= Word(1) << q0;
+      if (((cs >> q1) & 1) && !path_walk<Word>(s, true, L, off, edges, ea, cand, t, fm, slots))
+        fail |= Word(1) << q1;
     } else if (kind == 2) {
       for (uint32_t k = 0; k < t->walk_count; ++k) {
         const uint32_t x = t->walk_root[k];
@@ -1543,7 +1699,7 @@ __global__ void __launch_bounds__(kBlock) k_nlcc(int kind, const uint32_t* __res
       set_flag_atomic(flags, s, kModified);
     }
   }
-  if (memo && lane_id() == 0) ma.wave_epoch[wave_id] = m.epoch;
+  if (tables && lane_id() == 0) ma.wave_epoch[wave_id] = m.epoch;
   if (lane_id() == 0) {
     const int slot = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) & (kCounterSlots - 1);
     if (enumerated) atomicAdd(&counters[slot * kCounterStride + C_ENUM], static_cast<unsigned long long>(enumerated));
@@ -1608,9 +1764,17 @@ struct pj_ctx {
   bool dirty_enabled = true;
   bool rescan_all = false;  // every rescan mark is set (phase 1 need not push)
   bool memo_walks = false;  // run every cycle / path step as memoised walks (PJ_MEMO_WALKS=1)
+  int memo_log2 = 0;  // first-pass table entries per wave, log2 (PJ_MEMO_LOG2, 6..20; 0: by graph size)
   // memoised walks: per-wave tables (two passes), alreadyMatched, overflow list
-  MemoArgs memo1{}, memo2{};
+  MemoArgs memo1{}, memo_retry[kRetryPasses]{};
   void* d_already = nullptr;  // Word[V]
+  // active view for the memoised walks (k_view_*)
+  uint64_t* d_view_count = nullptr;  // V + 1
+  uint64_t* d_view_off = nullptr;    // V + 1
+  uint32_t* d_view_edges = nullptr;
+  uint64_t view_cap = 0;
+  void* d_scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
   uint32_t* d_ovf = nullptr;  // [0] first-pass overflows, [1] second-pass overflows, then the list
   uint32_t* d_bins = nullptr;        // kBins * V
   uint32_t* d_bin_count = nullptr;   // kBins
@@ -1691,10 +1855,14 @@ void free_graph(pj_ctx* ctx) {
                   ctx->d_hubs,
                   ctx->d_chunk_hub, ctx->d_chunk_start, ctx->d_hub_count, ctx->d_hub_elim, ctx->d_nhubs, ctx->d_hub_hit,
                   ctx->d_last_scan, ctx->d_hub_new, ctx->memo1.keys, ctx->memo1.vals, ctx->memo1.epochs,
-                  ctx->memo1.wave_epoch, ctx->memo2.keys, ctx->memo2.vals, ctx->memo2.epochs,
-                  ctx->memo2.wave_epoch, ctx->d_already, ctx->d_ovf};
+                  ctx->memo1.wave_epoch, ctx->d_already, ctx->d_ovf, ctx->d_view_count, ctx->d_view_off,
+                  ctx->d_view_edges, ctx->d_scan_tmp};
   for (void* b : bufs)
     if (b) hipFree(b);
+  for (auto& m : ctx->memo_retry)
+    for (void* b : {static_cast<void*>(m.keys), static_cast<void*>(m.vals), static_cast<void*>(m.epochs),
+                    static_cast<void*>(m.wave_epoch)})
+      if (b) hipFree(b);
   ctx->d_off = nullptr;
   ctx->d_edges = nullptr;
   ctx->d_labels = nullptr;
@@ -1710,9 +1878,14 @@ void free_graph(pj_ctx* ctx) {
   ctx->d_hub_hit = nullptr;
   ctx->d_last_scan = ctx->d_hub_new = nullptr;
   ctx->memo1 = MemoArgs{};
-  ctx->memo2 = MemoArgs{};
+  for (auto& m : ctx->memo_retry) m = MemoArgs{};
   ctx->d_already = nullptr;
   ctx->d_ovf = nullptr;
+  ctx->d_view_count = ctx->d_view_off = nullptr;
+  ctx->d_view_edges = nullptr;
+  ctx->d_scan_tmp = nullptr;
+  ctx->view_cap = 0;
+  ctx->scan_tmp_bytes = 0;
   ctx->V = 0;
   ctx->E = 0;
 }
@@ -1985,7 +2158,16 @@ int prepare_memo(pj_ctx* ctx) {
     if (dev_alloc(ctx, &a, V + 1) || dev_alloc(ctx, &ctx->d_ovf, 2 * V + 2)) return -1;
     ctx->d_already = a;
   }
-  if (alloc_memo(ctx, &ctx->memo1, kMemoWaves, kMemoLog2)) return -1;
+  // Walks on larger graphs reach more states: 2^14 entries per wave up to
+  // 2^23 vertices, doubling per doubling of V up to 2^18 (2048 waves x 2^18
+  // x 20 B = 10.7 GB at 2^27 vertices).
+  int log2 = ctx->memo_log2;
+  if (!log2) {
+    int bits = 0;
+    while (bits < 63 && (1ull << bits) < V) ++bits;
+    log2 = std::min(18, std::max(kMemoLog2, bits - 9));
+  }
+  if (alloc_memo(ctx, &ctx->memo1, kMemoWaves, log2)) return -1;
   ctx->memo1.ovf_count = ctx->d_ovf;
   ctx->memo1.ovf_list = ctx->d_ovf + 2;
   const uint64_t words = (V * ctx->word_bytes + 3) / 4;
@@ -1996,47 +2178,129 @@ int prepare_memo(pj_ctx* ctx) {
   return 0;
 }
 
+// Builds the active view (k_view_*) from the mode 1 bins (every active
+// vertex) and the hub list; the edge array grows to what a view needs.
+template <typename Word>
+int build_view(pj_ctx* ctx) {
+  const uint64_t V = ctx->V;
+  if (!ctx->d_view_off) {
+    if (dev_alloc(ctx, &ctx->d_view_count, V + 1) || dev_alloc(ctx, &ctx->d_view_off, V + 1)) return -1;
+    PJ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, ctx->scan_tmp_bytes, ctx->d_view_count, ctx->d_view_off, V + 1,
+                                            ctx->stream));
+    PJ_HIP(hipMalloc(&ctx->d_scan_tmp, ctx->scan_tmp_bytes));
+  }
+  const Word* cand = static_cast<const Word*>(ctx->d_cand);
+  const int g = ctx->cus * 8;
+  const uint32_t hub_blocks = std::max<uint32_t>(1, std::min<uint32_t>((ctx->nhubs + 3) / 4, ctx->cus * 4));
+  begin_launch(ctx, K_VIEW);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ctx, V + 1)), dim3(kBlock), 0, ctx->stream,
+                     reinterpret_cast<unsigned long long*>(ctx->d_view_count), V + 1, 0ull);
+  hipLaunchKernelGGL((k_view_count<Word, 4>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins, ctx->d_bin_count + 0,
+                     ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_count);
+  hipLaunchKernelGGL((k_view_count<Word, 16>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
+                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_count);
+  hipLaunchKernelGGL((k_view_count<Word, 64>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
+                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_count);
+  if (ctx->nhubs)
+    hipLaunchKernelGGL(k_view_hubs<Word>, dim3(hub_blocks), dim3(kBlock), 0, ctx->stream, ctx->d_hubs, ctx->nhubs,
+                       ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_count, nullptr, nullptr);
+  PJ_HIP(hipcub::DeviceScan::ExclusiveSum(ctx->d_scan_tmp, ctx->scan_tmp_bytes, ctx->d_view_count, ctx->d_view_off,
+                                          V + 1, ctx->stream));
+  uint64_t total = 0;
+  PJ_HIP(hipMemcpyAsync(&total, ctx->d_view_off + V, 8, hipMemcpyDeviceToHost, ctx->stream));
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  if (total + 1 > ctx->view_cap) {
+    if (ctx->d_view_edges) hipFree(ctx->d_view_edges);
+    ctx->d_view_edges = nullptr;
+    const uint64_t cap = std::min<uint64_t>(ctx->E + 1, total + total / 4 + 4096);
+    if (dev_alloc(ctx, &ctx->d_view_edges, cap)) return -1;
+    ctx->view_cap = cap;
+  }
+  hipLaunchKernelGGL((k_view_fill<Word, 4>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins, ctx->d_bin_count + 0,
+                     ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_off, ctx->d_view_edges);
+  hipLaunchKernelGGL((k_view_fill<Word, 16>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
+                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_off,
+                     ctx->d_view_edges);
+  hipLaunchKernelGGL((k_view_fill<Word, 64>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
+                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_off,
+                     ctx->d_view_edges);
+  if (ctx->nhubs)
+    hipLaunchKernelGGL(k_view_hubs<Word>, dim3(hub_blocks), dim3(kBlock), 0, ctx->stream, ctx->d_hubs, ctx->nhubs,
+                       ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_count, ctx->d_view_off,
+                       ctx->d_view_edges);
+  end_launch(ctx);
+  PJ_HIP(hipGetLastError());
+  return 0;
+}
+
+// Complete cycle walks of 5+ vertices and path walks of 4+ vertices keep a
+// failed-state memo (state_fail): several partial walks reach the same
+// (vertex, hop) there.
+bool long_walk(const pj_ctx* ctx, int kind) {
+  return (kind == 0 && ctx->h_t.nlcc_len >= 5) || (kind == 1 && ctx->h_t.nlcc_len >= 4);
+}
+
 template <typename Word>
 int launch_nlcc(pj_ctx* ctx, int kind) {
-  launch_bins<Word>(ctx, kind == 1 ? 2 : 1);
   const bool memo = ctx->h_t.nlcc_memo && kind != 3;
-  if (memo && prepare_memo(ctx)) return -1;
-  const MemoArgs ma = memo ? ctx->memo1 : MemoArgs{};
+  const bool tables = memo || (!ctx->h_t.nlcc_memo && long_walk(ctx, kind));
+  launch_bins<Word>(ctx, kind == 1 && !memo ? 2 : 1);
+  if (tables && prepare_memo(ctx)) return -1;
+  if (memo) {
+    if (build_view<Word>(ctx)) return -1;
+    if (kind == 1) launch_bins<Word>(ctx, 2);  // path sources only
+  }
+  const MemoArgs ma = tables ? ctx->memo1 : MemoArgs{};
+  // memoised walks run on the active view (every slot active: no bitset)
+  const uint64_t* off = memo ? ctx->d_view_off : ctx->d_off;
+  const uint32_t* edges = memo ? ctx->d_view_edges : ctx->d_edges;
+  const uint32_t* ea = memo ? nullptr : ctx->d_ea;
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   const uint64_t V = ctx->V;
-  const int g = memo ? kMemoWaves / (kBlock / 64) : ctx->cus * 8;
+  const int g = tables ? kMemoWaves / (kBlock / 64) : ctx->cus * 8;
   // Every bin holds active vertices; hubs are run from their own list.
   for (int b = 0; b < kBins; ++b) {
     unsigned long long* c = begin_launch(ctx, K_NLCC);
     hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, kind, ctx->d_bins + b * V,
-                       ctx->d_bin_count + b, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t,
-                       ma, c);
+                       ctx->d_bin_count + b, off, edges, ea, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
     end_launch(ctx);
   }
   return 0;
 }
 
-// Sources whose walks outgrew the first-pass tables run again, one wave
-// each, with tables of 2^kMemoLog2b entries.
+// Sources whose walks outgrew their tables run again with larger ones
+// (kRetryWaves / kRetryLog2), the overflow lists alternating between the two
+// halves of d_ovf.  A source's results are written only by the pass that
+// completes it.
 template <typename Word>
 int launch_nlcc_retry(pj_ctx* ctx, int kind) {
-  uint32_t n[2] = {0, 0};
-  PJ_HIP(hipMemcpyAsync(n, ctx->d_ovf, 8, hipMemcpyDeviceToHost, ctx->stream));
+  const uint64_t V = ctx->V;
+  for (int r = 0; r < kRetryPasses; ++r) {
+    uint32_t* in_count = ctx->d_ovf + (r & 1);
+    uint32_t* out_count = ctx->d_ovf + ((r + 1) & 1);
+    uint32_t* in_list = ctx->d_ovf + 2 + (r & 1) * V;
+    uint32_t* out_list = ctx->d_ovf + 2 + ((r + 1) & 1) * V;
+    uint32_t n = 0;
+    PJ_HIP(hipMemcpyAsync(&n, in_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PJ_HIP(hipStreamSynchronize(ctx->stream));
+    if (!n) return 0;
+    MemoArgs& m = ctx->memo_retry[r];
+    if (alloc_memo(ctx, &m, kRetryWaves[r], kRetryLog2[r])) return -1;
+    m.ovf_count = out_count;
+    m.ovf_list = out_list;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, out_count, static_cast<uint64_t>(1), 0u);
+    unsigned long long* c = begin_launch(ctx, K_NLCC);
+    hipLaunchKernelGGL(k_nlcc<Word>, dim3(kRetryWaves[r]), dim3(64), 0, ctx->stream, kind, in_list, in_count,
+                       ctx->d_view_off, ctx->d_view_edges, nullptr, static_cast<const Word*>(ctx->d_cand),
+                       static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->d_t, m, c);
+    end_launch(ctx);
+  }
+  uint32_t n = 0;
+  PJ_HIP(hipMemcpyAsync(&n, ctx->d_ovf + (kRetryPasses & 1), 4, hipMemcpyDeviceToHost, ctx->stream));
   PJ_HIP(hipStreamSynchronize(ctx->stream));
-  if (!n[0]) return 0;
-  if (alloc_memo(ctx, &ctx->memo2, kMemoWaves2, kMemoLog2b)) return -1;
-  ctx->memo2.ovf_count = ctx->d_ovf + 1;
-  ctx->memo2.ovf_list = ctx->d_ovf + 2 + ctx->V;
-  unsigned long long* c = begin_launch(ctx, K_NLCC);
-  hipLaunchKernelGGL(k_nlcc<Word>, dim3(kMemoWaves2), dim3(64), 0, ctx->stream, kind, ctx->d_ovf + 2, ctx->d_ovf,
-                     ctx->d_off, ctx->d_edges, ctx->d_ea, static_cast<const Word*>(ctx->d_cand),
-                     static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->d_t, ctx->memo2, c);
-  end_launch(ctx);
-  PJ_HIP(hipMemcpyAsync(n, ctx->d_ovf, 8, hipMemcpyDeviceToHost, ctx->stream));
-  PJ_HIP(hipStreamSynchronize(ctx->stream));
-  if (n[1]) {
-    ctx->err = "memoised walk of a source exceeds 2^22 memo entries";
+  if (n) {
+    ctx->err = "memoised walk of a source exceeds 2^29 memo entries";
     return -1;
   }
   return 0;
@@ -2051,11 +2315,13 @@ void launch_nlcc_hubs(pj_ctx* ctx, int kind, uint32_t* d_hub_n) {
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   const bool memo = ctx->h_t.nlcc_memo && kind != 3;
-  const MemoArgs ma = memo ? ctx->memo1 : MemoArgs{};
-  const uint32_t g = std::min<uint32_t>(ctx->nhubs, memo ? kMemoWaves : ctx->cus * 8);
+  const bool tables = memo || (!ctx->h_t.nlcc_memo && long_walk(ctx, kind));
+  const MemoArgs ma = tables ? ctx->memo1 : MemoArgs{};
+  const uint32_t g = std::min<uint32_t>(ctx->nhubs, tables ? kMemoWaves : ctx->cus * 8);
   unsigned long long* c = begin_launch(ctx, K_NLCC_HUB);
-  hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(64), 0, ctx->stream, kind, ctx->d_hubs, d_hub_n, ctx->d_off,
-                     ctx->d_edges, ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
+  hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(64), 0, ctx->stream, kind, ctx->d_hubs, d_hub_n,
+                     memo ? ctx->d_view_off : ctx->d_off, memo ? ctx->d_view_edges : ctx->d_edges,
+                     memo ? nullptr : ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
   end_launch(ctx);
 }
 
@@ -2346,6 +2612,9 @@ int pj_create(pj_ctx** out, int device) {
   // PJ_MEMO_WALKS=1 runs cycle and path steps as the reference's memoised
   // search even where the complete search is known to give the same result.
   if (const char* mw = getenv("PJ_MEMO_WALKS")) ctx->memo_walks = strcmp(mw, "0") != 0;
+  // PJ_MEMO_LOG2 sizes the first-pass memo tables (small values send most
+  // sources through the retry passes; the results are the same).
+  if (const char* ml = getenv("PJ_MEMO_LOG2")) ctx->memo_log2 = std::min(20, std::max(6, atoi(ml)));
   *out = ctx;
   return 0;
 }

@@ -113,7 +113,7 @@ void reset_modified(MultipleLabelDeviceState* s) {
 
 int run_step_machine(MultipleLabelDeviceState* state, const ConstraintSet& cs,
                      const std::function<void(const StepRecord&)>& on_step, double* total_ms,
-                     bool stop_before_enumeration) {
+                     bool stop_before_enumeration, bool stop_before_tds) {
   MultipleLabelHip::Lcc0Type lcc0;
   MultipleLabelHip::LccType lcc;
   MultipleLabelHip::CcType cc;
@@ -148,6 +148,7 @@ int run_step_machine(MultipleLabelDeviceState* state, const ConstraintSet& cs,
   machine.getNextStep(results, &step, &name);
   while (!finished && iteration < 1000) {
     if (stop_before_enumeration && step == MultipleLabelStep::E_ENUMERATION) break;
+    if (stop_before_tds && step == MultipleLabelStep::E_TDS) break;
     switch (step) {
       case MultipleLabelStep::E_CC:
         cc.resetState(state);

@@ -144,10 +144,11 @@ struct StepRecord {
 
 // Runs the step machine (multiple_label_cpu.cc:113-185) on a state whose
 // context holds the graph and template.  Calls `on_step` after every step.
-// With stop_before_enumeration the machine ends at the pruning fixed point.
+// With stop_before_enumeration the machine ends at the pruning fixed point;
+// with stop_before_tds it ends when the first TDS step is due.
 int run_step_machine(MultipleLabelDeviceState* state, const ConstraintSet& cs,
                      const std::function<void(const StepRecord&)>& on_step, double* total_ms,
-                     bool stop_before_enumeration = false);
+                     bool stop_before_enumeration = false, bool stop_before_tds = false);
 
 struct Options {
   std::string graph, graphBin, vertexData, edgeData, patternDir, outputDir;

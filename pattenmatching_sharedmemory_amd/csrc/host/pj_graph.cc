@@ -375,6 +375,7 @@ void rmat_generate(const RmatParams& p, Graph* g) {
     *d = dst;
   };
 
+  const bool trace = std::getenv("PJ_TRACE") != nullptr;  // progress of large generations on stderr
   std::vector<uint64_t> deg(static_cast<size_t>(V) + 1, 0);
 #pragma omp parallel for num_threads(threads) schedule(static)
   for (int64_t i = 0; i < static_cast<int64_t>(pairs); ++i) {
@@ -384,6 +385,7 @@ void rmat_generate(const RmatParams& p, Graph* g) {
     __atomic_fetch_add(&deg[s], 1ull, __ATOMIC_RELAXED);
     __atomic_fetch_add(&deg[d], 1ull, __ATOMIC_RELAXED);
   }
+  if (trace) fprintf(stderr, "[rmat] scale %d: degrees of %llu pairs counted\n", p.scale, (unsigned long long)pairs);
   std::vector<uint64_t> off(static_cast<size_t>(V) + 1, 0);
   for (uint32_t v = 0; v < V; ++v) off[v + 1] = off[v] + deg[v];
   std::vector<uint32_t> buf(off[V]);
@@ -396,6 +398,7 @@ void rmat_generate(const RmatParams& p, Graph* g) {
     buf[__atomic_fetch_add(&cur[s], 1ull, __ATOMIC_RELAXED)] = d;
     buf[__atomic_fetch_add(&cur[d], 1ull, __ATOMIC_RELAXED)] = s;
   }
+  if (trace) fprintf(stderr, "[rmat] scale %d: %llu slots placed\n", p.scale, (unsigned long long)off[V]);
   std::vector<uint64_t> ndeg(V, 0);
 #pragma omp parallel for num_threads(threads) schedule(dynamic, 256)
   for (int64_t v = 0; v < static_cast<int64_t>(V); ++v) {
@@ -404,6 +407,7 @@ void rmat_generate(const RmatParams& p, Graph* g) {
     std::sort(b, e);
     ndeg[v] = static_cast<uint64_t>(std::unique(b, e) - b);
   }
+  if (trace) fprintf(stderr, "[rmat] scale %d: rows sorted and deduplicated\n", p.scale);
   Graph out;
   out.vertex_count = V;
   out.directed = false;

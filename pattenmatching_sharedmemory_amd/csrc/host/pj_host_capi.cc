@@ -209,7 +209,7 @@ int pjh_run_pipeline(pj_ctx* ctx, const pjh_constraints* c, int flags, pjh_row* 
                             r.deviceMs};
         ++n;
       },
-      total_ms, (flags & PJH_STOP_BEFORE_ENUMERATION) != 0);
+      total_ms, (flags & PJH_STOP_BEFORE_ENUMERATION) != 0, (flags & PJH_STOP_BEFORE_TDS) != 0);
   return rc < 0 ? -1 : n;
 }
 

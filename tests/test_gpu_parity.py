@@ -308,3 +308,29 @@ def test_schedule_dependent_steps_stay_within_the_reference_outcomes(pj, orc, ca
         checked += 1
     assert checked
     pr.close()
+
+
+@pytest.mark.parametrize("scale,seed", [(12, 7), (11, 4)])
+def test_memo_retry_passes(pj, orc, scale, seed, monkeypatch):
+    """With 64-entry first-pass memo tables most TDS sources outgrow them and
+    are rerun by the retry passes; the results stay the reference's."""
+    monkeypatch.setenv("PJ_MEMO_LOG2", "6")
+    g = pj.Graph.rmat(scale, 8, seed=seed).labels_zipf(8, 1.0, seed=seed + 1)
+    p = pj.Graph.pattern(8, TDS_EDGES, [0, 1, 2, 3, 4, 5, 6, 7])
+    _compare_pipeline(pj, orc, g, p, f"TDS retry s{scale} seed {seed}", enumerate_matches=False)
+
+
+@pytest.mark.parametrize("case", ["path of 5", "cycle of 6", "cycle of 5 + chord"])
+def test_long_walks_with_failed_state_memo(pj, orc, case):
+    """Complete cycle walks of 5+ vertices and path walks of 4+ vertices skip
+    (vertex, hop) states already known to fail; the results are unchanged."""
+    g = pj.Graph.rmat(14, 16, seed=11).labels_uniform(6, seed=12)
+    if case == "path of 5":
+        p = pj.Graph.pattern(5, [(0, 1), (1, 2), (2, 3), (3, 4)], [5, 1, 2, 3, 5])
+    elif case == "cycle of 6":
+        p = pj.Graph.pattern(6, [(i, (i + 1) % 6) for i in range(6)], [0, 1, 2, 3, 4, 5])
+    else:
+        p = pj.Graph.pattern(5, [(i, (i + 1) % 5) for i in range(5)] + [(0, 2)], [0, 1, 2, 3, 4])
+    c = pj.Constraints(p)
+    assert max([len(x) for x in c.cycles()] + [len(x) for x in c.paths()]) >= 5
+    _compare_pipeline(pj, orc, g, p, case, enumerate_matches=False)
