@@ -206,12 +206,123 @@ def cpu_baseline(scale: int, threads: int, name: str = "plain", run_tds: bool = 
                                 "trace_equal": trace_equal}}
 
 
+QUERY_FORMS = {
+    # batched form: templates of 3-10 vertices, 0-3 extra edges, on the plain graph
+    "batch": {"scale": 22, "min_vertices": 3, "max_vertices": 10, "max_extra": 3, "queries": 256, "seed": 4242},
+    # served form: templates of 3-12 vertices, 0-4 extra edges, Poisson arrivals
+    "served": {"scale": 24, "min_vertices": 3, "max_vertices": 12, "max_extra": 4, "queries": 200, "seed": 777},
+}
+# Templates whose longest cycle constraint exceeds this are redrawn: the
+# reference's cycle check is a per-source walk search, and 4-cycles through
+# the hubs of a scale-22 R-MAT graph with frequent labels make single
+# queries run for minutes (up to 10^12 slots examined per step).
+MAX_CYCLE = 3
+
+
+def query_templates(form: dict, n: int):
+    """n random connected templates with distinct Zipf-drawn labels whose
+    cycle constraints have at most MAX_CYCLE vertices."""
+    from pattenmatching_sharedmemory_amd import Constraints, Graph
+    from pattenmatching_sharedmemory_amd.templates import longest_cycle, random_templates
+
+    out, specs, seed = [], [], form["seed"]
+    while len(out) < n:
+        for k, e, l in random_templates(4 * n, seed, form["min_vertices"], form["max_vertices"], form["max_extra"],
+                                        labels=32, zipf=1.2, distinct_labels=True):
+            c = Constraints(Graph.pattern(k, e, l))
+            if longest_cycle(c) <= MAX_CYCLE:
+                out.append(c)
+                specs.append((k, len(e)))
+                if len(out) == n:
+                    break
+        seed += 1
+    return out, specs
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, int(round(q / 100.0 * (len(xs) - 1))))] if xs else None
+
+
+def bench_queries(args) -> int:
+    """Batched and served forms: many templates over one resident graph,
+    solved to the pruning fixed point before TDS by worker contexts that
+    share the graph (one stream and host thread each)."""
+    import numpy as np
+
+    from pattenmatching_sharedmemory_amd import Graph, Pruner
+
+    form = QUERY_FORMS[args.workload]
+    scale = args.scale or form["scale"]
+    n = args.queries or form["queries"]
+    t0 = time.time()
+    g = Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=GRAPH_SEED).labels_zipf(32, 1.2, seed=LABEL_SEED)
+    log(f"[bench] graph scale {scale}: V={g.vertex_count} E={g.edge_count} built in {time.time() - t0:.1f}s")
+    cs, specs = query_templates(form, n)
+    owner = Pruner(g, device=0)
+    owner.run_queries(cs[: min(8, n)], workers=1, run_tds=False)  # warm the contexts' allocations
+    seq, seq_ms = owner.run_queries(cs, workers=1, run_tds=False)
+    log(f"[bench] one at a time: {n} queries in {seq_ms:.0f} ms ({n / seq_ms * 1e3:.1f} q/s)")
+    result = {"metric": METRIC, "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+              "dtype": "u8", "data": "synthetic", "steps": n, "warmup": 1}
+    service = [r.end_ms - r.start_ms for r in seq]
+    cfg = {"workload": f"{args.workload} form: {n} random connected templates ({form['min_vertices']}-"
+                       f"{form['max_vertices']} vertices, spanning tree + 0-{form['max_extra']} extra edges, "
+                       f"distinct labels drawn from Zipf(1.2) over 32, cycles of at most {MAX_CYCLE} vertices) on "
+                       f"R-MAT scale {scale} EF16, each solved to the pruning fixed point before TDS",
+           "vertices": g.vertex_count, "directed_slots": g.edge_count, "workers": args.workers,
+           "service_ms_p50": pct(service, 50), "service_ms_p99": pct(service, 99),
+           "template_vertices_mean": float(np.mean([k for k, _ in specs]))}
+    if args.workload == "batch":
+        bat, bat_ms = owner.run_queries(cs, workers=args.workers, run_tds=False)
+        equal = sum(a.result() == b.result() for a, b in zip(seq, bat))
+        log(f"[bench] batched ({args.workers} workers): {n} queries in {bat_ms:.0f} ms "
+            f"({n / bat_ms * 1e3:.1f} q/s); {equal}/{n} results equal to one-at-a-time")
+        result.update({"value": n / bat_ms * 1e3, "unit": "queries/s", "ms_per_step": bat_ms / n,
+                       "config": cfg,
+                       "batched": {"queries_per_s": n / bat_ms * 1e3, "wall_ms": bat_ms},
+                       "plain_sequential": {"queries_per_s": n / seq_ms * 1e3, "wall_ms": seq_ms},
+                       "speedup": seq_ms / bat_ms, "results_equal": equal})
+    else:
+        rng = np.random.default_rng(form["seed"])
+        rates = [float(x) for x in args.rates.split(",")]
+        table = []
+        budget = int(args.budget_gb * (1 << 30))
+        for rate in rates:
+            arrivals = np.cumsum(rng.exponential(1e3 / rate, n))
+            arrivals -= arrivals[0]
+            srv, srv_ms = owner.run_queries(cs, workers=args.workers, run_tds=False, arrivals_ms=arrivals,
+                                            budget_bytes=budget)
+            one, one_ms = owner.run_queries(cs, workers=1, run_tds=False, arrivals_ms=arrivals)
+            lat_s = [r.end_ms - r.arrival_ms for r in srv]
+            lat_1 = [r.end_ms - r.arrival_ms for r in one]
+            equal = sum(a.result() == b.result() for a, b in zip(seq, srv))
+            row = {"rate_qps": rate, "served_qps": n / srv_ms * 1e3, "served_p50_ms": pct(lat_s, 50),
+                   "served_p99_ms": pct(lat_s, 99), "plain_qps": n / one_ms * 1e3, "plain_p50_ms": pct(lat_1, 50),
+                   "plain_p99_ms": pct(lat_1, 99), "results_equal_alone": equal}
+            log(f"[bench] rate {rate:.0f} q/s: served {row['served_qps']:.1f} q/s p50 {row['served_p50_ms']:.1f} "
+                f"p99 {row['served_p99_ms']:.1f} ms | one at a time {row['plain_qps']:.1f} q/s p50 "
+                f"{row['plain_p50_ms']:.1f} p99 {row['plain_p99_ms']:.1f} ms | equal {equal}/{n}")
+            table.append(row)
+        top = table[-1]
+        cfg["budget_bytes"] = budget
+        result.update({"value": top["served_qps"], "unit": "queries/s", "ms_per_step": 1e3 / top["served_qps"],
+                       "config": cfg, "rates": table, "plain_solve_ms_p99": pct(service, 99)})
+    owner.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="plain")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + sorted(QUERY_FORMS), default="plain")
+    ap.add_argument("--queries", type=int, default=0, help="batch / served: number of templates")
+    ap.add_argument("--workers", type=int, default=8, help="batch / served: worker contexts")
+    ap.add_argument("--rates", default="50,200,1000,2000", help="served: arrival rates (queries/s)")
+    ap.add_argument("--budget-gb", type=float, default=64.0, help="served: device bytes admitted queries may hold")
     ap.add_argument("--scale", type=int, default=None, help="default: the workload's own scale")
     ap.add_argument("--template-labels", default=None, help="comma-separated label ranks of the template vertices")
     ap.add_argument("--stop-before-tds", action="store_true",
@@ -227,6 +338,8 @@ def main() -> int:
         return 2
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.workload in QUERY_FORMS:
+        return bench_queries(args)
 
     from pattenmatching_sharedmemory_amd import Pruner
 

@@ -190,6 +190,24 @@ int pj_get_counters(pj_ctx* ctx, uint64_t* active_vertices, uint64_t* active_edg
  *   ea_words  [ceil(E/32)] edge-active bitset, bit e%32 of word e/32 */
 int pj_download_state(pj_ctx* ctx, uint64_t* cand, uint8_t* vflags, uint32_t* ea_words);
 
+/* Order-dependent 64-bit checksum of the device state (candidate words,
+ * active / modified / scheduled flags, edge-active bits): equal for equal
+ * states, so a solve can be compared without downloading the state.  No
+ * reference counterpart (the reference dumps the state to files,
+ * multiple_label_cpu.cc:252-297). */
+int pj_state_checksum(pj_ctx* ctx, uint64_t* checksum);
+
+/* Makes ctx use the graph uploaded into `owner` (same device) without a
+ * copy: CSR, labels and hub lists are shared read-only, the pruning state is
+ * ctx's own.  `owner` must outlive ctx and keep its graph.  Many contexts
+ * over one resident graph serve the batched and served forms (one stream
+ * each).  No reference counterpart (the reference holds one graph and one
+ * state, multiple_label_cpu.cc:20-57). */
+int pj_share_graph(pj_ctx* ctx, const pj_ctx* owner);
+
+/* The device index the context was created on (pj_create). */
+int pj_get_device(const pj_ctx* ctx, int* device);
+
 /* Overwrites the pruning state from host arrays of the same layout as
  * pj_download_state (used to start a step from a given state). */
 int pj_upload_state(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags,

@@ -88,6 +88,35 @@ typedef struct pjh_row {
 int pjh_run_pipeline(pj_ctx* ctx, const pjh_constraints* c, int flags, pjh_row* rows, int max_rows,
                      double* total_ms);
 
+/* Batched and served forms ------------------------------------------------
+ * Runs n templates over the graph of `owner` with `workers` contexts sharing
+ * it (pj_share_graph), one host thread and one stream each.  Query i may
+ * start once the clock reaches arrival_ms[i] (NULL: all at 0), a worker is
+ * free and, when budget_bytes > 0, the device bytes projected for the
+ * queries in flight (pjh_projected_bytes) stay within the budget; queries
+ * are admitted in index order.  Each query sets its template, runs the step
+ * machine with `flags` (PJH_STOP_*) and takes pj_state_checksum.  Worker
+ * contexts are created before the clock starts.  Returns 0, or -1 with the
+ * message of the first failing query in err.  No reference counterpart: the
+ * reference runs one template per process (multiple_label_cpu.cc:113-185). */
+typedef struct pjh_query_result {
+  int32_t steps;            /* rows of the solve */
+  int32_t worker;           /* worker that ran it */
+  uint64_t rows_hash;       /* hash of every row's iteration, step, counters and active counts */
+  uint64_t state_checksum;  /* pj_state_checksum after the solve */
+  uint64_t active_vertices, active_edges;
+  uint64_t projected_bytes;
+  double arrival_ms, start_ms, end_ms; /* on the batch clock */
+  double device_ms;         /* summed step device time */
+} pjh_query_result;
+int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int n, int workers, int flags,
+                    const double* arrival_ms, uint64_t budget_bytes, pjh_query_result* out, double* total_ms,
+                    char* err, size_t errlen);
+/* Device bytes a query is projected to need beyond the shared graph:
+ * candidate, toUnmatch and alreadyMatched words (V each), the frontier bins,
+ * flags and marks, plus the memo tables when a constraint needs them. */
+uint64_t pjh_projected_bytes(const pjh_constraints* c, uint32_t vertex_count, uint64_t edge_count);
+
 /* patternmatching_binary: -i graph -b binary-cache -v vertex-data -p pattern-dir
  * -o output-dir [-d device]; writes results.totem, graph_%04d.totem,
  * vertex_%04d.totem and log.totem.  Returns the process exit code. */

@@ -306,6 +306,32 @@ class Pruner:
         av, ae = self.counters()
         return State(cand, vf, ea, av, ae)
 
+    def checksum(self) -> int:
+        """pj_state_checksum of the current state."""
+        h = C.c_uint64()
+        self._check(native.hip().pj_state_checksum(self._ctx, C.byref(h)))
+        return h.value
+
+    def run_queries(self, templates: list, workers: int = 4, enumerate_matches: bool = False,
+                    run_tds: bool = True, arrivals_ms=None, budget_bytes: int = 0) -> tuple[list, float]:
+        """Batched / served form (pjh_run_queries): every template is solved
+        over this context's resident graph by `workers` contexts sharing it.
+        Returns (QueryResult per template, total wall ms)."""
+        n = len(templates)
+        handles = (C.c_void_p * max(n, 1))(*[t.handle.value for t in templates])
+        out = (native.QueryResult * max(n, 1))()
+        arr = None
+        if arrivals_ms is not None:
+            arr = (C.c_double * max(n, 1))(*[float(a) for a in arrivals_ms])
+        total = C.c_double()
+        err = C.create_string_buffer(512)
+        flags = (0 if enumerate_matches else 1) | (0 if run_tds else 2)
+        rc = native.host().pjh_run_queries(self._ctx, handles, n, workers, flags, arr, budget_bytes, out,
+                                           C.byref(total), err, 512)
+        if rc:
+            raise RuntimeError(err.value.decode())
+        return list(out[:n]), total.value
+
     def load_state(self, st: State) -> None:
         cand = np.ascontiguousarray(st.cand, dtype=np.uint64)
         vf = np.ascontiguousarray(st.vflags, dtype=np.uint8)

@@ -1759,6 +1759,7 @@ struct pj_ctx {
   uint64_t ea_words = 0;
   uint32_t ea_tail = 0;  // host staging of the last edge-active word
   bool rows_sorted = false;
+  bool graph_shared = false;  // CSR, labels and hub lists belong to another context (pj_share_graph)
   bool symmetric = false;   // every slot u->v has a reverse slot v->u
   bool skip_clean = false;  // phase 1 skips clean scheduled vertices (rows sorted and symmetric)
   bool dirty_enabled = true;
@@ -1850,6 +1851,14 @@ int dev_alloc(pj_ctx* ctx, T** p, uint64_t count) {
 }
 
 void free_graph(pj_ctx* ctx) {
+  if (ctx->graph_shared) {  // the owner frees these
+    ctx->d_off = nullptr;
+    ctx->d_edges = nullptr;
+    ctx->d_labels = nullptr;
+    ctx->d_hubs = ctx->d_chunk_hub = nullptr;
+    ctx->d_chunk_start = nullptr;
+    ctx->graph_shared = false;
+  }
   void* bufs[] = {ctx->d_off, ctx->d_edges, ctx->d_labels, ctx->d_code, ctx->d_cand, ctx->d_unmatch, ctx->d_u64,
                   ctx->d_flags, ctx->d_vflags, ctx->d_dirty, ctx->d_rescan, ctx->d_ea, ctx->d_bins, ctx->d_bin_count,
                   ctx->d_hubs,
@@ -2159,13 +2168,13 @@ int prepare_memo(pj_ctx* ctx) {
     ctx->d_already = a;
   }
   // Walks on larger graphs reach more states: 2^14 entries per wave up to
-  // 2^23 vertices, doubling per doubling of V up to 2^18 (2048 waves x 2^18
-  // x 20 B = 10.7 GB at 2^27 vertices).
+  // 2^20 vertices, doubling per doubling of V up to 2^18 (2048 waves x 2^18
+  // x 20 B = 10.7 GB from 2^24 vertices).
   int log2 = ctx->memo_log2;
   if (!log2) {
     int bits = 0;
     while (bits < 63 && (1ull << bits) < V) ++bits;
-    log2 = std::min(18, std::max(kMemoLog2, bits - 9));
+    log2 = std::min(18, std::max(kMemoLog2, bits - 6));
   }
   if (alloc_memo(ctx, &ctx->memo1, kMemoWaves, log2)) return -1;
   ctx->memo1.ovf_count = ctx->d_ovf;
@@ -2233,11 +2242,10 @@ int build_view(pj_ctx* ctx) {
   return 0;
 }
 
-// Complete cycle walks of 5+ vertices and path walks of 4+ vertices keep a
-// failed-state memo (state_fail): several partial walks reach the same
-// (vertex, hop) there.
+// Complete cycle and path walks of 4+ vertices keep a failed-state memo
+// (state_fail): several partial walks reach the same (vertex, hop) there.
 bool long_walk(const pj_ctx* ctx, int kind) {
-  return (kind == 0 && ctx->h_t.nlcc_len >= 5) || (kind == 1 && ctx->h_t.nlcc_len >= 4);
+  return (kind == 0 || kind == 1) && ctx->h_t.nlcc_len >= 4;
 }
 
 template <typename Word>
@@ -2562,6 +2570,41 @@ int upload_state_t(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags, con
   return 0;
 }
 
+// Order-dependent checksum of a state: sum over vertices of a hash of (v,
+// candidate word, active / modified / scheduled) plus, over edge-active
+// words, a hash of (index, word).  Equal states give equal sums; a single
+// changed bit changes the sum (up to 64-bit hash collisions).
+template <typename Word>
+__global__ void k_state_checksum(const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
+                                 const uint32_t* __restrict__ ea, uint32_t V, uint64_t words,
+                                 unsigned long long* out) {
+  uint64_t acc = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += stride) {
+    const uint64_t c = static_cast<uint64_t>(cand[v]);
+    const uint64_t f = (flags[v] & (kModified | kScheduled)) | (c ? 1u : 0u);
+    acc += mix64(mix64(v * 0x9E3779B97F4A7C15ull ^ c) + f);
+  }
+  for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < words; w += stride)
+    acc += mix64((w + 0xD1B54A32D192ED03ull) * 0x94D049BB133111EBull ^ ea[w]);
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane_id() == 0) atomicAdd(out, static_cast<unsigned long long>(acc));
+}
+
+template <typename Word>
+int checksum_t(pj_ctx* ctx, uint64_t* out) {
+  unsigned long long* d = ctx->d_sums;  // free outside pj_run_step
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, d, static_cast<uint64_t>(1), 0ull);
+  hipLaunchKernelGGL(k_state_checksum<Word>, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream,
+                     static_cast<const Word*>(ctx->d_cand), ctx->d_flags, ctx->d_ea, ctx->V, ctx->ea_words, d);
+  PJ_HIP(hipGetLastError());
+  unsigned long long h = 0;
+  PJ_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, ctx->stream));
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  *out = h;
+  return 0;
+}
+
 // Candidate words are the narrowest unsigned type holding one bit per
 // template vertex (1, 2, 4 or 8 bytes), so the per-slot gathers of
 // neighbour candidates touch as few cache lines as possible.
@@ -2688,6 +2731,70 @@ int pj_upload_graph_device(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* 
                            const uint64_t* labels) {
   if (!ctx || !offsets || (E && !edges) || (V && !labels)) return -1;
   return upload_common(ctx, V, E, offsets, edges, labels, hipMemcpyDeviceToDevice);
+}
+
+int pj_share_graph(pj_ctx* ctx, const pj_ctx* owner) {
+  if (!ctx || !owner || ctx == owner) return -1;
+  if (!owner->d_off) {
+    ctx->err = "the owner context holds no graph";
+    return -1;
+  }
+  if (owner->device != ctx->device) {
+    ctx->err = "contexts on different devices cannot share a graph";
+    return -1;
+  }
+  PJ_HIP(hipSetDevice(ctx->device));
+  free_graph(ctx);
+  ctx->graph_shared = true;
+  ctx->V = owner->V;
+  ctx->E = owner->E;
+  ctx->ea_words = owner->ea_words;
+  ctx->d_off = owner->d_off;
+  ctx->d_edges = owner->d_edges;
+  ctx->d_labels = owner->d_labels;
+  ctx->rows_sorted = owner->rows_sorted;
+  ctx->symmetric = owner->symmetric;
+  ctx->skip_clean = ctx->rows_sorted && ctx->symmetric && ctx->dirty_enabled;
+  ctx->nhubs = owner->nhubs;
+  ctx->nchunks = owner->nchunks;
+  ctx->d_hubs = owner->d_hubs;
+  ctx->d_chunk_hub = owner->d_chunk_hub;
+  ctx->d_chunk_start = owner->d_chunk_start;
+  const uint64_t nh = ctx->nhubs;
+  if (alloc_state(ctx) || dev_alloc(ctx, &ctx->d_hub_count, nh * kMaxLabels) || dev_alloc(ctx, &ctx->d_hub_elim, nh) ||
+      dev_alloc(ctx, &ctx->d_nhubs, 1) || dev_alloc(ctx, &ctx->d_hub_hit, nh) || dev_alloc(ctx, &ctx->d_hub_new, nh))
+    return -1;
+  PJ_HIP(hipMemcpyAsync(ctx->d_nhubs, &ctx->nhubs, 4, hipMemcpyHostToDevice, ctx->stream));
+  if (nh) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, nh * kMaxLabels)), dim3(kBlock), 0, ctx->stream,
+                       ctx->d_hub_count, nh * kMaxLabels, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, nh)), dim3(kBlock), 0, ctx->stream, ctx->d_hub_elim, nh, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, nh)), dim3(kBlock), 0, ctx->stream, ctx->d_hub_hit, nh,
+                       0xFFFFFFFFu);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, nh)), dim3(kBlock), 0, ctx->stream, ctx->d_hub_new, nh, 0u);
+  }
+  PJ_HIP(hipGetLastError());
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->have_template = false;
+  ctx->active_vertices = ctx->V;
+  ctx->active_edges = ctx->E;
+  return 0;
+}
+
+int pj_get_device(const pj_ctx* ctx, int* device) {
+  if (!ctx || !device) return -1;
+  *device = ctx->device;
+  return 0;
+}
+
+int pj_state_checksum(pj_ctx* ctx, uint64_t* out) {
+  if (!ctx || !out) return -1;
+  if (!ctx->have_template) {
+    ctx->err = "no template installed";
+    return -1;
+  }
+  PJ_HIP(hipSetDevice(ctx->device));
+  return with_word(ctx, [&](auto w) { return checksum_t<decltype(w)>(ctx, out); });
 }
 
 int pj_set_template(pj_ctx* ctx, const pj_template_desc* d) {

@@ -1,9 +1,15 @@
 // extern "C" surface of the host library (include/prunejuice_host.h).
 #include <getopt.h>
 
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <iostream>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../../include/prunejuice_host.h"
 #include "pj_constraints.h"
@@ -211,6 +217,135 @@ int pjh_run_pipeline(pj_ctx* ctx, const pjh_constraints* c, int flags, pjh_row* 
       },
       total_ms, (flags & PJH_STOP_BEFORE_ENUMERATION) != 0, (flags & PJH_STOP_BEFORE_TDS) != 0);
   return rc < 0 ? -1 : n;
+}
+
+uint64_t pjh_projected_bytes(const pjh_constraints* c, uint32_t vertex_count, uint64_t edge_count) {
+  const auto& cs = c->c;
+  const size_t P = cs.vertex_label.size();
+  const uint64_t W = P <= 8 ? 1 : P <= 16 ? 2 : P <= 32 ? 4 : 8;
+  const uint64_t V = vertex_count;
+  // candidate / toUnmatch / alreadyMatched words, flags and marks (5 bytes),
+  // scan lengths (4), frontier bins (12), edge-active bits
+  uint64_t bytes = V * (3 * W + 5 + 4 + 12) + edge_count / 8;
+  bool tables = !cs.tds.empty();
+  for (const auto& cy : cs.cycles) tables = tables || cy.vertex.size() >= 4;
+  for (const auto& pa : cs.paths) tables = tables || pa.vertex.size() >= 4;
+  if (tables) {  // first-pass memo tables: 2048 waves x 2^log2 x 20 bytes (pj_device.hip, prepare_memo)
+    int bits = 0;
+    while (bits < 63 && (1ull << bits) < V) ++bits;
+    const int log2 = std::min(18, std::max(14, bits - 6));
+    bytes += 2048ull * (1ull << log2) * 20 + 2 * V * 4;
+  }
+  return bytes;
+}
+
+int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int n, int workers, int flags,
+                    const double* arrival_ms, uint64_t budget_bytes, pjh_query_result* out, double* total_ms,
+                    char* err, size_t errlen) {
+  if (!owner || (n > 0 && (!templates || !out)) || workers < 1) return -1;
+  // every worker owns a context sharing the graph; created before the clock
+  std::vector<pj_ctx*> ctxs(workers, nullptr);
+  int device = 0;
+  if (pj_get_device(owner, &device) != 0) return -1;
+  for (int w = 0; w < workers; ++w) {
+    if (pj_create(&ctxs[w], device) || pj_share_graph(ctxs[w], owner)) {
+      put_err(ctxs[w] ? pj_last_error(ctxs[w]) : "pj_create failed", err, errlen);
+      for (auto* c : ctxs)
+        if (c) pj_destroy(c);
+      return -1;
+    }
+  }
+  uint64_t vc = 0, ec = 0;
+  pj_get_counters(ctxs[0], &vc, &ec);  // a fresh context: V and E
+  std::vector<uint64_t> projected(n);
+  for (int i = 0; i < n; ++i) projected[i] = pjh_projected_bytes(templates[i], static_cast<uint32_t>(vc), ec);
+  std::mutex mu;
+  std::condition_variable cv;
+  int next = 0;
+  uint64_t reserved = 0;
+  std::string first_error;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto now_ms = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
+
+  auto worker = [&](int w) {
+    pj_ctx* ctx = ctxs[w];
+    while (true) {
+      int i;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        while (true) {
+          if (next >= n || !first_error.empty()) return;
+          const double due = arrival_ms ? arrival_ms[next] : 0.0;
+          const double t = now_ms();
+          const bool fits = budget_bytes == 0 || reserved == 0 || reserved + projected[next] <= budget_bytes;
+          if (t >= due && fits) break;
+          if (t < due)
+            cv.wait_for(lk, std::chrono::duration<double, std::milli>(due - t));
+          else
+            cv.wait(lk);
+        }
+        i = next++;
+        reserved += projected[i];
+      }
+      pjh_query_result& r = out[i];
+      r = pjh_query_result{};
+      r.worker = w;
+      r.projected_bytes = projected[i];
+      r.arrival_ms = arrival_ms ? arrival_ms[i] : 0.0;
+      r.start_ms = now_ms();
+      std::string e;
+      uint64_t h = 1469598103934665603ull;
+      auto mixin = [&h](uint64_t x) {
+        h ^= x + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+        h *= 1099511628211ull;
+      };
+      int steps = 0;
+      double dev = 0.0;
+      if (pj_set_template(ctx, templates[i]->c.desc()) != 0) {
+        e = pj_last_error(ctx);
+      } else {
+        pj::MultipleLabelDeviceState st;
+        st.ctx = ctx;
+        const int rc = pj::run_step_machine(
+            &st, templates[i]->c,
+            [&](const pj::StepRecord& rec) {
+              ++steps;
+              mixin(static_cast<uint64_t>(rec.iteration));
+              mixin(static_cast<uint64_t>(rec.step));
+              mixin(rec.results.vertexEliminated);
+              mixin(rec.results.edgeEliminated);
+              mixin(rec.results.matchEliminated);
+              mixin(rec.results.enumeration);
+              mixin(rec.activeVertices);
+              mixin(rec.activeEdges);
+            },
+            &dev, (flags & PJH_STOP_BEFORE_ENUMERATION) != 0, (flags & PJH_STOP_BEFORE_TDS) != 0);
+        if (rc < 0) e = st.message.empty() ? std::string(pj_last_error(ctx)) : st.message;
+        if (e.empty() && pj_state_checksum(ctx, &r.state_checksum) != 0) e = pj_last_error(ctx);
+        pj_get_counters(ctx, &r.active_vertices, &r.active_edges);
+      }
+      r.steps = steps;
+      r.rows_hash = h;
+      r.device_ms = dev;
+      r.end_ms = now_ms();
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        reserved -= projected[i];
+        if (!e.empty() && first_error.empty()) first_error = "query " + std::to_string(i) + ": " + e;
+      }
+      cv.notify_all();
+    }
+  };
+  std::vector<std::thread> threads;
+  for (int w = 0; w < workers; ++w) threads.emplace_back(worker, w);
+  for (auto& t : threads) t.join();
+  if (total_ms) *total_ms = now_ms();
+  for (auto* c : ctxs) pj_destroy(c);
+  if (!first_error.empty()) {
+    put_err(first_error, err, errlen);
+    return -1;
+  }
+  return 0;
 }
 
 // patternmatching_binary.cc:13-42 with the CLI of cmd_line_option.cc:22-78

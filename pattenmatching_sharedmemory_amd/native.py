@@ -74,6 +74,18 @@ class Row(C.Structure):
                 self.enumeration, self.active_vertices, self.active_edges)
 
 
+class QueryResult(C.Structure):
+    """pjh_query_result: one template of a batched / served run."""
+    _fields_ = [("steps", C.c_int32), ("worker", C.c_int32), ("rows_hash", C.c_uint64),
+                ("state_checksum", C.c_uint64), ("active_vertices", C.c_uint64), ("active_edges", C.c_uint64),
+                ("projected_bytes", C.c_uint64), ("arrival_ms", C.c_double), ("start_ms", C.c_double),
+                ("end_ms", C.c_double), ("device_ms", C.c_double)]
+
+    def result(self) -> tuple:
+        """What the solve decided (its rows and final state), times excluded."""
+        return (self.steps, self.rows_hash, self.state_checksum, self.active_vertices, self.active_edges)
+
+
 _hip = None
 _host = None
 
@@ -95,6 +107,9 @@ HIP_SIGNATURES = {
     "pj_get_counters": (C.c_int, [P, u64p, u64p]),
     "pj_download_state": (C.c_int, [P, u64p, u8p, u32p]),
     "pj_upload_state": (C.c_int, [P, u64p, u8p, u32p, C.c_uint64, C.c_uint64]),
+    "pj_state_checksum": (C.c_int, [P, u64p]),
+    "pj_share_graph": (C.c_int, [P, P]),
+    "pj_get_device": (C.c_int, [P, C.POINTER(C.c_int)]),
     "pj_last_error": (C.c_char_p, [P]),
     "pj_destroy": (None, [P]),
 }
@@ -122,6 +137,9 @@ HOST_SIGNATURES = {
     "pjh_constraints_memo_inert": (C.c_int, [P, C.c_int, C.c_uint32]),
     "pjh_constraints_free": (None, [P]),
     "pjh_run_pipeline": (C.c_int, [P, P, C.c_int, C.POINTER(Row), C.c_int, C.POINTER(C.c_double)]),
+    "pjh_run_queries": (C.c_int, [P, C.POINTER(P), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_uint64,
+                                  C.POINTER(QueryResult), C.POINTER(C.c_double), C.c_char_p, C.c_size_t]),
+    "pjh_projected_bytes": (C.c_uint64, [P, C.c_uint32, C.c_uint64]),
     "pjh_main": (C.c_int, [C.c_int, C.POINTER(C.c_char_p)]),
 }
 

@@ -1,0 +1,70 @@
+"""Batched and served forms (pjh_run_queries): worker contexts sharing one
+resident graph solve every template exactly as a context of its own does,
+alone or in the mix, and admission keeps the projected bytes in flight
+within the budget."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(pj, n, seed=5):
+    from pattenmatching_sharedmemory_amd.templates import longest_cycle, random_templates
+
+    g = pj.Graph.rmat(13, 16, seed=3).labels_zipf(32, 1.2, seed=4)
+    cs = []
+    for k, e, l in random_templates(4 * n, seed, 3, 10, 3, distinct_labels=True):
+        c = pj.Constraints(pj.Graph.pattern(k, e, l))
+        if longest_cycle(c) <= 4:
+            cs.append(c)
+        if len(cs) == n:
+            break
+    return g, cs
+
+
+def test_batched_results_equal_separate_solves(pj):
+    g, cs = _setup(pj, 16)
+    owner = pj.Pruner(g)
+    seq, _ = owner.run_queries(cs, workers=1, run_tds=False)
+    bat, _ = owner.run_queries(cs, workers=4, run_tds=False)
+    assert [r.result() for r in seq] == [r.result() for r in bat]
+    assert len({r.worker for r in bat}) > 1
+    for c, r in zip(cs, bat):  # a context of its own, through the plain entry points
+        pr = pj.Pruner(g, c)
+        rows, _ = pr.run(enumerate_matches=False, run_tds=False)
+        assert (r.steps, r.active_vertices, r.active_edges) == (len(rows), rows[-1].active_vertices,
+                                                                 rows[-1].active_edges)
+        assert r.state_checksum == pr.checksum()
+        pr.close()
+    owner.close()
+
+
+def test_served_results_equal_alone_and_admission_respects_budget(pj):
+    g, cs = _setup(pj, 24, seed=9)
+    owner = pj.Pruner(g)
+    alone, _ = owner.run_queries(cs, workers=1, run_tds=False)
+    rng = np.random.default_rng(1)
+    arrivals = np.cumsum(rng.exponential(2.0, len(cs)))
+    budget = int(max(r.projected_bytes for r in alone) * 2.5)
+    srv, _ = owner.run_queries(cs, workers=6, run_tds=False, arrivals_ms=arrivals, budget_bytes=budget)
+    assert [r.result() for r in alone] == [r.result() for r in srv]
+    for r, a in zip(srv, arrivals):
+        assert r.start_ms + 1e-6 >= a and r.end_ms >= r.start_ms
+    for r in srv:  # bytes of the queries running when r started
+        live = sum(q.projected_bytes for q in srv if q.start_ms <= r.start_ms < q.end_ms)
+        assert live <= budget
+    owner.close()
+
+
+def test_checksum_tracks_the_state(pj):
+    g, cs = _setup(pj, 1)
+    a, b = pj.Pruner(g, cs[0]), pj.Pruner(g, cs[0])
+    assert a.checksum() == b.checksum()
+    from pattenmatching_sharedmemory_amd import STEP_LCC0
+
+    a.step(STEP_LCC0)
+    assert a.checksum() != b.checksum()
+    b.step(STEP_LCC0)
+    assert a.checksum() == b.checksum()
+    a.close()
+    b.close()
