@@ -910,6 +910,7 @@ struct MemoArgs {
   void* already;         // alreadyMatched (Word[V])
   uint32_t* ovf_list;    // sources whose table overflowed
   uint32_t* ovf_count;
+  uint32_t lds_limit;    // live entries allowed in the LDS tables of k_memo_lds (0: 3/4 of them)
 };
 
 struct MemoTable {
@@ -1602,6 +1603,92 @@ __global__ void __launch_bounds__(kBlock) k_view_hubs(const uint32_t* __restrict
   }
 }
 
+// The memoised walks of one source: plans in walk order; a root already
+// matched by another source's walk is taken as matched (cc_cpu.tpp:102-108,
+// pc_cpu.tpp:104-142, tds_cpu.tpp:160-188).  Returns the failing roots;
+// *overflow when the table overflowed (then nothing of s may be written).
+template <typename Word>
+__device__ Word memo_source(uint32_t s, Word cs, const DevTemplate* __restrict__ t, const uint64_t* __restrict__ off,
+                            const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
+                            const Word* __restrict__ cand, Word* already, MemoTable& m, uint64_t& slots,
+                            bool* overflow) {
+  Word fail = 0;
+  *overflow = false;
+  for (uint32_t k = 0; k < t->walk_count; ++k) {
+    const uint32_t x = t->walk_root[k];
+    if (!((cs >> x) & 1)) continue;
+    const Word done = __hip_atomic_load(&already[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((done >> x) & 1) continue;
+    const uint32_t o = t->walk_off[k];
+    const uint32_t len = t->walk_off[k + 1] - o;
+    m.epoch = static_cast<uint32_t>(uniform(static_cast<int>(m.epoch + 1)));
+    m.used = 0;
+    m.next_node = 0;
+    m.overflow = false;
+    const int r = memo_walk<Word>(s, len, t->walk_move + o, t->walk_vertex + o, t->walk_test + o, t->walk_mark + o,
+                                  t->walk_mb + o, t->walk_hidx + o, t->walk_hyp != 0, off, edges, ea, cand, already,
+                                  m, slots);
+    if (r < 0) {
+      *overflow = true;
+      return 0;
+    }
+    if (r == 0) fail |= Word(1) << x;
+  }
+  return fail;
+}
+
+// First pass of the memoised walks: one wave per block with its memo table
+// in LDS (kLdsMemo entries, 20 KB), so each probe is an LDS round trip.
+// Sources whose walks outgrow it go to the overflow list and are rerun with
+// tables in HBM (k_nlcc with MemoArgs, launch_nlcc_retry).
+constexpr int kLdsMemo = 1024;
+
+template <typename Word>
+__global__ void __launch_bounds__(64) k_memo_lds(const uint32_t* __restrict__ list,
+                                                 const uint32_t* __restrict__ list_count,
+                                                 const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+                                                 const Word* __restrict__ cand, Word* __restrict__ unmatch,
+                                                 uint8_t* __restrict__ flags, const DevTemplate* __restrict__ t,
+                                                 MemoArgs ma, unsigned long long* counters) {
+  __shared__ uint64_t s_keys[kLdsMemo];
+  __shared__ uint64_t s_vals[kLdsMemo];
+  __shared__ uint32_t s_epochs[kLdsMemo];
+  const int lane = lane_id();
+  for (int i = lane; i < kLdsMemo; i += 64) s_epochs[i] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  MemoTable m{};
+  m.keys = s_keys;
+  m.vals = s_vals;
+  m.epochs = s_epochs;
+  m.mask = kLdsMemo - 1;
+  m.limit = ma.lds_limit ? min(ma.lds_limit, 3u * kLdsMemo / 4) : 3u * kLdsMemo / 4;
+  m.epoch = 0;
+  Word* already = static_cast<Word*>(ma.already);
+  const uint32_t n = *list_count;
+  if (blockIdx.x == 0 && lane == 0) counters[C_ITEMS] = n;
+  uint64_t slots = 0;
+  for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+    const uint32_t s = list[w];
+    const Word cs = cand[s];
+    bool overflow = false;
+    const Word fail = memo_source<Word>(s, cs, t, off, edges, nullptr, cand, already, m, slots, &overflow);
+    if (lane == 0) {
+      if (overflow) {
+        ma.ovf_list[atomicAdd(ma.ovf_count, 1u)] = s;
+      } else if (fail) {  // s is in one task list once: this wave is its only writer
+        unmatch[s] = static_cast<Word>(unmatch[s] | fail);
+        set_flag_atomic(flags, s, kModified);
+      }
+    }
+  }
+  if (lane == 0 && slots) {
+    const int slot = blockIdx.x & (kCounterSlots - 1);
+    atomicAdd(&counters[slot * kCounterStride + C_SLOTS], static_cast<unsigned long long>(slots));
+  }
+}
+
 // One wave per source in the task list (active vertices); failing
 // (source, template vertex) pairs go to toUnmatch and mark the source
 // modified (cc_cpu.tpp:94-129, pc_cpu.tpp:95-147, tds_cpu.tpp:153-188).
@@ -1639,31 +1726,9 @@ __global__ void __launch_bounds__(kBlock) k_nlcc(int kind, const uint32_t* __res
     Word fail = 0;
     const uint32_t L = t->nlcc_len;
     if (memo) {
-      // Plans in walk order; a root already matched by another source's
-      // walk is taken as matched (cc_cpu.tpp:102-108, pc_cpu.tpp:104-142,
-      // tds_cpu.tpp:160-188).
       bool overflow = false;
-      for (uint32_t k = 0; k < t->walk_count && !overflow; ++k) {
-        const uint32_t x = t->walk_root[k];
-        if (!((cs >> x) & 1)) continue;
-        const Word done = __hip_atomic_load(&already[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((done >> x) & 1) continue;
-        const uint32_t o = t->walk_off[k];
-        const uint32_t len = t->walk_off[k + 1] - o;
-        m.epoch = uniform(static_cast<int>(m.epoch + 1));
-        m.used = 0;
-        m.next_node = 0;
-        m.overflow = false;
-        const int r = memo_walk<Word>(s, len, t->walk_move + o, t->walk_vertex + o, t->walk_test + o,
-                                      t->walk_mark + o, t->walk_mb + o, t->walk_hidx + o, t->walk_hyp != 0, off,
-                                      edges, ea, cand, already, m, slots);
-        if (r < 0) overflow = true;
-        else if (r == 0) fail |= Word(1) << x;
-      }
-      if (overflow) {  // retried with a larger table; nothing of s is written here
-        if (lane_id() == 0) ma.ovf_list[atomicAdd(ma.ovf_count, 1u)] = s;
-        fail = 0;
-      }
+      fail = memo_source<Word>(s, cs, t, off, edges, ea, cand, already, m, slots, &overflow);
+      if (overflow && lane_id() == 0) ma.ovf_list[atomicAdd(ma.ovf_count, 1u)] = s;  // retried with a larger table
     } else if (kind == 0) {
       for (uint32_t j = 0; j < L; ++j) {
         const uint32_t q = t->nlcc_vertex[j];
@@ -1766,6 +1831,7 @@ struct pj_ctx {
   bool rescan_all = false;  // every rescan mark is set (phase 1 need not push)
   bool memo_walks = false;  // run every cycle / path step as memoised walks (PJ_MEMO_WALKS=1)
   int memo_log2 = 0;  // first-pass table entries per wave, log2 (PJ_MEMO_LOG2, 6..20; 0: by graph size)
+  uint32_t memo_lds_limit = 0;  // PJ_MEMO_LDS_LIMIT: live entries of the LDS tables (0: 768)
   // memoised walks: per-wave tables (two passes), alreadyMatched, overflow list
   MemoArgs memo1{}, memo_retry[kRetryPasses]{};
   void* d_already = nullptr;  // Word[V]
@@ -2179,6 +2245,7 @@ int prepare_memo(pj_ctx* ctx) {
   if (alloc_memo(ctx, &ctx->memo1, kMemoWaves, log2)) return -1;
   ctx->memo1.ovf_count = ctx->d_ovf;
   ctx->memo1.ovf_list = ctx->d_ovf + 2;
+  ctx->memo1.lds_limit = ctx->memo_lds_limit;
   const uint64_t words = (V * ctx->word_bytes + 3) / 4;
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, words)), dim3(kBlock), 0, ctx->stream,
                      static_cast<uint32_t*>(ctx->d_already), words, 0u);
@@ -2270,8 +2337,12 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
   // Every bin holds active vertices; hubs are run from their own list.
   for (int b = 0; b < kBins; ++b) {
     unsigned long long* c = begin_launch(ctx, K_NLCC);
-    hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, kind, ctx->d_bins + b * V,
-                       ctx->d_bin_count + b, off, edges, ea, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
+    if (memo)  // first pass with LDS tables; overflows go to the list of d_ovf[0]
+      hipLaunchKernelGGL(k_memo_lds<Word>, dim3(kMemoWaves), dim3(64), 0, ctx->stream, ctx->d_bins + b * V,
+                         ctx->d_bin_count + b, off, edges, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
+    else
+      hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, kind, ctx->d_bins + b * V,
+                         ctx->d_bin_count + b, off, edges, ea, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
     end_launch(ctx);
   }
   return 0;
@@ -2284,7 +2355,9 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
 template <typename Word>
 int launch_nlcc_retry(pj_ctx* ctx, int kind) {
   const uint64_t V = ctx->V;
-  for (int r = 0; r < kRetryPasses; ++r) {
+  // pass 0: the first-pass HBM tables (memo1, kMemoWaves waves); then the
+  // retry tables
+  for (int r = 0; r <= kRetryPasses; ++r) {
     uint32_t* in_count = ctx->d_ovf + (r & 1);
     uint32_t* out_count = ctx->d_ovf + ((r + 1) & 1);
     uint32_t* in_list = ctx->d_ovf + 2 + (r & 1) * V;
@@ -2293,22 +2366,27 @@ int launch_nlcc_retry(pj_ctx* ctx, int kind) {
     PJ_HIP(hipMemcpyAsync(&n, in_count, 4, hipMemcpyDeviceToHost, ctx->stream));
     PJ_HIP(hipStreamSynchronize(ctx->stream));
     if (!n) return 0;
-    MemoArgs& m = ctx->memo_retry[r];
-    if (alloc_memo(ctx, &m, kRetryWaves[r], kRetryLog2[r])) return -1;
+    MemoArgs m = ctx->memo1;
+    int waves = kMemoWaves;
+    if (r > 0) {
+      if (alloc_memo(ctx, &ctx->memo_retry[r - 1], kRetryWaves[r - 1], kRetryLog2[r - 1])) return -1;
+      m = ctx->memo_retry[r - 1];
+      waves = kRetryWaves[r - 1];
+    }
     m.ovf_count = out_count;
     m.ovf_list = out_list;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, out_count, static_cast<uint64_t>(1), 0u);
     unsigned long long* c = begin_launch(ctx, K_NLCC);
-    hipLaunchKernelGGL(k_nlcc<Word>, dim3(kRetryWaves[r]), dim3(64), 0, ctx->stream, kind, in_list, in_count,
+    hipLaunchKernelGGL(k_nlcc<Word>, dim3(waves), dim3(64), 0, ctx->stream, kind, in_list, in_count,
                        ctx->d_view_off, ctx->d_view_edges, nullptr, static_cast<const Word*>(ctx->d_cand),
                        static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->d_t, m, c);
     end_launch(ctx);
   }
   uint32_t n = 0;
-  PJ_HIP(hipMemcpyAsync(&n, ctx->d_ovf + (kRetryPasses & 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+  PJ_HIP(hipMemcpyAsync(&n, ctx->d_ovf + ((kRetryPasses + 1) & 1), 4, hipMemcpyDeviceToHost, ctx->stream));
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   if (n) {
-    ctx->err = "memoised walk of a source exceeds 2^29 memo entries";
+    ctx->err = "memoised walk of a source exceeds 2^29 memo entries";  // after the last retry table
     return -1;
   }
   return 0;
@@ -2327,9 +2405,12 @@ void launch_nlcc_hubs(pj_ctx* ctx, int kind, uint32_t* d_hub_n) {
   const MemoArgs ma = tables ? ctx->memo1 : MemoArgs{};
   const uint32_t g = std::min<uint32_t>(ctx->nhubs, tables ? kMemoWaves : ctx->cus * 8);
   unsigned long long* c = begin_launch(ctx, K_NLCC_HUB);
-  hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(64), 0, ctx->stream, kind, ctx->d_hubs, d_hub_n,
-                     memo ? ctx->d_view_off : ctx->d_off, memo ? ctx->d_view_edges : ctx->d_edges,
-                     memo ? nullptr : ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
+  if (memo)
+    hipLaunchKernelGGL(k_memo_lds<Word>, dim3(g), dim3(64), 0, ctx->stream, ctx->d_hubs, d_hub_n, ctx->d_view_off,
+                       ctx->d_view_edges, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
+  else
+    hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(64), 0, ctx->stream, kind, ctx->d_hubs, d_hub_n, ctx->d_off,
+                       ctx->d_edges, ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
   end_launch(ctx);
 }
 
@@ -2658,6 +2739,7 @@ int pj_create(pj_ctx** out, int device) {
   // PJ_MEMO_LOG2 sizes the first-pass memo tables (small values send most
   // sources through the retry passes; the results are the same).
   if (const char* ml = getenv("PJ_MEMO_LOG2")) ctx->memo_log2 = std::min(20, std::max(6, atoi(ml)));
+  if (const char* ll = getenv("PJ_MEMO_LDS_LIMIT")) ctx->memo_lds_limit = static_cast<uint32_t>(std::max(1, atoi(ll)));
   *out = ctx;
   return 0;
 }

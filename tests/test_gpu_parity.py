@@ -312,8 +312,10 @@ def test_schedule_dependent_steps_stay_within_the_reference_outcomes(pj, orc, ca
 
 @pytest.mark.parametrize("scale,seed", [(12, 7), (11, 4)])
 def test_memo_retry_passes(pj, orc, scale, seed, monkeypatch):
-    """With 64-entry first-pass memo tables most TDS sources outgrow them and
-    are rerun by the retry passes; the results stay the reference's."""
+    """With 16 live entries in the LDS tables and 64-entry HBM tables most TDS
+    sources outgrow both and are rerun by the retry passes; the results stay
+    the reference's."""
+    monkeypatch.setenv("PJ_MEMO_LDS_LIMIT", "16")
     monkeypatch.setenv("PJ_MEMO_LOG2", "6")
     g = pj.Graph.rmat(scale, 8, seed=seed).labels_zipf(8, 1.0, seed=seed + 1)
     p = pj.Graph.pattern(8, TDS_EDGES, [0, 1, 2, 3, 4, 5, 6, 7])
