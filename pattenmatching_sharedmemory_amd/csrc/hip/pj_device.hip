@@ -89,6 +89,7 @@ struct DevTemplate {
   uint32_t walk_hyp;                // memoised walks use the hypothesis memo (TDS)
   uint32_t path_code;               // code of the path's initial label
   uint32_t rows_sorted;             // every CSR row ascending (binary search allowed)
+  uint32_t symmetric;               // every slot u->v has a reverse slot v->u
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -1178,6 +1179,57 @@ __device__ bool close_cycle(uint32_t v, uint32_t s, int q, const uint64_t* __res
   return found;
 }
 
+// Triangle existence for a source of at most 64 slots (complete search of a
+// 3-vertex cycle, sorted rows, symmetric graph).  The walk s -> a -> u -> s
+// exists iff some a in A = {a : slot s->a active, q1 in C(a)} and some u in
+// U = {u in row(s) : q2 in C(u), slot u->s active} have an active slot a->u
+// (u reaches s over some slot, so u is in row(s)).  Lane i examines slot i
+// of row(s) for both roles; the |A| x |U| pairs are then checked 64 at a
+// time, each lane binary-searching row(a) for u.  Existence does not depend
+// on the order, so any pair may decide.
+template <typename Word>
+__device__ bool triangle_small(uint32_t s, int q1, int q2, const uint64_t* __restrict__ off,
+                               const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
+                               const Word* __restrict__ cand, uint64_t& slots) {
+  const int lane = lane_id();
+  const uint64_t beg = off[s];
+  const int deg = static_cast<int>(off[s + 1] - beg);
+  uint64_t probes = 0;
+  bool is_a = false, is_u = false;
+  uint32_t w = 0;
+  if (lane < deg) {
+    const uint64_t e = beg + lane;
+    w = edges[e];
+    const Word cw = cand[w];
+    is_a = ea_test(ea, e) && ((cw >> q1) & 1);
+    is_u = ((cw >> q2) & 1) && active_slot(w, s, off, edges, ea, probes);
+  }
+  const uint64_t ma = __ballot(is_a), mu = __ballot(is_u);
+  bool found = false;
+  if (ma && mu) {
+    // lane r learns the slot of the r-th a and of the r-th u
+    int a_from = 0, u_from = 0, r = 0;
+    for (uint64_t m = ma; m; m &= m - 1, ++r)
+      if (lane == r) a_from = __builtin_ctzll(m);
+    r = 0;
+    for (uint64_t m = mu; m; m &= m - 1, ++r)
+      if (lane == r) u_from = __builtin_ctzll(m);
+    const uint32_t a_list = __shfl(w, a_from, 64), u_list = __shfl(w, u_from, 64);
+    const int na = __popcll(ma), nu = __popcll(mu);
+    const int pairs = na * nu;
+    for (int p0 = 0; p0 < pairs && !found; p0 += 64) {
+      const int p = p0 + lane;
+      const int ia = p / nu, iu = p - ia * nu;
+      const uint32_t a = __shfl(a_list, ia & 63, 64), u = __shfl(u_list, iu & 63, 64);
+      const bool ok = p < pairs && active_slot(a, u, off, edges, ea, probes);
+      found = __ballot(ok) != 0;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) probes += __shfl_xor(probes, o, 64);
+  slots += static_cast<uint64_t>(deg) + probes;
+  return found;
+}
+
 // Cycle constraint idx[0..L-1] from start position j (cc_cpu.tpp:12-70).
 // With sorted rows the last two hops are decided by close_cycle.
 template <typename Word>
@@ -1733,7 +1785,12 @@ __global__ void __launch_bounds__(kBlock) k_nlcc(int kind, const uint32_t* __res
       for (uint32_t j = 0; j < L; ++j) {
         const uint32_t q = t->nlcc_vertex[j];
         if (!((cs >> q) & 1)) continue;
-        if (!cycle_walk<Word>(s, j, L, t->rows_sorted != 0, off, edges, ea, cand, t, fm, slots)) fail |= Word(1) << q;
+        const bool ok = L == 3 && t->symmetric && off[s + 1] - off[s] <= 64
+                            ? triangle_small<Word>(s, static_cast<int>(t->nlcc_vertex[(j + 1) % 3]),
+                                                   static_cast<int>(t->nlcc_vertex[(j + 2) % 3]), off, edges, ea,
+                                                   cand, slots)
+                            : cycle_walk<Word>(s, j, L, t->rows_sorted != 0, off, edges, ea, cand, t, fm, slots);
+        if (!ok) fail |= Word(1) << q;
       }
     } else if (kind == 1) {
       const uint32_t q0 = t->nlcc_vertex[0], q1 = t->nlcc_vertex[L - 1];
@@ -2463,6 +2520,7 @@ int add_plan(pj_ctx* ctx, uint32_t root, const std::vector<PlanMove>& mv, uint32
 int install_nlcc(pj_ctx* ctx, int step, uint32_t index) {
   DevTemplate& t = ctx->h_t;
   t.rows_sorted = ctx->rows_sorted ? 1u : 0u;
+  t.symmetric = ctx->rows_sorted && ctx->symmetric ? 1u : 0u;
   t.nlcc_memo = 0;
   t.walk_hyp = 0;
   const bool force_memo = ctx->memo_walks;
