@@ -179,6 +179,11 @@ typedef struct pj_kernel_stat {
 } pj_kernel_stat;
 int pj_set_kernel_timing(pj_ctx* ctx, int enable);
 int pj_kernel_stats(pj_ctx* ctx, pj_kernel_stat* out, int max);
+/* As pj_set_kernel_timing(ctx, 1), but only launches of the named kind
+ * (a pj_kernel_stat name) are bracketed with events and counted; NULL or ""
+ * times every launch.  -1 for an unknown name.  Lets a caller time one
+ * kernel without the cost of events around every other launch. */
+int pj_set_kernel_timing_kind(pj_ctx* ctx, const char* name);
 
 /* Active vertex / edge counters: graphActiveVertexCount and
  * graphActiveEdgeCount (multiple_label_common_cpu.h:43-46), as 64-bit values. */

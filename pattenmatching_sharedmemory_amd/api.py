@@ -275,9 +275,13 @@ class Pruner:
         native.hip().pj_last_step_ms(self._ctx, C.byref(ms))
         return ms.value
 
-    def kernel_timing(self, enable: bool) -> None:
-        """Per-kernel HIP-event timing of later steps (clears the totals)."""
-        self._check(native.hip().pj_set_kernel_timing(self._ctx, 1 if enable else 0))
+    def kernel_timing(self, enable: bool, only: str | None = None) -> None:
+        """Per-kernel HIP-event timing of later steps (clears the totals);
+        `only` names the one kernel kind to time."""
+        if enable and only:
+            self._check(native.hip().pj_set_kernel_timing_kind(self._ctx, only.encode()))
+        else:
+            self._check(native.hip().pj_set_kernel_timing(self._ctx, 1 if enable else 0))
 
     def kernel_stats(self) -> list[dict]:
         n = native.hip().pj_kernel_stats(self._ctx, None, 0)

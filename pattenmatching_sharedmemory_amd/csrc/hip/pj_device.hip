@@ -89,7 +89,6 @@ struct DevTemplate {
   uint32_t walk_hyp;                // memoised walks use the hypothesis memo (TDS)
   uint32_t path_code;               // code of the path's initial label
   uint32_t rows_sorted;             // every CSR row ascending (binary search allowed)
-  uint32_t symmetric;               // every slot u->v has a reverse slot v->u
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -204,10 +203,14 @@ __global__ void k_rows_sorted(const uint32_t* __restrict__ edges, uint64_t E, co
 // flag[1] / flag[2] count the slots u->v with v before / after u.  When all
 // lookups succeed and the counts agree, the graph is symmetric: reversing the
 // "v before u" slots maps them injectively (rows hold no duplicates) onto
-// "after" slots, and equal counts make that map onto.
+// "after" slots, and equal counts make that map onto.  With `rev` given, every
+// successful lookup also records the reverse positions of the pair: rev[e] of
+// a slot u->v is the position of u in row(v), and the reverse slot is
+// off[v] + rev[e] (a self-loop is its own reverse).  rev is complete exactly
+// when the graph is found symmetric.
 __global__ void __launch_bounds__(kBlock) k_rows_symmetric(const uint64_t* __restrict__ off,
                                                            const uint32_t* __restrict__ edges, uint32_t V,
-                                                           unsigned long long* flag) {
+                                                           unsigned long long* flag, uint32_t* __restrict__ rev) {
   const int lane = __lane_id();
   const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
   const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -222,7 +225,10 @@ __global__ void __launch_bounds__(kBlock) k_rows_symmetric(const uint64_t* __res
         bad = true;
         continue;
       }
-      if (v == u) continue;
+      if (v == u) {
+        if (rev) rev[e] = static_cast<uint32_t>(e - beg);
+        continue;
+      }
       uint64_t lo = off[v], hi = off[v + 1];
       const uint64_t dv = hi - lo;
       if (dv < du || (dv == du && v < u)) {
@@ -233,7 +239,12 @@ __global__ void __launch_bounds__(kBlock) k_rows_symmetric(const uint64_t* __res
           if (edges[mid] < u) lo = mid + 1;
           else hi = mid;
         }
-        if (lo == last || edges[lo] != u) bad = true;
+        if (lo == last || edges[lo] != u) {
+          bad = true;
+        } else if (rev) {
+          rev[e] = static_cast<uint32_t>(lo - off[v]);
+          rev[lo] = static_cast<uint32_t>(e - beg);
+        }
       } else {
         ++after;
       }
@@ -329,47 +340,69 @@ __global__ void k_lcc0(Word* __restrict__ cand, const uint8_t* __restrict__ code
 constexpr int kBins = 3;
 __constant__ uint32_t c_bin_limit[kBins] = {4, 32, kHubDegree};
 
-constexpr int kBinIters = 16;                 // vertices per thread
+constexpr int kBinIters = 4;                  // vertices per thread
 constexpr int kBinVerts = kBlock * kBinIters;  // vertices per block
 
-// mode 0 (LCC phase 1 frontier): active, scheduled and, when `dirty` is
-// given, dirty.  A scheduled vertex whose candidate set and whose neighbours'
-// candidate sets are unchanged since its last evaluation would remove nothing
-// and eliminate no slot: its previous removals were applied, its eliminated
-// slots stay eliminated and the surviving slots and label counts are the
-// same.  Skipping it leaves every result, flag and counter unchanged.
+// mode 0 (LCC phase 1 frontier): active, scheduled and, when `marks` (the
+// dirty marks) are given, dirty.  A scheduled vertex whose candidate set and
+// whose neighbours' candidate sets are unchanged since its last evaluation
+// would remove nothing and eliminate no slot: its previous removals were
+// applied, its eliminated slots stay eliminated and the surviving slots and
+// label counts are the same.  Skipping it leaves every result, flag and
+// counter unchanged.
 //
-// mode 3 (phase 2 frontier): active, and modified or, when `dirty` (then the
-// rescan marks) is given, marked for a rescan.  An unmodified vertex keeps
+// mode 3 (phase 2 frontier): active, and modified or, when `marks` (the
+// rescan marks) are given, marked for a rescan.  An unmodified vertex keeps
 // the result of its last scan until a neighbour turns modified (the only
 // change that can move its first hit: its own edge-active bits change only
 // when it is evaluated, and a vertex that loses a slot turns modified), so
 // its scheduled flag stands and last_scan[v] holds the slots the scan reads.
+// A modified vertex that is scheduled and has nothing to drop is left out:
+// phase 2 would change nothing for it and reads no slot.
+//
+// Returns the bin, -1 (not in the frontier) or -2 (skipped; its slots are
+// counted as examined).  Every load is issued before the tests.
 template <typename Word>
 __device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __restrict__ cand,
                                       const uint8_t* __restrict__ flags, const uint8_t* __restrict__ code,
-                                      const uint8_t* __restrict__ dirty, const uint64_t* __restrict__ off, int mode,
-                                      uint32_t path_code) {
-  if (v >= V || !cand[v]) return -1;
-  if (mode == 0 && !(flags[v] & kScheduled)) return -1;
-  if (mode == 0 && dirty && !dirty[v]) return -2;  // clean: skipped
-  if (mode == 3 && dirty && !dirty[v] && !(flags[v] & kModified)) return -2;  // scan result stands
-  if (mode == 2 && code[v] != path_code) return -1;
-  const uint64_t d = off[v + 1] - off[v];
-  if (d <= c_bin_limit[0]) return 0;
-  if (d <= c_bin_limit[1]) return 1;
-  if (d <= c_bin_limit[2]) return 2;
+                                      const uint8_t* __restrict__ marks, const Word* __restrict__ unmatch,
+                                      const uint64_t* __restrict__ off, int mode, uint32_t path_code, uint64_t& deg) {
+  deg = 0;
+  if (v >= V) return -1;
+  const Word c = cand[v];
+  const uint8_t f = flags[v];
+  const uint8_t mk = marks ? marks[v] : uint8_t(1);
+  const uint8_t cd = mode == 2 ? code[v] : uint8_t(0);
+  const Word tu = mode == 3 ? unmatch[v] : Word(0);
+  deg = off[v + 1] - off[v];
+  if (!c) return -1;
+  if (mode == 0) {
+    if (!(f & kScheduled)) return -1;
+    if (!mk) return -2;  // clean: skipped
+  } else if (mode == 3) {
+    if (f & kModified) {
+      if (!tu && (f & kScheduled)) return -1;  // nothing to apply
+    } else if (!mk) {
+      return -2;  // scan result stands
+    }
+  } else if (mode == 2 && cd != path_code) {
+    return -1;
+  }
+  if (deg <= c_bin_limit[0]) return 0;
+  if (deg <= c_bin_limit[1]) return 1;
+  if (deg <= c_bin_limit[2]) return 2;
   return -1;
 }
 
-// Each block compacts kBinVerts consecutive vertices: pass 1 counts per bin
-// with ballots, one global atomic per block and bin reserves space, pass 2
-// writes the vertex ids (each wave's run stays contiguous in its bin).
+// Each block compacts kBinVerts consecutive vertices: pass 1 classifies them
+// (kept in registers) and counts per bin with ballots, one global atomic per
+// block and bin reserves space, pass 2 writes the vertex ids (each wave's run
+// stays contiguous in its bin).
 template <typename Word>
 __global__ void __launch_bounds__(kBlock) k_build_bins(
     const Word* __restrict__ cand, const uint8_t* __restrict__ flags, const uint8_t* __restrict__ code,
-    const uint8_t* __restrict__ dirty, const uint64_t* __restrict__ off, uint32_t V, int mode,
-    const DevTemplate* __restrict__ t, uint32_t* __restrict__ bins, uint32_t* __restrict__ bin_count,
+    const uint8_t* __restrict__ marks, const Word* __restrict__ unmatch, const uint64_t* __restrict__ off, uint32_t V,
+    int mode, const DevTemplate* __restrict__ t, uint32_t* __restrict__ bins, uint32_t* __restrict__ bin_count,
     const uint32_t* __restrict__ last_scan, unsigned long long* counters) {
   __shared__ uint32_t s_wave[kBlock / 64][kBins];
   __shared__ uint32_t s_base[kBlock / 64][kBins];
@@ -377,17 +410,21 @@ __global__ void __launch_bounds__(kBlock) k_build_bins(
   const int wave = threadIdx.x >> 6;
   const uint32_t path_code = mode == 2 ? t->path_code : 0u;
   const uint64_t v0 = static_cast<uint64_t>(blockIdx.x) * kBinVerts + threadIdx.x;
+  int bin[kBinIters];
+  uint64_t deg[kBinIters];
+#pragma unroll
+  for (int i = 0; i < kBinIters; ++i)
+    bin[i] = bin_of<Word>(v0 + static_cast<uint64_t>(i) * kBlock, V, cand, flags, code, marks, unmatch, off, mode,
+                          path_code, deg[i]);
   uint32_t cnt[kBins] = {0, 0, 0};
   uint64_t skipped = 0;
-#pragma unroll 4
-  for (int i = 0; i < kBinIters; ++i) {
-    const uint64_t v = v0 + static_cast<uint64_t>(i) * kBlock;
-    const int b = bin_of<Word>(v, V, cand, flags, code, dirty, off, mode, path_code);
-    if (b == -2) skipped += mode == 0 ? off[v + 1] - off[v] : last_scan[v];
 #pragma unroll
-    for (int k = 0; k < kBins; ++k) cnt[k] += __popcll(__ballot(b == k));
+  for (int i = 0; i < kBinIters; ++i) {
+    if (bin[i] == -2) skipped += mode == 0 ? deg[i] : last_scan[v0 + static_cast<uint64_t>(i) * kBlock];
+#pragma unroll
+    for (int k = 0; k < kBins; ++k) cnt[k] += __popcll(__ballot(bin[i] == k));
   }
-  if (dirty) wave_add(counters, C_SKIP, skipped);
+  if (marks) wave_add(counters, C_SKIP, skipped);
   if (lane == 0)
     for (int k = 0; k < kBins; ++k) s_wave[wave][k] = cnt[k];
   __syncthreads();
@@ -406,14 +443,13 @@ __global__ void __launch_bounds__(kBlock) k_build_bins(
 #pragma unroll
   for (int k = 0; k < kBins; ++k) pos[k] = s_base[wave][k];
   const uint64_t below = (1ull << lane) - 1;
-#pragma unroll 4
+#pragma unroll
   for (int i = 0; i < kBinIters; ++i) {
     const uint64_t v = v0 + static_cast<uint64_t>(i) * kBlock;
-    const int b = bin_of<Word>(v, V, cand, flags, code, dirty, off, mode, path_code);
 #pragma unroll
     for (int k = 0; k < kBins; ++k) {
-      const uint64_t m = __ballot(b == k);
-      if (b == k) bins[static_cast<uint64_t>(k) * V + pos[k] + __popcll(m & below)] = static_cast<uint32_t>(v);
+      const uint64_t m = __ballot(bin[i] == k);
+      if (bin[i] == k) bins[static_cast<uint64_t>(k) * V + pos[k] + __popcll(m & below)] = static_cast<uint32_t>(v);
       pos[k] += __popcll(m);
     }
   }
@@ -806,6 +842,9 @@ __global__ void __launch_bounds__(kBlock) k_phase2_hub_chunks(
     const uint64_t row = off[v];
     const uint64_t beg = chunk_start[c];
     const uint64_t end = min(off[v + 1], beg + kHubChunk);
+    // A hit already found in an earlier chunk of the row decides the scan:
+    // this chunk cannot lower the minimum (one load per wave).
+    if (*reinterpret_cast<const volatile uint32_t*>(&hub_hit[h]) < beg - row) continue;
     constexpr int kPer = kHubChunk / kBlock;
     uint32_t word[kPer], u[kPer];
 #pragma unroll
@@ -1141,15 +1180,41 @@ __device__ __forceinline__ bool active_slot(uint32_t a, uint32_t b, const uint64
   return false;
 }
 
+// active_slot on a symmetric graph with reverse positions: the shorter of
+// row(a) and row(b) is searched (rows hold no duplicate slots), a hit in
+// row(b) being mapped to the slot a->b through rev.
+__device__ __forceinline__ bool active_slot_rev(uint32_t a, uint32_t b, const uint64_t* __restrict__ off,
+                                                const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
+                                                const uint32_t* __restrict__ rev, uint64_t& probes) {
+  const uint64_t ab = off[a], bb = off[b];
+  const uint64_t da = off[a + 1] - ab, db = off[b + 1] - bb;
+  const bool in_a = da <= db;
+  uint64_t lo = in_a ? ab : bb, hi = lo + (in_a ? da : db);
+  const uint64_t end = hi;
+  const uint32_t key = in_a ? b : a;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    ++probes;
+    if (edges[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo == end || edges[lo] != key) return false;
+  return ea_test(ea, in_a ? lo : ab + rev[lo]);
+}
+
 // Last two hops of a cycle on sorted rows: is there a u with an active slot
 // v->u, template vertex q among u's candidates, and an active slot u->s?
 // The shorter of row(v) and row(s) is read 64 slots at a time, each lane
 // binary-searching the other side, so a low-degree source next to a hub
-// never reads the hub's row.
+// never reads the hub's row.  With reverse positions (symmetric graphs) the
+// slot u->s of a u in row(s) is read directly and the remaining lookups
+// search the shorter of the two rows.
 template <typename Word>
 __device__ bool close_cycle(uint32_t v, uint32_t s, int q, const uint64_t* __restrict__ off,
                             const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
-                            const Word* __restrict__ cand, uint64_t& slots) {
+                            const uint32_t* __restrict__ rev, const Word* __restrict__ cand, uint64_t& slots) {
   const int lane = lane_id();
   const uint64_t vb = off[v], ve = off[v + 1], sb = off[s], se = off[s + 1];
   const bool from_v = (ve - vb) <= (se - sb);
@@ -1162,10 +1227,16 @@ __device__ bool close_cycle(uint32_t v, uint32_t s, int q, const uint64_t* __res
     if (e < end) {
       const uint32_t u = edges[e];
       if ((cand[u] >> q) & 1) {
-        if (from_v)
+        if (rev) {
+          if (from_v)
+            ok = ea_test(ea, e) && active_slot_rev(u, s, off, edges, ea, rev, probes);
+          else
+            ok = ea_test(ea, off[u] + rev[e]) && active_slot_rev(v, u, off, edges, ea, rev, probes);
+        } else if (from_v) {
           ok = ea_test(ea, e) && active_slot(u, s, off, edges, ea, probes);
-        else
+        } else {
           ok = active_slot(v, u, off, edges, ea, probes) && active_slot(u, s, off, edges, ea, probes);
+        }
       }
     }
     read += min<uint64_t>(64, end - base);
@@ -1180,31 +1251,35 @@ __device__ bool close_cycle(uint32_t v, uint32_t s, int q, const uint64_t* __res
 }
 
 // Triangle existence for a source of at most 64 slots (complete search of a
-// 3-vertex cycle, sorted rows, symmetric graph).  The walk s -> a -> u -> s
-// exists iff some a in A = {a : slot s->a active, q1 in C(a)} and some u in
+// 3-vertex cycle, sorted rows without duplicates, symmetric graph with
+// reverse positions).  The walk s -> a -> u -> s exists iff some a in
+// A = {a : slot s->a active, q1 in C(a)} and some u in
 // U = {u in row(s) : q2 in C(u), slot u->s active} have an active slot a->u
 // (u reaches s over some slot, so u is in row(s)).  Lane i examines slot i
-// of row(s) for both roles; the |A| x |U| pairs are then checked 64 at a
-// time, each lane binary-searching row(a) for u.  Existence does not depend
-// on the order, so any pair may decide.
+// of row(s) for both roles (the slot u->s is off[u] + rev[i]); the |A| x |U|
+// pairs are then checked 64 at a time, each lane searching the shorter of
+// row(a) and row(u).  Existence does not depend on the order, so any pair
+// may decide.
 template <typename Word>
 __device__ bool triangle_small(uint32_t s, int q1, int q2, const uint64_t* __restrict__ off,
                                const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
-                               const Word* __restrict__ cand, uint64_t& slots) {
+                               const uint32_t* __restrict__ rev, const Word* __restrict__ cand, uint64_t& slots) {
   const int lane = lane_id();
   const uint64_t beg = off[s];
   const int deg = static_cast<int>(off[s + 1] - beg);
   uint64_t probes = 0;
   bool is_a = false, is_u = false;
   uint32_t w = 0;
+  Word cw = 0;
+  const uint64_t e = beg + lane;
   if (lane < deg) {
-    const uint64_t e = beg + lane;
     w = edges[e];
-    const Word cw = cand[w];
+    cw = cand[w];
     is_a = ea_test(ea, e) && ((cw >> q1) & 1);
-    is_u = ((cw >> q2) & 1) && active_slot(w, s, off, edges, ea, probes);
   }
-  const uint64_t ma = __ballot(is_a), mu = __ballot(is_u);
+  const uint64_t ma = __ballot(is_a);
+  if (ma && lane < deg && ((cw >> q2) & 1)) is_u = ea_test(ea, off[w] + rev[e]);
+  const uint64_t mu = __ballot(is_u);
   bool found = false;
   if (ma && mu) {
     // lane r learns the slot of the r-th a and of the r-th u
@@ -1221,7 +1296,7 @@ __device__ bool triangle_small(uint32_t s, int q1, int q2, const uint64_t* __res
       const int p = p0 + lane;
       const int ia = p / nu, iu = p - ia * nu;
       const uint32_t a = __shfl(a_list, ia & 63, 64), u = __shfl(u_list, iu & 63, 64);
-      const bool ok = p < pairs && active_slot(a, u, off, edges, ea, probes);
+      const bool ok = p < pairs && active_slot_rev(a, u, off, edges, ea, rev, probes);
       found = __ballot(ok) != 0;
     }
   }
@@ -1235,8 +1310,8 @@ __device__ bool triangle_small(uint32_t s, int q1, int q2, const uint64_t* __res
 template <typename Word>
 __device__ bool cycle_walk(uint32_t s, uint32_t j, uint32_t L, bool sorted, const uint64_t* __restrict__ off,
                            const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
-                           const Word* __restrict__ cand, const DevTemplate* __restrict__ t, MemoTable* fm,
-                           uint64_t& slots) {
+                           const uint32_t* __restrict__ rev, const Word* __restrict__ cand,
+                           const DevTemplate* __restrict__ t, MemoTable* fm, uint64_t& slots) {
   const int lane = lane_id();
   uint32_t st_v = s;          // lane i: vertex at level i
   uint64_t st_c = off[s];     // lane i: resume slot at level i
@@ -1250,7 +1325,7 @@ __device__ bool cycle_walk(uint32_t s, uint32_t j, uint32_t L, bool sorted, cons
   while (true) {
     if (sorted && depth == static_cast<int>(L) - 2) {
       const int q = static_cast<int>(t->nlcc_vertex[(j + L - 1) % L]);
-      if (close_cycle<Word>(v, s, q, off, edges, ea, cand, slots)) return true;
+      if (close_cycle<Word>(v, s, q, off, edges, ea, rev, cand, slots)) return true;
       if (fm && depth >= 2) state_fail(*fm, v, depth);
       depth = uniform(depth - 1);
       v = rl32(st_v, depth);
@@ -1749,10 +1824,10 @@ template <typename Word>
 __global__ void __launch_bounds__(kBlock) k_nlcc(int kind, const uint32_t* __restrict__ list,
                                                  const uint32_t* __restrict__ list_count,
                                                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
-                                                 const uint32_t* __restrict__ ea, const Word* __restrict__ cand,
-                                                 Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
-                                                 const DevTemplate* __restrict__ t, MemoArgs ma,
-                                                 unsigned long long* counters) {
+                                                 const uint32_t* __restrict__ ea, const uint32_t* __restrict__ rev,
+                                                 const Word* __restrict__ cand, Word* __restrict__ unmatch,
+                                                 uint8_t* __restrict__ flags, const DevTemplate* __restrict__ t,
+                                                 MemoArgs ma, unsigned long long* counters) {
   const uint32_t n = *list_count;
   if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_ITEMS] = n;
   const uint32_t waves = gridDim.x * (blockDim.x / 64);
@@ -1785,11 +1860,11 @@ __global__ void __launch_bounds__(kBlock) k_nlcc(int kind, const uint32_t* __res
       for (uint32_t j = 0; j < L; ++j) {
         const uint32_t q = t->nlcc_vertex[j];
         if (!((cs >> q) & 1)) continue;
-        const bool ok = L == 3 && t->symmetric && off[s + 1] - off[s] <= 64
+        const bool ok = L == 3 && rev && off[s + 1] - off[s] <= 64
                             ? triangle_small<Word>(s, static_cast<int>(t->nlcc_vertex[(j + 1) % 3]),
                                                    static_cast<int>(t->nlcc_vertex[(j + 2) % 3]), off, edges, ea,
-                                                   cand, slots)
-                            : cycle_walk<Word>(s, j, L, t->rows_sorted != 0, off, edges, ea, cand, t, fm, slots);
+                                                   rev, cand, slots)
+                            : cycle_walk<Word>(s, j, L, t->rows_sorted != 0, off, edges, ea, rev, cand, t, fm, slots);
         if (!ok) fail |= Word(1) << q;
       }
     } else if (kind == 1) {
@@ -1878,6 +1953,7 @@ struct pj_ctx {
   uint32_t* d_last_scan = nullptr;   // per vertex: slots its last phase 2 scan read
   uint32_t* d_hub_new = nullptr;     // per hub: turned modified in the last phase 1
   uint32_t* d_ea = nullptr;
+  uint32_t* d_rev = nullptr;  // per slot u->v: position of u in row(v) (symmetric graphs only)
   uint64_t ea_words = 0;
   uint32_t ea_tail = 0;  // host staging of the last edge-active word
   bool rows_sorted = false;
@@ -1886,6 +1962,7 @@ struct pj_ctx {
   bool skip_clean = false;  // phase 1 skips clean scheduled vertices (rows sorted and symmetric)
   bool dirty_enabled = true;
   bool rescan_all = false;  // every rescan mark is set (phase 1 need not push)
+  bool reverse_enabled = true;  // keep d_rev for symmetric graphs (PJ_REVERSE_SLOTS=0: no)
   bool memo_walks = false;  // run every cycle / path step as memoised walks (PJ_MEMO_WALKS=1)
   int memo_log2 = 0;  // first-pass table entries per wave, log2 (PJ_MEMO_LOG2, 6..20; 0: by graph size)
   uint32_t memo_lds_limit = 0;  // PJ_MEMO_LDS_LIMIT: live entries of the LDS tables (0: 768)
@@ -1934,6 +2011,7 @@ struct pj_ctx {
   bool timing = false, cur_timed = false;
   int nlaunch = 0;
   int launch_kid[kMaxLaunches] = {};
+  int timing_kid = -1;  // time only launches of this kind (-1: every launch)
   hipEvent_t lev[2 * kMaxLaunches] = {};
   uint64_t k_launches[K_COUNT] = {}, k_items[K_COUNT] = {}, k_slots[K_COUNT] = {}, k_elim[K_COUNT] = {};
   double k_ms[K_COUNT] = {};
@@ -1978,6 +2056,7 @@ void free_graph(pj_ctx* ctx) {
     ctx->d_off = nullptr;
     ctx->d_edges = nullptr;
     ctx->d_labels = nullptr;
+    ctx->d_rev = nullptr;
     ctx->d_hubs = ctx->d_chunk_hub = nullptr;
     ctx->d_chunk_start = nullptr;
     ctx->graph_shared = false;
@@ -1988,7 +2067,7 @@ void free_graph(pj_ctx* ctx) {
                   ctx->d_chunk_hub, ctx->d_chunk_start, ctx->d_hub_count, ctx->d_hub_elim, ctx->d_nhubs, ctx->d_hub_hit,
                   ctx->d_last_scan, ctx->d_hub_new, ctx->memo1.keys, ctx->memo1.vals, ctx->memo1.epochs,
                   ctx->memo1.wave_epoch, ctx->d_already, ctx->d_ovf, ctx->d_view_count, ctx->d_view_off,
-                  ctx->d_view_edges, ctx->d_scan_tmp};
+                  ctx->d_view_edges, ctx->d_scan_tmp, ctx->d_rev};
   for (void* b : bufs)
     if (b) hipFree(b);
   for (auto& m : ctx->memo_retry)
@@ -2002,6 +2081,7 @@ void free_graph(pj_ctx* ctx) {
   ctx->d_cand = ctx->d_unmatch = ctx->d_u64 = nullptr;
   ctx->d_flags = ctx->d_vflags = ctx->d_dirty = ctx->d_rescan = nullptr;
   ctx->d_ea = nullptr;
+  ctx->d_rev = nullptr;
   ctx->d_bins = ctx->d_bin_count = nullptr;
   ctx->d_hubs = ctx->d_chunk_hub = nullptr;
   ctx->d_chunk_start = nullptr;
@@ -2119,7 +2199,7 @@ int state_reset_t(pj_ctx* ctx) {
 // adds into block 0; with it each launch gets its own block and a pair of
 // HIP events so that pj_kernel_stats can attribute time and work per kernel.
 unsigned long long* begin_launch(pj_ctx* ctx, int kid) {
-  if (!ctx->timing || ctx->nlaunch >= kMaxLaunches) {
+  if (!ctx->timing || (ctx->timing_kid >= 0 && kid != ctx->timing_kid) || ctx->nlaunch >= kMaxLaunches) {
     ctx->cur_timed = false;
     return ctx->d_counters;
   }
@@ -2156,8 +2236,9 @@ void launch_bins(pj_ctx* ctx, int mode) {
   const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(ctx->V) + kBinVerts - 1) / kBinVerts);
   const uint8_t* marks = !ctx->skip_clean ? nullptr : mode == 0 ? ctx->d_dirty : mode == 3 ? ctx->d_rescan : nullptr;
   hipLaunchKernelGGL(k_build_bins<Word>, dim3(std::max<uint32_t>(blocks, 1)), dim3(kBlock), 0, ctx->stream,
-                     static_cast<const Word*>(ctx->d_cand), ctx->d_flags, ctx->d_code, marks, ctx->d_off, ctx->V, mode,
-                     ctx->d_t, ctx->d_bins, ctx->d_bin_count, ctx->d_last_scan, c);
+                     static_cast<const Word*>(ctx->d_cand), ctx->d_flags, ctx->d_code, marks,
+                     static_cast<const Word*>(ctx->d_unmatch), ctx->d_off, ctx->V, mode, ctx->d_t, ctx->d_bins,
+                     ctx->d_bin_count, ctx->d_last_scan, c);
   end_launch(ctx);
 }
 
@@ -2372,6 +2453,12 @@ bool long_walk(const pj_ctx* ctx, int kind) {
   return (kind == 0 || kind == 1) && ctx->h_t.nlcc_len >= 4;
 }
 
+// Reverse positions of the full CSR for complete walks (null where the graph
+// is not known to be sorted and symmetric, or they are switched off).
+const uint32_t* reverse_positions(const pj_ctx* ctx) {
+  return ctx->rows_sorted && ctx->symmetric ? ctx->d_rev : nullptr;
+}
+
 template <typename Word>
 int launch_nlcc(pj_ctx* ctx, int kind) {
   const bool memo = ctx->h_t.nlcc_memo && kind != 3;
@@ -2399,7 +2486,8 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
                          ctx->d_bin_count + b, off, edges, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
     else
       hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, kind, ctx->d_bins + b * V,
-                         ctx->d_bin_count + b, off, edges, ea, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
+                         ctx->d_bin_count + b, off, edges, ea, reverse_positions(ctx), cand, unm, ctx->d_flags,
+                         ctx->d_t, ma, c);
     end_launch(ctx);
   }
   return 0;
@@ -2435,7 +2523,7 @@ int launch_nlcc_retry(pj_ctx* ctx, int kind) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, out_count, static_cast<uint64_t>(1), 0u);
     unsigned long long* c = begin_launch(ctx, K_NLCC);
     hipLaunchKernelGGL(k_nlcc<Word>, dim3(waves), dim3(64), 0, ctx->stream, kind, in_list, in_count,
-                       ctx->d_view_off, ctx->d_view_edges, nullptr, static_cast<const Word*>(ctx->d_cand),
+                       ctx->d_view_off, ctx->d_view_edges, nullptr, nullptr, static_cast<const Word*>(ctx->d_cand),
                        static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->d_t, m, c);
     end_launch(ctx);
   }
@@ -2467,7 +2555,7 @@ void launch_nlcc_hubs(pj_ctx* ctx, int kind, uint32_t* d_hub_n) {
                        ctx->d_view_edges, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
   else
     hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(64), 0, ctx->stream, kind, ctx->d_hubs, d_hub_n, ctx->d_off,
-                       ctx->d_edges, ctx->d_ea, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
+                       ctx->d_edges, ctx->d_ea, reverse_positions(ctx), cand, unm, ctx->d_flags, ctx->d_t, ma, c);
   end_launch(ctx);
 }
 
@@ -2520,7 +2608,6 @@ int add_plan(pj_ctx* ctx, uint32_t root, const std::vector<PlanMove>& mv, uint32
 int install_nlcc(pj_ctx* ctx, int step, uint32_t index) {
   DevTemplate& t = ctx->h_t;
   t.rows_sorted = ctx->rows_sorted ? 1u : 0u;
-  t.symmetric = ctx->rows_sorted && ctx->symmetric ? 1u : 0u;
   t.nlcc_memo = 0;
   t.walk_hyp = 0;
   const bool force_memo = ctx->memo_walks;
@@ -2798,6 +2885,9 @@ int pj_create(pj_ctx** out, int device) {
   // sources through the retry passes; the results are the same).
   if (const char* ml = getenv("PJ_MEMO_LOG2")) ctx->memo_log2 = std::min(20, std::max(6, atoi(ml)));
   if (const char* ll = getenv("PJ_MEMO_LDS_LIMIT")) ctx->memo_lds_limit = static_cast<uint32_t>(std::max(1, atoi(ll)));
+  // PJ_REVERSE_SLOTS=0 keeps no reverse positions (4 bytes per slot) and
+  // closes cycles by binary searches instead (same results).
+  if (const char* rs = getenv("PJ_REVERSE_SLOTS")) ctx->reverse_enabled = strcmp(rs, "0") != 0;
   *out = ctx;
   return 0;
 }
@@ -2845,12 +2935,17 @@ static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* of
     if (unsorted == 0) {
       unsigned long long* f = ctx->d_sums;  // three counters; d_sums is free outside pj_run_step
       hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, f, static_cast<uint64_t>(3), 0ull);
+      if (ctx->reverse_enabled && dev_alloc(ctx, &ctx->d_rev, E)) return -1;
       hipLaunchKernelGGL(k_rows_symmetric, dim3(grid_for(ctx, static_cast<uint64_t>(V) * 64)), dim3(kBlock), 0,
-                         ctx->stream, ctx->d_off, ctx->d_edges, V, f);
+                         ctx->stream, ctx->d_off, ctx->d_edges, V, f, ctx->d_rev);
       unsigned long long h[3] = {1, 0, 0};
       PJ_HIP(hipMemcpyAsync(h, f, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
       PJ_HIP(hipStreamSynchronize(ctx->stream));
       ctx->symmetric = h[0] == 0 && h[1] == h[2];
+      if (!ctx->symmetric && ctx->d_rev) {  // incomplete: never read
+        PJ_HIP(hipFree(ctx->d_rev));
+        ctx->d_rev = nullptr;
+      }
     }
     ctx->skip_clean = ctx->rows_sorted && ctx->symmetric && ctx->dirty_enabled;
   }
@@ -2892,6 +2987,7 @@ int pj_share_graph(pj_ctx* ctx, const pj_ctx* owner) {
   ctx->d_off = owner->d_off;
   ctx->d_edges = owner->d_edges;
   ctx->d_labels = owner->d_labels;
+  ctx->d_rev = owner->d_rev;
   ctx->rows_sorted = owner->rows_sorted;
   ctx->symmetric = owner->symmetric;
   ctx->skip_clean = ctx->rows_sorted && ctx->symmetric && ctx->dirty_enabled;
@@ -3080,10 +3176,27 @@ int pj_set_kernel_timing(pj_ctx* ctx, int enable) {
   if (enable && !ctx->lev[0])
     for (int i = 0; i < 2 * kMaxLaunches; ++i) PJ_HIP(hipEventCreate(&ctx->lev[i]));
   ctx->timing = enable != 0;
+  ctx->timing_kid = -1;
   for (int k = 0; k < K_COUNT; ++k) {
     ctx->k_launches[k] = ctx->k_items[k] = ctx->k_slots[k] = ctx->k_elim[k] = 0;
     ctx->k_ms[k] = 0.0;
   }
+  return 0;
+}
+
+int pj_set_kernel_timing_kind(pj_ctx* ctx, const char* name) {
+  if (!ctx) return -1;
+  int kid = -1;
+  if (name && *name) {
+    for (int k = 0; k < K_COUNT; ++k)
+      if (std::strcmp(kKernelNames[k], name) == 0) kid = k;
+    if (kid < 0) {
+      ctx->err = std::string("unknown kernel kind: ") + name;
+      return -1;
+    }
+  }
+  if (pj_set_kernel_timing(ctx, 1)) return -1;
+  ctx->timing_kid = kid;
   return 0;
 }
 

@@ -103,6 +103,7 @@ HIP_SIGNATURES = {
     "pj_last_step_ms": (C.c_int, [P, C.POINTER(C.c_double)]),
     "pj_last_step_slots": (C.c_int, [P, u64p]),
     "pj_set_kernel_timing": (C.c_int, [P, C.c_int]),
+    "pj_set_kernel_timing_kind": (C.c_int, [P, C.c_char_p]),
     "pj_kernel_stats": (C.c_int, [P, C.POINTER(KernelStat), C.c_int]),
     "pj_get_counters": (C.c_int, [P, u64p, u64p]),
     "pj_download_state": (C.c_int, [P, u64p, u8p, u32p]),
