@@ -502,13 +502,20 @@ __device__ __forceinline__ void count_labels(uint64_t (&cnt)[KW], bool counted, 
   }
 }
 
+// With `derive` set, every active vertex's candidates carry its own label
+// (true once LCC0 has run: later steps only remove candidates), so the code
+// of a neighbour u is that of any template vertex in C(u) and is read from a
+// table in LDS instead of gathering code[u].
 template <typename Word, int G, int KW>
 __global__ void __launch_bounds__(kBlock) k_lcc_phase1(
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, const uint64_t* __restrict__ off,
-    const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code, const Word* __restrict__ cand,
+    const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code, bool derive, const Word* __restrict__ cand,
     Word* __restrict__ unmatch, uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty,
     uint8_t* __restrict__ rescan, uint32_t* __restrict__ ea, const DevTemplate* __restrict__ t,
     unsigned long long* counters) {
+  __shared__ uint8_t s_pcode[64];
+  if (threadIdx.x < 64) s_pcode[threadIdx.x] = t->pcode[threadIdx.x];
+  __syncthreads();
   const int lane = lane_id();
   const int gl = lane & (G - 1);
   const int gbase = lane & ~(G - 1);
@@ -549,7 +556,7 @@ __global__ void __launch_bounds__(kBlock) k_lcc_phase1(
           const uint64_t e = base + k * G + gl;
           const bool act = (word[k] >> (e & 31)) & 1u;
           cu[k] = act ? cand[u[k]] : Word(0);
-          lb[k] = act ? code[u[k]] : kNoCode;
+          lb[k] = act && !derive ? code[u[k]] : kNoCode;
         }
 #pragma unroll
         for (int k = 0; k < kUnroll; ++k) {
@@ -560,7 +567,7 @@ __global__ void __launch_bounds__(kBlock) k_lcc_phase1(
               el = true;
               ++elim;
             } else {
-              lab = lb[k];
+              lab = derive ? s_pcode[ctz<Word>(cu[k])] : lb[k];
               counted = lab != kNoCode;
             }
           }
@@ -596,13 +603,17 @@ __global__ void __launch_bounds__(kBlock) k_lcc_phase1(
 // Label counts and eliminations are summed in hub_count[h][code] and
 // hub_elim[h]; k_lcc_hub_finish evaluates the requirements and clears them.
 
+// `derive`: neighbour codes from candidates, as in k_lcc_phase1.
 template <typename Word>
 __global__ void __launch_bounds__(kBlock) k_lcc_hub_chunks(
     const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
     const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
-    const uint8_t* __restrict__ code, const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
+    const uint8_t* __restrict__ code, bool derive, const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
     const uint8_t* __restrict__ dirty, uint32_t* __restrict__ ea, uint32_t* __restrict__ hub_count,
     uint32_t* __restrict__ hub_elim, const DevTemplate* __restrict__ t, unsigned long long* counters) {
+  __shared__ uint8_t s_pcode[64];
+  if (threadIdx.x < 64) s_pcode[threadIdx.x] = t->pcode[threadIdx.x];
+  __syncthreads();
   // Each thread owns slots threadIdx.x + k * kBlock of the chunk (k < kPer);
   // ids and edge-active words of all of them are loaded before the gathers.
   constexpr int kPer = kHubChunk / kBlock;
@@ -631,7 +642,12 @@ __global__ void __launch_bounds__(kBlock) k_lcc_hub_chunks(
       const uint64_t e = beg + threadIdx.x + k * kBlock;
       const bool act = (word[k] >> (e & 31)) & 1u;
       cu[k] = act ? cand[u[k]] : Word(0);
-      lb[k] = act ? code[u[k]] : kNoCode;
+      lb[k] = act && !derive ? code[u[k]] : kNoCode;
+    }
+    if (derive) {
+#pragma unroll
+      for (int k = 0; k < kPer; ++k)
+        if (cu[k]) lb[k] = s_pcode[ctz<Word>(cu[k])];
     }
     uint32_t elim = 0, mine = 0;  // lane L counts label code L
 #pragma unroll
@@ -1963,6 +1979,7 @@ struct pj_ctx {
   bool dirty_enabled = true;
   bool rescan_all = false;  // every rescan mark is set (phase 1 need not push)
   bool reverse_enabled = true;  // keep d_rev for symmetric graphs (PJ_REVERSE_SLOTS=0: no)
+  bool labels_consistent = false;  // every candidate of every vertex carries the vertex's label (after LCC0)
   bool memo_walks = false;  // run every cycle / path step as memoised walks (PJ_MEMO_WALKS=1)
   int memo_log2 = 0;  // first-pass table entries per wave, log2 (PJ_MEMO_LOG2, 6..20; 0: by graph size)
   uint32_t memo_lds_limit = 0;  // PJ_MEMO_LDS_LIMIT: live entries of the LDS tables (0: 768)
@@ -2192,6 +2209,7 @@ int state_reset_t(pj_ctx* ctx) {
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   ctx->active_vertices = ctx->V;
   ctx->active_edges = ctx->E;
+  ctx->labels_consistent = false;  // every template vertex is a candidate again
   return 0;
 }
 
@@ -2285,6 +2303,7 @@ void launch_phase2(pj_ctx* ctx) {
 template <typename Word, int KW>
 void launch_lcc_phase1(pj_ctx* ctx) {
   launch_bins<Word>(ctx, 0);
+  const bool derive = ctx->labels_consistent;
   uint8_t* dirty = ctx->skip_clean ? ctx->d_dirty : nullptr;
   uint8_t* rescan = ctx->skip_clean && !ctx->rescan_all ? ctx->d_rescan : nullptr;  // push target
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
@@ -2293,24 +2312,24 @@ void launch_lcc_phase1(pj_ctx* ctx) {
   const int g = ctx->cus * 8;
   unsigned long long* c = begin_launch(ctx, K_LCC_P1_G4);
   hipLaunchKernelGGL((k_lcc_phase1<Word, 4, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
-                     ctx->d_bin_count + 0, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, dirty,
-                     rescan, ctx->d_ea, ctx->d_t, c);
+                     ctx->d_bin_count + 0, ctx->d_off, ctx->d_edges, ctx->d_code, derive, cand, unm, ctx->d_flags,
+                     dirty, rescan, ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
   c = begin_launch(ctx, K_LCC_P1_G16);
   hipLaunchKernelGGL((k_lcc_phase1<Word, 16, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
-                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, dirty,
-                     rescan, ctx->d_ea, ctx->d_t, c);
+                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_code, derive, cand, unm, ctx->d_flags,
+                     dirty, rescan, ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
   c = begin_launch(ctx, K_LCC_P1_G64);
   hipLaunchKernelGGL((k_lcc_phase1<Word, 64, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
-                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_code, cand, unm, ctx->d_flags, dirty,
-                     rescan, ctx->d_ea, ctx->d_t, c);
+                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_code, derive, cand, unm, ctx->d_flags,
+                     dirty, rescan, ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
   if (ctx->nhubs) {
     c = begin_launch(ctx, K_LCC_HUB);
     hipLaunchKernelGGL(k_lcc_hub_chunks<Word>, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
                        ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
-                       ctx->d_edges, ctx->d_code, cand, ctx->d_flags, dirty, ctx->d_ea, ctx->d_hub_count,
+                       ctx->d_edges, ctx->d_code, derive, cand, ctx->d_flags, dirty, ctx->d_ea, ctx->d_hub_count,
                        ctx->d_hub_elim, ctx->d_t, c);
     end_launch(ctx);
     begin_launch(ctx, K_LCC_HUB_FINISH);
@@ -2742,6 +2761,7 @@ int run_step_t(pj_ctx* ctx, int step, uint32_t index) {
                          static_cast<Word*>(ctx->d_cand), ctx->d_code, ctx->V, ctx->d_t, c);
       end_launch(ctx);
       mark_all_dirty(ctx);  // every candidate set was rewritten
+      ctx->labels_consistent = true;
       break;
     }
     case PJ_STEP_LCC:
@@ -2791,6 +2811,7 @@ int upload_state_t(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags, con
                      static_cast<const uint64_t*>(ctx->d_u64), ctx->d_vflags, static_cast<Word*>(ctx->d_cand),
                      static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->V);
   mark_all_dirty(ctx);
+  ctx->labels_consistent = false;  // an uploaded state may hold candidates of other labels
   PJ_HIP(hipGetLastError());
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
