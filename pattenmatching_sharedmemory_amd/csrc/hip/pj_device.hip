@@ -1832,6 +1832,87 @@ __global__ void __launch_bounds__(64) k_memo_lds(const uint32_t* __restrict__ li
   }
 }
 
+// 3-vertex cycle constraint for sources of at most G slots, one group of G
+// lanes per source (64 / G sources per wave, so more independent searches
+// are in flight than with one wave per source).  Same test as triangle_small
+// with group-local ballots: lane i examines slot i of row(s) for both roles,
+// then the |A| x |U| pairs are checked G at a time.  Needs sorted rows
+// without duplicates, a symmetric graph and reverse positions.
+template <typename Word, int G>
+__global__ void __launch_bounds__(kBlock) k_triangles(const uint32_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ list_count,
+                                                      const uint64_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ edges,
+                                                      const uint32_t* __restrict__ ea, const uint32_t* __restrict__ rev,
+                                                      const Word* __restrict__ cand, Word* __restrict__ unmatch,
+                                                      uint8_t* __restrict__ flags, const DevTemplate* __restrict__ t,
+                                                      unsigned long long* counters) {
+  const int lane = lane_id();
+  const int gl = lane & (G - 1);
+  const int gbase = lane & ~(G - 1);
+  const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
+  const uint32_t n = *list_count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_ITEMS] = n;
+  const uint64_t groups_per_grid = (uint64_t)gridDim.x * (blockDim.x / G);
+  const uint64_t span = ((n + groups_per_grid - 1) / groups_per_grid) * groups_per_grid;
+  const uint32_t qv[3] = {t->nlcc_vertex[0], t->nlcc_vertex[1], t->nlcc_vertex[2]};
+  uint64_t slots = 0;
+  for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G; g < span; g += groups_per_grid) {
+    if (g >= n) continue;
+    const uint32_t s = list[g];
+    const Word cs = cand[s];
+    const uint64_t beg = off[s];
+    const int deg = static_cast<int>(off[s + 1] - beg);
+    const bool mine = gl < deg;
+    const uint64_t e = beg + gl;
+    uint32_t w = 0;
+    Word cw = 0;
+    bool out_ok = false, back_ok = false;
+    if (mine) {
+      w = edges[e];
+      cw = cand[w];
+      out_ok = ea_test(ea, e);
+      if (cw) back_ok = ea_test(ea, off[w] + rev[e]);  // slot w->s
+    }
+    uint64_t probes = 0;
+    Word fail = 0;
+    for (int j = 0; j < 3; ++j) {
+      const uint32_t q = qv[j];
+      if (!((cs >> q) & 1)) continue;
+      const uint32_t q1 = qv[(j + 1) % 3], q2 = qv[(j + 2) % 3];
+      const uint64_t ma = (__ballot(out_ok && ((cw >> q1) & 1)) & gmask) >> gbase;
+      const uint64_t mu = (__ballot(back_ok && ((cw >> q2) & 1)) & gmask) >> gbase;
+      bool found = false;
+      if (ma && mu) {
+        int a_from = 0, u_from = 0, r = 0;  // lane r of the group: slot of the r-th a / u
+        for (uint64_t m = ma; m; m &= m - 1, ++r)
+          if (gl == r) a_from = __builtin_ctzll(m);
+        r = 0;
+        for (uint64_t m = mu; m; m &= m - 1, ++r)
+          if (gl == r) u_from = __builtin_ctzll(m);
+        const uint32_t a_list = __shfl(w, gbase + a_from, 64), u_list = __shfl(w, gbase + u_from, 64);
+        const int na = __popcll(ma), nu = __popcll(mu);
+        const int pairs = na * nu;
+        for (int p0 = 0; p0 < pairs && !found; p0 += G) {
+          const int p = p0 + gl;
+          const int ia = p / nu, iu = p - ia * nu;
+          const uint32_t a = __shfl(a_list, gbase + (ia & (G - 1)), 64);
+          const uint32_t u = __shfl(u_list, gbase + (iu & (G - 1)), 64);
+          const bool ok = p < pairs && active_slot_rev(a, u, off, edges, ea, rev, probes);
+          found = (__ballot(ok) & gmask) != 0;
+        }
+      }
+      if (!found) fail |= Word(1) << q;
+    }
+    slots += probes + (gl == 0 ? static_cast<uint64_t>(deg) : 0);
+    if (fail && gl == 0) {  // s is in one task list once: this group is its only writer
+      unmatch[s] = static_cast<Word>(unmatch[s] | fail);
+      set_flag_atomic(flags, s, kModified);
+    }
+  }
+  wave_add(counters, C_SLOTS, slots);
+}
+
 // One wave per source in the task list (active vertices); failing
 // (source, template vertex) pairs go to toUnmatch and mark the source
 // modified (cc_cpu.tpp:94-129, pc_cpu.tpp:95-147, tds_cpu.tpp:153-188).
@@ -2497,10 +2578,19 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   const uint64_t V = ctx->V;
   const int g = tables ? kMemoWaves / (kBlock / 64) : ctx->cus * 8;
+  // 3-vertex cycles from low-degree sources: several sources per wave
+  const uint32_t* rev = reverse_positions(ctx);
+  const bool triangles = kind == 0 && !tables && rev && ctx->h_t.nlcc_len == 3;
   // Every bin holds active vertices; hubs are run from their own list.
   for (int b = 0; b < kBins; ++b) {
     unsigned long long* c = begin_launch(ctx, K_NLCC);
-    if (memo)  // first pass with LDS tables; overflows go to the list of d_ovf[0]
+    if (triangles && b == 0)
+      hipLaunchKernelGGL((k_triangles<Word, 4>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
+                         ctx->d_bin_count + b, off, edges, ea, rev, cand, unm, ctx->d_flags, ctx->d_t, c);
+    else if (triangles && b == 1)
+      hipLaunchKernelGGL((k_triangles<Word, 32>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
+                         ctx->d_bin_count + b, off, edges, ea, rev, cand, unm, ctx->d_flags, ctx->d_t, c);
+    else if (memo)  // first pass with LDS tables; overflows go to the list of d_ovf[0]
       hipLaunchKernelGGL(k_memo_lds<Word>, dim3(kMemoWaves), dim3(64), 0, ctx->stream, ctx->d_bins + b * V,
                          ctx->d_bin_count + b, off, edges, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
     else

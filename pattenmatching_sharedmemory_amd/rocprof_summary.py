@@ -28,6 +28,7 @@ _TEMPLATE_NAMES = {
     "k_lcc_hub_finish": "lcc_phase1_hub_finish",
     "k_phase2_hub_chunks": "phase2_hub_chunks",
     "k_phase2_hub_finish": "phase2_hub_finish",
+    "k_triangles": "nlcc_walk",
 }
 
 
