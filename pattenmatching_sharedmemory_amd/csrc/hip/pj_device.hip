@@ -394,33 +394,40 @@ __device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __rest
   return -1;
 }
 
-// Each block compacts kBinVerts consecutive vertices: pass 1 classifies them
-// (kept in registers) and counts per bin with ballots, one global atomic per
-// block and bin reserves space, pass 2 writes the vertex ids (each wave's run
-// stays contiguous in its bin).
+// Each block compacts kBinVerts consecutive entries of the live list (the
+// vertices that may still be active; `live` null: every vertex): pass 1
+// classifies them (kept in registers) and counts per bin with ballots, one
+// global atomic per block and bin reserves space, pass 2 writes the vertex
+// ids (each wave's run stays contiguous in its bin).
 template <typename Word>
 __global__ void __launch_bounds__(kBlock) k_build_bins(
-    const Word* __restrict__ cand, const uint8_t* __restrict__ flags, const uint8_t* __restrict__ code,
-    const uint8_t* __restrict__ marks, const Word* __restrict__ unmatch, const uint64_t* __restrict__ off, uint32_t V,
-    int mode, const DevTemplate* __restrict__ t, uint32_t* __restrict__ bins, uint32_t* __restrict__ bin_count,
+    const uint32_t* __restrict__ live, uint32_t n_live, const Word* __restrict__ cand,
+    const uint8_t* __restrict__ flags, const uint8_t* __restrict__ code, const uint8_t* __restrict__ marks,
+    const Word* __restrict__ unmatch, const uint64_t* __restrict__ off, uint32_t V, int mode,
+    const DevTemplate* __restrict__ t, uint32_t* __restrict__ bins, uint32_t* __restrict__ bin_count,
     const uint32_t* __restrict__ last_scan, unsigned long long* counters) {
   __shared__ uint32_t s_wave[kBlock / 64][kBins];
   __shared__ uint32_t s_base[kBlock / 64][kBins];
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
   const uint32_t path_code = mode == 2 ? t->path_code : 0u;
-  const uint64_t v0 = static_cast<uint64_t>(blockIdx.x) * kBinVerts + threadIdx.x;
+  const uint64_t i0 = static_cast<uint64_t>(blockIdx.x) * kBinVerts + threadIdx.x;
+  uint32_t vid[kBinIters];
+#pragma unroll
+  for (int i = 0; i < kBinIters; ++i) {
+    const uint64_t idx = i0 + static_cast<uint64_t>(i) * kBlock;
+    vid[i] = idx >= n_live ? V : live ? live[idx] : static_cast<uint32_t>(idx);
+  }
   int bin[kBinIters];
   uint64_t deg[kBinIters];
 #pragma unroll
   for (int i = 0; i < kBinIters; ++i)
-    bin[i] = bin_of<Word>(v0 + static_cast<uint64_t>(i) * kBlock, V, cand, flags, code, marks, unmatch, off, mode,
-                          path_code, deg[i]);
+    bin[i] = bin_of<Word>(vid[i], V, cand, flags, code, marks, unmatch, off, mode, path_code, deg[i]);
   uint32_t cnt[kBins] = {0, 0, 0};
   uint64_t skipped = 0;
 #pragma unroll
   for (int i = 0; i < kBinIters; ++i) {
-    if (bin[i] == -2) skipped += mode == 0 ? deg[i] : last_scan[v0 + static_cast<uint64_t>(i) * kBlock];
+    if (bin[i] == -2) skipped += mode == 0 ? deg[i] : last_scan[vid[i]];
 #pragma unroll
     for (int k = 0; k < kBins; ++k) cnt[k] += __popcll(__ballot(bin[i] == k));
   }
@@ -445,13 +452,55 @@ __global__ void __launch_bounds__(kBlock) k_build_bins(
   const uint64_t below = (1ull << lane) - 1;
 #pragma unroll
   for (int i = 0; i < kBinIters; ++i) {
-    const uint64_t v = v0 + static_cast<uint64_t>(i) * kBlock;
 #pragma unroll
     for (int k = 0; k < kBins; ++k) {
       const uint64_t m = __ballot(bin[i] == k);
-      if (bin[i] == k) bins[static_cast<uint64_t>(k) * V + pos[k] + __popcll(m & below)] = static_cast<uint32_t>(v);
+      if (bin[i] == k) bins[static_cast<uint64_t>(k) * V + pos[k] + __popcll(m & below)] = vid[i];
       pos[k] += __popcll(m);
     }
+  }
+}
+
+// Live list compaction: the entries of `in` (null: every vertex) that are
+// still active, in blocks of kBinVerts entries whose runs keep their order.
+template <typename Word>
+__global__ void __launch_bounds__(kBlock) k_live_compact(const uint32_t* __restrict__ in, uint32_t n_in,
+                                                        const Word* __restrict__ cand, uint32_t* __restrict__ out,
+                                                        uint32_t* __restrict__ n_out) {
+  __shared__ uint32_t s_wave[kBlock / 64];
+  __shared__ uint32_t s_base[kBlock / 64];
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const uint64_t i0 = static_cast<uint64_t>(blockIdx.x) * kBinVerts + threadIdx.x;
+  uint32_t vid[kBinIters];
+  bool keep[kBinIters];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int i = 0; i < kBinIters; ++i) {
+    const uint64_t idx = i0 + static_cast<uint64_t>(i) * kBlock;
+    vid[i] = idx < n_in ? (in ? in[idx] : static_cast<uint32_t>(idx)) : 0u;
+    keep[i] = idx < n_in && cand[vid[i]] != Word(0);
+    cnt += __popcll(__ballot(keep[i]));
+  }
+  if (lane == 0) s_wave[wave] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) tot += s_wave[w];
+    uint32_t base = tot ? atomicAdd(n_out, tot) : 0u;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      s_base[w] = base;
+      base += s_wave[w];
+    }
+  }
+  __syncthreads();
+  uint32_t pos = s_base[wave];
+  const uint64_t below = (1ull << lane) - 1;
+#pragma unroll
+  for (int i = 0; i < kBinIters; ++i) {
+    const uint64_t m = __ballot(keep[i]);
+    if (keep[i]) out[pos + __popcll(m & below)] = vid[i];
+    pos += __popcll(m);
   }
 }
 
@@ -2051,6 +2100,12 @@ struct pj_ctx {
   uint32_t* d_hub_new = nullptr;     // per hub: turned modified in the last phase 1
   uint32_t* d_ea = nullptr;
   uint32_t* d_rev = nullptr;  // per slot u->v: position of u in row(v) (symmetric graphs only)
+  // Live list: the vertices that may still be active (a superset of the
+  // active ones; candidates only shrink until the state is replaced).
+  // live == nullptr: every vertex.  Two buffers for the compaction.
+  uint32_t* d_live[2] = {nullptr, nullptr};
+  const uint32_t* live = nullptr;
+  uint32_t n_live = 0;
   uint64_t ea_words = 0;
   uint32_t ea_tail = 0;  // host staging of the last edge-active word
   bool rows_sorted = false;
@@ -2165,7 +2220,7 @@ void free_graph(pj_ctx* ctx) {
                   ctx->d_chunk_hub, ctx->d_chunk_start, ctx->d_hub_count, ctx->d_hub_elim, ctx->d_nhubs, ctx->d_hub_hit,
                   ctx->d_last_scan, ctx->d_hub_new, ctx->memo1.keys, ctx->memo1.vals, ctx->memo1.epochs,
                   ctx->memo1.wave_epoch, ctx->d_already, ctx->d_ovf, ctx->d_view_count, ctx->d_view_off,
-                  ctx->d_view_edges, ctx->d_scan_tmp, ctx->d_rev};
+                  ctx->d_view_edges, ctx->d_scan_tmp, ctx->d_rev, ctx->d_live[0], ctx->d_live[1]};
   for (void* b : bufs)
     if (b) hipFree(b);
   for (auto& m : ctx->memo_retry)
@@ -2180,6 +2235,9 @@ void free_graph(pj_ctx* ctx) {
   ctx->d_flags = ctx->d_vflags = ctx->d_dirty = ctx->d_rescan = nullptr;
   ctx->d_ea = nullptr;
   ctx->d_rev = nullptr;
+  ctx->d_live[0] = ctx->d_live[1] = nullptr;
+  ctx->live = nullptr;
+  ctx->n_live = 0;
   ctx->d_bins = ctx->d_bin_count = nullptr;
   ctx->d_hubs = ctx->d_chunk_hub = nullptr;
   ctx->d_chunk_start = nullptr;
@@ -2251,6 +2309,7 @@ int alloc_state(pj_ctx* ctx) {
       dev_alloc(ctx, &ctx->d_rescan, V + 4) || dev_alloc(ctx, &ctx->d_last_scan, V) ||
       dev_alloc(ctx, &ctx->d_ea, ctx->ea_words) ||
       dev_alloc(ctx, &ctx->d_bins, V * kBins) || dev_alloc(ctx, &ctx->d_bin_count, 4) ||
+      dev_alloc(ctx, &ctx->d_live[0], V + 1) || dev_alloc(ctx, &ctx->d_live[1], V + 1) ||
       dev_alloc(ctx, &ctx->d_counters, (kMaxLaunches + 1) * kCounterBlock))
     return -1;
   return 0;
@@ -2291,6 +2350,8 @@ int state_reset_t(pj_ctx* ctx) {
   ctx->active_vertices = ctx->V;
   ctx->active_edges = ctx->E;
   ctx->labels_consistent = false;  // every template vertex is a candidate again
+  ctx->live = nullptr;              // every vertex is active again
+  ctx->n_live = ctx->V;
   return 0;
 }
 
@@ -2332,10 +2393,10 @@ template <typename Word>
 void launch_bins(pj_ctx* ctx, int mode) {
   zero_bins(ctx);
   unsigned long long* c = begin_launch(ctx, K_BINS);
-  const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(ctx->V) + kBinVerts - 1) / kBinVerts);
+  const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(ctx->n_live) + kBinVerts - 1) / kBinVerts);
   const uint8_t* marks = !ctx->skip_clean ? nullptr : mode == 0 ? ctx->d_dirty : mode == 3 ? ctx->d_rescan : nullptr;
   hipLaunchKernelGGL(k_build_bins<Word>, dim3(std::max<uint32_t>(blocks, 1)), dim3(kBlock), 0, ctx->stream,
-                     static_cast<const Word*>(ctx->d_cand), ctx->d_flags, ctx->d_code, marks,
+                     ctx->live, ctx->n_live, static_cast<const Word*>(ctx->d_cand), ctx->d_flags, ctx->d_code, marks,
                      static_cast<const Word*>(ctx->d_unmatch), ctx->d_off, ctx->V, mode, ctx->d_t, ctx->d_bins,
                      ctx->d_bin_count, ctx->d_last_scan, c);
   end_launch(ctx);
@@ -2879,6 +2940,26 @@ int run_step_t(pj_ctx* ctx, int step, uint32_t index) {
   return 0;
 }
 
+// Live lists shorter than this are not compacted further.
+constexpr uint32_t kLiveMin = 1u << 16;
+
+template <typename Word>
+int compact_live_t(pj_ctx* ctx) {
+  uint32_t* out = ctx->live == ctx->d_live[0] ? ctx->d_live[1] : ctx->d_live[0];
+  uint32_t* n_out = out + ctx->V;  // count in the spare last entry
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, n_out, static_cast<uint64_t>(1), 0u);
+  const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(ctx->n_live) + kBinVerts - 1) / kBinVerts);
+  hipLaunchKernelGGL(k_live_compact<Word>, dim3(std::max<uint32_t>(blocks, 1)), dim3(kBlock), 0, ctx->stream,
+                     ctx->live, ctx->n_live, static_cast<const Word*>(ctx->d_cand), out, n_out);
+  PJ_HIP(hipGetLastError());
+  uint32_t n = 0;
+  PJ_HIP(hipMemcpyAsync(&n, n_out, 4, hipMemcpyDeviceToHost, ctx->stream));
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->live = out;
+  ctx->n_live = n;
+  return 0;
+}
+
 template <typename Word>
 int download_t(pj_ctx* ctx, uint64_t* cand, uint8_t* vflags, uint32_t* ea_words) {
   hipLaunchKernelGGL(k_cand_to_u64<Word>, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream,
@@ -2902,6 +2983,8 @@ int upload_state_t(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags, con
                      static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->V);
   mark_all_dirty(ctx);
   ctx->labels_consistent = false;  // an uploaded state may hold candidates of other labels
+  ctx->live = nullptr;              // and any vertex may be active
+  ctx->n_live = ctx->V;
   PJ_HIP(hipGetLastError());
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
@@ -3064,6 +3147,8 @@ static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* of
   ctx->have_template = false;
   ctx->active_vertices = V;
   ctx->active_edges = E;
+  ctx->live = nullptr;
+  ctx->n_live = V;
   return 0;
 }
 
@@ -3125,6 +3210,8 @@ int pj_share_graph(pj_ctx* ctx, const pj_ctx* owner) {
   ctx->have_template = false;
   ctx->active_vertices = ctx->V;
   ctx->active_edges = ctx->E;
+  ctx->live = nullptr;
+  ctx->n_live = ctx->V;
   return 0;
 }
 
@@ -3279,6 +3366,11 @@ int pj_run_step(pj_ctx* ctx, int step, uint32_t index, pj_algo_results* out) {
   ctx->active_vertices -= r.vertex_eliminated;
   ctx->active_edges -= r.edge_eliminated;
   if (out) *out = r;
+  // Once fewer than half of the live list can be active, the list is
+  // compacted, so frontier construction reads only live vertices.
+  if (ctx->n_live > kLiveMin && ctx->active_vertices < ctx->n_live / 2 &&
+      with_word(ctx, [&](auto w) { return compact_live_t<decltype(w)>(ctx); }))
+    return -1;
   return 0;
 }
 
