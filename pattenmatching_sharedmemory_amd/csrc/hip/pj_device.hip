@@ -360,8 +360,18 @@ constexpr int kBinVerts = kBlock * kBinIters;  // vertices per block
 // A modified vertex that is scheduled and has nothing to drop is left out:
 // phase 2 would change nothing for it and reads no slot.
 //
-// Returns the bin, -1 (not in the frontier) or -2 (skipped; its slots are
-// counted as examined).  Every load is issued before the tests.
+// mode 4 (phase 2 of an NLCC step whose modified flags were all cleared
+// before it, symmetric graph): the only modified vertices are the failing
+// sources, and each has marked the vertices of its row (`marks`, the rescan
+// marks).  By symmetry an unmarked vertex has no slot to a modified vertex:
+// its scan would read the whole row and find nothing (-3: the caller records
+// that result).  Marked and modified vertices are binned as in mode 3.
+//
+// mode 5: active and modified (the sources that mark their neighbours).
+//
+// Returns the bin, -1 (not in the frontier), -2 (skipped; its slots are
+// counted as examined) or -3 (mode 4 default).  Every load is issued before
+// the tests.
 template <typename Word>
 __device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __restrict__ cand,
                                       const uint8_t* __restrict__ flags, const uint8_t* __restrict__ code,
@@ -373,18 +383,20 @@ __device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __rest
   const uint8_t f = flags[v];
   const uint8_t mk = marks ? marks[v] : uint8_t(1);
   const uint8_t cd = mode == 2 ? code[v] : uint8_t(0);
-  const Word tu = mode == 3 ? unmatch[v] : Word(0);
+  const Word tu = mode == 3 || mode == 4 ? unmatch[v] : Word(0);
   deg = off[v + 1] - off[v];
   if (!c) return -1;
   if (mode == 0) {
     if (!(f & kScheduled)) return -1;
     if (!mk) return -2;  // clean: skipped
-  } else if (mode == 3) {
+  } else if (mode == 3 || mode == 4) {
     if (f & kModified) {
       if (!tu && (f & kScheduled)) return -1;  // nothing to apply
     } else if (!mk) {
-      return -2;  // scan result stands
+      return mode == 3 ? -2 : -3;  // scan result stands / no modified neighbour
     }
+  } else if (mode == 5) {
+    if (!(f & kModified)) return -1;
   } else if (mode == 2 && cd != path_code) {
     return -1;
   }
@@ -401,11 +413,10 @@ __device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __rest
 // ids (each wave's run stays contiguous in its bin).
 template <typename Word>
 __global__ void __launch_bounds__(kBlock) k_build_bins(
-    const uint32_t* __restrict__ live, uint32_t n_live, const Word* __restrict__ cand,
-    const uint8_t* __restrict__ flags, const uint8_t* __restrict__ code, const uint8_t* __restrict__ marks,
+    const uint32_t* __restrict__ live, uint32_t n_live, const Word* __restrict__ cand, uint8_t* __restrict__ flags, const uint8_t* __restrict__ code, const uint8_t* __restrict__ marks,
     const Word* __restrict__ unmatch, const uint64_t* __restrict__ off, uint32_t V, int mode,
     const DevTemplate* __restrict__ t, uint32_t* __restrict__ bins, uint32_t* __restrict__ bin_count,
-    const uint32_t* __restrict__ last_scan, unsigned long long* counters) {
+    uint32_t* __restrict__ last_scan, unsigned long long* counters) {
   __shared__ uint32_t s_wave[kBlock / 64][kBins];
   __shared__ uint32_t s_base[kBlock / 64][kBins];
   const int lane = lane_id();
@@ -428,6 +439,13 @@ __global__ void __launch_bounds__(kBlock) k_build_bins(
 #pragma unroll
   for (int i = 0; i < kBinIters; ++i) {
     if (bin[i] == -2) skipped += mode == 0 ? deg[i] : last_scan[vid[i]];
+    if (bin[i] == -3) {  // the scan finds no modified neighbour
+      const uint32_t v = vid[i];
+      const uint8_t f = flags[v];
+      if (f & kScheduled) flags[v] = static_cast<uint8_t>(f & ~kScheduled);
+      last_scan[v] = static_cast<uint32_t>(deg[i]);
+      skipped += deg[i];
+    }
 #pragma unroll
     for (int k = 0; k < kBins; ++k) cnt[k] += __popcll(__ballot(bin[i] == k));
   }
@@ -770,11 +788,12 @@ __global__ void k_lcc_hub_finish(const uint32_t* __restrict__ hubs, uint32_t nhu
 __global__ void __launch_bounds__(kBlock) k_hub_rescan_push(
     const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
     const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
-    const uint32_t* __restrict__ hub_new, uint8_t* __restrict__ rescan, unsigned long long* counters) {
+    const uint32_t* __restrict__ hub_new, const uint8_t* __restrict__ flags, uint8_t* __restrict__ rescan,
+    unsigned long long* counters) {
   uint64_t pushed = 0;
   for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const uint32_t h = chunk_hub[c];
-    if (!hub_new[h]) continue;
+    if (flags ? !(flags[hubs[h]] & kModified) : !hub_new[h]) continue;  // flags given: every modified hub
     const uint32_t v = hubs[h];
     const uint64_t beg = chunk_start[c];
     const uint64_t end = min(off[v + 1], beg + kHubChunk);
@@ -782,6 +801,31 @@ __global__ void __launch_bounds__(kBlock) k_hub_rescan_push(
     if (threadIdx.x == 0) pushed += end - beg;
   }
   wave_add(counters, C_XSLOTS, pushed);
+}
+
+// Vertices of a bin (mode 5: modified) mark the vertices of their rows for a
+// phase 2 rescan, one group of G lanes per vertex.
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_push_rescan(const uint32_t* __restrict__ list,
+                                                        const uint32_t* __restrict__ list_count,
+                                                        const uint64_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ edges,
+                                                        uint8_t* __restrict__ rescan, unsigned long long* counters) {
+  const int gl = lane_id() & (G - 1);
+  const uint32_t n = *list_count;
+  const uint64_t groups_per_grid = (uint64_t)gridDim.x * (blockDim.x / G);
+  uint64_t pushed = 0;
+  for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G; g < n; g += groups_per_grid) {
+    const uint32_t v = list[g];
+    const uint64_t beg = off[v], end = off[v + 1];
+    for (uint64_t e = beg + gl; e < end; e += G) rescan[edges[e]] = 1;
+    if (gl == 0) pushed += end - beg;
+  }
+  for (int o = 32; o > 0; o >>= 1) pushed += __shfl_xor(pushed, o, 64);
+  if (lane_id() == 0 && pushed)
+    atomicAdd(&counters[((blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) & (kCounterSlots - 1)) * kCounterStride +
+                        C_XSLOTS],
+              static_cast<unsigned long long>(pushed));
 }
 
 // ---------------------------------------------------------------------------
@@ -2116,6 +2160,7 @@ struct pj_ctx {
   bool rescan_all = false;  // every rescan mark is set (phase 1 need not push)
   bool reverse_enabled = true;  // keep d_rev for symmetric graphs (PJ_REVERSE_SLOTS=0: no)
   bool labels_consistent = false;  // every candidate of every vertex carries the vertex's label (after LCC0)
+  bool modified_cleared = false;   // no modified flag set since pj_reset_modified
   bool memo_walks = false;  // run every cycle / path step as memoised walks (PJ_MEMO_WALKS=1)
   int memo_log2 = 0;  // first-pass table entries per wave, log2 (PJ_MEMO_LOG2, 6..20; 0: by graph size)
   uint32_t memo_lds_limit = 0;  // PJ_MEMO_LDS_LIMIT: live entries of the LDS tables (0: 768)
@@ -2350,6 +2395,7 @@ int state_reset_t(pj_ctx* ctx) {
   ctx->active_vertices = ctx->V;
   ctx->active_edges = ctx->E;
   ctx->labels_consistent = false;  // every template vertex is a candidate again
+  ctx->modified_cleared = false;
   ctx->live = nullptr;              // every vertex is active again
   ctx->n_live = ctx->V;
   return 0;
@@ -2394,7 +2440,10 @@ void launch_bins(pj_ctx* ctx, int mode) {
   zero_bins(ctx);
   unsigned long long* c = begin_launch(ctx, K_BINS);
   const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(ctx->n_live) + kBinVerts - 1) / kBinVerts);
-  const uint8_t* marks = !ctx->skip_clean ? nullptr : mode == 0 ? ctx->d_dirty : mode == 3 ? ctx->d_rescan : nullptr;
+  const uint8_t* marks = !ctx->skip_clean  ? nullptr
+                         : mode == 0       ? ctx->d_dirty
+                         : mode == 3 || mode == 4 ? ctx->d_rescan
+                                                  : nullptr;
   hipLaunchKernelGGL(k_build_bins<Word>, dim3(std::max<uint32_t>(blocks, 1)), dim3(kBlock), 0, ctx->stream,
                      ctx->live, ctx->n_live, static_cast<const Word*>(ctx->d_cand), ctx->d_flags, ctx->d_code, marks,
                      static_cast<const Word*>(ctx->d_unmatch), ctx->d_off, ctx->V, mode, ctx->d_t, ctx->d_bins,
@@ -2404,8 +2453,8 @@ void launch_bins(pj_ctx* ctx, int mode) {
 
 // Phase 2 over every active vertex (bins + hubs).
 template <typename Word>
-void launch_phase2(pj_ctx* ctx) {
-  launch_bins<Word>(ctx, 3);
+void launch_phase2(pj_ctx* ctx, int mode = 3) {
+  launch_bins<Word>(ctx, mode);
   uint8_t* dirty = ctx->skip_clean ? ctx->d_dirty : nullptr;
   uint8_t* rescan = ctx->skip_clean ? ctx->d_rescan : nullptr;
   uint32_t* last = ctx->d_last_scan;
@@ -2440,6 +2489,30 @@ void launch_phase2(pj_ctx* ctx) {
     end_launch(ctx);
   }
   ctx->rescan_all = false;  // the scans cleared the marks of the vertices they read
+}
+
+// Rescan marks of an NLCC phase 2: cleared, then set on the rows of every
+// modified vertex (bins of mode 5, hubs by chunks).
+template <typename Word>
+void push_modified(pj_ctx* ctx) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, (ctx->V + 4) / 4)), dim3(kBlock), 0, ctx->stream,
+                     reinterpret_cast<uint32_t*>(ctx->d_rescan), static_cast<uint64_t>((ctx->V + 4) / 4), 0u);
+  launch_bins<Word>(ctx, 5);
+  const uint64_t V = ctx->V;
+  const int g = ctx->cus * 8;
+  unsigned long long* c = begin_launch(ctx, K_HUB_PUSH);
+  hipLaunchKernelGGL(k_push_rescan<4>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins, ctx->d_bin_count + 0,
+                     ctx->d_off, ctx->d_edges, ctx->d_rescan, c);
+  hipLaunchKernelGGL(k_push_rescan<16>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
+                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_rescan, c);
+  hipLaunchKernelGGL(k_push_rescan<64>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
+                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_rescan, c);
+  if (ctx->nhubs)
+    hipLaunchKernelGGL(k_hub_rescan_push, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
+                       ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
+                       ctx->d_edges, ctx->d_hub_new, ctx->d_flags, ctx->d_rescan, c);
+  end_launch(ctx);
+  ctx->rescan_all = false;
 }
 
 template <typename Word, int KW>
@@ -2483,7 +2556,7 @@ void launch_lcc_phase1(pj_ctx* ctx) {
       c = begin_launch(ctx, K_HUB_PUSH);
       hipLaunchKernelGGL(k_hub_rescan_push, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
                          ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
-                         ctx->d_edges, ctx->d_hub_new, rescan, c);
+                         ctx->d_edges, ctx->d_hub_new, nullptr, rescan, c);
       end_launch(ctx);
     }
   }
@@ -2927,8 +3000,15 @@ int run_step_t(pj_ctx* ctx, int step, uint32_t index) {
       launch_nlcc_hubs<Word>(ctx, kind, ctx->d_nhubs);
       if (ctx->h_t.nlcc_memo && kind != 3 && launch_nlcc_retry<Word>(ctx, kind)) return -1;
       if (step != PJ_STEP_ENUMERATION) {
-        mark_all_rescan(ctx);  // sources that failed turned modified
-        launch_phase2<Word>(ctx);
+        if (ctx->skip_clean && ctx->modified_cleared) {
+          // Only the failing sources are modified: they mark their rows and
+          // every unmarked vertex has no modified neighbour.
+          push_modified<Word>(ctx);
+          launch_phase2<Word>(ctx, 4);
+        } else {
+          mark_all_rescan(ctx);  // sources that failed turned modified
+          launch_phase2<Word>(ctx);
+        }
       }
       break;
     }
@@ -2983,6 +3063,7 @@ int upload_state_t(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags, con
                      static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->V);
   mark_all_dirty(ctx);
   ctx->labels_consistent = false;  // an uploaded state may hold candidates of other labels
+  ctx->modified_cleared = false;
   ctx->live = nullptr;              // and any vertex may be active
   ctx->n_live = ctx->V;
   PJ_HIP(hipGetLastError());
@@ -3321,6 +3402,7 @@ int pj_reset_modified(pj_ctx* ctx) {
                      ctx->V);
   mark_all_rescan(ctx);
   PJ_HIP(hipGetLastError());
+  ctx->modified_cleared = true;
   return 0;
 }
 
@@ -3335,6 +3417,7 @@ int pj_run_step(pj_ctx* ctx, int step, uint32_t index, pj_algo_results* out) {
     if (install_nlcc(ctx, step, index)) return -1;
   PJ_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   int rc = with_word(ctx, [&](auto w) { return run_step_t<decltype(w)>(ctx, step, index); });
+  ctx->modified_cleared = false;
   if (rc) return rc;
   PJ_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   const size_t blocks = ctx->timing ? static_cast<size_t>(ctx->nlaunch) + 1 : 1;
