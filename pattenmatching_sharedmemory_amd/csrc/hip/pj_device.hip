@@ -2304,19 +2304,34 @@ void free_graph(pj_ctx* ctx) {
 }
 
 // Builds the static hub chunk list from host offsets.
+// The chunk list holds the first chunk of every hub, then the second of every
+// hub with two or more, and so on: the grid reaches the start of every row
+// first, which lets the phase 2 scan (first hit in slot order) skip the later
+// chunks of rows whose hit is already found.
 int setup_hubs(pj_ctx* ctx, const uint64_t* h_off) {
   std::vector<uint32_t> hubs, chunk_hub;
   std::vector<uint64_t> chunk_start;
+  uint64_t max_chunks = 0;
   for (uint32_t v = 0; v < ctx->V; ++v) {
     uint64_t d = h_off[v + 1] - h_off[v];
     if (d <= kHubDegree) continue;
-    uint32_t h = static_cast<uint32_t>(hubs.size());
     hubs.push_back(v);
-    for (uint64_t s = h_off[v]; s < h_off[v + 1]; s += kHubChunk) {
-      chunk_hub.push_back(h);
-      chunk_start.push_back(s);
-    }
+    max_chunks = std::max<uint64_t>(max_chunks, (d + kHubChunk - 1) / kHubChunk);
   }
+  // hubs by chunk count, descending: those with more than r chunks are a prefix
+  std::vector<uint32_t> by_count(hubs.size());
+  std::vector<uint64_t> count(hubs.size());
+  for (uint32_t h = 0; h < hubs.size(); ++h) {
+    by_count[h] = h;
+    count[h] = (h_off[hubs[h] + 1] - h_off[hubs[h]] + kHubChunk - 1) / kHubChunk;
+  }
+  std::stable_sort(by_count.begin(), by_count.end(), [&](uint32_t x, uint32_t y) { return count[x] > count[y]; });
+  for (uint64_t r = 0; r < max_chunks; ++r)
+    for (uint32_t i = 0; i < by_count.size() && count[by_count[i]] > r; ++i) {
+      const uint32_t h = by_count[i];
+      chunk_hub.push_back(h);
+      chunk_start.push_back(h_off[hubs[h]] + r * kHubChunk);
+    }
   ctx->nhubs = static_cast<uint32_t>(hubs.size());
   ctx->nchunks = static_cast<uint32_t>(chunk_hub.size());
   if (dev_alloc(ctx, &ctx->d_hubs, hubs.size()) || dev_alloc(ctx, &ctx->d_chunk_hub, chunk_hub.size()) ||
