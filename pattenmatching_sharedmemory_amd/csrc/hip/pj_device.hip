@@ -53,13 +53,13 @@ constexpr int kMaxLaunches = 32;  // timed launches per step
 
 // Kernel kinds reported by pj_kernel_stats.
 enum KernelId {
-  K_LCC0, K_BINS, K_LCC_P1_G4, K_LCC_P1_G16, K_LCC_P1_G64, K_LCC_HUB, K_LCC_HUB_FINISH,
-  K_P2_G4, K_P2_G16, K_P2_G64, K_P2_HUB, K_P2_HUB_FINISH, K_NLCC, K_NLCC_HUB, K_HUB_PUSH, K_VIEW, K_COUNT
+  K_LCC0, K_BINS, K_LCC_P1, K_LCC_HUB, K_LCC_HUB_FINISH, K_P2, K_P2_HUB_FINISH, K_NLCC, K_NLCC_HUB, K_HUB_PUSH,
+  K_VIEW, K_COUNT
 };
 const char* const kKernelNames[K_COUNT] = {
-    "lcc0", "build_bins", "lcc_phase1_g4", "lcc_phase1_g16", "lcc_phase1_g64", "lcc_phase1_hub_chunks",
-    "lcc_phase1_hub_finish", "phase2_g4", "phase2_g16", "phase2_g64", "phase2_hub_chunks", "phase2_hub_finish",
-    "nlcc_walk", "nlcc_walk_hubs", "hub_rescan_push", "active_view"};
+    "lcc0",      "build_bins",        "lcc_phase1", "lcc_phase1_hub_chunks", "lcc_phase1_hub_finish",
+    "phase2",    "phase2_hub_finish", "nlcc_walk",  "nlcc_walk_hubs",        "hub_rescan_push",
+    "active_view"};
 
 // Template tables in device memory.
 struct DevTemplate {
@@ -574,26 +574,23 @@ __device__ __forceinline__ void count_labels(uint64_t (&cnt)[KW], bool counted, 
 // of a neighbour u is that of any template vertex in C(u) and is read from a
 // table in LDS instead of gathering code[u].
 template <typename Word, int G, int KW>
-__global__ void __launch_bounds__(kBlock) k_lcc_phase1(
-    const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, const uint64_t* __restrict__ off,
-    const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code, bool derive, const Word* __restrict__ cand,
-    Word* __restrict__ unmatch, uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty,
-    uint8_t* __restrict__ rescan, uint32_t* __restrict__ ea, const DevTemplate* __restrict__ t,
-    unsigned long long* counters) {
-  __shared__ uint8_t s_pcode[64];
-  if (threadIdx.x < 64) s_pcode[threadIdx.x] = t->pcode[threadIdx.x];
-  __syncthreads();
+__device__ void lcc_phase1_bin(uint32_t block, uint32_t nblocks, const uint8_t* s_pcode,
+                               const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
+                               const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+                               const uint8_t* __restrict__ code, bool derive, const Word* __restrict__ cand,
+                               Word* __restrict__ unmatch, uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty,
+                               uint8_t* __restrict__ rescan, uint32_t* __restrict__ ea,
+                               const DevTemplate* __restrict__ t, unsigned long long* counters) {
   const int lane = lane_id();
   const int gl = lane & (G - 1);
   const int gbase = lane & ~(G - 1);
   const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
   const uint32_t n = *list_count;
   const uint32_t K = t->K;
-  if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_ITEMS] = n;
-  const uint64_t groups_per_grid = (uint64_t)gridDim.x * (blockDim.x / G);
+  const uint64_t groups_per_grid = (uint64_t)nblocks * (blockDim.x / G);
   const uint64_t span = ((n + groups_per_grid - 1) / groups_per_grid) * groups_per_grid;
   uint64_t elim_total = 0, slots_total = 0, xslots = 0;
-  for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G; g < span; g += groups_per_grid) {
+  for (uint64_t g = (block * (uint64_t)blockDim.x + threadIdx.x) / G; g < span; g += groups_per_grid) {
     if (g < n) {
       const uint32_t v = list[g];
       const Word cv = cand[v];
@@ -670,23 +667,22 @@ __global__ void __launch_bounds__(kBlock) k_lcc_phase1(
 // Label counts and eliminations are summed in hub_count[h][code] and
 // hub_elim[h]; k_lcc_hub_finish evaluates the requirements and clears them.
 
-// `derive`: neighbour codes from candidates, as in k_lcc_phase1.
+// `derive`: neighbour codes from candidates, as in lcc_phase1_bin.
 template <typename Word>
-__global__ void __launch_bounds__(kBlock) k_lcc_hub_chunks(
-    const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
-    const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
-    const uint8_t* __restrict__ code, bool derive, const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
-    const uint8_t* __restrict__ dirty, uint32_t* __restrict__ ea, uint32_t* __restrict__ hub_count,
-    uint32_t* __restrict__ hub_elim, const DevTemplate* __restrict__ t, unsigned long long* counters) {
-  __shared__ uint8_t s_pcode[64];
-  if (threadIdx.x < 64) s_pcode[threadIdx.x] = t->pcode[threadIdx.x];
-  __syncthreads();
+__device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* s_pcode,
+                               const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start,
+                               uint32_t nchunks, const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off,
+                               const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code, bool derive,
+                               const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
+                               const uint8_t* __restrict__ dirty, uint32_t* __restrict__ ea,
+                               uint32_t* __restrict__ hub_count, uint32_t* __restrict__ hub_elim,
+                               const DevTemplate* __restrict__ t, unsigned long long* counters) {
   // Each thread owns slots threadIdx.x + k * kBlock of the chunk (k < kPer);
   // ids and edge-active words of all of them are loaded before the gathers.
   constexpr int kPer = kHubChunk / kBlock;
   const int lane = lane_id();
   uint64_t elim_total = 0, slots_total = 0, items = 0;
-  for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+  for (uint32_t c = block; c < nchunks; c += nblocks) {
     const uint32_t h = chunk_hub[c];
     const uint32_t v = hubs[h];
     const Word cv = cand[v];
@@ -749,6 +745,45 @@ __global__ void __launch_bounds__(kBlock) k_lcc_hub_chunks(
   wave_add(counters, C_EDGE, elim_total);
   wave_add(counters, C_SLOTS, slots_total);
   wave_add(counters, C_ITEMS, items);
+}
+
+// Hub rows of LCC phase 1 in a launch of their own (fused with the bins,
+// the unrolled chunk loop raises the register count of the whole kernel).
+template <typename Word>
+__global__ void __launch_bounds__(kBlock) k_lcc_hub_chunks(
+    const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
+    const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+    const uint8_t* __restrict__ code, bool derive, const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
+    const uint8_t* __restrict__ dirty, uint32_t* __restrict__ ea, uint32_t* __restrict__ hub_count,
+    uint32_t* __restrict__ hub_elim, const DevTemplate* __restrict__ t, unsigned long long* counters) {
+  __shared__ uint8_t s_pcode[64];
+  if (threadIdx.x < 64) s_pcode[threadIdx.x] = t->pcode[threadIdx.x];
+  __syncthreads();
+  lcc_hub_chunks<Word>(blockIdx.x, gridDim.x, s_pcode, chunk_hub, chunk_start, nchunks, hubs, off, edges, code, derive,
+                       cand, flags, dirty, ea, hub_count, hub_elim, t, counters);
+}
+
+// LCC phase 1 of the degree bins in one launch: G = 64, 16 and 4 lanes per
+// vertex, each bin grid-strided over the whole grid.
+template <typename Word, int KW>
+__global__ void __launch_bounds__(kBlock) k_lcc_phase1_all(
+    const uint32_t* __restrict__ bins, const uint32_t* __restrict__ bin_count, uint32_t V,
+    const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code,
+    bool derive, const Word* __restrict__ cand, Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
+    uint8_t* __restrict__ dirty, uint8_t* __restrict__ rescan, uint32_t* __restrict__ ea,
+    const DevTemplate* __restrict__ t, unsigned long long* counters) {
+  __shared__ uint8_t s_pcode[64];
+  if (threadIdx.x < 64) s_pcode[threadIdx.x] = t->pcode[threadIdx.x];
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    atomicAdd(&counters[C_ITEMS], static_cast<unsigned long long>(bin_count[0]) + bin_count[1] + bin_count[2]);
+  const uint64_t V64 = V;
+  lcc_phase1_bin<Word, 64, KW>(blockIdx.x, gridDim.x, s_pcode, bins + 2 * V64, bin_count + 2, off, edges, code,
+                               derive, cand, unmatch, flags, dirty, rescan, ea, t, counters);
+  lcc_phase1_bin<Word, 16, KW>(blockIdx.x, gridDim.x, s_pcode, bins + V64, bin_count + 1, off, edges, code, derive,
+                               cand, unmatch, flags, dirty, rescan, ea, t, counters);
+  lcc_phase1_bin<Word, 4, KW>(blockIdx.x, gridDim.x, s_pcode, bins, bin_count + 0, off, edges, code, derive, cand,
+                              unmatch, flags, dirty, rescan, ea, t, counters);
 }
 
 template <typename Word, int KW>
@@ -836,21 +871,21 @@ __global__ void __launch_bounds__(kBlock) k_push_rescan(const uint32_t* __restri
 // (active or not).  Modified flags are left set.
 
 template <typename Word, int G>
-__global__ void __launch_bounds__(kBlock) k_phase2(
-    const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, const uint64_t* __restrict__ off,
-    const uint32_t* __restrict__ edges, Word* __restrict__ cand, Word* __restrict__ unmatch,
-    uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty, uint8_t* __restrict__ rescan,
-    uint32_t* __restrict__ last_scan, const uint32_t* __restrict__ ea, unsigned long long* counters) {
+__device__ void phase2_bin(uint32_t block, uint32_t nblocks, const uint32_t* __restrict__ list,
+                           const uint32_t* __restrict__ list_count, const uint64_t* __restrict__ off,
+                           const uint32_t* __restrict__ edges, Word* __restrict__ cand, Word* __restrict__ unmatch,
+                           uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty, uint8_t* __restrict__ rescan,
+                           uint32_t* __restrict__ last_scan, const uint32_t* __restrict__ ea,
+                           unsigned long long* counters) {
   const int lane = lane_id();
   const int gl = lane & (G - 1);
   const int gbase = lane & ~(G - 1);
   const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
   const uint32_t n = *list_count;
-  if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_ITEMS] = n;
-  const uint64_t groups_per_grid = (uint64_t)gridDim.x * (blockDim.x / G);
+  const uint64_t groups_per_grid = (uint64_t)nblocks * (blockDim.x / G);
   const uint64_t span = ((n + groups_per_grid - 1) / groups_per_grid) * groups_per_grid;
   uint64_t velim = 0, melim = 0, slots = 0, xslots = 0;
-  for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G; g < span; g += groups_per_grid) {
+  for (uint64_t g = (block * (uint64_t)blockDim.x + threadIdx.x) / G; g < span; g += groups_per_grid) {
     if (g < n) {
       const uint32_t v = list[g];
       const uint8_t f = flags[v];
@@ -927,14 +962,16 @@ __global__ void __launch_bounds__(kBlock) k_phase2(
 // such a slot in the row (hub_hit[h], 0xFFFFFFFF when none) and
 // k_phase2_hub_finish applies the result and resets hub_hit.
 template <typename Word>
-__global__ void __launch_bounds__(kBlock) k_phase2_hub_chunks(
-    const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
-    const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
-    const Word* __restrict__ cand, const Word* __restrict__ unmatch, const uint8_t* __restrict__ flags,
-    uint8_t* __restrict__ dirty, const uint8_t* __restrict__ rescan, const uint32_t* __restrict__ ea,
-    uint32_t* __restrict__ hub_hit, unsigned long long* counters) {
+__device__ void phase2_hub_chunks(uint32_t block, uint32_t nblocks, const uint32_t* __restrict__ chunk_hub,
+                                  const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
+                                  const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off,
+                                  const uint32_t* __restrict__ edges, const Word* __restrict__ cand,
+                                  const Word* __restrict__ unmatch, const uint8_t* __restrict__ flags,
+                                  uint8_t* __restrict__ dirty, const uint8_t* __restrict__ rescan,
+                                  const uint32_t* __restrict__ ea, uint32_t* __restrict__ hub_hit,
+                                  unsigned long long* counters) {
   uint64_t examined = 0;  // slots read here; the reference-order count is taken in k_phase2_hub_finish
-  for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+  for (uint32_t c = block; c < nchunks; c += nblocks) {
     const uint32_t h = chunk_hub[c];
     const uint32_t v = hubs[h];
     if (!cand[v]) continue;
@@ -977,6 +1014,32 @@ __global__ void __launch_bounds__(kBlock) k_phase2_hub_chunks(
     if (threadIdx.x == 0) examined += end - beg;
   }
   wave_add(counters, C_XSLOTS, examined);
+}
+
+// Phase 2 in one launch: hub chunks first, then the degree bins of G = 64,
+// 16 and 4 lanes per vertex.  The hub chunks touch only hub rows and the
+// bins only their own vertices; every vertex reads of its neighbours only
+// the modified bit, which phase 2 leaves as it is.
+template <typename Word>
+__global__ void __launch_bounds__(kBlock) k_phase2_all(
+    const uint32_t* __restrict__ bins, const uint32_t* __restrict__ bin_count, uint32_t V,
+    const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
+    const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+    Word* __restrict__ cand, Word* __restrict__ unmatch, uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty,
+    uint8_t* __restrict__ rescan, uint32_t* __restrict__ last_scan, const uint32_t* __restrict__ ea,
+    uint32_t* __restrict__ hub_hit, unsigned long long* counters) {
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    atomicAdd(&counters[C_ITEMS], static_cast<unsigned long long>(bin_count[0]) + bin_count[1] + bin_count[2]);
+  const uint64_t V64 = V;
+  if (nchunks)
+    phase2_hub_chunks<Word>(blockIdx.x, gridDim.x, chunk_hub, chunk_start, nchunks, hubs, off, edges, cand, unmatch,
+                            flags, dirty, rescan, ea, hub_hit, counters);
+  phase2_bin<Word, 64>(blockIdx.x, gridDim.x, bins + 2 * V64, bin_count + 2, off, edges, cand, unmatch, flags, dirty,
+                       rescan, last_scan, ea, counters);
+  phase2_bin<Word, 16>(blockIdx.x, gridDim.x, bins + V64, bin_count + 1, off, edges, cand, unmatch, flags, dirty,
+                       rescan, last_scan, ea, counters);
+  phase2_bin<Word, 4>(blockIdx.x, gridDim.x, bins, bin_count + 0, off, edges, cand, unmatch, flags, dirty, rescan,
+                      last_scan, ea, counters);
 }
 
 template <typename Word>
@@ -2475,28 +2538,13 @@ void launch_phase2(pj_ctx* ctx, int mode = 3) {
   uint32_t* last = ctx->d_last_scan;
   Word* cand = static_cast<Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
-  const uint64_t V = ctx->V;
-  const int g = ctx->cus * 8;
-  unsigned long long* c = begin_launch(ctx, K_P2_G4);
-  hipLaunchKernelGGL((k_phase2<Word, 4>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins, ctx->d_bin_count + 0,
-                     ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, dirty, rescan, last, ctx->d_ea, c);
-  end_launch(ctx);
-  c = begin_launch(ctx, K_P2_G16);
-  hipLaunchKernelGGL((k_phase2<Word, 16>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
-                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, dirty, rescan, last,
-                     ctx->d_ea, c);
-  end_launch(ctx);
-  c = begin_launch(ctx, K_P2_G64);
-  hipLaunchKernelGGL((k_phase2<Word, 64>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
-                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, dirty, rescan, last,
-                     ctx->d_ea, c);
+  unsigned long long* c = begin_launch(ctx, K_P2);
+  hipLaunchKernelGGL(k_phase2_all<Word>, dim3(ctx->cus * 8), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
+                     ctx->d_bin_count, ctx->V, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nhubs ? ctx->nchunks : 0u,
+                     ctx->d_hubs, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, dirty, rescan, last, ctx->d_ea,
+                     ctx->d_hub_hit, c);
   end_launch(ctx);
   if (ctx->nhubs) {
-    c = begin_launch(ctx, K_P2_HUB);
-    hipLaunchKernelGGL(k_phase2_hub_chunks<Word>, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
-                       ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
-                       ctx->d_edges, cand, unm, ctx->d_flags, dirty, rescan, ctx->d_ea, ctx->d_hub_hit, c);
-    end_launch(ctx);
     c = begin_launch(ctx, K_P2_HUB_FINISH);
     hipLaunchKernelGGL(k_phase2_hub_finish<Word>, dim3(grid_for(ctx, ctx->nhubs)), dim3(kBlock), 0, ctx->stream,
                        ctx->d_hubs, ctx->nhubs, ctx->d_off, cand, unm, ctx->d_flags, dirty, rescan, last,
@@ -2538,21 +2586,9 @@ void launch_lcc_phase1(pj_ctx* ctx) {
   uint8_t* rescan = ctx->skip_clean && !ctx->rescan_all ? ctx->d_rescan : nullptr;  // push target
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
-  const uint64_t V = ctx->V;
-  const int g = ctx->cus * 8;
-  unsigned long long* c = begin_launch(ctx, K_LCC_P1_G4);
-  hipLaunchKernelGGL((k_lcc_phase1<Word, 4, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
-                     ctx->d_bin_count + 0, ctx->d_off, ctx->d_edges, ctx->d_code, derive, cand, unm, ctx->d_flags,
-                     dirty, rescan, ctx->d_ea, ctx->d_t, c);
-  end_launch(ctx);
-  c = begin_launch(ctx, K_LCC_P1_G16);
-  hipLaunchKernelGGL((k_lcc_phase1<Word, 16, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
-                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_code, derive, cand, unm, ctx->d_flags,
-                     dirty, rescan, ctx->d_ea, ctx->d_t, c);
-  end_launch(ctx);
-  c = begin_launch(ctx, K_LCC_P1_G64);
-  hipLaunchKernelGGL((k_lcc_phase1<Word, 64, KW>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
-                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_code, derive, cand, unm, ctx->d_flags,
+  unsigned long long* c = begin_launch(ctx, K_LCC_P1);
+  hipLaunchKernelGGL((k_lcc_phase1_all<Word, KW>), dim3(ctx->cus * 8), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
+                     ctx->d_bin_count, ctx->V, ctx->d_off, ctx->d_edges, ctx->d_code, derive, cand, unm, ctx->d_flags,
                      dirty, rescan, ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
   if (ctx->nhubs) {

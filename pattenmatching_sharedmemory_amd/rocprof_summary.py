@@ -9,7 +9,7 @@
         KiB per dispatch), each scaled by what the same counter reads for a
         plain device copy of N bytes (the copy passes)
 
-Kernels are reported under the names pj_kernel_stats uses (lcc_phase1_g64,
+Kernels are reported under the names pj_kernel_stats uses (lcc_phase1,
 nlcc_walk, ...), so the figures line up with the live per-kernel timing.
 """
 from __future__ import annotations
@@ -24,9 +24,10 @@ from collections import defaultdict
 _TEMPLATE_NAMES = {
     "k_lcc0": "lcc0",
     "k_build_bins": "build_bins",
+    "k_lcc_phase1_all": "lcc_phase1",
     "k_lcc_hub_chunks": "lcc_phase1_hub_chunks",
     "k_lcc_hub_finish": "lcc_phase1_hub_finish",
-    "k_phase2_hub_chunks": "phase2_hub_chunks",
+    "k_phase2_all": "phase2",
     "k_phase2_hub_finish": "phase2_hub_finish",
     "k_triangles": "nlcc_walk",
 }
@@ -56,10 +57,6 @@ def stat_name(full: str, workgroup: int | None = None) -> str:
     args = [a.strip() for a in args]
     if base in _TEMPLATE_NAMES:
         return _TEMPLATE_NAMES[base]
-    if base == "k_lcc_phase1" and len(args) >= 2:
-        return f"lcc_phase1_g{args[1]}"
-    if base == "k_phase2" and len(args) >= 2:
-        return f"phase2_g{args[1]}"
     if base == "k_nlcc":
         return "nlcc_walk_hubs" if workgroup == 64 else "nlcc_walk"
     return s
