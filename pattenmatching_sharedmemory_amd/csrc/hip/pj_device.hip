@@ -1330,7 +1330,7 @@ __device__ __forceinline__ RowScan scan_row(uint64_t from, uint64_t end, const u
 }
 
 // Whether row(a) holds an active slot to b, by binary search (rows sorted
-// ascending); `probes` counts the slots read.
+// ascending; ea null: every slot is active); `probes` counts the slots read.
 __device__ __forceinline__ bool active_slot(uint32_t a, uint32_t b, const uint64_t* __restrict__ off,
                                             const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
                                             uint64_t& probes) {
@@ -1347,7 +1347,7 @@ __device__ __forceinline__ bool active_slot(uint32_t a, uint32_t b, const uint64
   for (; lo < end; ++lo) {
     ++probes;
     if (edges[lo] != b) break;
-    if (ea_test(ea, lo)) return true;
+    if (walk_slot(ea, lo)) return true;
   }
   return false;
 }
@@ -1695,7 +1695,7 @@ template <typename Word>
 __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ mv, const uint32_t* __restrict__ vx,
                          const uint8_t* __restrict__ tb, const uint8_t* __restrict__ mk,
                          const uint32_t* __restrict__ mb, const uint32_t* __restrict__ hidx, bool hyp,
-                         const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+                         bool sorted, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
                          const uint32_t* __restrict__ ea, const Word* __restrict__ cand, Word* already,
                          MemoTable& m, uint64_t& slots) {
   const int lane = lane_id();
@@ -1774,9 +1774,18 @@ __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ m
         continue;
       }
     } else if (mo == PJ_MOVE_CHECK || mo == kMoveClose) {
+      // an existence test (the walk goes on from cur either way): on sorted
+      // rows a binary search, else a scan in row order
       const uint32_t target = mo == kMoveClose ? s : rl32(hist, static_cast<int>(mb[pos]));
-      RowScan r = scan_row<Word>(off[cur], off[cur + 1], edges, ea, cand, -1, target, 0xFFFFFFFFu, slots);
-      if (r.found) {
+      bool hit;
+      if (sorted) {
+        uint64_t probes = 0;
+        hit = active_slot(cur, target, off, edges, ea, probes);
+        slots += probes;
+      } else {
+        hit = scan_row<Word>(off[cur], off[cur + 1], edges, ea, cand, -1, target, 0xFFFFFFFFu, slots).found;
+      }
+      if (hit) {
         pos = uniform(pos + 1);
         if (lane == pos) {
           st_v = cur;
@@ -1925,8 +1934,8 @@ __device__ Word memo_source(uint32_t s, Word cs, const DevTemplate* __restrict__
     m.next_node = 0;
     m.overflow = false;
     const int r = memo_walk<Word>(s, len, t->walk_move + o, t->walk_vertex + o, t->walk_test + o, t->walk_mark + o,
-                                  t->walk_mb + o, t->walk_hidx + o, t->walk_hyp != 0, off, edges, ea, cand, already,
-                                  m, slots);
+                                  t->walk_mb + o, t->walk_hidx + o, t->walk_hyp != 0, t->rows_sorted != 0, off, edges,
+                                  ea, cand, already, m, slots);
     if (r < 0) {
       *overflow = true;
       return 0;
