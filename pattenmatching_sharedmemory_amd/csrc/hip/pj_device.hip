@@ -682,65 +682,88 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
   constexpr int kPer = kHubChunk / kBlock;
   const int lane = lane_id();
   uint64_t elim_total = 0, slots_total = 0, items = 0;
-  for (uint32_t c = block; c < nchunks; c += nblocks) {
-    const uint32_t h = chunk_hub[c];
-    const uint32_t v = hubs[h];
-    const Word cv = cand[v];
-    if (!cv || !(flags[v] & kScheduled) || (dirty && !dirty[v])) continue;  // block-uniform
-    Word U = 0;
-    for (Word m = cv; m; m &= m - 1) U |= static_cast<Word>(t->nbr[ctz<Word>(m)]);
-    const uint64_t beg = chunk_start[c];
-    const uint64_t end = min(off[v + 1], beg + kHubChunk);
-    uint32_t word[kPer], u[kPer];
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const uint64_t e = beg + threadIdx.x + k * kBlock;
-      word[k] = e < end ? ea[e >> 5] : 0u;
-      u[k] = e < end ? edges[e] : 0u;
+  // The activity of up to kBlock of this block's chunks (block + k * nblocks)
+  // is tested at once, one thread each; the block then works through the
+  // active ones in order.
+  __shared__ uint64_t s_act[kBlock / 64];
+  const uint32_t per = block < nchunks ? (nchunks - 1 - block) / nblocks + 1 : 0;
+  for (uint32_t k0 = 0; k0 < per; k0 += kBlock) {
+    {
+      const uint32_t kk = k0 + threadIdx.x;
+      bool act = false;
+      if (kk < per) {
+        const uint32_t c = block + kk * nblocks;
+        const uint32_t v = hubs[chunk_hub[c]];
+        act = cand[v] && (flags[v] & kScheduled) && (!dirty || dirty[v]);
+      }
+      const uint64_t am = __ballot(act);
+      if (lane_id() == 0) s_act[threadIdx.x >> 6] = am;
+      __syncthreads();
     }
-    Word cu[kPer];
-    uint8_t lb[kPer];
+    for (int aw = 0; aw < kBlock / 64; ++aw) {
+      for (uint64_t mm = s_act[aw]; mm; mm &= mm - 1) {
+        const uint32_t c = block + (k0 + aw * 64 + static_cast<uint32_t>(__builtin_ctzll(mm))) * nblocks;
+        const uint32_t h = chunk_hub[c];
+        const uint32_t v = hubs[h];
+        const Word cv = cand[v];
+        if (!cv || !(flags[v] & kScheduled) || (dirty && !dirty[v])) continue;  // block-uniform
+        Word U = 0;
+        for (Word m = cv; m; m &= m - 1) U |= static_cast<Word>(t->nbr[ctz<Word>(m)]);
+        const uint64_t beg = chunk_start[c];
+        const uint64_t end = min(off[v + 1], beg + kHubChunk);
+        uint32_t word[kPer], u[kPer];
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const uint64_t e = beg + threadIdx.x + k * kBlock;
-      const bool act = (word[k] >> (e & 31)) & 1u;
-      cu[k] = act ? cand[u[k]] : Word(0);
-      lb[k] = act && !derive ? code[u[k]] : kNoCode;
-    }
-    if (derive) {
+        for (int k = 0; k < kPer; ++k) {
+          const uint64_t e = beg + threadIdx.x + k * kBlock;
+          word[k] = e < end ? ea[e >> 5] : 0u;
+          u[k] = e < end ? edges[e] : 0u;
+        }
+        Word cu[kPer];
+        uint8_t lb[kPer];
 #pragma unroll
-      for (int k = 0; k < kPer; ++k)
-        if (cu[k]) lb[k] = s_pcode[ctz<Word>(cu[k])];
-    }
-    uint32_t elim = 0, mine = 0;  // lane L counts label code L
+        for (int k = 0; k < kPer; ++k) {
+          const uint64_t e = beg + threadIdx.x + k * kBlock;
+          const bool act = (word[k] >> (e & 31)) & 1u;
+          cu[k] = act ? cand[u[k]] : Word(0);
+          lb[k] = act && !derive ? code[u[k]] : kNoCode;
+        }
+        if (derive) {
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      bool counted = false, el = false;
-      if (cu[k]) {
-        if ((cu[k] & U) == 0) {
-          el = true;
-          ++elim;
-        } else {
-          counted = lb[k] != kNoCode;
+          for (int k = 0; k < kPer; ++k)
+            if (cu[k]) lb[k] = s_pcode[ctz<Word>(cu[k])];
+        }
+        uint32_t elim = 0, mine = 0;  // lane L counts label code L
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+          bool counted = false, el = false;
+          if (cu[k]) {
+            if ((cu[k] & U) == 0) {
+              el = true;
+              ++elim;
+            } else {
+              counted = lb[k] != kNoCode;
+            }
+          }
+          clear_slots<64>(ea, beg + k * kBlock + (threadIdx.x & ~63u), el, lane, 0);
+          // one ballot per distinct label code among the counted slots
+          for (uint64_t m = __ballot(counted); m; m = __ballot(counted)) {
+            const uint32_t L = rl32(lb[k], __builtin_ctzll(m));
+            const uint64_t same = __ballot(counted && lb[k] == L);
+            if (lane == static_cast<int>(L)) mine += __popcll(same);
+            counted = counted && lb[k] != L;
+          }
+        }
+        if (mine) atomicAdd(&hub_count[h * kMaxLabels + lane], mine);
+        for (int o = 32; o > 0; o >>= 1) elim += __shfl_xor(elim, o, 64);
+        if (lane == 0 && elim) atomicAdd(&hub_elim[h], elim);
+        elim_total += lane == 0 ? elim : 0u;
+        if (threadIdx.x == 0) {
+          slots_total += end - beg;
+          ++items;
         }
       }
-      clear_slots<64>(ea, beg + k * kBlock + (threadIdx.x & ~63u), el, lane, 0);
-      // one ballot per distinct label code among the counted slots
-      for (uint64_t m = __ballot(counted); m; m = __ballot(counted)) {
-        const uint32_t L = rl32(lb[k], __builtin_ctzll(m));
-        const uint64_t same = __ballot(counted && lb[k] == L);
-        if (lane == static_cast<int>(L)) mine += __popcll(same);
-        counted = counted && lb[k] != L;
-      }
     }
-    if (mine) atomicAdd(&hub_count[h * kMaxLabels + lane], mine);
-    for (int o = 32; o > 0; o >>= 1) elim += __shfl_xor(elim, o, 64);
-    if (lane == 0 && elim) atomicAdd(&hub_elim[h], elim);
-    elim_total += lane == 0 ? elim : 0u;
-    if (threadIdx.x == 0) {
-      slots_total += end - beg;
-      ++items;
-    }
+    __syncthreads();
   }
   wave_add(counters, C_EDGE, elim_total);
   wave_add(counters, C_SLOTS, slots_total);
@@ -826,14 +849,37 @@ __global__ void __launch_bounds__(kBlock) k_hub_rescan_push(
     const uint32_t* __restrict__ hub_new, const uint8_t* __restrict__ flags, uint8_t* __restrict__ rescan,
     unsigned long long* counters) {
   uint64_t pushed = 0;
-  for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    const uint32_t h = chunk_hub[c];
-    if (flags ? !(flags[hubs[h]] & kModified) : !hub_new[h]) continue;  // flags given: every modified hub
-    const uint32_t v = hubs[h];
-    const uint64_t beg = chunk_start[c];
-    const uint64_t end = min(off[v + 1], beg + kHubChunk);
-    for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) rescan[edges[e]] = 1;
-    if (threadIdx.x == 0) pushed += end - beg;
+  // The activity of up to kBlock of this block's chunks (blockIdx.x + k * gridDim.x)
+  // is tested at once, one thread each; the block then works through the
+  // active ones in order.
+  __shared__ uint64_t s_act[kBlock / 64];
+  const uint32_t per = blockIdx.x < nchunks ? (nchunks - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+  for (uint32_t k0 = 0; k0 < per; k0 += kBlock) {
+    {
+      const uint32_t kk = k0 + threadIdx.x;
+      bool act = false;
+      if (kk < per) {
+        const uint32_t c = blockIdx.x + kk * gridDim.x;
+        const uint32_t h = chunk_hub[c];
+        act = flags ? (flags[hubs[h]] & kModified) != 0 : hub_new[h] != 0;
+      }
+      const uint64_t am = __ballot(act);
+      if (lane_id() == 0) s_act[threadIdx.x >> 6] = am;
+      __syncthreads();
+    }
+    for (int aw = 0; aw < kBlock / 64; ++aw) {
+      for (uint64_t mm = s_act[aw]; mm; mm &= mm - 1) {
+        const uint32_t c = blockIdx.x + (k0 + aw * 64 + static_cast<uint32_t>(__builtin_ctzll(mm))) * gridDim.x;
+        const uint32_t h = chunk_hub[c];
+        if (flags ? !(flags[hubs[h]] & kModified) : !hub_new[h]) continue;  // flags given: every modified hub
+        const uint32_t v = hubs[h];
+        const uint64_t beg = chunk_start[c];
+        const uint64_t end = min(off[v + 1], beg + kHubChunk);
+        for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) rescan[edges[e]] = 1;
+        if (threadIdx.x == 0) pushed += end - beg;
+      }
+    }
+    __syncthreads();
   }
   wave_add(counters, C_XSLOTS, pushed);
 }
@@ -971,47 +1017,70 @@ __device__ void phase2_hub_chunks(uint32_t block, uint32_t nblocks, const uint32
                                   const uint32_t* __restrict__ ea, uint32_t* __restrict__ hub_hit,
                                   unsigned long long* counters) {
   uint64_t examined = 0;  // slots read here; the reference-order count is taken in k_phase2_hub_finish
-  for (uint32_t c = block; c < nchunks; c += nblocks) {
-    const uint32_t h = chunk_hub[c];
-    const uint32_t v = hubs[h];
-    if (!cand[v]) continue;
-    if (flags[v] & kModified) {
-      if (dirty && unmatch[v]) {  // the hub's candidates change: dirty its neighbours
+  // The activity of up to kBlock of this block's chunks (block + k * nblocks)
+  // is tested at once, one thread each; the block then works through the
+  // active ones in order.
+  __shared__ uint64_t s_act[kBlock / 64];
+  const uint32_t per = block < nchunks ? (nchunks - 1 - block) / nblocks + 1 : 0;
+  for (uint32_t k0 = 0; k0 < per; k0 += kBlock) {
+    {
+      const uint32_t kk = k0 + threadIdx.x;
+      bool act = false;
+      if (kk < per) {
+        const uint32_t c = block + kk * nblocks;
+        const uint32_t v = hubs[chunk_hub[c]];
+        if (cand[v]) act = (flags[v] & kModified) ? (dirty && unmatch[v]) : (!rescan || rescan[v]);
+      }
+      const uint64_t am = __ballot(act);
+      if (lane_id() == 0) s_act[threadIdx.x >> 6] = am;
+      __syncthreads();
+    }
+    for (int aw = 0; aw < kBlock / 64; ++aw) {
+      for (uint64_t mm = s_act[aw]; mm; mm &= mm - 1) {
+        const uint32_t c = block + (k0 + aw * 64 + static_cast<uint32_t>(__builtin_ctzll(mm))) * nblocks;
+        const uint32_t h = chunk_hub[c];
+        const uint32_t v = hubs[h];
+        if (!cand[v]) continue;
+        if (flags[v] & kModified) {
+          if (dirty && unmatch[v]) {  // the hub's candidates change: dirty its neighbours
+            const uint64_t beg = chunk_start[c];
+            const uint64_t end = min(off[v + 1], beg + kHubChunk);
+            for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) dirty[edges[e]] = 1;
+            if (threadIdx.x == 0) examined += end - beg;
+          }
+          continue;
+        }
+        if (rescan && !rescan[v]) continue;  // scan result stands (counted in k_build_bins)
+        const uint64_t row = off[v];
         const uint64_t beg = chunk_start[c];
         const uint64_t end = min(off[v + 1], beg + kHubChunk);
-        for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) dirty[edges[e]] = 1;
+        // A hit already found in an earlier chunk of the row decides the scan:
+        // this chunk cannot lower the minimum (one load per wave).
+        if (*reinterpret_cast<const volatile uint32_t*>(&hub_hit[h]) < beg - row) continue;
+        constexpr int kPer = kHubChunk / kBlock;
+        uint32_t word[kPer], u[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+          const uint64_t e = beg + threadIdx.x + k * kBlock;
+          word[k] = e < end ? ea[e >> 5] : 0u;
+          u[k] = e < end ? edges[e] : 0u;
+        }
+        uint8_t fl[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+          const uint64_t e = beg + threadIdx.x + k * kBlock;
+          fl[k] = ((word[k] >> (e & 31)) & 1u) ? flags[u[k]] : uint8_t(0);
+        }
+        uint32_t first = 0xFFFFFFFFu;  // this thread's lowest hit (row offset)
+#pragma unroll
+        for (int k = kPer - 1; k >= 0; --k)
+          if (fl[k] & kModified) first = static_cast<uint32_t>(beg + threadIdx.x + k * kBlock - row);
+        for (int o = 32; o > 0; o >>= 1) first = min(first, static_cast<uint32_t>(__shfl_xor(first, o, 64)));
+        if (lane_id() == 0 && first != 0xFFFFFFFFu) atomicMin(&hub_hit[h], first);
         if (threadIdx.x == 0) examined += end - beg;
       }
-      continue;
     }
-    if (rescan && !rescan[v]) continue;  // scan result stands (counted in k_build_bins)
-    const uint64_t row = off[v];
-    const uint64_t beg = chunk_start[c];
-    const uint64_t end = min(off[v + 1], beg + kHubChunk);
-    // A hit already found in an earlier chunk of the row decides the scan:
-    // this chunk cannot lower the minimum (one load per wave).
-    if (*reinterpret_cast<const volatile uint32_t*>(&hub_hit[h]) < beg - row) continue;
-    constexpr int kPer = kHubChunk / kBlock;
-    uint32_t word[kPer], u[kPer];
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const uint64_t e = beg + threadIdx.x + k * kBlock;
-      word[k] = e < end ? ea[e >> 5] : 0u;
-      u[k] = e < end ? edges[e] : 0u;
-    }
-    uint8_t fl[kPer];
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const uint64_t e = beg + threadIdx.x + k * kBlock;
-      fl[k] = ((word[k] >> (e & 31)) & 1u) ? flags[u[k]] : uint8_t(0);
-    }
-    uint32_t first = 0xFFFFFFFFu;  // this thread's lowest hit (row offset)
-#pragma unroll
-    for (int k = kPer - 1; k >= 0; --k)
-      if (fl[k] & kModified) first = static_cast<uint32_t>(beg + threadIdx.x + k * kBlock - row);
-    for (int o = 32; o > 0; o >>= 1) first = min(first, static_cast<uint32_t>(__shfl_xor(first, o, 64)));
-    if (lane_id() == 0 && first != 0xFFFFFFFFu) atomicMin(&hub_hit[h], first);
-    if (threadIdx.x == 0) examined += end - beg;
+    __syncthreads();
   }
   wave_add(counters, C_XSLOTS, examined);
 }
