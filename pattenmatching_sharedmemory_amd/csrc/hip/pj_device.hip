@@ -50,6 +50,13 @@ enum { C_VERTEX = 0, C_EDGE = 1, C_MATCH = 2, C_ENUM = 3, C_SLOTS = 4, C_ITEMS =
 constexpr int kCounterStride = 8;
 constexpr size_t kCounterBlock = static_cast<size_t>(kCounterSlots) * kCounterStride;
 constexpr int kMaxLaunches = 32;  // timed launches per step
+// Grids of the grid-strided sweeps, in blocks per compute unit: more blocks
+// than fit at once let blocks that finish early be replaced (measured on the
+// scale-22 solve: 16 for LCC phase 1 and the walks, 8 for phase 2, whose
+// tail steps are mostly empty launches).
+constexpr int kPhase1BlocksPerCu = 16;
+constexpr int kPhase2BlocksPerCu = 8;
+constexpr int kWalkBlocksPerCu = 16;
 
 // Kernel kinds reported by pj_kernel_stats.
 enum KernelId {
@@ -2617,7 +2624,7 @@ void launch_phase2(pj_ctx* ctx, int mode = 3) {
   Word* cand = static_cast<Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   unsigned long long* c = begin_launch(ctx, K_P2);
-  hipLaunchKernelGGL(k_phase2_all<Word>, dim3(ctx->cus * 8), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
+  hipLaunchKernelGGL(k_phase2_all<Word>, dim3(ctx->cus * kPhase2BlocksPerCu), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
                      ctx->d_bin_count, ctx->V, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nhubs ? ctx->nchunks : 0u,
                      ctx->d_hubs, ctx->d_off, ctx->d_edges, cand, unm, ctx->d_flags, dirty, rescan, last, ctx->d_ea,
                      ctx->d_hub_hit, c);
@@ -2665,7 +2672,7 @@ void launch_lcc_phase1(pj_ctx* ctx) {
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   unsigned long long* c = begin_launch(ctx, K_LCC_P1);
-  hipLaunchKernelGGL((k_lcc_phase1_all<Word, KW>), dim3(ctx->cus * 8), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
+  hipLaunchKernelGGL((k_lcc_phase1_all<Word, KW>), dim3(ctx->cus * kPhase1BlocksPerCu), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
                      ctx->d_bin_count, ctx->V, ctx->d_off, ctx->d_edges, ctx->d_code, derive, cand, unm, ctx->d_flags,
                      dirty, rescan, ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
@@ -2840,7 +2847,8 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   const uint64_t V = ctx->V;
-  const int g = tables ? kMemoWaves / (kBlock / 64) : ctx->cus * 8;
+  // memo tables are indexed by wave: those launches have exactly kMemoWaves
+  const int g = tables ? kMemoWaves / (kBlock / 64) : ctx->cus * kWalkBlocksPerCu;
   // 3-vertex cycles from low-degree sources: several sources per wave
   const uint32_t* rev = reverse_positions(ctx);
   const bool triangles = kind == 0 && !tables && rev && ctx->h_t.nlcc_len == 3;
