@@ -2678,7 +2678,8 @@ void launch_lcc_phase1(pj_ctx* ctx) {
   end_launch(ctx);
   if (ctx->nhubs) {
     c = begin_launch(ctx, K_LCC_HUB);
-    hipLaunchKernelGGL(k_lcc_hub_chunks<Word>, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_lcc_hub_chunks<Word>, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * kPhase1BlocksPerCu)),
+                       dim3(kBlock), 0,
                        ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
                        ctx->d_edges, ctx->d_code, derive, cand, ctx->d_flags, dirty, ctx->d_ea, ctx->d_hub_count,
                        ctx->d_hub_elim, ctx->d_t, c);
