@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(kBlock) k_rows_symmetric(const uint64_t* __res
   const int lane = __lane_id();
   const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
   const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  bool bad = false;
+  bool bad = false, loop = false;
   uint64_t before = 0, after = 0;
   for (uint64_t u = wave; u < V; u += waves) {
     const uint64_t beg = off[u], end = off[u + 1];
@@ -234,6 +234,7 @@ __global__ void __launch_bounds__(kBlock) k_rows_symmetric(const uint64_t* __res
       }
       if (v == u) {
         if (rev) rev[e] = static_cast<uint32_t>(e - beg);
+        loop = true;
         continue;
       }
       uint64_t lo = off[v], hi = off[v + 1];
@@ -262,6 +263,7 @@ __global__ void __launch_bounds__(kBlock) k_rows_symmetric(const uint64_t* __res
     after += __shfl_xor(after, o, 64);
   }
   if (__ballot(bad) && lane == 0) atomicOr(&flag[0], 1ull);
+  if (__ballot(loop) && lane == 0) atomicOr(&flag[3], 1ull);  // a self-loop slot
   if (lane == 0) {
     if (before) atomicAdd(&flag[1], before);
     if (after) atomicAdd(&flag[2], after);
@@ -2154,6 +2156,111 @@ __global__ void __launch_bounds__(kBlock) k_triangles(const uint32_t* __restrict
   wave_add(counters, C_SLOTS, slots);
 }
 
+// Memoised 3-vertex cycle constraint (cc_cpu.tpp:12-70 with L = 3) on a
+// graph without duplicate slots or self-loops (sorted rows, reverse
+// positions).  For start j the reference walks s -> a -> u -> s with
+// a in row(s) order (slot active, q1 in C(a)) and u in row(a) order (slot
+// active, q2 in C(u)); its edge memo, cleared per (s, j), stores template ids
+// and tests positions.  On such a graph the only test that can succeed is the
+// one at the second hop, position (j+1) % 3, on the edge {a, u}, and it holds
+// exactly when q2 equals that position and u was itself an earlier first hop
+// whose second hops reached a (u in row(s) before a, slot s->u active, q1 in
+// C(u); slot u->a active, q2 in C(a)): the first hop's test on {s, a} sees
+// only ids inserted for {s, a}, which no earlier step inserts, and a second
+// hop to u = s (the only other insertion on {a, s}) cannot close without a
+// self-loop.  So the search takes the first (a, u) in that order that is not
+// blocked and has an active slot u -> s, and marks a with q1 and u with q2 in
+// alreadyMatched, as the walk does.  Rows of a are read 64 slots at a time
+// and the first lane in row order decides.  One wave per source.
+template <typename Word>
+__global__ void __launch_bounds__(kBlock) k_triangles_memo(const uint32_t* __restrict__ list,
+                                                           const uint32_t* __restrict__ list_count,
+                                                           const uint64_t* __restrict__ off,
+                                                           const uint32_t* __restrict__ edges,
+                                                           const uint32_t* __restrict__ ea,
+                                                           const uint32_t* __restrict__ rev,
+                                                           const Word* __restrict__ cand, Word* __restrict__ unmatch,
+                                                           uint8_t* __restrict__ flags, Word* already,
+                                                           const DevTemplate* __restrict__ t,
+                                                           unsigned long long* counters) {
+  const int lane = lane_id();
+  const uint32_t n = *list_count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[C_ITEMS], static_cast<unsigned long long>(n));
+  const uint32_t waves = gridDim.x * (blockDim.x / 64);
+  const uint32_t wave_id = uniform(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6));
+  uint64_t slots = 0, probes = 0;
+  for (uint32_t wv = wave_id; wv < n; wv += waves) {
+    const uint32_t s = list[wv];
+    const Word cs = cand[s];
+    const uint64_t sb = off[s], se = off[s + 1];
+    Word fail = 0;
+    for (uint32_t j = 0; j < 3; ++j) {
+      const uint32_t q0 = t->nlcc_vertex[j];
+      if (!((cs >> q0) & 1)) continue;
+      const Word done = __hip_atomic_load(&already[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((done >> q0) & 1) continue;  // alreadyMatched (cc_cpu.tpp:102-108)
+      const uint32_t q1 = t->nlcc_vertex[(j + 1) % 3], q2 = t->nlcc_vertex[(j + 2) % 3];
+      const bool may_block = q2 == (j + 1) % 3;
+      bool found = false;
+      uint64_t cursor = sb;
+      while (!found) {
+        // next first hop a in row(s) order
+        const RowScan r = scan_row<Word>(cursor, se, edges, ea, cand, static_cast<int>(q1), 0, 0xFFFFFFFFu, slots);
+        if (!r.found) break;
+        cursor = r.next;
+        const uint32_t a = r.u;
+        const uint64_t pa = r.next - 1 - sb;  // position of a in row(s)
+        const bool block_a = may_block && ((cand[a] >> q2) & 1);
+        const uint64_t ab = off[a], ae = off[a + 1];
+        for (uint64_t base = ab; base < ae && !found; base += 64) {
+          const uint64_t e = base + lane;
+          bool ok = false;
+          if (e < ae && ea_test(ea, e)) {
+            const uint32_t u = edges[e];
+            const Word cu = cand[u];
+            ok = (cu >> q2) & 1;
+            if (ok && block_a && ((cu >> q1) & 1) && ea_test(ea, off[u] + rev[e])) {
+              // u an earlier first hop?  Its position in row(s), by binary search.
+              uint64_t lo = sb, hi = se;
+              while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                ++probes;
+                if (edges[mid] < u)
+                  lo = mid + 1;
+                else
+                  hi = mid;
+              }
+              if (lo < se && edges[lo] == u && lo - sb < pa && ea_test(ea, lo)) ok = false;
+            }
+            if (ok) ok = active_slot_rev(u, s, off, edges, ea, rev, probes);
+          }
+          const uint64_t m = __ballot(ok);
+          slots += m ? __builtin_ctzll(m) + 1 : min<uint64_t>(64, ae - base);
+          if (m) {
+            found = true;
+            const uint32_t u = rl32(e < ae ? edges[e] : 0u, __builtin_ctzll(m));
+            if (lane == 0) {
+              atomic_or_word<Word>(&already[a], Word(1) << q1);
+              atomic_or_word<Word>(&already[u], Word(1) << q2);
+            }
+          }
+        }
+      }
+      if (!found) fail |= Word(1) << q0;
+    }
+    if (fail && lane == 0) {  // s is in one task list once: this wave is its only writer
+      unmatch[s] = static_cast<Word>(unmatch[s] | fail);
+      set_flag_atomic(flags, s, kModified);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) probes += __shfl_xor(probes, o, 64);
+  if (lane == 0) {
+    const int slot = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) & (kCounterSlots - 1);
+    slots += probes;
+    if (slots) atomicAdd(&counters[slot * kCounterStride + C_SLOTS], static_cast<unsigned long long>(slots));
+  }
+}
+
 // One wave per source in the task list (active vertices); failing
 // (source, template vertex) pairs go to toUnmatch and mark the source
 // modified (cc_cpu.tpp:94-129, pc_cpu.tpp:95-147, tds_cpu.tpp:153-188).
@@ -2306,6 +2413,8 @@ struct pj_ctx {
   bool skip_clean = false;  // phase 1 skips clean scheduled vertices (rows sorted and symmetric)
   bool dirty_enabled = true;
   bool rescan_all = false;  // every rescan mark is set (phase 1 need not push)
+  bool self_loops = true;       // some row holds its own vertex (known only for symmetric graphs)
+  bool triangle_memo_enabled = true;  // PJ_TRIANGLE_MEMO=0: memoised 3-cycles as walks
   bool reverse_enabled = true;  // keep d_rev for symmetric graphs (PJ_REVERSE_SLOTS=0: no)
   bool labels_consistent = false;  // every candidate of every vertex carries the vertex's label (after LCC0)
   bool modified_cleared = false;   // no modified flag set since pj_reset_modified
@@ -2735,12 +2844,19 @@ int alloc_memo(pj_ctx* ctx, MemoArgs* m, int waves, int log2) {
   return 0;
 }
 
-int prepare_memo(pj_ctx* ctx) {
+int prepare_memo(pj_ctx* ctx, bool tables = true) {
   const uint64_t V = ctx->V;
   if (!ctx->d_already) {
     uint64_t* a = nullptr;
     if (dev_alloc(ctx, &a, V + 1) || dev_alloc(ctx, &ctx->d_ovf, 2 * V + 2)) return -1;
     ctx->d_already = a;
+  }
+  if (!tables) {  // alreadyMatched only
+    const uint64_t words = (V * ctx->word_bytes + 3) / 4;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, words)), dim3(kBlock), 0, ctx->stream,
+                       static_cast<uint32_t*>(ctx->d_already), words, 0u);
+    PJ_HIP(hipGetLastError());
+    return 0;
   }
   // Walks on larger graphs reach more states: 2^14 entries per wave up to
   // 2^20 vertices, doubling per doubling of V up to 2^18 (2048 waves x 2^18
@@ -2830,8 +2946,31 @@ const uint32_t* reverse_positions(const pj_ctx* ctx) {
   return ctx->rows_sorted && ctx->symmetric ? ctx->d_rev : nullptr;
 }
 
+// Memoised 3-vertex cycle steps that k_triangles_memo decides exactly:
+// graphs with sorted rows, no duplicate slots (implied by the reverse
+// positions) and no self-loops.  PJ_TRIANGLE_MEMO=0 runs them as memoised
+// walks instead (same results).
+bool triangle_memo(const pj_ctx* ctx, int kind) {
+  return kind == 0 && ctx->h_t.nlcc_memo && ctx->h_t.nlcc_len == 3 && reverse_positions(ctx) && !ctx->self_loops &&
+         ctx->triangle_memo_enabled;
+}
+
 template <typename Word>
 int launch_nlcc(pj_ctx* ctx, int kind) {
+  if (triangle_memo(ctx, kind)) {
+    if (prepare_memo(ctx, false)) return -1;
+    launch_bins<Word>(ctx, 1);
+    for (int b = 0; b < kBins; ++b) {
+      unsigned long long* c = begin_launch(ctx, K_NLCC);
+      hipLaunchKernelGGL(k_triangles_memo<Word>, dim3(ctx->cus * kWalkBlocksPerCu), dim3(kBlock), 0, ctx->stream,
+                         ctx->d_bins + b * static_cast<uint64_t>(ctx->V), ctx->d_bin_count + b, ctx->d_off,
+                         ctx->d_edges, ctx->d_ea, reverse_positions(ctx), static_cast<const Word*>(ctx->d_cand),
+                         static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, static_cast<Word*>(ctx->d_already),
+                         ctx->d_t, c);
+      end_launch(ctx);
+    }
+    return 0;
+  }
   const bool memo = ctx->h_t.nlcc_memo && kind != 3;
   const bool tables = memo || (!ctx->h_t.nlcc_memo && long_walk(ctx, kind));
   launch_bins<Word>(ctx, kind == 1 && !memo ? 2 : 1);
@@ -2931,7 +3070,11 @@ void launch_nlcc_hubs(pj_ctx* ctx, int kind, uint32_t* d_hub_n) {
   const MemoArgs ma = tables ? ctx->memo1 : MemoArgs{};
   const uint32_t g = std::min<uint32_t>(ctx->nhubs, tables ? kMemoWaves : ctx->cus * 8);
   unsigned long long* c = begin_launch(ctx, K_NLCC_HUB);
-  if (memo)
+  if (triangle_memo(ctx, kind))
+    hipLaunchKernelGGL(k_triangles_memo<Word>, dim3(std::min<uint32_t>(ctx->nhubs, ctx->cus * 8)), dim3(64), 0,
+                       ctx->stream, ctx->d_hubs, d_hub_n, ctx->d_off, ctx->d_edges, ctx->d_ea, reverse_positions(ctx),
+                       cand, unm, ctx->d_flags, static_cast<Word*>(ctx->d_already), ctx->d_t, c);
+  else if (memo)
     hipLaunchKernelGGL(k_memo_lds<Word>, dim3(g), dim3(64), 0, ctx->stream, ctx->d_hubs, d_hub_n, ctx->d_view_off,
                        ctx->d_view_edges, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
   else
@@ -3136,7 +3279,8 @@ int run_step_t(pj_ctx* ctx, int step, uint32_t index) {
       const int kind = step == PJ_STEP_CC ? 0 : step == PJ_STEP_PC ? 1 : step == PJ_STEP_TDS ? 2 : 3;
       if (launch_nlcc<Word>(ctx, kind)) return -1;
       launch_nlcc_hubs<Word>(ctx, kind, ctx->d_nhubs);
-      if (ctx->h_t.nlcc_memo && kind != 3 && launch_nlcc_retry<Word>(ctx, kind)) return -1;
+      if (ctx->h_t.nlcc_memo && kind != 3 && !triangle_memo(ctx, kind) && launch_nlcc_retry<Word>(ctx, kind))
+        return -1;
       if (step != PJ_STEP_ENUMERATION) {
         if (ctx->skip_clean && ctx->modified_cleared) {
           // Only the failing sources are modified: they mark their rows and
@@ -3301,6 +3445,7 @@ int pj_create(pj_ctx** out, int device) {
   // PJ_REVERSE_SLOTS=0 keeps no reverse positions (4 bytes per slot) and
   // closes cycles by binary searches instead (same results).
   if (const char* rs = getenv("PJ_REVERSE_SLOTS")) ctx->reverse_enabled = strcmp(rs, "0") != 0;
+  if (const char* tm = getenv("PJ_TRIANGLE_MEMO")) ctx->triangle_memo_enabled = strcmp(tm, "0") != 0;
   *out = ctx;
   return 0;
 }
@@ -3347,14 +3492,15 @@ static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* of
     ctx->symmetric = false;
     if (unsorted == 0) {
       unsigned long long* f = ctx->d_sums;  // three counters; d_sums is free outside pj_run_step
-      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, f, static_cast<uint64_t>(3), 0ull);
+      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, f, static_cast<uint64_t>(4), 0ull);
       if (ctx->reverse_enabled && dev_alloc(ctx, &ctx->d_rev, E)) return -1;
       hipLaunchKernelGGL(k_rows_symmetric, dim3(grid_for(ctx, static_cast<uint64_t>(V) * 64)), dim3(kBlock), 0,
                          ctx->stream, ctx->d_off, ctx->d_edges, V, f, ctx->d_rev);
-      unsigned long long h[3] = {1, 0, 0};
+      unsigned long long h[4] = {1, 0, 0, 1};
       PJ_HIP(hipMemcpyAsync(h, f, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
       PJ_HIP(hipStreamSynchronize(ctx->stream));
       ctx->symmetric = h[0] == 0 && h[1] == h[2];
+      ctx->self_loops = h[3] != 0;
       if (!ctx->symmetric && ctx->d_rev) {  // incomplete: never read
         PJ_HIP(hipFree(ctx->d_rev));
         ctx->d_rev = nullptr;
@@ -3403,6 +3549,7 @@ int pj_share_graph(pj_ctx* ctx, const pj_ctx* owner) {
   ctx->d_edges = owner->d_edges;
   ctx->d_labels = owner->d_labels;
   ctx->d_rev = owner->d_rev;
+  ctx->self_loops = owner->self_loops;
   ctx->rows_sorted = owner->rows_sorted;
   ctx->symmetric = owner->symmetric;
   ctx->skip_clean = ctx->rows_sorted && ctx->symmetric && ctx->dirty_enabled;

@@ -336,3 +336,53 @@ def test_long_walks_with_failed_state_memo(pj, orc, case):
     c = pj.Constraints(p)
     assert max([len(x) for x in c.cycles()] + [len(x) for x in c.paths()]) >= 5
     _compare_pipeline(pj, orc, g, p, case, enumerate_matches=False)
+
+
+# Triangle 2-3-4 whose memo can block walks (template id 2 is the position
+# tested at the second hop of start 1; vertices 4 and 2 share a label).
+MEMO_TRIANGLE = (5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 2)], [0, 3, 1, 2, 1])
+
+
+@pytest.mark.parametrize("kernel", ["triangle", "walk"])
+@pytest.mark.parametrize("seed,perturb", [(5, False), (6, True), (7, True)])
+def test_memoised_triangle_steps_stay_within_the_reference_outcomes(pj, orc, seed, perturb, kernel, monkeypatch):
+    """The memoised 3-vertex cycle step (k_triangles_memo, or the memoised
+    walk with PJ_TRIANGLE_MEMO=0) from the reference's state, and from states
+    with a fifth of the edge-active bits cleared, lands within the outcomes of
+    the reference's schedules: F0 - M0 <= F <= F0 (failures F0 and marks M0
+    with every walk run)."""
+    from pattenmatching_sharedmemory_amd import STEP_CC, STEP_LCC, STEP_LCC0
+
+    if kernel == "walk":
+        monkeypatch.setenv("PJ_TRIANGLE_MEMO", "0")
+    n, edges, labels = MEMO_TRIANGLE
+    g = pj.Graph.rmat(12, 8, seed=seed).labels_uniform(4, seed=seed + 10)
+    c = pj.Constraints(pj.Graph.pattern(n, edges, labels))
+    k = [list(x) for x in c.cycles()].index([2, 3, 4])
+    o = orc.Oracle(g, c, threads=8)
+    o.step(STEP_LCC0, 0, orc.FAITHFUL)
+    while any(o.step(STEP_LCC, 0, orc.FAITHFUL)):
+        pass
+    o.reset_modified()
+    cand, vf, ea, av, ae = o.state()
+    if perturb:
+        rng = np.random.default_rng(seed)
+        bits = np.unpackbits(ea.view(np.uint8), bitorder="little")
+        bits &= (rng.random(bits.size) >= 0.2).astype(np.uint8)
+        ea = np.packbits(bits, bitorder="little").view(np.uint32)[: ea.size].copy()
+    cand = np.where(vf & 1, cand, 0).astype(np.uint64)
+    x = orc.Oracle(g, c, threads=8)
+    x.load_state(cand, vf, ea, av, ae)
+    x.step(STEP_CC, k, orc.FAITHFUL_NO_SHORTCUT)
+    f0 = cand & ~x.state()[0]
+    m0 = x.last_marks()
+    pr = pj.Pruner(g, c)
+    pr.load_state(pj.State(cand, vf, ea, av, ae))
+    pr.step(STEP_CC, k)
+    f = cand & ~pr.state().cand
+    print(f"[{kernel} seed {seed}] failures {int(np.count_nonzero(f))}, "
+          f"bounds {int(np.count_nonzero(f0 & ~m0))}..{int(np.count_nonzero(f0))}", flush=True)
+    assert np.any(f0)
+    assert np.all((f & ~f0) == 0)
+    assert np.all((f0 & ~m0 & ~f) == 0)
+    pr.close()

@@ -242,3 +242,105 @@ def test_faithful_outcomes_lie_between_the_no_shortcut_bounds(pj, orc):
         assert np.all((f & ~f0) == 0) and np.all((f0 & ~m0 & ~f) == 0), threads
         outcomes.add(f.tobytes())
     print(f"{len(outcomes)} distinct outcomes over the thread counts")
+
+
+# Triangle 2-3-4 whose smallest template id (2) equals the position the
+# reference's memo tests at the second hop of start 1, with template
+# vertices 4 and 2 sharing a label: the memo can block walks there.
+MEMO_TRIANGLE = (5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 2)], [0, 3, 1, 2, 1])
+
+
+def _triangle_memo_rule(g, cand, ea_bits, idx):
+    """The memoised 3-vertex cycle step with every walk run, as the device's
+    triangle kernel decides it (graphs without duplicate slots or
+    self-loops): for start j the first (a, u) in row order with active slots
+    s->a, a->u, u->s, q1 in C(a), q2 in C(u), not blocked, succeeds and marks
+    a with q1 and u with q2; (a, u) is blocked iff q2 equals position
+    (j+1) % 3, q2 in C(a), and u is a first hop before a (slot s->u active,
+    q1 in C(u)) with slot u->a active.  Returns (failures, marks)."""
+    off, edges = g.offsets(), g.edges()
+    V = g.vertex_count
+    fail = np.zeros(V, dtype=np.uint64)
+    marks = np.zeros(V, dtype=np.uint64)
+
+    def slot(x, y):  # index of slot x->y or -1 (rows ascending)
+        lo, hi = int(off[x]), int(off[x + 1])
+        k = lo + int(np.searchsorted(edges[lo:hi], y))
+        return k if k < hi and edges[k] == y else -1
+
+    def act(e):
+        return e >= 0 and bool(ea_bits[e])
+
+    has = lambda v, q: (int(cand[v]) >> q) & 1
+    for s in range(V):
+        if not cand[s]:
+            continue
+        beg = int(off[s])
+        row = [int(x) for x in edges[beg:int(off[s + 1])]]
+        for j in range(3):
+            q0, q1, q2 = idx[j], idx[(j + 1) % 3], idx[(j + 2) % 3]
+            if not has(s, q0):
+                continue
+            found = False
+            for pa, a in enumerate(row):
+                if not (act(beg + pa) and has(a, q1)):
+                    continue
+                for e in range(int(off[a]), int(off[a + 1])):
+                    u = int(edges[e])
+                    if not (act(e) and has(u, q2)):
+                        continue
+                    if q2 == (j + 1) % 3 and has(a, q2) and has(u, q1):
+                        pu = slot(s, u)
+                        if 0 <= pu - beg < pa and act(pu) and act(slot(u, a)):
+                            continue  # blocked by the edge memo
+                    if act(slot(u, s)):
+                        found = True
+                        marks[a] |= np.uint64(1 << q1)
+                        marks[u] |= np.uint64(1 << q2)
+                        break
+                if found:
+                    break
+            if not found:
+                fail[s] |= np.uint64(1 << q0)
+    return fail, marks
+
+
+@pytest.mark.parametrize("seed,perturb", [(3, False), (4, False), (5, True), (6, True)])
+def test_memoised_triangle_rule_matches_the_reference_search(pj, orc, seed, perturb):
+    """Pins the rule the device's memoised-triangle kernel uses against the
+    oracle's memoised search with every walk run (failures and
+    alreadyMatched marks, bit for bit), from the pipeline's state and from
+    states with a fifth of the edge-active bits cleared at random."""
+    n, edges, labels = MEMO_TRIANGLE
+    g = pj.Graph.rmat(9, 8, seed=seed).labels_uniform(4, seed=seed + 10)
+    c = pj.Constraints(pj.Graph.pattern(n, edges, labels))
+    k = [list(x) for x in c.cycles()].index([2, 3, 4])
+    o = orc.Oracle(g, c, threads=4)
+    o.step(STEP_LCC0, 0, orc.FAITHFUL)
+    while any(o.step(STEP_LCC, 0, orc.FAITHFUL)):
+        pass
+    o.reset_modified()
+    cand, vf, ea, av, ae = o.state()
+    if perturb:
+        rng = np.random.default_rng(seed)
+        bits = np.unpackbits(ea.view(np.uint8), bitorder="little")
+        bits &= (rng.random(bits.size) >= 0.2).astype(np.uint8)
+        ea = np.packbits(bits, bitorder="little").view(np.uint32)[: ea.size].copy()
+    cand = np.where(vf & 1, cand, 0).astype(np.uint64)
+    x = orc.Oracle(g, c, threads=4)
+    x.load_state(cand, vf, ea, av, ae)
+    x.step(STEP_CC, k, orc.FAITHFUL_NO_SHORTCUT)
+    f0 = cand & ~x.state()[0]
+    m0 = x.last_marks()
+    bits = np.unpackbits(ea.view(np.uint8), bitorder="little")
+    f, m = _triangle_memo_rule(g, cand, bits, [2, 3, 4])
+    assert np.any(f0) and np.any(m0)
+    assert np.array_equal(f, f0)
+    assert np.array_equal(m, m0)
+    y = orc.Oracle(g, c, threads=4)
+    y.load_state(cand, vf, ea, av, ae)
+    y.step(STEP_CC, k, orc.COMPLETE)
+    fc = cand & ~y.state()[0]
+    print(f"seed {seed}: {int(np.count_nonzero(f0))} sources fail with the memo, {int(np.count_nonzero(fc))} without")
+    if perturb:  # on these states the memo blocks walks the complete search finds
+        assert not np.array_equal(f0, fc)
