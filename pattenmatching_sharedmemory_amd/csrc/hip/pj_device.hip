@@ -2405,6 +2405,7 @@ struct pj_ctx {
   uint32_t* d_live[2] = {nullptr, nullptr};
   const uint32_t* live = nullptr;
   uint32_t n_live = 0;
+  uint32_t live_min = 1u << 16;  // lists this short are not compacted further (PJ_LIVE_MIN)
   uint64_t ea_words = 0;
   uint32_t ea_tail = 0;  // host staging of the last edge-active word
   bool rows_sorted = false;
@@ -3302,8 +3303,6 @@ int run_step_t(pj_ctx* ctx, int step, uint32_t index) {
   return 0;
 }
 
-// Live lists shorter than this are not compacted further.
-constexpr uint32_t kLiveMin = 1u << 16;
 
 template <typename Word>
 int compact_live_t(pj_ctx* ctx) {
@@ -3446,6 +3445,7 @@ int pj_create(pj_ctx** out, int device) {
   // closes cycles by binary searches instead (same results).
   if (const char* rs = getenv("PJ_REVERSE_SLOTS")) ctx->reverse_enabled = strcmp(rs, "0") != 0;
   if (const char* tm = getenv("PJ_TRIANGLE_MEMO")) ctx->triangle_memo_enabled = strcmp(tm, "0") != 0;
+  if (const char* lm = getenv("PJ_LIVE_MIN")) ctx->live_min = static_cast<uint32_t>(std::max(0, atoi(lm)));
   *out = ctx;
   return 0;
 }
@@ -3736,7 +3736,7 @@ int pj_run_step(pj_ctx* ctx, int step, uint32_t index, pj_algo_results* out) {
   if (out) *out = r;
   // Once fewer than half of the live list can be active, the list is
   // compacted, so frontier construction reads only live vertices.
-  if (ctx->n_live > kLiveMin && ctx->active_vertices < ctx->n_live / 2 &&
+  if (ctx->n_live > ctx->live_min && ctx->active_vertices < ctx->n_live / 2 &&
       with_word(ctx, [&](auto w) { return compact_live_t<decltype(w)>(ctx); }))
     return -1;
   return 0;

@@ -386,3 +386,66 @@ def test_memoised_triangle_steps_stay_within_the_reference_outcomes(pj, orc, see
     assert np.all((f & ~f0) == 0)
     assert np.all((f0 & ~m0 & ~f) == 0)
     pr.close()
+
+
+@pytest.mark.parametrize("case", ["C2-shape s16", "hub"])
+def test_reverse_positions_change_nothing(pj, orc, case, monkeypatch):
+    """Cycle steps closing through the reverse slot positions and through
+    binary searches (PJ_REVERSE_SLOTS=0) give identical traces and states,
+    and the reference's."""
+    if case == "hub":
+        g = pj.Graph.rmat(14, 32, seed=3).labels_uniform(4, seed=4)
+        p = pj.Graph.pattern(4, [(0, 1), (1, 2), (2, 0), (2, 3)], [0, 1, 2, 3])
+    else:
+        g = pj.Graph.rmat(16, 16, seed=20221).labels_zipf(32, 1.2, seed=7)
+        p = pj.Graph.pattern(6, TRIANGLE_TAIL, [0, 1, 2, 3, 4, 3])
+    c = pj.Constraints(p)
+    runs = []
+    for rev in ("0", "1"):
+        monkeypatch.setenv("PJ_REVERSE_SLOTS", rev)
+        pr = pj.Pruner(g, c)
+        rows, _ = pr.run(enumerate_matches=False)
+        runs.append((rows, pr.state()))
+        pr.close()
+    (r0, s0), (r1, s1) = runs
+    assert [r.key() for r in r0] == [r.key() for r in r1]
+    assert np.array_equal(s0.cand, s1.cand) and np.array_equal(s0.vflags, s1.vflags)
+    assert np.array_equal(s0.ea_words, s1.ea_words)
+    o = orc.Oracle(g, c, threads=8)
+    orows, _ = o.run(orc.FAITHFUL, enumerate_matches=False)
+    assert [r.key() for r in r1] == [r.key() for r in orows]
+
+
+def test_live_list_compaction_follows_the_reference(pj, orc, monkeypatch):
+    """With the live-vertex list compacted at every halving of the active
+    vertices (PJ_LIVE_MIN=16), the pipeline still matches the reference."""
+    monkeypatch.setenv("PJ_LIVE_MIN", "16")
+    g = pj.Graph.rmat(14, 16, seed=1).labels_uniform(8, seed=2)
+    p = pj.Graph.pattern(4, [(0, 1), (1, 2), (2, 3), (3, 0)], [0, 1, 2, 3])
+    _compare_pipeline(pj, orc, g, p, "C1 s14 compacted")
+    g = pj.Graph.rmat(16, 16, seed=20221).labels_zipf(32, 1.2, seed=7)
+    p = pj.Graph.pattern(6, TRIANGLE_TAIL, [0, 1, 2, 3, 4, 3])
+    _compare_pipeline(pj, orc, g, p, "C2-shape s16 compacted", enumerate_matches=False)
+
+
+def test_gathered_and_derived_label_codes_agree(pj):
+    """LCC phase 1 takes neighbour label codes from candidate words after
+    LCC0 and gathers them after a state upload; both give the same sweep."""
+    from pattenmatching_sharedmemory_amd import STEP_LCC, STEP_LCC0
+
+    g = pj.Graph.rmat(15, 16, seed=11).labels_zipf(16, 1.1, seed=12)
+    c = pj.Constraints(pj.Graph.pattern(6, TRIANGLE_TAIL, [0, 1, 2, 3, 4, 3]))
+    a, b = pj.Pruner(g, c), pj.Pruner(g, c)
+    a.step(STEP_LCC0)
+    b.load_state(a.state())  # uploaded: codes gathered until LCC0 runs again
+    while True:
+        ra, rb = a.step(STEP_LCC), b.step(STEP_LCC)
+        assert (ra.vertex_eliminated, ra.edge_eliminated, ra.match_eliminated) == \
+            (rb.vertex_eliminated, rb.edge_eliminated, rb.match_eliminated)
+        if not (ra.vertex_eliminated or ra.edge_eliminated or ra.match_eliminated):
+            break
+    sa, sb = a.state(), b.state()
+    assert np.array_equal(sa.cand, sb.cand) and np.array_equal(sa.vflags, sb.vflags)
+    assert np.array_equal(sa.ea_words, sb.ea_words)
+    a.close()
+    b.close()
