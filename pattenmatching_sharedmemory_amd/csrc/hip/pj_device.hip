@@ -1146,7 +1146,7 @@ __global__ void k_phase2_hub_finish(const uint32_t* __restrict__ hubs, uint32_t 
         cand[v] = nc;
         if (!nc) ++velim;
         unmatch[v] = 0;
-        if (dirty) dirty[v] = 1;  // k_phase2_hub_chunks dirtied the neighbours
+        if (dirty) dirty[v] = 1;  // phase2_hub_chunks dirtied the neighbours
       }
       flags[v] = f | kScheduled;
     } else if (!rescan || rescan[v]) {
@@ -1162,7 +1162,7 @@ __global__ void k_phase2_hub_finish(const uint32_t* __restrict__ hubs, uint32_t 
   }
   wave_add(counters, C_VERTEX, velim);
   wave_add(counters, C_MATCH, melim);
-  wave_add(counters, C_SKIP, slots);  // reference-order slots of the hub scans (read by k_phase2_hub_chunks)
+  wave_add(counters, C_SKIP, slots);  // reference-order slots of the hub scans (read by phase2_hub_chunks)
 }
 
 // ---------------------------------------------------------------------------

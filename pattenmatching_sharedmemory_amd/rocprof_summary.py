@@ -34,7 +34,7 @@ _TEMPLATE_NAMES = {
 
 
 def short_name(full: str) -> str:
-    """'void (anonymous namespace)::k_phase2<unsigned int, 64>(...)' -> 'k_phase2<unsigned int, 64>'."""
+    """'void (anonymous namespace)::k_triangles<unsigned int, 32>(...)' -> 'k_triangles<unsigned int, 32>'."""
     s = full.strip().replace("(anonymous namespace)::", "")
     if s.startswith("void "):
         s = s[5:]
