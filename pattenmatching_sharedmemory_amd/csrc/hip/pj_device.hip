@@ -346,8 +346,8 @@ __global__ void k_lcc0(Word* __restrict__ cand, const uint8_t* __restrict__ code
 // list); mode 2: active with label code == t->path_code (PC); mode 3: the
 // phase 2 frontier (see bin_of).
 
-constexpr int kBins = 3;
-__constant__ uint32_t c_bin_limit[kBins] = {4, 32, kHubDegree};
+constexpr int kBins = 4;
+__constant__ uint32_t c_bin_limit[kBins] = {4, 32, 256, kHubDegree};
 
 constexpr int kBinIters = 4;                  // vertices per thread
 constexpr int kBinVerts = kBlock * kBinIters;  // vertices per block
@@ -377,6 +377,10 @@ constexpr int kBinVerts = kBlock * kBinIters;  // vertices per block
 // that result).  Marked and modified vertices are binned as in mode 3.
 //
 // mode 5: active and modified (the sources that mark their neighbours).
+//
+// The walk task lists (modes 1 and 2) leave bin 2 empty: their kernels take
+// one wave per source above 32 slots, so bins 2 and 3 would be launched
+// alike.
 //
 // Returns the bin, -1 (not in the frontier), -2 (skipped; its slots are
 // counted as examined) or -3 (mode 4 default).  Every load is issued before
@@ -411,7 +415,8 @@ __device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __rest
   }
   if (deg <= c_bin_limit[0]) return 0;
   if (deg <= c_bin_limit[1]) return 1;
-  if (deg <= c_bin_limit[2]) return 2;
+  if (deg <= c_bin_limit[2] && mode != 1 && mode != 2) return 2;  // walk task lists: one bin above 32 slots
+  if (deg <= c_bin_limit[3]) return 3;
   return -1;
 }
 
@@ -443,7 +448,7 @@ __global__ void __launch_bounds__(kBlock) k_build_bins(
 #pragma unroll
   for (int i = 0; i < kBinIters; ++i)
     bin[i] = bin_of<Word>(vid[i], V, cand, flags, code, marks, unmatch, off, mode, path_code, deg[i]);
-  uint32_t cnt[kBins] = {0, 0, 0};
+  uint32_t cnt[kBins] = {};
   uint64_t skipped = 0;
 #pragma unroll
   for (int i = 0; i < kBinIters; ++i) {
@@ -613,7 +618,7 @@ __device__ void lcc_phase1_bin(uint32_t block, uint32_t nblocks, const uint8_t* 
       // kUnroll slices of G slots per pass: the edge-active words and ids of
       // all slices are loaded first, then all neighbour gathers, so each pass
       // waits on two rounds of memory latency.
-      constexpr int kUnroll = G == 64 ? 4 : G == 16 ? 2 : 1;
+      constexpr int kUnroll = G >= 32 ? 4 : G == 16 ? 2 : 1;
       for (uint64_t base = beg; base < end; base += static_cast<uint64_t>(G) * kUnroll) {
         uint32_t word[kUnroll], u[kUnroll];
 #pragma unroll
@@ -808,9 +813,12 @@ __global__ void __launch_bounds__(kBlock) k_lcc_phase1_all(
   if (threadIdx.x < 64) s_pcode[threadIdx.x] = t->pcode[threadIdx.x];
   __syncthreads();
   if (blockIdx.x == 0 && threadIdx.x == 0)
-    atomicAdd(&counters[C_ITEMS], static_cast<unsigned long long>(bin_count[0]) + bin_count[1] + bin_count[2]);
+    atomicAdd(&counters[C_ITEMS],
+              static_cast<unsigned long long>(bin_count[0]) + bin_count[1] + bin_count[2] + bin_count[3]);
   const uint64_t V64 = V;
-  lcc_phase1_bin<Word, 64, KW>(blockIdx.x, gridDim.x, s_pcode, bins + 2 * V64, bin_count + 2, off, edges, code,
+  lcc_phase1_bin<Word, 64, KW>(blockIdx.x, gridDim.x, s_pcode, bins + 3 * V64, bin_count + 3, off, edges, code,
+                               derive, cand, unmatch, flags, dirty, rescan, ea, t, counters);
+  lcc_phase1_bin<Word, 32, KW>(blockIdx.x, gridDim.x, s_pcode, bins + 2 * V64, bin_count + 2, off, edges, code,
                                derive, cand, unmatch, flags, dirty, rescan, ea, t, counters);
   lcc_phase1_bin<Word, 16, KW>(blockIdx.x, gridDim.x, s_pcode, bins + V64, bin_count + 1, off, edges, code, derive,
                                cand, unmatch, flags, dirty, rescan, ea, t, counters);
@@ -971,7 +979,7 @@ __device__ void phase2_bin(uint32_t block, uint32_t nblocks, const uint32_t* __r
         uint64_t scanned = end - beg;
         // kUnroll slices of G slots per pass, loads batched as in phase 1;
         // the first hit in slot order ends the scan.
-        constexpr int kUnroll = G == 64 ? 4 : G == 16 ? 2 : 1;
+        constexpr int kUnroll = G >= 32 ? 4 : G == 16 ? 2 : 1;
         for (uint64_t base = beg; base < end && !hit; base += static_cast<uint64_t>(G) * kUnroll) {
           uint32_t word[kUnroll], u[kUnroll];
 #pragma unroll
@@ -1107,12 +1115,15 @@ __global__ void __launch_bounds__(kBlock) k_phase2_all(
     uint8_t* __restrict__ rescan, uint32_t* __restrict__ last_scan, const uint32_t* __restrict__ ea,
     uint32_t* __restrict__ hub_hit, unsigned long long* counters) {
   if (blockIdx.x == 0 && threadIdx.x == 0)
-    atomicAdd(&counters[C_ITEMS], static_cast<unsigned long long>(bin_count[0]) + bin_count[1] + bin_count[2]);
+    atomicAdd(&counters[C_ITEMS],
+              static_cast<unsigned long long>(bin_count[0]) + bin_count[1] + bin_count[2] + bin_count[3]);
   const uint64_t V64 = V;
   if (nchunks)
     phase2_hub_chunks<Word>(blockIdx.x, gridDim.x, chunk_hub, chunk_start, nchunks, hubs, off, edges, cand, unmatch,
                             flags, dirty, rescan, ea, hub_hit, counters);
-  phase2_bin<Word, 64>(blockIdx.x, gridDim.x, bins + 2 * V64, bin_count + 2, off, edges, cand, unmatch, flags, dirty,
+  phase2_bin<Word, 64>(blockIdx.x, gridDim.x, bins + 3 * V64, bin_count + 3, off, edges, cand, unmatch, flags, dirty,
+                       rescan, last_scan, ea, counters);
+  phase2_bin<Word, 32>(blockIdx.x, gridDim.x, bins + 2 * V64, bin_count + 2, off, edges, cand, unmatch, flags, dirty,
                        rescan, last_scan, ea, counters);
   phase2_bin<Word, 16>(blockIdx.x, gridDim.x, bins + V64, bin_count + 1, off, edges, cand, unmatch, flags, dirty,
                        rescan, last_scan, ea, counters);
@@ -2626,7 +2637,7 @@ int alloc_state(pj_ctx* ctx) {
       dev_alloc(ctx, &ctx->d_vflags, V + 4) || dev_alloc(ctx, &ctx->d_dirty, V + 4) ||
       dev_alloc(ctx, &ctx->d_rescan, V + 4) || dev_alloc(ctx, &ctx->d_last_scan, V) ||
       dev_alloc(ctx, &ctx->d_ea, ctx->ea_words) ||
-      dev_alloc(ctx, &ctx->d_bins, V * kBins) || dev_alloc(ctx, &ctx->d_bin_count, 4) ||
+      dev_alloc(ctx, &ctx->d_bins, V * kBins) || dev_alloc(ctx, &ctx->d_bin_count, kBins) ||
       dev_alloc(ctx, &ctx->d_live[0], V + 1) || dev_alloc(ctx, &ctx->d_live[1], V + 1) ||
       dev_alloc(ctx, &ctx->d_counters, (kMaxLaunches + 1) * kCounterBlock))
     return -1;
@@ -2704,7 +2715,7 @@ int zero_counters(pj_ctx* ctx) {
 }
 
 int zero_bins(pj_ctx* ctx) {
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bin_count, static_cast<uint64_t>(4), 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bin_count, static_cast<uint64_t>(kBins), 0u);
   return 0;
 }
 
@@ -2763,8 +2774,10 @@ void push_modified(pj_ctx* ctx) {
                      ctx->d_off, ctx->d_edges, ctx->d_rescan, c);
   hipLaunchKernelGGL(k_push_rescan<16>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
                      ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_rescan, c);
-  hipLaunchKernelGGL(k_push_rescan<64>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
+  hipLaunchKernelGGL(k_push_rescan<32>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
                      ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_rescan, c);
+  hipLaunchKernelGGL(k_push_rescan<64>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 3 * V,
+                     ctx->d_bin_count + 3, ctx->d_off, ctx->d_edges, ctx->d_rescan, c);
   if (ctx->nhubs)
     hipLaunchKernelGGL(k_hub_rescan_push, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
                        ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
@@ -2901,8 +2914,10 @@ int build_view(pj_ctx* ctx) {
                      ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_count);
   hipLaunchKernelGGL((k_view_count<Word, 16>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
                      ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_count);
-  hipLaunchKernelGGL((k_view_count<Word, 64>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
+  hipLaunchKernelGGL((k_view_count<Word, 32>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
                      ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_count);
+  hipLaunchKernelGGL((k_view_count<Word, 64>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 3 * V,
+                     ctx->d_bin_count + 3, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_count);
   if (ctx->nhubs)
     hipLaunchKernelGGL(k_view_hubs<Word>, dim3(hub_blocks), dim3(kBlock), 0, ctx->stream, ctx->d_hubs, ctx->nhubs,
                        ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_count, nullptr, nullptr);
@@ -2923,8 +2938,11 @@ int build_view(pj_ctx* ctx) {
   hipLaunchKernelGGL((k_view_fill<Word, 16>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
                      ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_off,
                      ctx->d_view_edges);
-  hipLaunchKernelGGL((k_view_fill<Word, 64>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
+  hipLaunchKernelGGL((k_view_fill<Word, 32>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
                      ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_off,
+                     ctx->d_view_edges);
+  hipLaunchKernelGGL((k_view_fill<Word, 64>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 3 * V,
+                     ctx->d_bin_count + 3, ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_off,
                      ctx->d_view_edges);
   if (ctx->nhubs)
     hipLaunchKernelGGL(k_view_hubs<Word>, dim3(hub_blocks), dim3(kBlock), 0, ctx->stream, ctx->d_hubs, ctx->nhubs,
@@ -2962,6 +2980,7 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
     if (prepare_memo(ctx, false)) return -1;
     launch_bins<Word>(ctx, 1);
     for (int b = 0; b < kBins; ++b) {
+      if (b == 2) continue;  // empty in walk task lists
       unsigned long long* c = begin_launch(ctx, K_NLCC);
       hipLaunchKernelGGL(k_triangles_memo<Word>, dim3(ctx->cus * kWalkBlocksPerCu), dim3(kBlock), 0, ctx->stream,
                          ctx->d_bins + b * static_cast<uint64_t>(ctx->V), ctx->d_bin_count + b, ctx->d_off,
@@ -2995,6 +3014,7 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
   const bool triangles = kind == 0 && !tables && rev && ctx->h_t.nlcc_len == 3;
   // Every bin holds active vertices; hubs are run from their own list.
   for (int b = 0; b < kBins; ++b) {
+    if (b == 2) continue;  // empty in walk task lists
     unsigned long long* c = begin_launch(ctx, K_NLCC);
     if (triangles && b == 0)
       hipLaunchKernelGGL((k_triangles<Word, 4>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
