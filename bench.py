@@ -87,7 +87,7 @@ def algorithmic_bytes(k: dict, vertices: int) -> float:
     flags 1, edge-active 1/8 per slot read or written.
       phase 1 (LCC): items*(9.25 + 2W) + slots*(5.25 + W) + 0.125*eliminated
       phase 2:       items*(3W + 0.5) + 4.25*slots
-      NLCC walks:    8*items + (4.25 + W)*slots
+      NLCC walks (nlcc_walk*, triangles*, memo_walk): 8*items + (4.25 + W)*slots
       bins:          vertices*(W + 9) + 4*listed"""
     W = k["word_bytes"]
     name = k["name"]
@@ -95,7 +95,7 @@ def algorithmic_bytes(k: dict, vertices: int) -> float:
         return k["items"] * (9.25 + 2 * W) + k["slots"] * (5.25 + W) + 0.125 * k["edges_eliminated"]
     if name.startswith("phase2"):
         return k["items"] * (3 * W + 0.5) + 4.25 * k["slots"]
-    if name.startswith("nlcc"):
+    if name.startswith("nlcc") or name.startswith("triangles") or name == "memo_walk":
         return 8 * k["items"] + (4.25 + W) * k["slots"]
     if name == "build_bins":
         return k["launches"] * vertices * (W + 9)

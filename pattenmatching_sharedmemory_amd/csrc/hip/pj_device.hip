@@ -60,13 +60,17 @@ constexpr int kWalkBlocksPerCu = 16;
 
 // Kernel kinds reported by pj_kernel_stats.
 enum KernelId {
-  K_LCC0, K_BINS, K_LCC_P1, K_LCC_HUB, K_LCC_HUB_FINISH, K_P2, K_P2_HUB_FINISH, K_NLCC, K_NLCC_HUB, K_HUB_PUSH,
-  K_VIEW, K_COUNT
+  K_LCC0, K_BINS, K_LCC_P1, K_LCC_HUB, K_LCC_HUB_FINISH, K_P2, K_P2_HUB_FINISH, K_NLCC, K_NLCC_HUB, K_TRIANGLES,
+  K_TRIANGLES_MEMO, K_MEMO_WALK, K_HUB_PUSH, K_VIEW, K_COUNT
 };
+// nlcc_walk: k_nlcc (cycle / path / TDS / enumeration walks over the bins,
+// and the memo retry passes); triangles: k_triangles (3-vertex cycles from
+// sources of at most 4 / 32 slots); triangles_memo: k_triangles_memo;
+// memo_walk: the first pass of the memoised walks (k_memo_lds).
 const char* const kKernelNames[K_COUNT] = {
-    "lcc0",      "build_bins",        "lcc_phase1", "lcc_phase1_hub_chunks", "lcc_phase1_hub_finish",
-    "phase2",    "phase2_hub_finish", "nlcc_walk",  "nlcc_walk_hubs",        "hub_rescan_push",
-    "active_view"};
+    "lcc0",          "build_bins",        "lcc_phase1", "lcc_phase1_hub_chunks", "lcc_phase1_hub_finish",
+    "phase2",        "phase2_hub_finish", "nlcc_walk",  "nlcc_walk_hubs",        "triangles",
+    "triangles_memo", "memo_walk",        "hub_rescan_push", "active_view"};
 
 // Template tables in device memory.
 struct DevTemplate {
@@ -2981,7 +2985,7 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
     launch_bins<Word>(ctx, 1);
     for (int b = 0; b < kBins; ++b) {
       if (b == 2) continue;  // empty in walk task lists
-      unsigned long long* c = begin_launch(ctx, K_NLCC);
+      unsigned long long* c = begin_launch(ctx, K_TRIANGLES_MEMO);
       hipLaunchKernelGGL(k_triangles_memo<Word>, dim3(ctx->cus * kWalkBlocksPerCu), dim3(kBlock), 0, ctx->stream,
                          ctx->d_bins + b * static_cast<uint64_t>(ctx->V), ctx->d_bin_count + b, ctx->d_off,
                          ctx->d_edges, ctx->d_ea, reverse_positions(ctx), static_cast<const Word*>(ctx->d_cand),
@@ -3015,7 +3019,8 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
   // Every bin holds active vertices; hubs are run from their own list.
   for (int b = 0; b < kBins; ++b) {
     if (b == 2) continue;  // empty in walk task lists
-    unsigned long long* c = begin_launch(ctx, K_NLCC);
+    unsigned long long* c =
+        begin_launch(ctx, triangles && b <= 1 ? K_TRIANGLES : memo ? K_MEMO_WALK : K_NLCC);
     if (triangles && b == 0)
       hipLaunchKernelGGL((k_triangles<Word, 4>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
                          ctx->d_bin_count + b, off, edges, ea, rev, cand, unm, ctx->d_flags, ctx->d_t, c);

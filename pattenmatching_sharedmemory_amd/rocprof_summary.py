@@ -29,7 +29,9 @@ _TEMPLATE_NAMES = {
     "k_lcc_hub_finish": "lcc_phase1_hub_finish",
     "k_phase2_all": "phase2",
     "k_phase2_hub_finish": "phase2_hub_finish",
-    "k_triangles": "nlcc_walk",
+    "k_triangles": "triangles",
+    "k_triangles_memo": "triangles_memo",
+    "k_memo_lds": "memo_walk",
 }
 
 
