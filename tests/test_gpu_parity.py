@@ -449,3 +449,52 @@ def test_gathered_and_derived_label_codes_agree(pj):
     assert np.array_equal(sa.ea_words, sb.ea_words)
     a.close()
     b.close()
+
+
+def test_full_scale_plain_workload_invariants(pj):
+    """The bench's plain configuration at full size (R-MAT scale 22, 128 M
+    slots), checked through properties that do not need the oracle:
+    LCC0's closed form, the counters against the state (every eliminated
+    edge is one cleared edge-active bit, every active vertex has
+    candidates), symmetric edge-active bits between active vertices at the
+    LCC fixed point (the template adjacency is symmetric), and the fixed
+    point itself (one more LCC step removes nothing)."""
+    from pattenmatching_sharedmemory_amd import STEP_LCC, STEP_LCC0
+
+    g = pj.Graph.rmat(22, 16, 0.57, 0.19, 0.19, seed=20221).labels_zipf(32, 1.2, seed=7)
+    plab = [0, 1, 2, 3, 4, 3]
+    c = pj.Constraints(pj.Graph.pattern(6, TRIANGLE_TAIL, plab))
+    pr = pj.Pruner(g, c)
+    V, E = g.vertex_count, g.edge_count
+    # LCC0: a vertex keeps the template vertices of its label
+    r0 = pr.step(STEP_LCC0)
+    lab = g.labels()
+    per_label = np.bincount(np.array(plab), minlength=int(lab.max()) + 1)
+    keep = per_label[np.minimum(lab, len(per_label) - 1)] * (lab < len(per_label))
+    assert r0.match_eliminated == int(np.sum(len(plab) - keep))
+    assert r0.vertex_eliminated == int(np.count_nonzero(keep == 0))
+    pr.reset()
+    rows, _ = pr.run(enumerate_matches=False)
+    st = pr.state()
+    active = (st.vflags & 1).astype(bool)
+    assert np.array_equal(active, st.cand != 0)
+    assert rows[-1].active_vertices == int(np.count_nonzero(active))
+    bits = np.unpackbits(st.ea_words.view(np.uint8), bitorder="little")[:E].astype(bool)
+    eliminated = sum(r.edge_eliminated for r in rows)
+    assert int(np.count_nonzero(bits)) == E - eliminated
+    assert rows[-1].active_edges == E - eliminated
+    # edge-active bits agree in both directions between active vertices
+    off, edges = g.offsets(), g.edges()
+    src = np.repeat(np.arange(V, dtype=np.int64), np.diff(off).astype(np.int64))
+    both = active[src] & active[edges]
+    key = src * V + edges.astype(np.int64)  # ascending: rows in order, each row ascending
+    rkey = edges[both].astype(np.int64) * V + src[both]
+    pos = np.searchsorted(key, rkey)
+    assert np.all(key[pos] == rkey)  # symmetric graph
+    assert np.array_equal(bits[both], bits[pos])
+    # fixed point
+    r = pr.step(STEP_LCC)
+    assert (r.vertex_eliminated, r.edge_eliminated, r.match_eliminated) == (0, 0, 0)
+    print(f"scale 22: {len(rows)} steps, {int(np.count_nonzero(active))} active vertices, "
+          f"{E - eliminated} active slots", flush=True)
+    pr.close()
