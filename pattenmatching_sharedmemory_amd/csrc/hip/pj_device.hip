@@ -208,8 +208,8 @@ __global__ void k_rows_sorted(const uint32_t* __restrict__ edges, uint64_t E, co
     }
 }
 
-// Symmetry check on rows sorted strictly ascending, one wave per vertex u.
-// Order vertices by (degree, id).  Every slot u->v with v before u is looked
+// Symmetry check on rows sorted strictly ascending (k_rows_symmetric and,
+// for hub rows, k_rows_symmetric_chunks).  Order vertices by (degree, id).  Every slot u->v with v before u is looked
 // up in the shorter row(v) by binary search (flag[0] set when u is missing);
 // flag[1] / flag[2] count the slots u->v with v before / after u.  When all
 // lookups succeed and the counts agree, the graph is symmetric: reversing the
@@ -219,6 +219,63 @@ __global__ void k_rows_sorted(const uint32_t* __restrict__ edges, uint64_t E, co
 // a slot u->v is the position of u in row(v), and the reverse slot is
 // off[v] + rev[e] (a self-loop is its own reverse).  rev is complete exactly
 // when the graph is found symmetric.
+
+// One slot u->v of the symmetry check (row(u) starts at beg, has du slots).
+__device__ __forceinline__ void symmetric_slot(uint64_t e, uint64_t u, uint64_t beg, uint64_t du,
+                                               const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+                                               uint32_t V, uint32_t* __restrict__ rev, bool& bad, bool& loop,
+                                               uint64_t& before, uint64_t& after) {
+  const uint32_t v = edges[e];
+  if (v >= V) {
+    bad = true;
+    return;
+  }
+  if (v == u) {
+    if (rev) rev[e] = static_cast<uint32_t>(e - beg);
+    loop = true;
+    return;
+  }
+  uint64_t lo = off[v], hi = off[v + 1];
+  const uint64_t dv = hi - lo;
+  if (dv < du || (dv == du && v < u)) {
+    ++before;
+    const uint64_t last = hi;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (edges[mid] < u)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    if (lo == last || edges[lo] != u) {
+      bad = true;
+    } else if (rev) {
+      rev[e] = static_cast<uint32_t>(lo - off[v]);
+      rev[lo] = static_cast<uint32_t>(e - beg);
+    }
+  } else {
+    ++after;
+  }
+}
+
+__device__ __forceinline__ void symmetric_flags(unsigned long long* flag, bool bad, bool loop, uint64_t before,
+                                                uint64_t after) {
+  const int lane = __lane_id();
+  for (int o = 32; o > 0; o >>= 1) {
+    before += __shfl_xor(before, o, 64);
+    after += __shfl_xor(after, o, 64);
+  }
+  if (__ballot(bad) && lane == 0) atomicOr(&flag[0], 1ull);
+  if (__ballot(loop) && lane == 0) atomicOr(&flag[3], 1ull);  // a self-loop slot
+  if (lane == 0) {
+    if (before) atomicAdd(&flag[1], before);
+    if (after) atomicAdd(&flag[2], after);
+  }
+}
+
+// Rows up to kHubDegree slots, one wave per vertex; hub rows are split
+// into chunks by k_rows_symmetric_chunks (a single wave working through the
+// largest row would hold up the whole check).
 __global__ void __launch_bounds__(kBlock) k_rows_symmetric(const uint64_t* __restrict__ off,
                                                            const uint32_t* __restrict__ edges, uint32_t V,
                                                            unsigned long long* flag, uint32_t* __restrict__ rev) {
@@ -230,48 +287,27 @@ __global__ void __launch_bounds__(kBlock) k_rows_symmetric(const uint64_t* __res
   for (uint64_t u = wave; u < V; u += waves) {
     const uint64_t beg = off[u], end = off[u + 1];
     const uint64_t du = end - beg;
-    for (uint64_t e = beg + lane; e < end; e += 64) {
-      const uint32_t v = edges[e];
-      if (v >= V) {
-        bad = true;
-        continue;
-      }
-      if (v == u) {
-        if (rev) rev[e] = static_cast<uint32_t>(e - beg);
-        loop = true;
-        continue;
-      }
-      uint64_t lo = off[v], hi = off[v + 1];
-      const uint64_t dv = hi - lo;
-      if (dv < du || (dv == du && v < u)) {
-        ++before;
-        const uint64_t last = hi;
-        while (lo < hi) {
-          const uint64_t mid = (lo + hi) >> 1;
-          if (edges[mid] < u) lo = mid + 1;
-          else hi = mid;
-        }
-        if (lo == last || edges[lo] != u) {
-          bad = true;
-        } else if (rev) {
-          rev[e] = static_cast<uint32_t>(lo - off[v]);
-          rev[lo] = static_cast<uint32_t>(e - beg);
-        }
-      } else {
-        ++after;
-      }
-    }
+    if (du > kHubDegree) continue;
+    for (uint64_t e = beg + lane; e < end; e += 64) symmetric_slot(e, u, beg, du, off, edges, V, rev, bad, loop, before, after);
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    before += __shfl_xor(before, o, 64);
-    after += __shfl_xor(after, o, 64);
+  symmetric_flags(flag, bad, loop, before, after);
+}
+
+// Hub rows of the symmetry check: one block per chunk of kHubChunk slots.
+__global__ void __launch_bounds__(kBlock) k_rows_symmetric_chunks(
+    const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
+    const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+    uint32_t V, unsigned long long* flag, uint32_t* __restrict__ rev) {
+  bool bad = false, loop = false;
+  uint64_t before = 0, after = 0;
+  for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const uint64_t u = hubs[chunk_hub[c]];
+    const uint64_t beg = off[u], end = off[u + 1];
+    const uint64_t cb = chunk_start[c], ce = min(end, cb + kHubChunk);
+    for (uint64_t e = cb + threadIdx.x; e < ce; e += blockDim.x)
+      symmetric_slot(e, u, beg, end - beg, off, edges, V, rev, bad, loop, before, after);
   }
-  if (__ballot(bad) && lane == 0) atomicOr(&flag[0], 1ull);
-  if (__ballot(loop) && lane == 0) atomicOr(&flag[3], 1ull);  // a self-loop slot
-  if (lane == 0) {
-    if (before) atomicAdd(&flag[1], before);
-    if (after) atomicAdd(&flag[2], after);
-  }
+  symmetric_flags(flag, bad, loop, before, after);
 }
 
 // Sums the kCounterSlots slots of counter block blockIdx.x.
@@ -3521,6 +3557,10 @@ static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* of
       if (ctx->reverse_enabled && dev_alloc(ctx, &ctx->d_rev, E)) return -1;
       hipLaunchKernelGGL(k_rows_symmetric, dim3(grid_for(ctx, static_cast<uint64_t>(V) * 64)), dim3(kBlock), 0,
                          ctx->stream, ctx->d_off, ctx->d_edges, V, f, ctx->d_rev);
+      if (ctx->nchunks)
+        hipLaunchKernelGGL(k_rows_symmetric_chunks, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 16)),
+                           dim3(kBlock), 0, ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks,
+                           ctx->d_hubs, ctx->d_off, ctx->d_edges, V, f, ctx->d_rev);
       unsigned long long h[4] = {1, 0, 0, 1};
       PJ_HIP(hipMemcpyAsync(h, f, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
       PJ_HIP(hipStreamSynchronize(ctx->stream));
