@@ -436,17 +436,17 @@ __device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __rest
   const uint8_t f = flags[v];
   const uint8_t mk = marks ? marks[v] : uint8_t(1);
   const uint8_t cd = mode == 2 ? code[v] : uint8_t(0);
-  const Word tu = mode == 3 || mode == 4 ? unmatch[v] : Word(0);
+  const Word tu = mode == 3 || mode == 4 || mode == 6 ? unmatch[v] : Word(0);
   deg = off[v + 1] - off[v];
   if (!c) return -1;
   if (mode == 0) {
     if (!(f & kScheduled)) return -1;
     if (!mk) return -2;  // clean: skipped
-  } else if (mode == 3 || mode == 4) {
+  } else if (mode == 3 || mode == 4 || mode == 6) {
     if (f & kModified) {
       if (!tu && (f & kScheduled)) return -1;  // nothing to apply
-    } else if (!mk) {
-      return mode == 3 ? -2 : -3;  // scan result stands / no modified neighbour
+    } else if (mode == 6 || !mk) {
+      return mode == 3 ? -2 : -3;  // scan result stands / no modified neighbour / pushed first hit
     }
   } else if (mode == 5) {
     if (!(f & kModified)) return -1;
@@ -470,7 +470,7 @@ __global__ void __launch_bounds__(kBlock) k_build_bins(
     const uint32_t* __restrict__ live, uint32_t n_live, const Word* __restrict__ cand, uint8_t* __restrict__ flags, const uint8_t* __restrict__ code, const uint8_t* __restrict__ marks,
     const Word* __restrict__ unmatch, const uint64_t* __restrict__ off, uint32_t V, int mode,
     const DevTemplate* __restrict__ t, uint32_t* __restrict__ bins, uint32_t* __restrict__ bin_count,
-    uint32_t* __restrict__ last_scan, unsigned long long* counters) {
+    uint32_t* __restrict__ last_scan, const uint32_t* __restrict__ first, unsigned long long* counters) {
   __shared__ uint32_t s_wave[kBlock / 64][kBins];
   __shared__ uint32_t s_base[kBlock / 64][kBins];
   const int lane = lane_id();
@@ -493,17 +493,20 @@ __global__ void __launch_bounds__(kBlock) k_build_bins(
 #pragma unroll
   for (int i = 0; i < kBinIters; ++i) {
     if (bin[i] == -2) skipped += mode == 0 ? deg[i] : last_scan[vid[i]];
-    if (bin[i] == -3) {  // the scan finds no modified neighbour
+    if (bin[i] == -3) {  // the scan finds no modified neighbour, or (first given) stops at first[v]
       const uint32_t v = vid[i];
       const uint8_t f = flags[v];
-      if (f & kScheduled) flags[v] = static_cast<uint8_t>(f & ~kScheduled);
-      last_scan[v] = static_cast<uint32_t>(deg[i]);
-      skipped += deg[i];
+      const uint32_t h = first ? first[v] : 0xFFFFFFFFu;
+      const uint64_t scanned = h != 0xFFFFFFFFu ? h + 1ull : deg[i];
+      const uint8_t nf = h != 0xFFFFFFFFu ? static_cast<uint8_t>(f | kScheduled) : static_cast<uint8_t>(f & ~kScheduled);
+      if (nf != f) flags[v] = nf;
+      last_scan[v] = static_cast<uint32_t>(scanned);
+      skipped += scanned;
     }
 #pragma unroll
     for (int k = 0; k < kBins; ++k) cnt[k] += __popcll(__ballot(bin[i] == k));
   }
-  if (marks) wave_add(counters, C_SKIP, skipped);
+  if (marks || first) wave_add(counters, C_SKIP, skipped);
   if (lane == 0)
     for (int k = 0; k < kBins; ++k) s_wave[wave][k] = cnt[k];
   __syncthreads();
@@ -898,12 +901,36 @@ __global__ void k_lcc_hub_finish(const uint32_t* __restrict__ hubs, uint32_t nhu
   }
 }
 
-// Hubs that turned modified in the last phase 1 mark their neighbours for a
-// phase 2 rescan, one block per chunk of their rows.
+// The push of slot e = v->u of a modified vertex v: with reverse positions
+// (`first` given), u's phase 2 scan would stop at the slot u->v, position
+// rev[e] of row(u), if that slot is active: first[u] keeps the smallest such
+// position over all modified neighbours.  Otherwise u is marked for a rescan.
+__device__ __forceinline__ void push_one(uint64_t e, const uint32_t* __restrict__ edges,
+                                         const uint64_t* __restrict__ off, uint8_t* __restrict__ rescan,
+                                         const uint32_t* __restrict__ ea, const uint32_t* __restrict__ rev,
+                                         uint32_t* __restrict__ first) {
+  const uint32_t u = edges[e];
+  if (!first) {
+    rescan[u] = 1;
+    return;
+  }
+  const uint32_t p = rev[e];
+  // A plain load of first[u] beside off[u]: most pushes to a vertex of many
+  // modified neighbours cannot lower its minimum (a stale value is only
+  // larger) and skip the slot test and the atomic.
+  const uint64_t ou = off[u];
+  const uint32_t cur = first[u];
+  if (p < cur && ea_test(ea, ou + p)) atomicMin(&first[u], p);
+}
+
+// Hubs that turned modified in the last phase 1 (hub_new), or every modified
+// hub (flags given), push to the vertices of their rows (push_one), one block
+// per chunk of their rows.
 __global__ void __launch_bounds__(kBlock) k_hub_rescan_push(
     const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
     const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
     const uint32_t* __restrict__ hub_new, const uint8_t* __restrict__ flags, uint8_t* __restrict__ rescan,
+    const uint32_t* __restrict__ ea, const uint32_t* __restrict__ rev, uint32_t* __restrict__ first,
     unsigned long long* counters) {
   uint64_t pushed = 0;
   // The activity of up to kBlock of this block's chunks (blockIdx.x + k * gridDim.x)
@@ -932,7 +959,7 @@ __global__ void __launch_bounds__(kBlock) k_hub_rescan_push(
         const uint32_t v = hubs[h];
         const uint64_t beg = chunk_start[c];
         const uint64_t end = min(off[v + 1], beg + kHubChunk);
-        for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) rescan[edges[e]] = 1;
+        for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) push_one(e, edges, off, rescan, ea, rev, first);
         if (threadIdx.x == 0) pushed += end - beg;
       }
     }
@@ -941,14 +968,16 @@ __global__ void __launch_bounds__(kBlock) k_hub_rescan_push(
   wave_add(counters, C_XSLOTS, pushed);
 }
 
-// Vertices of a bin (mode 5: modified) mark the vertices of their rows for a
-// phase 2 rescan, one group of G lanes per vertex.
+// Vertices of a bin (mode 5: modified) push to the vertices of their rows
+// (push_one), one group of G lanes per vertex.
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_push_rescan(const uint32_t* __restrict__ list,
                                                         const uint32_t* __restrict__ list_count,
                                                         const uint64_t* __restrict__ off,
                                                         const uint32_t* __restrict__ edges,
-                                                        uint8_t* __restrict__ rescan, unsigned long long* counters) {
+                                                        uint8_t* __restrict__ rescan, const uint32_t* __restrict__ ea,
+                                                        const uint32_t* __restrict__ rev, uint32_t* __restrict__ first,
+                                                        unsigned long long* counters) {
   const int gl = lane_id() & (G - 1);
   const uint32_t n = *list_count;
   const uint64_t groups_per_grid = (uint64_t)gridDim.x * (blockDim.x / G);
@@ -956,7 +985,7 @@ __global__ void __launch_bounds__(kBlock) k_push_rescan(const uint32_t* __restri
   for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G; g < n; g += groups_per_grid) {
     const uint32_t v = list[g];
     const uint64_t beg = off[v], end = off[v + 1];
-    for (uint64_t e = beg + gl; e < end; e += G) rescan[edges[e]] = 1;
+    for (uint64_t e = beg + gl; e < end; e += G) push_one(e, edges, off, rescan, ea, rev, first);
     if (gl == 0) pushed += end - beg;
   }
   for (int o = 32; o > 0; o >>= 1) pushed += __shfl_xor(pushed, o, 64);
@@ -2441,7 +2470,7 @@ struct pj_ctx {
   uint8_t* d_code = nullptr;
   void* d_cand = nullptr;
   void* d_unmatch = nullptr;
-  void* d_u64 = nullptr;   // staging for downloads / uploads
+  void* d_u64 = nullptr;   // staging for downloads / uploads; first hits of an NLCC phase 2 (push_modified)
   uint8_t* d_flags = nullptr;
   uint8_t* d_vflags = nullptr;
   uint8_t* d_dirty = nullptr;  // per vertex: candidates of v or of a neighbour changed since v's last evaluation
@@ -2771,7 +2800,8 @@ void launch_bins(pj_ctx* ctx, int mode) {
   hipLaunchKernelGGL(k_build_bins<Word>, dim3(std::max<uint32_t>(blocks, 1)), dim3(kBlock), 0, ctx->stream,
                      ctx->live, ctx->n_live, static_cast<const Word*>(ctx->d_cand), ctx->d_flags, ctx->d_code, marks,
                      static_cast<const Word*>(ctx->d_unmatch), ctx->d_off, ctx->V, mode, ctx->d_t, ctx->d_bins,
-                     ctx->d_bin_count, ctx->d_last_scan, c);
+                     ctx->d_bin_count, ctx->d_last_scan, mode == 6 ? static_cast<const uint32_t*>(ctx->d_u64) : nullptr,
+                     c);
   end_launch(ctx);
 }
 
@@ -2800,30 +2830,48 @@ void launch_phase2(pj_ctx* ctx, int mode = 3) {
   ctx->rescan_all = false;  // the scans cleared the marks of the vertices they read
 }
 
-// Rescan marks of an NLCC phase 2: cleared, then set on the rows of every
-// modified vertex (bins of mode 5, hubs by chunks).
+// Reverse positions of the full CSR for complete walks and NLCC pushes (null
+// where the graph is not known to be sorted and symmetric, or they are
+// switched off).
+const uint32_t* reverse_positions(const pj_ctx* ctx) {
+  return ctx->rows_sorted && ctx->symmetric ? ctx->d_rev : nullptr;
+}
+
+// The push of an NLCC phase 2: rescan marks cleared, then every modified
+// vertex (bins of mode 5, hubs by chunks) pushes to the vertices of its row.
+// With reverse positions the push computes each neighbour's first hit
+// (first[u], in the staging array) and the phase 2 that follows (mode 6)
+// scans no row of an unmodified vertex; without them it sets rescan marks
+// (mode 4).  Returns the phase 2 mode.
 template <typename Word>
-void push_modified(pj_ctx* ctx) {
+int push_modified(pj_ctx* ctx) {
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, (ctx->V + 4) / 4)), dim3(kBlock), 0, ctx->stream,
                      reinterpret_cast<uint32_t*>(ctx->d_rescan), static_cast<uint64_t>((ctx->V + 4) / 4), 0u);
+  const uint32_t* rev = reverse_positions(ctx);
+  uint32_t* first = rev ? static_cast<uint32_t*>(ctx->d_u64) : nullptr;
+  const uint32_t* ea = rev ? ctx->d_ea : nullptr;
+  if (first)
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, ctx->V)), dim3(kBlock), 0, ctx->stream, first,
+                       static_cast<uint64_t>(ctx->V), 0xFFFFFFFFu);
   launch_bins<Word>(ctx, 5);
   const uint64_t V = ctx->V;
   const int g = ctx->cus * 8;
   unsigned long long* c = begin_launch(ctx, K_HUB_PUSH);
   hipLaunchKernelGGL(k_push_rescan<4>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins, ctx->d_bin_count + 0,
-                     ctx->d_off, ctx->d_edges, ctx->d_rescan, c);
+                     ctx->d_off, ctx->d_edges, ctx->d_rescan, ea, rev, first, c);
   hipLaunchKernelGGL(k_push_rescan<16>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + V,
-                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_rescan, c);
+                     ctx->d_bin_count + 1, ctx->d_off, ctx->d_edges, ctx->d_rescan, ea, rev, first, c);
   hipLaunchKernelGGL(k_push_rescan<32>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 2 * V,
-                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_rescan, c);
+                     ctx->d_bin_count + 2, ctx->d_off, ctx->d_edges, ctx->d_rescan, ea, rev, first, c);
   hipLaunchKernelGGL(k_push_rescan<64>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + 3 * V,
-                     ctx->d_bin_count + 3, ctx->d_off, ctx->d_edges, ctx->d_rescan, c);
+                     ctx->d_bin_count + 3, ctx->d_off, ctx->d_edges, ctx->d_rescan, ea, rev, first, c);
   if (ctx->nhubs)
     hipLaunchKernelGGL(k_hub_rescan_push, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
                        ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
-                       ctx->d_edges, ctx->d_hub_new, ctx->d_flags, ctx->d_rescan, c);
+                       ctx->d_edges, ctx->d_hub_new, ctx->d_flags, ctx->d_rescan, ea, rev, first, c);
   end_launch(ctx);
   ctx->rescan_all = false;
+  return first ? 6 : 4;
 }
 
 template <typename Word, int KW>
@@ -2856,7 +2904,7 @@ void launch_lcc_phase1(pj_ctx* ctx) {
       c = begin_launch(ctx, K_HUB_PUSH);
       hipLaunchKernelGGL(k_hub_rescan_push, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * 8)), dim3(kBlock), 0,
                          ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
-                         ctx->d_edges, ctx->d_hub_new, nullptr, rescan, c);
+                         ctx->d_edges, ctx->d_hub_new, nullptr, rescan, nullptr, nullptr, nullptr, c);
       end_launch(ctx);
     }
   }
@@ -2997,12 +3045,6 @@ int build_view(pj_ctx* ctx) {
 // (state_fail): several partial walks reach the same (vertex, hop) there.
 bool long_walk(const pj_ctx* ctx, int kind) {
   return (kind == 0 || kind == 1) && ctx->h_t.nlcc_len >= 4;
-}
-
-// Reverse positions of the full CSR for complete walks (null where the graph
-// is not known to be sorted and symmetric, or they are switched off).
-const uint32_t* reverse_positions(const pj_ctx* ctx) {
-  return ctx->rows_sorted && ctx->symmetric ? ctx->d_rev : nullptr;
 }
 
 // Memoised 3-vertex cycle steps that k_triangles_memo decides exactly:
@@ -3347,8 +3389,7 @@ int run_step_t(pj_ctx* ctx, int step, uint32_t index) {
         if (ctx->skip_clean && ctx->modified_cleared) {
           // Only the failing sources are modified: they mark their rows and
           // every unmarked vertex has no modified neighbour.
-          push_modified<Word>(ctx);
-          launch_phase2<Word>(ctx, 4);
+          launch_phase2<Word>(ctx, push_modified<Word>(ctx));
         } else {
           mark_all_rescan(ctx);  // sources that failed turned modified
           launch_phase2<Word>(ctx);
