@@ -136,6 +136,9 @@ __device__ __forceinline__ bool walk_slot(const uint32_t* __restrict__ ea, uint6
 // Wave sum of a per-lane value, added by lane 0 to a spread counter slot.
 // Must be called with every lane of the wave active.
 __device__ __forceinline__ void wave_add(unsigned long long* counters, int which, uint64_t v) {
+  // The shuffle reduction is a chain of twelve cross-lane moves; a wave whose
+  // lanes all hold 0 (most waves of a launch over a small frontier) skips it.
+  if (!__ballot(v != 0)) return;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   if (lane_id() == 0 && v) {
     int slot = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) & (kCounterSlots - 1);
@@ -811,8 +814,10 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
           }
         }
         if (mine) atomicAdd(&hub_count[h * kMaxLabels + lane], mine);
-        for (int o = 32; o > 0; o >>= 1) elim += __shfl_xor(elim, o, 64);
-        if (lane == 0 && elim) atomicAdd(&hub_elim[h], elim);
+        if (__ballot(elim != 0)) {
+          for (int o = 32; o > 0; o >>= 1) elim += __shfl_xor(elim, o, 64);
+          if (lane == 0) atomicAdd(&hub_elim[h], elim);
+        }
         elim_total += lane == 0 ? elim : 0u;
         if (threadIdx.x == 0) {
           slots_total += end - beg;
@@ -988,11 +993,7 @@ __global__ void __launch_bounds__(kBlock) k_push_rescan(const uint32_t* __restri
     for (uint64_t e = beg + gl; e < end; e += G) push_one(e, edges, off, rescan, ea, rev, first);
     if (gl == 0) pushed += end - beg;
   }
-  for (int o = 32; o > 0; o >>= 1) pushed += __shfl_xor(pushed, o, 64);
-  if (lane_id() == 0 && pushed)
-    atomicAdd(&counters[((blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) & (kCounterSlots - 1)) * kCounterStride +
-                        C_XSLOTS],
-              static_cast<unsigned long long>(pushed));
+  wave_add(counters, C_XSLOTS, pushed);
 }
 
 // ---------------------------------------------------------------------------
@@ -1161,8 +1162,10 @@ __device__ void phase2_hub_chunks(uint32_t block, uint32_t nblocks, const uint32
 #pragma unroll
         for (int k = kPer - 1; k >= 0; --k)
           if (fl[k] & kModified) first = static_cast<uint32_t>(beg + threadIdx.x + k * kBlock - row);
-        for (int o = 32; o > 0; o >>= 1) first = min(first, static_cast<uint32_t>(__shfl_xor(first, o, 64)));
-        if (lane_id() == 0 && first != 0xFFFFFFFFu) atomicMin(&hub_hit[h], first);
+        if (__ballot(first != 0xFFFFFFFFu)) {
+          for (int o = 32; o > 0; o >>= 1) first = min(first, static_cast<uint32_t>(__shfl_xor(first, o, 64)));
+          if (lane_id() == 0) atomicMin(&hub_hit[h], first);
+        }
         if (threadIdx.x == 0) examined += end - beg;
       }
     }
