@@ -133,13 +133,21 @@ __device__ __forceinline__ bool walk_slot(const uint32_t* __restrict__ ea, uint6
   return !ea || ea_test(ea, e);
 }
 
+}  // namespace
+
+// Wave reduction of the ROCm device library (ockl), declared by HIP's headers
+// only under HIP_ENABLE_EXTRA_WARP_SYNC_TYPES.
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_add_u64(unsigned long long);
+
+namespace {
+
 // Wave sum of a per-lane value, added by lane 0 to a spread counter slot.
 // Must be called with every lane of the wave active.
 __device__ __forceinline__ void wave_add(unsigned long long* counters, int which, uint64_t v) {
-  // The shuffle reduction is a chain of twelve cross-lane moves; a wave whose
-  // lanes all hold 0 (most waves of a launch over a small frontier) skips it.
+  // A wave whose lanes all hold 0 (most waves of a launch over a small
+  // frontier) skips the reduction (DPP steps of the device library).
   if (!__ballot(v != 0)) return;
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  v = __ockl_wfred_add_u64(v);
   if (lane_id() == 0 && v) {
     int slot = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) & (kCounterSlots - 1);
     atomicAdd(&counters[slot * kCounterStride + which], static_cast<unsigned long long>(v));
@@ -815,7 +823,7 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
         }
         if (mine) atomicAdd(&hub_count[h * kMaxLabels + lane], mine);
         if (__ballot(elim != 0)) {
-          for (int o = 32; o > 0; o >>= 1) elim += __shfl_xor(elim, o, 64);
+          elim = __ockl_wfred_add_u32(elim);
           if (lane == 0) atomicAdd(&hub_elim[h], elim);
         }
         elim_total += lane == 0 ? elim : 0u;
@@ -1163,7 +1171,7 @@ __device__ void phase2_hub_chunks(uint32_t block, uint32_t nblocks, const uint32
         for (int k = kPer - 1; k >= 0; --k)
           if (fl[k] & kModified) first = static_cast<uint32_t>(beg + threadIdx.x + k * kBlock - row);
         if (__ballot(first != 0xFFFFFFFFu)) {
-          for (int o = 32; o > 0; o >>= 1) first = min(first, static_cast<uint32_t>(__shfl_xor(first, o, 64)));
+          first = __ockl_wfred_min_u32(first);
           if (lane_id() == 0) atomicMin(&hub_hit[h], first);
         }
         if (threadIdx.x == 0) examined += end - beg;
