@@ -843,7 +843,7 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
 // Hub rows of LCC phase 1 in a launch of their own (fused with the bins,
 // the unrolled chunk loop raises the register count of the whole kernel).
 template <typename Word>
-__global__ void __launch_bounds__(kBlock) k_lcc_hub_chunks(
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_lcc_hub_chunks(
     const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
     const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
     const uint8_t* __restrict__ code, bool derive, const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
@@ -859,7 +859,7 @@ __global__ void __launch_bounds__(kBlock) k_lcc_hub_chunks(
 // LCC phase 1 of the degree bins in one launch: G = 64, 16 and 4 lanes per
 // vertex, each bin grid-strided over the whole grid.
 template <typename Word, int KW>
-__global__ void __launch_bounds__(kBlock) k_lcc_phase1_all(
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_lcc_phase1_all(
     const uint32_t* __restrict__ bins, const uint32_t* __restrict__ bin_count, uint32_t V,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code,
     bool derive, const Word* __restrict__ cand, Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
@@ -1187,7 +1187,7 @@ __device__ void phase2_hub_chunks(uint32_t block, uint32_t nblocks, const uint32
 // bins only their own vertices; every vertex reads of its neighbours only
 // the modified bit, which phase 2 leaves as it is.
 template <typename Word>
-__global__ void __launch_bounds__(kBlock) k_phase2_all(
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_phase2_all(
     const uint32_t* __restrict__ bins, const uint32_t* __restrict__ bin_count, uint32_t V,
     const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
     const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
