@@ -2264,7 +2264,7 @@ __global__ void __launch_bounds__(kBlock) k_triangles(const uint32_t* __restrict
 // alreadyMatched, as the walk does.  Rows of a are read 64 slots at a time
 // and the first lane in row order decides.  One wave per source.
 template <typename Word>
-__global__ void __launch_bounds__(kBlock) k_triangles_memo(const uint32_t* __restrict__ list,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_triangles_memo(const uint32_t* __restrict__ list,
                                                            const uint32_t* __restrict__ list_count,
                                                            const uint64_t* __restrict__ off,
                                                            const uint32_t* __restrict__ edges,
@@ -2357,7 +2357,7 @@ __global__ void __launch_bounds__(kBlock) k_triangles_memo(const uint32_t* __res
 // modified (cc_cpu.tpp:94-129, pc_cpu.tpp:95-147, tds_cpu.tpp:153-188).
 // kind 0: cycle, 1: path, 2: TDS walks, 3: enumeration (counts only).
 template <typename Word>
-__global__ void __launch_bounds__(kBlock) k_nlcc(int kind, const uint32_t* __restrict__ list,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_nlcc(int kind, const uint32_t* __restrict__ list,
                                                  const uint32_t* __restrict__ list_count,
                                                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
                                                  const uint32_t* __restrict__ ea, const uint32_t* __restrict__ rev,
