@@ -599,22 +599,48 @@ __global__ void __launch_bounds__(kBlock) k_live_compact(const uint32_t* __restr
 // candidate p of v is then kept only if v's label is p's label and every
 // required neighbour-label count of p is met.
 
-template <typename Word, int KW>
+// The tables are the template's pcode[p] and need[p][w] (row stride NS):
+// the DevTemplate itself, or a block's LDS copy of them.
+template <typename Word, int KW, int NS>
 __device__ __forceinline__ Word lcc_requirements(Word cv, uint8_t vcode, const uint64_t (&cnt)[KW],
-                                                 const DevTemplate* __restrict__ t) {
+                                                 const uint8_t* pcode, const uint64_t* need) {
   const uint64_t H = 0x8080808080808080ull;
   Word fail = 0;
   for (Word m = cv; m; m &= m - 1) {
     int p = ctz<Word>(m);
-    bool bad = t->pcode[p] != vcode;
+    bool bad = pcode[p] != vcode;
 #pragma unroll
     for (int w = 0; w < KW; ++w) {
-      uint64_t d = (cnt[w] | H) - t->need[p][w];
+      uint64_t d = (cnt[w] | H) - need[p * NS + w];
       bad |= ((~d) & H) != 0;
     }
     if (bad) fail |= Word(1) << p;
   }
   return fail;
+}
+
+template <typename Word, int KW>
+__device__ __forceinline__ Word lcc_requirements(Word cv, uint8_t vcode, const uint64_t (&cnt)[KW],
+                                                 const DevTemplate* __restrict__ t) {
+  return lcc_requirements<Word, KW, 8>(cv, vcode, cnt, t->pcode, &t->need[0][0]);
+}
+
+// A block's LDS copy of the template tables LCC phase 1 reads per vertex
+// (codes, template neighbours, required counts of the first KW words).
+template <int KW>
+struct LccTables {
+  uint8_t pcode[64];
+  uint64_t nbr[64];
+  uint64_t need[64 * KW];
+};
+
+template <int KW>
+__device__ __forceinline__ void load_lcc_tables(LccTables<KW>& s, const DevTemplate* __restrict__ t) {
+  for (int i = threadIdx.x; i < 64; i += blockDim.x) {
+    s.pcode[i] = t->pcode[i];
+    s.nbr[i] = t->nbr[i];
+  }
+  for (int i = threadIdx.x; i < 64 * KW; i += blockDim.x) s.need[i] = t->need[i / KW][i % KW];
 }
 
 template <int KW>
@@ -642,7 +668,7 @@ __device__ __forceinline__ void count_labels(uint64_t (&cnt)[KW], bool counted, 
 // of a neighbour u is that of any template vertex in C(u) and is read from a
 // table in LDS instead of gathering code[u].
 template <typename Word, int G, int KW>
-__device__ void lcc_phase1_bin(uint32_t block, uint32_t nblocks, const uint8_t* s_pcode,
+__device__ void lcc_phase1_bin(uint32_t block, uint32_t nblocks, const LccTables<KW>& tab,
                                const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
                                const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
                                const uint8_t* __restrict__ code, bool derive, const Word* __restrict__ cand,
@@ -663,7 +689,7 @@ __device__ void lcc_phase1_bin(uint32_t block, uint32_t nblocks, const uint8_t* 
       const uint32_t v = list[g];
       const Word cv = cand[v];
       Word U = 0;
-      for (Word m = cv; m; m &= m - 1) U |= static_cast<Word>(t->nbr[ctz<Word>(m)]);
+      for (Word m = cv; m; m &= m - 1) U |= static_cast<Word>(tab.nbr[ctz<Word>(m)]);
       const uint64_t beg = off[v], end = off[v + 1];
       uint64_t cnt[KW];
 #pragma unroll
@@ -699,7 +725,7 @@ __device__ void lcc_phase1_bin(uint32_t block, uint32_t nblocks, const uint8_t* 
               el = true;
               ++elim;
             } else {
-              lab = derive ? s_pcode[ctz<Word>(cu[k])] : lb[k];
+              lab = derive ? tab.pcode[ctz<Word>(cu[k])] : lb[k];
               counted = lab != kNoCode;
             }
           }
@@ -710,7 +736,7 @@ __device__ void lcc_phase1_bin(uint32_t block, uint32_t nblocks, const uint8_t* 
       const bool any_elim = (__ballot(elim != 0) & gmask) != 0;
       elim_total += elim;
       // every lane of the group holds the same counts, so each can evaluate
-      const Word fail = lcc_requirements<Word, KW>(cv, code[v], cnt, t);
+      const Word fail = lcc_requirements<Word, KW, KW>(cv, code[v], cnt, tab.pcode, tab.need);
       const uint8_t f = flags[v];
       if (gl == 0) {
         slots_total += end - beg;
@@ -859,26 +885,26 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
 // LCC phase 1 of the degree bins in one launch: G = 64, 16 and 4 lanes per
 // vertex, each bin grid-strided over the whole grid.
 template <typename Word, int KW>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_lcc_phase1_all(
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW <= 2 ? 8 : 1))) k_lcc_phase1_all(
     const uint32_t* __restrict__ bins, const uint32_t* __restrict__ bin_count, uint32_t V,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code,
     bool derive, const Word* __restrict__ cand, Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
     uint8_t* __restrict__ dirty, uint8_t* __restrict__ rescan, uint32_t* __restrict__ ea,
     const DevTemplate* __restrict__ t, unsigned long long* counters) {
-  __shared__ uint8_t s_pcode[64];
-  if (threadIdx.x < 64) s_pcode[threadIdx.x] = t->pcode[threadIdx.x];
+  __shared__ LccTables<KW> tab;
+  load_lcc_tables<KW>(tab, t);
   __syncthreads();
   if (blockIdx.x == 0 && threadIdx.x == 0)
     atomicAdd(&counters[C_ITEMS],
               static_cast<unsigned long long>(bin_count[0]) + bin_count[1] + bin_count[2] + bin_count[3]);
   const uint64_t V64 = V;
-  lcc_phase1_bin<Word, 64, KW>(blockIdx.x, gridDim.x, s_pcode, bins + 3 * V64, bin_count + 3, off, edges, code,
+  lcc_phase1_bin<Word, 64, KW>(blockIdx.x, gridDim.x, tab, bins + 3 * V64, bin_count + 3, off, edges, code,
                                derive, cand, unmatch, flags, dirty, rescan, ea, t, counters);
-  lcc_phase1_bin<Word, 32, KW>(blockIdx.x, gridDim.x, s_pcode, bins + 2 * V64, bin_count + 2, off, edges, code,
+  lcc_phase1_bin<Word, 32, KW>(blockIdx.x, gridDim.x, tab, bins + 2 * V64, bin_count + 2, off, edges, code,
                                derive, cand, unmatch, flags, dirty, rescan, ea, t, counters);
-  lcc_phase1_bin<Word, 16, KW>(blockIdx.x, gridDim.x, s_pcode, bins + V64, bin_count + 1, off, edges, code, derive,
+  lcc_phase1_bin<Word, 16, KW>(blockIdx.x, gridDim.x, tab, bins + V64, bin_count + 1, off, edges, code, derive,
                                cand, unmatch, flags, dirty, rescan, ea, t, counters);
-  lcc_phase1_bin<Word, 4, KW>(blockIdx.x, gridDim.x, s_pcode, bins, bin_count + 0, off, edges, code, derive, cand,
+  lcc_phase1_bin<Word, 4, KW>(blockIdx.x, gridDim.x, tab, bins, bin_count + 0, off, edges, code, derive, cand,
                               unmatch, flags, dirty, rescan, ea, t, counters);
 }
 
