@@ -429,9 +429,9 @@ constexpr int kBinVerts = kBlock * kBinIters;  // vertices per block
 //
 // mode 5: active and modified (the sources that mark their neighbours).
 //
-// The walk task lists (modes 1 and 2) leave bin 2 empty: their kernels take
-// one wave per source above 32 slots, so bins 2 and 3 would be launched
-// alike.
+// The walk task lists (modes 1 and 2) use the same bins: one wave per source
+// above 32 slots, bin 2 (33-256 slots) taken by k_triangles_mid in 3-vertex
+// cycle steps.
 //
 // Returns the bin, -1 (not in the frontier), -2 (skipped; its slots are
 // counted as examined) or -3 (mode 4 default).  Every load is issued before
@@ -466,7 +466,7 @@ __device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __rest
   }
   if (deg <= c_bin_limit[0]) return 0;
   if (deg <= c_bin_limit[1]) return 1;
-  if (deg <= c_bin_limit[2] && mode != 1 && mode != 2) return 2;  // walk task lists: one bin above 32 slots
+  if (deg <= c_bin_limit[2]) return 2;
   if (deg <= c_bin_limit[3]) return 3;
   return -1;
 }
@@ -2273,6 +2273,132 @@ __global__ void __launch_bounds__(kBlock) k_triangles(const uint32_t* __restrict
   wave_add(counters, C_SLOTS, slots);
 }
 
+// Triangle existence for a source of at most 64 * KC slots: the test of
+// triangle_small over KC chunks of 64 slots of row(s).  Every chunk is
+// classified at once (lane i keeps slot i of each chunk in registers, all
+// loads of a level issued together), and each lane learns the slot of the
+// r-th a and the r-th u of each chunk; the pairs of each (A chunk, U chunk)
+// are then checked 64 at a time, A chunks in row order.  The pairs are
+// packed densely (only a in A and u in U), so a source whose candidates fail
+// costs |A| x |U| / 64 rounds of searches, not one round per slot of row(s)
+// for every a as in the walk.
+template <typename Word, int KC>
+__device__ bool triangle_chunks(uint32_t s, int q1, int q2, const uint64_t* __restrict__ off,
+                                const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
+                                const uint32_t* __restrict__ rev, const Word* __restrict__ cand, uint64_t& slots) {
+  const int lane = lane_id();
+  const uint64_t beg = off[s];
+  const int deg = static_cast<int>(off[s + 1] - beg);
+  uint32_t w[KC];
+  Word cw[KC];
+  uint32_t rv[KC];
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int i = k * 64 + lane;
+    w[k] = i < deg ? edges[beg + i] : 0u;
+    rv[k] = i < deg ? rev[beg + i] : 0u;
+  }
+  uint64_t ma[KC], mu[KC];
+  uint64_t wo[KC];
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int i = k * 64 + lane;
+    cw[k] = i < deg ? cand[w[k]] : Word(0);
+    wo[k] = (i < deg && ((cw[k] >> q2) & 1)) ? off[w[k]] : 0ull;
+  }
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int i = k * 64 + lane;
+    const bool in = i < deg;
+    ma[k] = __ballot(in && ((cw[k] >> q1) & 1) && ea_test(ea, beg + i));
+    mu[k] = __ballot(in && ((cw[k] >> q2) & 1) && ea_test(ea, wo[k] + rv[k]));  // slot u->s
+  }
+  uint64_t probes = 0;
+  bool found = false;
+  uint64_t any_a = 0, any_u = 0;
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    any_a |= ma[k];
+    any_u |= mu[k];
+  }
+  if (any_a && any_u) {
+    uint32_t a_list[KC], u_list[KC];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      int a_from = 0, u_from = 0, r = 0;  // lane r: slot of the r-th a / u of chunk k
+      for (uint64_t m = ma[k]; m; m &= m - 1, ++r)
+        if (lane == r) a_from = __builtin_ctzll(m);
+      r = 0;
+      for (uint64_t m = mu[k]; m; m &= m - 1, ++r)
+        if (lane == r) u_from = __builtin_ctzll(m);
+      a_list[k] = __shfl(w[k], a_from, 64);
+      u_list[k] = __shfl(w[k], u_from, 64);
+    }
+#pragma unroll
+    for (int ka = 0; ka < KC; ++ka) {
+#pragma unroll
+      for (int ku = 0; ku < KC; ++ku) {
+        if (found || !ma[ka] || !mu[ku]) continue;
+        const int na = __popcll(ma[ka]), nu = __popcll(mu[ku]);
+        const int pairs = na * nu;
+        for (int p0 = 0; p0 < pairs && !found; p0 += 64) {
+          const int p = p0 + lane;
+          const int ia = p / nu, iu = p - ia * nu;
+          const uint32_t a = __shfl(a_list[ka], ia & 63, 64), u = __shfl(u_list[ku], iu & 63, 64);
+          const bool ok = p < pairs && active_slot_rev(a, u, off, edges, ea, rev, probes);
+          found = __ballot(ok) != 0;
+        }
+      }
+    }
+  }
+  probes = __ockl_wfred_add_u64(probes);
+  slots += static_cast<uint64_t>(deg) + probes;
+  return found;
+}
+
+// 3-vertex cycle sources of 33 to 64 * kMidChunks slots (complete search;
+// sorted rows without duplicates, symmetric graph, reverse positions): one
+// wave per source, triangle_small / triangle_chunks per start position.
+constexpr int kMidChunks = 4;
+static_assert(kMidChunks * 64 == 256, "bin 2 holds rows of 33 to 256 slots");
+
+template <typename Word>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
+k_triangles_mid(const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
+                const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+                const uint32_t* __restrict__ ea, const uint32_t* __restrict__ rev, const Word* __restrict__ cand,
+                Word* __restrict__ unmatch, uint8_t* __restrict__ flags, const DevTemplate* __restrict__ t,
+                unsigned long long* counters) {
+  const uint32_t n = *list_count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_ITEMS] = n;
+  const uint32_t waves = gridDim.x * (blockDim.x / 64);
+  const uint32_t wave_id = uniform(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6));
+  const uint32_t qv[3] = {t->nlcc_vertex[0], t->nlcc_vertex[1], t->nlcc_vertex[2]};
+  uint64_t slots = 0;
+  for (uint32_t w = wave_id; w < n; w += waves) {
+    const uint32_t s = list[w];
+    const Word cs = cand[s];
+    const uint64_t deg = off[s + 1] - off[s];
+    Word fail = 0;
+    for (int j = 0; j < 3; ++j) {
+      const uint32_t q = qv[j];
+      if (!((cs >> q) & 1)) continue;
+      const int q1 = static_cast<int>(qv[(j + 1) % 3]), q2 = static_cast<int>(qv[(j + 2) % 3]);
+      const bool ok = deg <= 64 ? triangle_small<Word>(s, q1, q2, off, edges, ea, rev, cand, slots)
+                                : triangle_chunks<Word, kMidChunks>(s, q1, q2, off, edges, ea, rev, cand, slots);
+      if (!ok) fail |= Word(1) << q;
+    }
+    if (fail && lane_id() == 0) {  // s is in one task list once: this wave is its only writer
+      unmatch[s] = static_cast<Word>(unmatch[s] | fail);
+      set_flag_atomic(flags, s, kModified);
+    }
+  }
+  if (lane_id() == 0 && slots) {
+    const int slot = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) & (kCounterSlots - 1);
+    atomicAdd(&counters[slot * kCounterStride + C_SLOTS], static_cast<unsigned long long>(slots));
+  }
+}
+
 // Memoised 3-vertex cycle constraint (cc_cpu.tpp:12-70 with L = 3) on a
 // graph without duplicate slots or self-loops (sorted rows, reverse
 // positions).  For start j the reference walks s -> a -> u -> s with
@@ -3099,7 +3225,6 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
     if (prepare_memo(ctx, false)) return -1;
     launch_bins<Word>(ctx, 1);
     for (int b = 0; b < kBins; ++b) {
-      if (b == 2) continue;  // empty in walk task lists
       unsigned long long* c = begin_launch(ctx, K_TRIANGLES_MEMO);
       hipLaunchKernelGGL(k_triangles_memo<Word>, dim3(ctx->cus * kWalkBlocksPerCu), dim3(kBlock), 0, ctx->stream,
                          ctx->d_bins + b * static_cast<uint64_t>(ctx->V), ctx->d_bin_count + b, ctx->d_off,
@@ -3133,14 +3258,16 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
   const bool triangles = kind == 0 && !tables && rev && ctx->h_t.nlcc_len == 3;
   // Every bin holds active vertices; hubs are run from their own list.
   for (int b = 0; b < kBins; ++b) {
-    if (b == 2) continue;  // empty in walk task lists
     unsigned long long* c =
-        begin_launch(ctx, triangles && b <= 1 ? K_TRIANGLES : memo ? K_MEMO_WALK : K_NLCC);
+        begin_launch(ctx, triangles && b <= 2 ? K_TRIANGLES : memo ? K_MEMO_WALK : K_NLCC);
     if (triangles && b == 0)
       hipLaunchKernelGGL((k_triangles<Word, 4>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
                          ctx->d_bin_count + b, off, edges, ea, rev, cand, unm, ctx->d_flags, ctx->d_t, c);
     else if (triangles && b == 1)
       hipLaunchKernelGGL((k_triangles<Word, 32>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
+                         ctx->d_bin_count + b, off, edges, ea, rev, cand, unm, ctx->d_flags, ctx->d_t, c);
+    else if (triangles && b == 2)
+      hipLaunchKernelGGL(k_triangles_mid<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
                          ctx->d_bin_count + b, off, edges, ea, rev, cand, unm, ctx->d_flags, ctx->d_t, c);
     else if (memo)  // first pass with LDS tables; overflows go to the list of d_ovf[0]
       hipLaunchKernelGGL(k_memo_lds<Word>, dim3(kMemoWaves), dim3(64), 0, ctx->stream, ctx->d_bins + b * V,
