@@ -2511,12 +2511,16 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
 template <typename Word>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_nlcc(int kind, const uint32_t* __restrict__ list,
                                                  const uint32_t* __restrict__ list_count,
+                                                 const uint32_t* __restrict__ list2,
+                                                 const uint32_t* __restrict__ list2_count,
                                                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
                                                  const uint32_t* __restrict__ ea, const uint32_t* __restrict__ rev,
                                                  const Word* __restrict__ cand, Word* __restrict__ unmatch,
                                                  uint8_t* __restrict__ flags, const DevTemplate* __restrict__ t,
                                                  MemoArgs ma, unsigned long long* counters) {
-  const uint32_t n = *list_count;
+  // list2 (may be null) follows list: sources w >= n1 are list2[w - n1]
+  const uint32_t n1 = *list_count;
+  const uint32_t n = n1 + (list2 ? *list2_count : 0u);
   if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_ITEMS] = n;
   const uint32_t waves = gridDim.x * (blockDim.x / 64);
   const uint32_t wave_id = uniform(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6));
@@ -2536,7 +2540,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
   }
   uint64_t enumerated = 0, slots = 0;
   for (uint32_t w = wave_id; w < n; w += waves) {
-    const uint32_t s = list[w];
+    const uint32_t s = w < n1 ? list[w] : list2[w - n1];
     const Word cs = cand[s];
     Word fail = 0;
     const uint32_t L = t->nlcc_len;
@@ -3258,6 +3262,7 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
   const bool triangles = kind == 0 && !tables && rev && ctx->h_t.nlcc_len == 3;
   // Every bin holds active vertices; hubs are run from their own list.
   for (int b = 0; b < kBins; ++b) {
+    if (b == 2 && !triangles && !memo) continue;  // taken by the k_nlcc launch of bin 3
     unsigned long long* c =
         begin_launch(ctx, triangles && b <= 2 ? K_TRIANGLES : memo ? K_MEMO_WALK : K_NLCC);
     if (triangles && b == 0)
@@ -3272,10 +3277,11 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
     else if (memo)  // first pass with LDS tables; overflows go to the list of d_ovf[0]
       hipLaunchKernelGGL(k_memo_lds<Word>, dim3(kMemoWaves), dim3(64), 0, ctx->stream, ctx->d_bins + b * V,
                          ctx->d_bin_count + b, off, edges, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
-    else
+    else  // bin 3 (with bin 2 unless k_triangles_mid took it)
       hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, kind, ctx->d_bins + b * V,
-                         ctx->d_bin_count + b, off, edges, ea, reverse_positions(ctx), cand, unm, ctx->d_flags,
-                         ctx->d_t, ma, c);
+                         ctx->d_bin_count + b, b == 3 && !triangles ? ctx->d_bins + 2 * V : nullptr,
+                         b == 3 && !triangles ? ctx->d_bin_count + 2 : nullptr, off, edges, ea,
+                         reverse_positions(ctx), cand, unm, ctx->d_flags, ctx->d_t, ma, c);
     end_launch(ctx);
   }
   return 0;
@@ -3310,8 +3316,8 @@ int launch_nlcc_retry(pj_ctx* ctx, int kind) {
     m.ovf_list = out_list;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, out_count, static_cast<uint64_t>(1), 0u);
     unsigned long long* c = begin_launch(ctx, K_NLCC);
-    hipLaunchKernelGGL(k_nlcc<Word>, dim3(waves), dim3(64), 0, ctx->stream, kind, in_list, in_count,
-                       ctx->d_view_off, ctx->d_view_edges, nullptr, nullptr, static_cast<const Word*>(ctx->d_cand),
+    hipLaunchKernelGGL(k_nlcc<Word>, dim3(waves), dim3(64), 0, ctx->stream, kind, in_list, in_count, nullptr,
+                       nullptr, ctx->d_view_off, ctx->d_view_edges, nullptr, nullptr, static_cast<const Word*>(ctx->d_cand),
                        static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->d_t, m, c);
     end_launch(ctx);
   }
@@ -3346,7 +3352,8 @@ void launch_nlcc_hubs(pj_ctx* ctx, int kind, uint32_t* d_hub_n) {
     hipLaunchKernelGGL(k_memo_lds<Word>, dim3(g), dim3(64), 0, ctx->stream, ctx->d_hubs, d_hub_n, ctx->d_view_off,
                        ctx->d_view_edges, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
   else
-    hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(64), 0, ctx->stream, kind, ctx->d_hubs, d_hub_n, ctx->d_off,
+    hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(64), 0, ctx->stream, kind, ctx->d_hubs, d_hub_n, nullptr, nullptr,
+                       ctx->d_off,
                        ctx->d_edges, ctx->d_ea, reverse_positions(ctx), cand, unm, ctx->d_flags, ctx->d_t, ma, c);
   end_launch(ctx);
 }
