@@ -2508,8 +2508,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
 // (source, template vertex) pairs go to toUnmatch and mark the source
 // modified (cc_cpu.tpp:94-129, pc_cpu.tpp:95-147, tds_cpu.tpp:153-188).
 // kind 0: cycle, 1: path, 2: TDS walks, 3: enumeration (counts only).
-template <typename Word>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_nlcc(int kind, const uint32_t* __restrict__ list,
+// kOcc: registers capped for 8 waves per SIMD (launches of many waves);
+// launches with per-wave memo tables have a fixed, smaller wave count and
+// are built without the cap.
+template <typename Word, bool kOcc>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kOcc ? 8 : 1))) k_nlcc(int kind, const uint32_t* __restrict__ list,
                                                  const uint32_t* __restrict__ list_count,
                                                  const uint32_t* __restrict__ list2,
                                                  const uint32_t* __restrict__ list2_count,
@@ -3278,7 +3281,8 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
       hipLaunchKernelGGL(k_memo_lds<Word>, dim3(kMemoWaves), dim3(64), 0, ctx->stream, ctx->d_bins + b * V,
                          ctx->d_bin_count + b, off, edges, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
     else  // bin 3 (with bin 2 unless k_triangles_mid took it)
-      hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, kind, ctx->d_bins + b * V,
+      hipLaunchKernelGGL((tables ? k_nlcc<Word, false> : k_nlcc<Word, true>), dim3(g), dim3(kBlock), 0, ctx->stream,
+                         kind, ctx->d_bins + b * V,
                          ctx->d_bin_count + b, b == 3 && !triangles ? ctx->d_bins + 2 * V : nullptr,
                          b == 3 && !triangles ? ctx->d_bin_count + 2 : nullptr, off, edges, ea,
                          reverse_positions(ctx), cand, unm, ctx->d_flags, ctx->d_t, ma, c);
@@ -3316,7 +3320,7 @@ int launch_nlcc_retry(pj_ctx* ctx, int kind) {
     m.ovf_list = out_list;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, out_count, static_cast<uint64_t>(1), 0u);
     unsigned long long* c = begin_launch(ctx, K_NLCC);
-    hipLaunchKernelGGL(k_nlcc<Word>, dim3(waves), dim3(64), 0, ctx->stream, kind, in_list, in_count, nullptr,
+    hipLaunchKernelGGL((k_nlcc<Word, false>), dim3(waves), dim3(64), 0, ctx->stream, kind, in_list, in_count, nullptr,
                        nullptr, ctx->d_view_off, ctx->d_view_edges, nullptr, nullptr, static_cast<const Word*>(ctx->d_cand),
                        static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->d_t, m, c);
     end_launch(ctx);
@@ -3352,7 +3356,8 @@ void launch_nlcc_hubs(pj_ctx* ctx, int kind, uint32_t* d_hub_n) {
     hipLaunchKernelGGL(k_memo_lds<Word>, dim3(g), dim3(64), 0, ctx->stream, ctx->d_hubs, d_hub_n, ctx->d_view_off,
                        ctx->d_view_edges, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
   else
-    hipLaunchKernelGGL(k_nlcc<Word>, dim3(g), dim3(64), 0, ctx->stream, kind, ctx->d_hubs, d_hub_n, nullptr, nullptr,
+    hipLaunchKernelGGL((tables ? k_nlcc<Word, false> : k_nlcc<Word, true>), dim3(g), dim3(64), 0, ctx->stream, kind,
+                       ctx->d_hubs, d_hub_n, nullptr, nullptr,
                        ctx->d_off,
                        ctx->d_edges, ctx->d_ea, reverse_positions(ctx), cand, unm, ctx->d_flags, ctx->d_t, ma, c);
   end_launch(ctx);
