@@ -416,6 +416,58 @@ def test_reverse_positions_change_nothing(pj, orc, case, monkeypatch):
     assert [r.key() for r in r1] == [r.key() for r in orows]
 
 
+@pytest.mark.parametrize("seed", [31, 32])
+def test_triangle_steps_from_random_states_in_every_degree_bin(pj, orc, seed):
+    """3-vertex cycle steps from random states (candidates, flags and
+    edge-active bits) on a graph with sources in every degree bin, so that
+    k_triangles (at most 4 / 32 slots), k_triangles_mid (33-64 slots in one
+    chunk, 65-256 in up to four), the walk (257-2048) and the hub walk all
+    decide sources, many of them failing through cleared slots."""
+    from pattenmatching_sharedmemory_amd import STEP_CC
+
+    rng = np.random.default_rng(seed)
+    g = pj.Graph.rmat(14, 32, seed=3).labels_uniform(3, seed=4)
+    deg = np.diff(g.offsets().astype(np.int64))
+    for lo, hi in [(1, 4), (5, 32), (33, 64), (65, 256), (257, 2048), (2049, 1 << 40)]:
+        assert np.any((deg >= lo) & (deg <= hi)), (lo, hi)
+    p = pj.Graph.pattern(3, [(0, 1), (1, 2), (2, 0)], [0, 1, 2])
+    c = pj.Constraints(p)
+    assert len(c.cycles()) == 1
+    pr = pj.Pruner(g, c)
+    o = orc.Oracle(g, c, threads=8)
+    V, E = g.vertex_count, g.edge_count
+    exact = 0
+    for rep in range(3):
+        cand, vf, ea = _random_state(rng, V, E, 3)
+        av = int((vf & 1).sum())
+        pr.load_state(pj.State(cand, vf, ea, av, E))
+        o.load_state(cand, vf, ea, av, E)
+        r = pr.step(STEP_CC, 0).as_tuple()
+        label = f"seed {seed} state {rep}"
+        print(label, r, flush=True)
+        st = pr.state()
+        # bounds every schedule of the reference obeys; exact comparison when
+        # no failing root is marked by another source's walk
+        o.step(STEP_CC, 0, orc.FAITHFUL_NO_SHORTCUT)
+        f0 = np.where(vf & 1, cand, 0) & ~o.state()[0]
+        m0 = o.last_marks()
+        f = np.where(vf & 1, cand, 0) & ~st.cand
+        assert np.any(f), label
+        assert np.all((f & ~f0) == 0) and np.all((f0 & ~m0 & ~f) == 0), label
+        if np.any(f0 & m0):
+            continue
+        o.load_state(cand, vf, ea, av, E)
+        ro = o.step(STEP_CC, 0, orc.FAITHFUL)
+        oc, ovf, oea, oav, oae = o.state()
+        assert r == ro, label
+        assert np.array_equal(st.cand, oc), label
+        assert np.array_equal(st.vflags, ovf), label
+        assert np.array_equal(st.ea_words, oea), label
+        exact += 1
+    print(f"seed {seed}: {exact} of 3 states compared exactly", flush=True)
+    pr.close()
+
+
 def test_live_list_compaction_follows_the_reference(pj, orc, monkeypatch):
     """With the live-vertex list compacted at every halving of the active
     vertices (PJ_LIVE_MIN=16), the pipeline still matches the reference."""
