@@ -777,9 +777,15 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
   const int lane = lane_id();
   uint64_t elim_total = 0, slots_total = 0, items = 0;
   // The activity of up to kBlock of this block's chunks (block + k * nblocks)
-  // is tested at once, one thread each; the block then works through the
-  // active ones in order.
+  // is tested at once, one thread each; that thread also leaves the chunk's
+  // hub, slot range and U (the template neighbours of the hub's candidates)
+  // in LDS, so the block then works through the active chunks in order
+  // without the chain of loads that leads to them.  The hubs' candidates,
+  // flags and dirty marks do not change during the launch.
   __shared__ uint64_t s_act[kBlock / 64];
+  __shared__ uint32_t s_h[kBlock];
+  __shared__ uint64_t s_beg[kBlock], s_end[kBlock];
+  __shared__ Word s_U[kBlock];
   const uint32_t per = block < nchunks ? (nchunks - 1 - block) / nblocks + 1 : 0;
   for (uint32_t k0 = 0; k0 < per; k0 += kBlock) {
     {
@@ -787,8 +793,19 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
       bool act = false;
       if (kk < per) {
         const uint32_t c = block + kk * nblocks;
-        const uint32_t v = hubs[chunk_hub[c]];
-        act = cand[v] && (flags[v] & kScheduled) && (!dirty || dirty[v]);
+        const uint32_t h = chunk_hub[c];
+        const uint32_t v = hubs[h];
+        const Word cv = cand[v];
+        act = cv && (flags[v] & kScheduled) && (!dirty || dirty[v]);
+        if (act) {
+          Word U = 0;
+          for (Word m = cv; m; m &= m - 1) U |= static_cast<Word>(t->nbr[ctz<Word>(m)]);
+          const uint64_t beg = chunk_start[c];
+          s_h[threadIdx.x] = h;
+          s_beg[threadIdx.x] = beg;
+          s_end[threadIdx.x] = min(off[v + 1], beg + kHubChunk);
+          s_U[threadIdx.x] = U;
+        }
       }
       const uint64_t am = __ballot(act);
       if (lane_id() == 0) s_act[threadIdx.x >> 6] = am;
@@ -796,15 +813,11 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
     }
     for (int aw = 0; aw < kBlock / 64; ++aw) {
       for (uint64_t mm = s_act[aw]; mm; mm &= mm - 1) {
-        const uint32_t c = block + (k0 + aw * 64 + static_cast<uint32_t>(__builtin_ctzll(mm))) * nblocks;
-        const uint32_t h = chunk_hub[c];
-        const uint32_t v = hubs[h];
-        const Word cv = cand[v];
-        if (!cv || !(flags[v] & kScheduled) || (dirty && !dirty[v])) continue;  // block-uniform
-        Word U = 0;
-        for (Word m = cv; m; m &= m - 1) U |= static_cast<Word>(t->nbr[ctz<Word>(m)]);
-        const uint64_t beg = chunk_start[c];
-        const uint64_t end = min(off[v + 1], beg + kHubChunk);
+        const int slot = aw * 64 + __builtin_ctzll(mm);
+        const uint32_t h = s_h[slot];
+        const Word U = s_U[slot];
+        const uint64_t beg = s_beg[slot];
+        const uint64_t end = s_end[slot];
         uint32_t word[kPer], u[kPer];
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
