@@ -1152,9 +1152,15 @@ __device__ void phase2_hub_chunks(uint32_t block, uint32_t nblocks, const uint32
                                   unsigned long long* counters) {
   uint64_t examined = 0;  // slots read here; the reference-order count is taken in k_phase2_hub_finish
   // The activity of up to kBlock of this block's chunks (block + k * nblocks)
-  // is tested at once, one thread each; the block then works through the
-  // active ones in order.
+  // is tested at once, one thread each; that thread also leaves the chunk's
+  // hub, row start, slot range and kind (a modified hub dirtying its
+  // neighbours, or a scan) in LDS, and the block then works through the
+  // active chunks in order.  Hub state does not change during the launch
+  // (phase 2 of the hubs is applied by k_phase2_hub_finish).
   __shared__ uint64_t s_act[kBlock / 64];
+  __shared__ uint32_t s_h[kBlock];
+  __shared__ uint64_t s_row[kBlock], s_beg[kBlock], s_end[kBlock];
+  __shared__ uint8_t s_mod[kBlock];
   const uint32_t per = block < nchunks ? (nchunks - 1 - block) / nblocks + 1 : 0;
   for (uint32_t k0 = 0; k0 < per; k0 += kBlock) {
     {
@@ -1162,8 +1168,18 @@ __device__ void phase2_hub_chunks(uint32_t block, uint32_t nblocks, const uint32
       bool act = false;
       if (kk < per) {
         const uint32_t c = block + kk * nblocks;
-        const uint32_t v = hubs[chunk_hub[c]];
-        if (cand[v]) act = (flags[v] & kModified) ? (dirty && unmatch[v]) : (!rescan || rescan[v]);
+        const uint32_t h = chunk_hub[c];
+        const uint32_t v = hubs[h];
+        const bool mod = (flags[v] & kModified) != 0;
+        if (cand[v]) act = mod ? (dirty && unmatch[v]) : (!rescan || rescan[v]);
+        if (act) {
+          const uint64_t row = off[v], beg = chunk_start[c];
+          s_h[threadIdx.x] = h;
+          s_row[threadIdx.x] = row;
+          s_beg[threadIdx.x] = beg;
+          s_end[threadIdx.x] = min(off[v + 1], beg + kHubChunk);
+          s_mod[threadIdx.x] = mod;
+        }
       }
       const uint64_t am = __ballot(act);
       if (lane_id() == 0) s_act[threadIdx.x >> 6] = am;
@@ -1171,23 +1187,16 @@ __device__ void phase2_hub_chunks(uint32_t block, uint32_t nblocks, const uint32
     }
     for (int aw = 0; aw < kBlock / 64; ++aw) {
       for (uint64_t mm = s_act[aw]; mm; mm &= mm - 1) {
-        const uint32_t c = block + (k0 + aw * 64 + static_cast<uint32_t>(__builtin_ctzll(mm))) * nblocks;
-        const uint32_t h = chunk_hub[c];
-        const uint32_t v = hubs[h];
-        if (!cand[v]) continue;
-        if (flags[v] & kModified) {
-          if (dirty && unmatch[v]) {  // the hub's candidates change: dirty its neighbours
-            const uint64_t beg = chunk_start[c];
-            const uint64_t end = min(off[v + 1], beg + kHubChunk);
-            for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) dirty[edges[e]] = 1;
-            if (threadIdx.x == 0) examined += end - beg;
-          }
+        const int slot = aw * 64 + __builtin_ctzll(mm);
+        const uint32_t h = s_h[slot];
+        const uint64_t beg = s_beg[slot];
+        const uint64_t end = s_end[slot];
+        if (s_mod[slot]) {  // the hub's candidates change: dirty its neighbours
+          for (uint64_t e = beg + threadIdx.x; e < end; e += blockDim.x) dirty[edges[e]] = 1;
+          if (threadIdx.x == 0) examined += end - beg;
           continue;
         }
-        if (rescan && !rescan[v]) continue;  // scan result stands (counted in k_build_bins)
-        const uint64_t row = off[v];
-        const uint64_t beg = chunk_start[c];
-        const uint64_t end = min(off[v + 1], beg + kHubChunk);
+        const uint64_t row = s_row[slot];
         // A hit already found in an earlier chunk of the row decides the scan:
         // this chunk cannot lower the minimum (one load per wave).
         if (*reinterpret_cast<const volatile uint32_t*>(&hub_hit[h]) < beg - row) continue;
