@@ -65,7 +65,7 @@ enum KernelId {
 };
 // nlcc_walk: k_nlcc (cycle / path / TDS / enumeration walks over the bins,
 // and the memo retry passes); triangles: k_triangles (3-vertex cycles from
-// sources of at most 4 / 32 slots); triangles_memo: k_triangles_memo;
+// sources of at most 8 / 32 slots); triangles_memo: k_triangles_memo;
 // memo_walk: the first pass of the memoised walks (k_memo_lds).
 const char* const kKernelNames[K_COUNT] = {
     "lcc0",          "build_bins",        "lcc_phase1", "lcc_phase1_hub_chunks", "lcc_phase1_hub_finish",
@@ -399,6 +399,9 @@ __global__ void k_lcc0(Word* __restrict__ cand, const uint8_t* __restrict__ code
 
 constexpr int kBins = 4;
 __constant__ uint32_t c_bin_limit[kBins] = {4, 32, 256, kHubDegree};
+// Bin 0 of the walk task lists holds rows of up to 8 slots (k_triangles with
+// 8 lanes per source).
+constexpr uint32_t kWalkBin0 = 8;
 
 constexpr int kBinIters = 4;                  // vertices per thread
 constexpr int kBinVerts = kBlock * kBinIters;  // vertices per block
@@ -464,7 +467,7 @@ __device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __rest
   } else if (mode == 2 && cd != path_code) {
     return -1;
   }
-  if (deg <= c_bin_limit[0]) return 0;
+  if (deg <= (mode == 1 || mode == 2 ? kWalkBin0 : c_bin_limit[0])) return 0;
   if (deg <= c_bin_limit[1]) return 1;
   if (deg <= c_bin_limit[2]) return 2;
   if (deg <= c_bin_limit[3]) return 3;
@@ -3291,7 +3294,7 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
     unsigned long long* c =
         begin_launch(ctx, triangles && b <= 2 ? K_TRIANGLES : memo ? K_MEMO_WALK : K_NLCC);
     if (triangles && b == 0)
-      hipLaunchKernelGGL((k_triangles<Word, 4>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
+      hipLaunchKernelGGL((k_triangles<Word, kWalkBin0>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
                          ctx->d_bin_count + b, off, edges, ea, rev, cand, unm, ctx->d_flags, ctx->d_t, c);
     else if (triangles && b == 1)
       hipLaunchKernelGGL((k_triangles<Word, 32>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,

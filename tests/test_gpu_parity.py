@@ -420,7 +420,7 @@ def test_reverse_positions_change_nothing(pj, orc, case, monkeypatch):
 def test_triangle_steps_from_random_states_in_every_degree_bin(pj, orc, seed):
     """3-vertex cycle steps from random states (candidates, flags and
     edge-active bits) on a graph with sources in every degree bin, so that
-    k_triangles (at most 4 / 32 slots), k_triangles_mid (33-64 slots in one
+    k_triangles (at most 8 / 32 slots), k_triangles_mid (33-64 slots in one
     chunk, 65-256 in up to four), the walk (257-2048) and the hub walk all
     decide sources, many of them failing through cleared slots."""
     from pattenmatching_sharedmemory_amd import STEP_CC
