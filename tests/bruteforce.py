@@ -173,3 +173,76 @@ def label_paths(p_rows, p_lab):
     for s in range(P):
         dfs([s])
     return out
+
+
+def walk_template_edges(root, moves):
+    """Undirected template edges a TDS walk covers: each VERTEX move is an
+    edge from the walk's current template vertex to the new one, each CHECK
+    an edge from the current template vertex to the checked one."""
+    hist, cur, edges = [root], root, set()
+    for mv, x, mb in moves[1:]:
+        if mv in (2, 3):  # VERTEX_STORE, VERTEX_NO_STORE
+            edges.add(frozenset((cur, x)))
+            hist.append(x)
+            cur = x
+        elif mv == 1:  # CHECK
+            edges.add(frozenset((cur, x)))
+        else:  # MOVE_BACK
+            cur = hist[mb]
+    return edges
+
+
+def homomorphism_exists(g_adj, cand, t_edges, root, v):
+    """Is there a map f from the template vertices of t_edges to graph
+    vertices with f(root) = v, every template vertex x among the candidates
+    of f(x), and every template edge {x, y} on an active undirected graph
+    edge {f(x), f(y)}?  g_adj[w] is the set of active neighbours of w.
+    Not necessarily injective.  Plain backtracking in BFS order from root."""
+    nbrs = {}
+    for e in t_edges:
+        a, b = tuple(e)
+        nbrs.setdefault(a, set()).add(b)
+        nbrs.setdefault(b, set()).add(a)
+    order, seen = [root], {root}
+    for x in order:
+        for y in sorted(nbrs[x]):
+            if y not in seen:
+                seen.add(y)
+                order.append(y)
+    if not (cand[v] >> root) & 1:
+        return False
+    f = {root: v}
+
+    def extend(i):
+        if i == len(order):
+            return True
+        x = order[i]
+        placed = [y for y in nbrs[x] if y in f]
+        pool = set(g_adj[f[placed[0]]])
+        for y in placed[1:]:
+            pool &= g_adj[f[y]]
+        for w in sorted(pool):
+            if (cand[w] >> x) & 1:
+                f[x] = w
+                if extend(i + 1):
+                    return True
+                del f[x]
+        return False
+
+    return extend(1)
+
+
+def random_symmetric_graph(n, p, rng):
+    """Undirected simple graph on n vertices (each pair with probability p),
+    as ascending CSR rows (offsets, edges)."""
+    adj = [[] for _ in range(n)]
+    for a in range(n):
+        for b in range(a + 1, n):
+            if rng.random() < p:
+                adj[a].append(b)
+                adj[b].append(a)
+    off, ed = [0], []
+    for a in range(n):
+        ed += sorted(adj[a])
+        off.append(len(ed))
+    return off, ed
