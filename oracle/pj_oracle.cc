@@ -20,7 +20,10 @@
 //          thread the shortcut can make results depend on scheduling,
 //          exactly as in the reference.
 //   mode 1 "complete": the same walks without memo and without the
-//          shortcut, i.e. "does a walk satisfying the constraint exist".
+//          shortcut, i.e. "does a walk satisfying the constraint exist";
+//          on graphs whose rows ascend, a cycle's closing hop is a binary
+//          search instead of the reference's scan (same decision, fewer
+//          slots read: the slot counts of this mode are not the reference's).
 //          The two modes agree whenever the memo never fires on a needed
 //          branch (memo-inert constraints: pjh_constraints_memo_inert of the
 //          host library decides it for cycle and path constraints) — tests
@@ -89,6 +92,7 @@ struct or_state {
   uint64_t active_v = 0, active_e = 0;
   uint64_t slots = 0;  // slots examined by the last step
   int threads = 1;
+  bool sorted_rows = false;  // every row ascending (duplicates allowed)
 };
 
 namespace {
@@ -221,11 +225,26 @@ inline std::pair<uint32_t, uint32_t> ukey(uint32_t a, uint32_t b) {
   return a > b ? std::make_pair(a, b) : std::make_pair(b, a);
 }
 
+// Closing hop of the complete search on ascending rows: an active slot
+// cur -> src exists (binary search, then every duplicate of src).  Decides
+// the same as the reference's linear scan (cc_cpu.tpp:24-34); the faithful
+// mode keeps the scan so that its slot counts follow the reference.
+bool closes_sorted(const or_state* s, uint32_t cur, uint32_t src) {
+  const uint32_t* b = s->edges + s->off[cur];
+  const uint32_t* e = s->edges + s->off[cur + 1];
+  tl_slots += 1;
+  if (!s->active[src]) return false;
+  for (const uint32_t* p = std::lower_bound(b, e, src); p != e && *p == src; ++p)
+    if (s->edge_active[p - s->edges]) return true;
+  return false;
+}
+
 // ---- CC walk: multiple_label_cc_cpu.tpp:12-70 -------------------------------
 bool cc_walk(or_state* s, const std::vector<uint32_t>& idx, EdgeMemo* memo, bool faithful, uint32_t src, uint32_t cur,
              size_t start, size_t remaining) {
   const size_t L = idx.size();
   if (remaining == 1) {
+    if (!faithful && s->sorted_rows) return closes_sorted(s, cur, src);
     for (uint64_t e = s->off[cur]; e < s->off[cur + 1]; ++e) {
       ++tl_slots;
       if (!s->edge_active[e]) continue;
@@ -498,6 +517,13 @@ int or_set_graph(or_state* s, uint32_t V, uint64_t E, const uint64_t* off, const
   s->off = off;
   s->edges = edges;
   s->labels = labels;
+  s->sorted_rows = true;
+  for (uint32_t v = 0; v < V && s->sorted_rows; ++v)
+    for (uint64_t e = off[v] + 1; e < off[v + 1]; ++e)
+      if (edges[e - 1] > edges[e]) {
+        s->sorted_rows = false;
+        break;
+      }
   return 0;
 }
 

@@ -202,7 +202,7 @@ __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
 // Row-order check: k_row_starts marks the first slot of every row in a
 // zeroed bitset, k_rows_sorted sets bit 0 of *unsorted when a slot that starts
 // no row holds a smaller id than the slot before it, bit 1 when it holds the
-// same id.
+// same id, bit 2 when a slot holds an id that is not a vertex.
 __global__ void k_row_starts(const uint64_t* __restrict__ off, uint32_t V, uint64_t E, uint32_t* __restrict__ bits) {
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t e = off[v];
@@ -210,13 +210,16 @@ __global__ void k_row_starts(const uint64_t* __restrict__ off, uint32_t V, uint6
   }
 }
 
-__global__ void k_rows_sorted(const uint32_t* __restrict__ edges, uint64_t E, const uint32_t* __restrict__ bits,
-                              uint32_t* unsorted) {
-  for (uint64_t e = 1 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x)
-    if (!((bits[e >> 5] >> (e & 31)) & 1u)) {
-      if (edges[e] < edges[e - 1]) atomicOr(unsorted, 1u);
-      else if (edges[e] == edges[e - 1]) atomicOr(unsorted, 2u);  // duplicate slot
+__global__ void k_rows_sorted(const uint32_t* __restrict__ edges, uint64_t E, uint32_t V,
+                              const uint32_t* __restrict__ bits, uint32_t* unsorted) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = edges[e];
+    if (u >= V) atomicOr(unsorted, 4u);  // not a vertex: the upload is refused
+    if (e && !((bits[e >> 5] >> (e & 31)) & 1u)) {
+      if (u < edges[e - 1]) atomicOr(unsorted, 1u);
+      else if (u == edges[e - 1]) atomicOr(unsorted, 2u);  // duplicate slot
     }
+  }
 }
 
 // Symmetry check on rows sorted strictly ascending (k_rows_symmetric and,
@@ -3774,10 +3777,21 @@ static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* of
     PJ_HIP(hipStreamSynchronize(ctx->stream));
     host_off = h_off.data();
   }
-  if (host_off[V] != E) {
-    ctx->err = "offsets[V] != edge_count";
+  // Every kernel indexes off[v + 1] and cand[edges[e]] unchecked: refuse a
+  // CSR whose offsets decrease or leave [0, E] (here) or whose ids are not
+  // vertices (bit 2 of the row-order check below).
+  auto refuse = [ctx](const std::string& why) {
+    hipStreamSynchronize(ctx->stream);  // copies into the buffers may still run
+    free_graph(ctx);
+    ctx->V = 0;
+    ctx->E = 0;
+    ctx->ea_words = 0;
+    ctx->err = "malformed CSR: " + why;
     return -1;
-  }
+  };
+  if (host_off[0] != 0 || host_off[V] != E) return refuse("offsets must run from 0 to edge_count");
+  for (uint64_t v = 0; v < V; ++v)
+    if (host_off[v + 1] < host_off[v]) return refuse("offsets decrease at vertex " + std::to_string(v));
   if (setup_hubs(ctx, host_off)) return -1;
   {
     // The edge-active bitset is free until a template is installed.
@@ -3786,11 +3800,12 @@ static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* of
                        ctx->ea_words, 0u);
     hipLaunchKernelGGL(k_row_starts, dim3(grid_for(ctx, V)), dim3(kBlock), 0, ctx->stream, ctx->d_off, V, E,
                        ctx->d_ea);
-    hipLaunchKernelGGL(k_rows_sorted, dim3(grid_for(ctx, E)), dim3(kBlock), 0, ctx->stream, ctx->d_edges, E,
+    hipLaunchKernelGGL(k_rows_sorted, dim3(grid_for(ctx, E)), dim3(kBlock), 0, ctx->stream, ctx->d_edges, E, V,
                        ctx->d_ea, ctx->d_bin_count);
     uint32_t unsorted = 0;
     PJ_HIP(hipMemcpyAsync(&unsorted, ctx->d_bin_count, 4, hipMemcpyDeviceToHost, ctx->stream));
     PJ_HIP(hipStreamSynchronize(ctx->stream));
+    if (unsorted & 4u) return refuse("a neighbour id is not a vertex");
     ctx->rows_sorted = (unsorted & 1u) == 0;
     ctx->symmetric = false;
     if (unsorted == 0) {

@@ -228,6 +228,32 @@ bad:
   return -1;
 }
 
+int validate_csr(const Graph& g, std::string* err) {
+  const uint64_t V = g.vertex_count, E = g.edge_count;
+  if (g.vertices.size() != V + 1 || g.edges.size() != E || g.vertices[0] != 0 || g.vertices[V] != E) {
+    *err = "malformed CSR: offsets must run from 0 to the edge count";
+    return -1;
+  }
+  int64_t bad_row = -1;
+#pragma omp parallel for reduction(max : bad_row)
+  for (int64_t v = 0; v < static_cast<int64_t>(V); ++v)
+    if (g.vertices[v + 1] < g.vertices[v]) bad_row = std::max(bad_row, v);
+  if (bad_row >= 0) {
+    *err = "malformed CSR: offsets decrease at vertex " + std::to_string(bad_row);
+    return -1;
+  }
+  int64_t bad_slot = -1;
+#pragma omp parallel for reduction(max : bad_slot)
+  for (int64_t e = 0; e < static_cast<int64_t>(E); ++e)
+    if (g.edges[e] >= V) bad_slot = std::max(bad_slot, e);
+  if (bad_slot >= 0) {
+    *err = "malformed CSR: neighbour id " + std::to_string(g.edges[bad_slot]) + " at slot " +
+           std::to_string(bad_slot) + " is not a vertex";
+    return -1;
+  }
+  return 0;
+}
+
 int load_totem(const std::string& path, Graph* g, std::string* err) {
   FilePtr f(fopen(path.c_str(), "rb"));
   if (!f) {
@@ -239,9 +265,18 @@ int load_totem(const std::string& path, Graph* g, std::string* err) {
     *err = "empty file " + path;
     return -1;
   }
-  if (magic == kBinaryMagic) return load_totem_binary(f.get(), path, g, err);
-  f.reset();
-  return load_totem_text(path, g, err);
+  int rc;
+  if (magic == kBinaryMagic) {
+    rc = load_totem_binary(f.get(), path, g, err);
+  } else {
+    f.reset();
+    rc = load_totem_text(path, g, err);
+  }
+  if (rc == 0 && validate_csr(*g, err)) {
+    *err += " (" + path + ")";
+    return -1;
+  }
+  return rc;
 }
 
 int store_totem_binary(const Graph& g, const std::string& path, int eid_bytes, std::string* err) {

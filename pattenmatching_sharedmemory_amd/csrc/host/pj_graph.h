@@ -30,6 +30,12 @@ struct Graph {
 // Edge weights are skipped.  Returns 0 / -1 (message in *err).
 int load_totem(const std::string& path, Graph* g, std::string* err);
 
+// CSR well-formedness: offsets start at 0, never decrease, end at the edge
+// count, and every neighbour id is a vertex.  Every graph the loaders and
+// pjh_graph_from_csr return has passed it (kernels index cand[edges[e]] and
+// off[v + 1] without bounds checks).  Returns 0 / -1 (message in *err).
+int validate_csr(const Graph& g, std::string* err);
+
 // Text parser only (# Nodes / # Edges / # DIRECTED|UNDIRECTED header,
 // optional vertex list, edge list sorted by source), totem_graph.cu:30-350.
 int load_totem_text(const std::string& path, Graph* g, std::string* err);

@@ -100,6 +100,12 @@ int pjh_graph_from_csr(uint32_t V, uint64_t E, const uint64_t* off, const uint32
     h->g.values.assign(labels, labels + V);
   }
   h->g.directed = false;
+  std::string err;
+  if (pj::validate_csr(h->g, &err)) {
+    fprintf(stderr, "pjh_graph_from_csr: %s\n", err.c_str());
+    delete h;
+    return -1;
+  }
   *out = h;
   return 0;
 }

@@ -103,6 +103,30 @@ def test_truncated_binary_is_an_error(pj, tmp_path):
         pj.Graph.load(str(p))
 
 
+
+@pytest.mark.parametrize("off,ed,why", [
+    ([0, 2, 1, 3], [1, 2, 0], "offsets"),      # offsets decrease
+    ([0, 1, 2, 2], [1, 3], "not a vertex"),     # neighbour id 3 with V = 3
+    ([1, 2, 3, 3], [1, 2, 0], "offsets"),      # offsets do not start at 0
+])
+def test_malformed_csr_is_rejected(pj, off, ed, why):
+    with pytest.raises(ValueError):
+        pj.Graph.from_csr(off, ed, [0, 0, 0])
+
+
+def test_malformed_binary_file_is_rejected(pj, tmp_path):
+    """A binary TOTEM file whose edge list names a vertex beyond the count
+    fails to load instead of reaching the device."""
+    g = pj.Graph.from_csr([0, 1, 2, 2], [1, 0], [0, 0, 0])
+    p = tmp_path / "g.bin"
+    g.store_binary(str(p), 4)
+    data = bytearray(open(p, "rb").read())
+    # last 4 bytes hold the second neighbour id (no labels follow the edges)
+    data[-4:] = (7).to_bytes(4, "little")
+    open(p, "wb").write(bytes(data))
+    with pytest.raises(ValueError, match="not a vertex"):
+        pj.Graph.load(str(p))
+
 # --- R-MAT -----------------------------------------------------------------
 
 MASK = (1 << 64) - 1

@@ -420,7 +420,7 @@ def test_reverse_positions_change_nothing(pj, orc, case, monkeypatch):
 def test_triangle_steps_from_random_states_in_every_degree_bin(pj, orc, seed):
     """3-vertex cycle steps from random states (candidates, flags and
     edge-active bits) on a graph with sources in every degree bin, so that
-    k_triangles (at most 8 / 32 slots), k_triangles_mid (33-64 slots in one
+    k_triangles (1-8 / 9-32 slots), k_triangles_mid (33-64 slots in one
     chunk, 65-256 in up to four), the walk (257-2048) and the hub walk all
     decide sources, many of them failing through cleared slots."""
     from pattenmatching_sharedmemory_amd import STEP_CC
@@ -428,7 +428,10 @@ def test_triangle_steps_from_random_states_in_every_degree_bin(pj, orc, seed):
     rng = np.random.default_rng(seed)
     g = pj.Graph.rmat(14, 32, seed=3).labels_uniform(3, seed=4)
     deg = np.diff(g.offsets().astype(np.int64))
-    for lo, hi in [(1, 4), (5, 32), (33, 64), (65, 256), (257, 2048), (2049, 1 << 40)]:
+    # walk task list bin 0 holds rows of up to 8 slots (k_triangles<Word, 8>:
+    # groups of 8 lanes, so rows of 1-4 and of 5-8 slots both occur), bin 1
+    # rows of 9-32 (k_triangles<Word, 32>)
+    for lo, hi in [(1, 4), (5, 8), (9, 32), (33, 64), (65, 256), (257, 2048), (2049, 1 << 40)]:
         assert np.any((deg >= lo) & (deg <= hi)), (lo, hi)
     p = pj.Graph.pattern(3, [(0, 1), (1, 2), (2, 0)], [0, 1, 2])
     c = pj.Constraints(p)
@@ -550,3 +553,31 @@ def test_full_scale_plain_workload_invariants(pj):
     print(f"scale 22: {len(rows)} steps, {int(np.count_nonzero(active))} active vertices, "
           f"{E - eliminated} active slots", flush=True)
     pr.close()
+
+
+@pytest.mark.parametrize("off,ed,msg", [
+    ([0, 2, 1, 3], [1, 2, 0], "decrease"),
+    ([0, 1, 2, 2], [1, 3], "not a vertex"),
+    ([0, 1, 2, 5], [1, 0], "edge_count"),
+])
+def test_upload_refuses_malformed_csr(pj, off, ed, msg):
+    """pj_upload_graph checks offsets on the host and neighbour ids in its
+    row-order pass (which reads each slot once, in bounds) before any kernel
+    indexes with them; the context stays usable for a good graph."""
+    import ctypes as C
+
+    from pattenmatching_sharedmemory_amd import native
+
+    lib = native.hip()
+    ctx = C.c_void_p()
+    assert lib.pj_create(C.byref(ctx), 0) == 0
+    o = np.array(off, dtype=np.uint64)
+    e = np.array(ed, dtype=np.uint32)
+    lab = np.zeros(3, dtype=np.uint64)
+    u64, u32 = C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)
+    rc = lib.pj_upload_graph(ctx, 3, len(ed), o.ctypes.data_as(u64), e.ctypes.data_as(u32), lab.ctypes.data_as(u64))
+    assert rc == -1 and msg in lib.pj_last_error(ctx).decode()
+    good_o, good_e = np.array([0, 1, 2, 2], dtype=np.uint64), np.array([1, 0], dtype=np.uint32)
+    assert lib.pj_upload_graph(ctx, 3, 2, good_o.ctypes.data_as(u64), good_e.ctypes.data_as(u32),
+                               lab.ctypes.data_as(u64)) == 0
+    lib.pj_destroy(ctx)
