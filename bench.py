@@ -2,7 +2,7 @@
 
 Workload (the single-GPU plain configuration): R-MAT scale 22, edge factor 16,
 (a, b, c, d) = (0.57, 0.19, 0.19, 0.05), symmetrised / deduplicated / no
-self-loops (4,194,304 vertices, 128,312,408 directed CSR slots), 32 labels
+self-loops (4,194,304 vertices, 128,305,988 directed CSR slots), 32 labels
 drawn from Zipf(s=1.2); template = labelled triangle t0-t1-t2 plus the tail
 t2-t3-t4-t5 with labels [0, 1, 2, 3, 4, 3] (one cycle and one path
 constraint).  One bench step = one whole pipeline solve on the HBM-resident
@@ -14,13 +14,16 @@ on the active graph, so a solve sweeps the sum over its steps of the active
 CSR slots at the start of the step; value = that sum / solve time.  The sum
 comes from the step trace alone (it is the same for every implementation
 that produces the same trace), so the CPU baseline uses the same definition.
-The slots the kernels actually examined are reported beside it.  roofline =
+teps_edge_sweeps repeats value under its own name; teps_slots_examined =
+the CSR slots the kernels examined (a serial scan in the reference's loop
+order, binary searches counting their probes) per second.  roofline =
 the dominant kernel's algorithmic bytes per launch / its average launch time
 (HIP events on the context stream) against 8 TB/s; traffic = HBM bytes per
 launch of that kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
 summary of the same workload (profiles/), when there is one.  cpu_baseline =
-the oracle (CPU restatement of the reference, faithful mode, all host
-threads) on a bounded sample of the same workload family.
+the oracle (CPU restatement of the reference) solving the same scale-22
+graph and template once, on every CPU the process may use (OMP_NUM_THREADS
+when set), with the host's model and core counts.
 
 --copy-calibration MIB runs only a device-to-device copy of MIB MiB three
 times (the plain copy kernel the PMC byte counters are scaled by).
@@ -31,6 +34,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -170,40 +174,69 @@ def phases(rows, edge_count: int) -> dict:
     return out
 
 
-def cpu_baseline(scale: int, threads: int, name: str = "plain", run_tds: bool = True) -> dict:
-    import oracle
-    from pattenmatching_sharedmemory_amd import Pruner
+def host_cpu() -> dict:
+    """CPU model, sockets and logical CPUs of this host, and the CPUs this
+    process may run on."""
+    model, sockets = None, set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model is None:
+                model = line.split(":", 1)[1].strip()
+            if line.startswith("physical id"):
+                sockets.add(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "sockets": len(sockets) or None, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
-    g, _, c = workload(scale, name)
+
+def baseline_threads() -> int:
+    """Every CPU the process may use: OMP_NUM_THREADS when the environment
+    sets it (the host's share for this job), else the affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else host_cpu()["usable_cpus"]
+
+
+def cpu_baseline(g, c, rows_gpu, gpu_s: float, threads: int, run_tds: bool = True) -> dict:
+    """The oracle (CPU restatement of the reference) solving the same graph
+    and template as the timed device solves.  LCC0, LCC and every phase 2 run
+    the reference's loops; the cycle and path steps run the oracle's complete
+    search, which decides exactly as the reference's memoised search on
+    memo-inert constraints (checked here) and closes cycles by binary search
+    instead of the reference's scan of the last row, so it is faster than
+    the reference's own algorithm: a conservative baseline."""
+    import oracle
+    from pattenmatching_sharedmemory_amd import STEP_CC, STEP_PC
+
+    inert = all(c.memo_inert(STEP_CC, i) == 1 for i in range(len(c.cycles()))) and \
+        all(c.memo_inert(STEP_PC, i) == 1 for i in range(len(c.paths()))) and not (run_tds and c.tds())
+    mode = oracle.COMPLETE if inert else oracle.FAITHFUL
     o = oracle.Oracle(g, c, threads=threads)
     t0 = time.time()
-    rows, total_ms = o.run(oracle.FAITHFUL, enumerate_matches=False, run_tds=run_tds)
+    rows, total_ms = o.run(mode, enumerate_matches=False, run_tds=run_tds)
     wall = time.time() - t0
     sweeps = edge_sweeps(rows, g.edge_count)
-    log(f"[bench] cpu baseline scale {scale}: {len(rows)} steps, {total_ms:.0f} ms of step time "
-        f"({wall:.1f}s wall), {sweeps} edge sweeps, {sum(r.slots for r in rows)} slots read")
-    # the GPU on the same sample, for a ratio at equal work
-    pr = Pruner(g, c, device=0)
-    t0 = time.perf_counter()
-    grows, _ = pr.run(enumerate_matches=False, run_tds=run_tds)
-    # equal unless a memoised step's result depends on thread timing in the
-    # reference (alreadyMatched), where each run is one of its outcomes
-    trace_equal = [r.key() for r in grows] == [r.key() for r in rows]
-    if not trace_equal:
-        log("[bench] cpu baseline: GPU and CPU traces differ on the sample")
-    reps = max(1, min(20, int(2.0 / max(time.perf_counter() - t0, 1e-3))))
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        pr.run(enumerate_matches=False, run_tds=run_tds)
-    gpu_s = (time.perf_counter() - t0) / reps
-    pr.close()
+    trace_equal = [r.key() for r in rows] == [r.key() for r in rows_gpu]
+    log(f"[bench] cpu baseline: {len(rows)} steps, {total_ms:.0f} ms of step time ({wall:.1f}s wall) on {threads} "
+        f"threads, {sweeps} edge sweeps; trace equal to the device's: {trace_equal}")
+    per_phase = {}
+    active = g.edge_count
+    for r in rows:
+        ph = per_phase.setdefault(STEP_NAMES.get(r.step, str(r.step)), {"ms": 0.0, "edge_sweeps": 0})
+        ph["ms"] = round(ph["ms"] + r.ms, 3)
+        ph["edge_sweeps"] += active
+        active = r.active_edges
     return {"value": sweeps / (total_ms / 1e3), "unit": "TEPS", "cores": threads, "kind": "port",
-            "sample": f"oracle faithful mode (the reference's loops and data layout, OpenMP threads), whole pruning "
-                      f"solve on R-MAT scale {scale} of the same generator, labels and template; time = sum of step "
-                      f"times",
-            "steps": len(rows), "seconds": total_ms / 1e3, "slots_read": sum(r.slots for r in rows),
-            "gpu_same_sample": {"seconds": gpu_s, "value": sweeps / gpu_s, "speedup": (total_ms / 1e3) / gpu_s,
-                                "trace_equal": trace_equal}}
+            "sample": f"the whole measured workload: one pruning solve of the same R-MAT scale-"
+                      f"{int(round(math.log2(g.vertex_count)))} graph and template by the oracle (CPU restatement "
+                      f"of the reference: its LCC loops and layout; "
+                      f"{'complete-search cycle and path steps on memo-inert constraints' if inert else 'its memoised walks'}"
+                      f"), OpenMP over {threads} threads; time = sum of step times as the reference measures",
+            "host": host_cpu(), "seconds": total_ms / 1e3, "steps": len(rows), "phases": per_phase,
+            "trace_equal": trace_equal, "speedup_gpu_vs_cpu": (total_ms / 1e3) / gpu_s}
 
 
 QUERY_FORMS = {
@@ -327,7 +360,6 @@ def main() -> int:
     ap.add_argument("--template-labels", default=None, help="comma-separated label ranks of the template vertices")
     ap.add_argument("--stop-before-tds", action="store_true",
                     help="end each solve when the first TDS step is due (phases up to TDS only)")
-    ap.add_argument("--cpu-scale", type=int, default=19)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
@@ -406,6 +438,8 @@ def main() -> int:
         "metric": METRIC,
         "value": sweeps / sec,
         "unit": "TEPS",
+        "teps_edge_sweeps": sweeps / sec,
+        "teps_slots_examined": slots / sec,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -431,8 +465,7 @@ def main() -> int:
     }
     pr.close()
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        result["cpu_baseline"] = cpu_baseline(args.cpu_scale, threads, args.workload, not args.stop_before_tds)
+        result["cpu_baseline"] = cpu_baseline(g, c, rows, sec, baseline_threads(), not args.stop_before_tds)
     print(json.dumps(result), flush=True)
     return 0
 

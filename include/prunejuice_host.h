@@ -112,10 +112,20 @@ typedef struct pjh_query_result {
 int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int n, int workers, int flags,
                     const double* arrival_ms, uint64_t budget_bytes, pjh_query_result* out, double* total_ms,
                     char* err, size_t errlen);
-/* Device bytes a query is projected to need beyond the shared graph:
- * candidate, toUnmatch and alreadyMatched words (V each), the frontier bins,
- * flags and marks, plus the memo tables when a constraint needs them. */
+/* Device bytes a query's context is projected to need beyond the shared
+ * graph, mirroring the device library's allocations: the pruning state
+ * (candidate, toUnmatch and staging words, flags, marks, scan lengths,
+ * frontier bins, live lists, edge-active bits); with memoised walks (TDS,
+ * cycle / path constraints whose memo may fire) alreadyMatched, overflow
+ * lists, the first-pass memo tables and the active view; with complete walks
+ * of 4+ vertices the memo tables of their failed-state memo.  The retry
+ * tables are not per query: one set per device, shared by every context and
+ * used under a lock, of PJ_RETRY_POOL_BYTES; pjh_run_queries holds them back
+ * from the budget once when any query uses memo tables. */
+#define PJ_RETRY_POOL_BYTES ((32ull << 22) * 20 + (4ull << 26) * 20 + (1ull << 29) * 20)
 uint64_t pjh_projected_bytes(const pjh_constraints* c, uint32_t vertex_count, uint64_t edge_count);
+/* 1 when a query of these constraints allocates memo tables (see above). */
+int pjh_query_uses_memo(const pjh_constraints* c);
 
 /* patternmatching_binary: -i graph -b binary-cache -v vertex-data -p pattern-dir
  * -o output-dir [-d device]; writes results.totem, graph_%04d.totem,

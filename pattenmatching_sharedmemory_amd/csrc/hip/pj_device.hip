@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2701,7 +2702,7 @@ struct pj_ctx {
   int memo_log2 = 0;  // first-pass table entries per wave, log2 (PJ_MEMO_LOG2, 6..20; 0: by graph size)
   uint32_t memo_lds_limit = 0;  // PJ_MEMO_LDS_LIMIT: live entries of the LDS tables (0: 768)
   // memoised walks: per-wave tables (two passes), alreadyMatched, overflow list
-  MemoArgs memo1{}, memo_retry[kRetryPasses]{};
+  MemoArgs memo1{};
   void* d_already = nullptr;  // Word[V]
   // active view for the memoised walks (k_view_*)
   uint64_t* d_view_count = nullptr;  // V + 1
@@ -2804,10 +2805,6 @@ void free_graph(pj_ctx* ctx) {
                   ctx->d_view_edges, ctx->d_scan_tmp, ctx->d_rev, ctx->d_live[0], ctx->d_live[1]};
   for (void* b : bufs)
     if (b) hipFree(b);
-  for (auto& m : ctx->memo_retry)
-    for (void* b : {static_cast<void*>(m.keys), static_cast<void*>(m.vals), static_cast<void*>(m.epochs),
-                    static_cast<void*>(m.wave_epoch)})
-      if (b) hipFree(b);
   ctx->d_off = nullptr;
   ctx->d_edges = nullptr;
   ctx->d_labels = nullptr;
@@ -2827,7 +2824,6 @@ void free_graph(pj_ctx* ctx) {
   ctx->d_hub_hit = nullptr;
   ctx->d_last_scan = ctx->d_hub_new = nullptr;
   ctx->memo1 = MemoArgs{};
-  for (auto& m : ctx->memo_retry) m = MemoArgs{};
   ctx->d_already = nullptr;
   ctx->d_ovf = nullptr;
   ctx->d_view_count = ctx->d_view_off = nullptr;
@@ -3319,6 +3315,20 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
   return 0;
 }
 
+// Retry tables (2.7, 5.4 and 10.7 GB) serve the rare sources whose walk
+// outgrows the first-pass tables.  One set per device, shared by every
+// context of the process and allocated on first use: a context takes the
+// lock for its retry passes and leaves them with its stream drained, so the
+// passes of different contexts never overlap and each holds the per-wave
+// epochs to itself.  The set lives until the process ends.
+constexpr int kMaxDevices = 64;
+struct RetryPool {
+  std::mutex mu;
+  MemoArgs tables[kRetryPasses]{};
+};
+RetryPool g_retry_pool[kMaxDevices];
+
+
 // Sources whose walks outgrew their tables run again with larger ones
 // (kRetryWaves / kRetryLog2), the overflow lists alternating between the two
 // halves of d_ovf.  A source's results are written only by the pass that
@@ -3326,8 +3336,14 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
 template <typename Word>
 int launch_nlcc_retry(pj_ctx* ctx, int kind) {
   const uint64_t V = ctx->V;
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) {
+    ctx->err = "device index beyond the retry table pool";
+    return -1;
+  }
+  RetryPool& pool = g_retry_pool[ctx->device];
+  std::unique_lock<std::mutex> pool_lock(pool.mu, std::defer_lock);  // held from the first retry pass
   // pass 0: the first-pass HBM tables (memo1, kMemoWaves waves); then the
-  // retry tables
+  // shared retry tables
   for (int r = 0; r <= kRetryPasses; ++r) {
     uint32_t* in_count = ctx->d_ovf + (r & 1);
     uint32_t* out_count = ctx->d_ovf + ((r + 1) & 1);
@@ -3340,8 +3356,12 @@ int launch_nlcc_retry(pj_ctx* ctx, int kind) {
     MemoArgs m = ctx->memo1;
     int waves = kMemoWaves;
     if (r > 0) {
-      if (alloc_memo(ctx, &ctx->memo_retry[r - 1], kRetryWaves[r - 1], kRetryLog2[r - 1])) return -1;
-      m = ctx->memo_retry[r - 1];
+      if (!pool_lock.owns_lock()) pool_lock.lock();
+      MemoArgs& shared = pool.tables[r - 1];
+      if (alloc_memo(ctx, &shared, kRetryWaves[r - 1], kRetryLog2[r - 1])) return -1;
+      m = shared;
+      m.already = ctx->d_already;  // this context's alreadyMatched
+      m.lds_limit = ctx->memo_lds_limit;
       waves = kRetryWaves[r - 1];
     }
     m.ovf_count = out_count;

@@ -423,16 +423,20 @@ void ConstraintSet::flatten() {
   }
   f_walks.clear();
   f_tds_off.assign(1, 0);
+  auto walk_desc = [](uint32_t root, const Walk& wk) {
+    pj_walk_desc w;
+    w.root = root;
+    w.length = static_cast<uint32_t>(wk.move.size());
+    w.move = wk.move.data();
+    w.vertex = wk.vertex.data();
+    w.move_back = wk.move_back.data();
+    return w;
+  };
   for (const auto& t : tds) {
-    for (const auto& kv : t.walks) {
-      pj_walk_desc w;
-      w.root = kv.first;
-      w.length = static_cast<uint32_t>(kv.second.move.size());
-      w.move = kv.second.move.data();
-      w.vertex = kv.second.vertex.data();
-      w.move_back = kv.second.move_back.data();
-      f_walks.push_back(w);
-    }
+    if (t.order.empty())
+      for (const auto& kv : t.walks) f_walks.push_back(walk_desc(kv.first, kv.second));
+    else
+      for (uint32_t r : t.order) f_walks.push_back(walk_desc(r, t.walks.at(r)));
     f_tds_off.push_back(static_cast<uint32_t>(f_walks.size()));
   }
   pj_template_desc& d = f_desc;
@@ -454,12 +458,10 @@ void ConstraintSet::flatten() {
   d.tds_walk_offset = f_tds_off.data();
   d.walk_count = static_cast<uint32_t>(f_walks.size());
   d.walks = f_walks.data();
-  const auto& first = *enumeration.walks.cbegin();
-  d.enumeration_walk.root = first.first;
-  d.enumeration_walk.length = static_cast<uint32_t>(first.second.move.size());
-  d.enumeration_walk.move = first.second.move.data();
-  d.enumeration_walk.vertex = first.second.vertex.data();
-  d.enumeration_walk.move_back = first.second.move_back.data();
+  if (!enumeration.walks.empty()) {
+    const uint32_t root = enumeration.order.empty() ? enumeration.walks.cbegin()->first : enumeration.order[0];
+    d.enumeration_walk = walk_desc(root, enumeration.walks.at(root));
+  }
 }
 
 std::string ConstraintSet::dump() const {

@@ -69,6 +69,9 @@ struct TemplateConstraint {
   size_t length = 0;  // template vertices covered
   std::unordered_map<uint32_t, std::set<uint32_t>> adjacency;
   std::unordered_map<uint32_t, Walk> walks;
+  // Root order of `walks` when set from outside (the reference's walkMap
+  // iteration order); empty: the order of `walks` itself.
+  std::vector<uint32_t> order;
 
   static TemplateConstraint from_cycle(const CycleConstraint& c);
   static TemplateConstraint from_path(const PathConstraint& p);
@@ -104,8 +107,8 @@ struct ConstraintSet {
   const pj_template_desc* desc() const { return &f_desc; }
   // Human-readable dump in the spirit of the reference's log.totem prints.
   std::string dump() const;
-
- private:
+  // Rebuilds desc() from the vectors above (build() calls it; call it after
+  // filling them directly).
   void flatten();
 };
 

@@ -82,11 +82,106 @@ bool MultipleLabelStep::getNextStep(const AlgoResults& r, Step* next, std::strin
   return false;
 }
 
+int MultipleLabelDeviceState::allocate(uint64_t vertexCount, uint64_t edgeCount, uint32_t patternCount) {
+  if (ctx) free();
+  graphVertexCount = graphActiveVertexCount = vertexCount;
+  graphEdgeCount = graphActiveEdgeCount = edgeCount;
+  patternVertexCount = patternCount;
+  error = 0;
+  message.clear();
+  uploaded = nullptr;
+  template_version = 0;
+  host_fresh_ = false;
+  const char* dev = std::getenv("PJ_DEVICE");
+  if (pj_create(&ctx, dev ? std::atoi(dev) : 0) != 0) {
+    ctx = nullptr;
+    error = 1;
+    message = "pj_create failed";
+    return -1;
+  }
+  return 0;
+}
+
+int MultipleLabelDeviceState::free() {
+  if (ctx) pj_destroy(ctx);
+  ctx = nullptr;
+  uploaded = nullptr;
+  template_version = 0;
+  host_fresh_ = false;
+  return 0;
+}
+
+void MultipleLabelDeviceState::sync_host() const {
+  if (host_fresh_ || !ctx) return;
+  h_cand_.assign(graphVertexCount, 0);
+  h_vflags_.assign(graphVertexCount, 0);
+  h_ea_.assign((graphEdgeCount + 31) / 32, 0);
+  if (pj_download_state(ctx, h_cand_.data(), h_vflags_.data(), h_ea_.data()) != 0) {
+    std::fprintf(stderr, "pj_download_state: %s\n", pj_last_error(ctx));
+    return;
+  }
+  host_fresh_ = true;
+}
+
+uint32_t MultipleLabelDeviceState::ActiveList::operator[](uint64_t i) const {
+  s->sync_host();
+  if (edges) return i < s->h_ea_.size() * 32 ? (s->h_ea_[i >> 5] >> (i & 31)) & 1u : 0u;
+  return i < s->h_vflags_.size() ? s->h_vflags_[i] & 1u : 0u;
+}
+
+MultipleLabelDeviceState::CandidateSet MultipleLabelDeviceState::CandidateList::operator[](uint64_t v) const {
+  s->sync_host();
+  return CandidateSet{v < s->h_cand_.size() ? s->h_cand_[v] : 0};
+}
+
 namespace detail {
+
+TemplateAssembly& reference_template() {
+  static TemplateAssembly assembly;
+  return assembly;
+}
+
+int upload_graph(MultipleLabelDeviceState* s, uint32_t V, uint64_t E, const std::vector<uint64_t>& offsets,
+                 const uint32_t* edges, const uint64_t* labels, const void* key) {
+  if (pj_upload_graph(s->ctx, V, E, offsets.data(), edges, labels) != 0) {
+    s->error = 1;
+    s->message = pj_last_error(s->ctx);
+    return -1;
+  }
+  s->uploaded = key;
+  s->template_version = 0;  // a new graph needs the template again
+  s->graphVertexCount = s->graphActiveVertexCount = V;
+  s->graphEdgeCount = s->graphActiveEdgeCount = E;
+  s->invalidate_host_copy();
+  return 0;
+}
+
+int install_reference_template(MultipleLabelDeviceState* s) {
+  TemplateAssembly& a = reference_template();
+  if (a.set.local.empty()) {
+    s->error = 1;
+    s->message = "no local constraints were given to the step classes";
+    return -1;
+  }
+  if (!a.have_enumeration) {  // setConstraint not called: enumerate along a walk of the first TDS constraint
+    a.set.enumeration = a.set.tds.empty() ? TemplateConstraint{} : a.set.tds.front();
+  }
+  a.set.flatten();
+  if (pj_set_template(s->ctx, a.set.desc()) != 0) {
+    s->error = 1;
+    s->message = pj_last_error(s->ctx);
+    return -1;
+  }
+  s->template_version = a.version;
+  pj_get_counters(s->ctx, &s->graphActiveVertexCount, &s->graphActiveEdgeCount);
+  s->invalidate_host_copy();
+  return 0;
+}
 
 AlgoResults run_device_step(MultipleLabelDeviceState* s, int step, uint32_t index) {
   AlgoResults out;
   if (s->error) return out;
+  s->invalidate_host_copy();
   pj_algo_results r{};
   if (pj_run_step(s->ctx, step, index, &r) != 0) {
     s->error = 1;
@@ -103,6 +198,7 @@ AlgoResults run_device_step(MultipleLabelDeviceState* s, int step, uint32_t inde
 
 void reset_modified(MultipleLabelDeviceState* s) {
   if (s->error) return;
+  s->invalidate_host_copy();
   if (pj_reset_modified(s->ctx) != 0) {
     s->error = 1;
     s->message = pj_last_error(s->ctx);
@@ -309,6 +405,7 @@ int MultipleLabelHip::allocate(const Options& options) {
     error_ = pj_last_error(state_.ctx);
     return -1;
   }
+  state_.managed = true;
   state_.graphVertexCount = graph_.vertex_count;
   state_.graphEdgeCount = graph_.edge_count;
   state_.patternVertexCount = pattern_.vertex_count;

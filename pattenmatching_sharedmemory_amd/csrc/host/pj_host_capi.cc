@@ -204,6 +204,7 @@ int pjh_run_pipeline(pj_ctx* ctx, const pjh_constraints* c, int flags, pjh_row* 
   if (!ctx || !c) return -1;
   pj::MultipleLabelDeviceState st;
   st.ctx = ctx;
+  st.managed = true;
   int n = 0;
   int rc = pj::run_step_machine(
       &st, c->c,
@@ -227,22 +228,45 @@ int pjh_run_pipeline(pj_ctx* ctx, const pjh_constraints* c, int flags, pjh_row* 
 
 uint64_t pjh_projected_bytes(const pjh_constraints* c, uint32_t vertex_count, uint64_t edge_count) {
   const auto& cs = c->c;
-  const size_t P = cs.vertex_label.size();
-  const uint64_t W = P <= 8 ? 1 : P <= 16 ? 2 : P <= 32 ? 4 : 8;
-  const uint64_t V = vertex_count;
-  // candidate / toUnmatch / alreadyMatched words, flags and marks (5 bytes),
-  // scan lengths (4), frontier bins (12), edge-active bits
-  uint64_t bytes = V * (3 * W + 5 + 4 + 12) + edge_count / 8;
-  bool tables = !cs.tds.empty();
-  for (const auto& cy : cs.cycles) tables = tables || cy.vertex.size() >= 4;
-  for (const auto& pa : cs.paths) tables = tables || pa.vertex.size() >= 4;
-  if (tables) {  // first-pass memo tables: 2048 waves x 2^log2 x 20 bytes (pj_device.hip, prepare_memo)
+  const uint64_t V = vertex_count, E = edge_count;
+  // Per-context state (pj_device.hip alloc_state): label codes 1, candidate /
+  // toUnmatch / staging words 3 x 8, four flag bytes, scan lengths 4,
+  // frontier bins 4 x 4, two live lists 2 x 4 bytes per vertex; edge-active
+  // bits.
+  uint64_t bytes = V * (1 + 24 + 4 + 4 + 16 + 8) + (E + 31) / 32 * 4;
+  // Memoised walks: every TDS step, and cycle / path constraints whose memo
+  // may fire (pjh_constraints_memo_inert == 0).  Complete cycle and path
+  // walks of 4+ vertices keep a failed-state memo in the same tables.
+  bool memo = !cs.tds.empty(), tables = memo;
+  for (uint32_t i = 0; i < cs.cycles.size(); ++i) {
+    const bool fires = pjh_constraints_memo_inert(c, PJ_STEP_CC, i) == 0;
+    memo = memo || fires;
+    tables = tables || fires || cs.cycles[i].vertex.size() >= 4;
+  }
+  for (uint32_t i = 0; i < cs.paths.size(); ++i) {
+    const bool fires = pjh_constraints_memo_inert(c, PJ_STEP_PC, i) == 0;
+    memo = memo || fires;
+    tables = tables || fires || cs.paths[i].vertex.size() >= 4;
+  }
+  if (memo || tables) bytes += V * 8 + 8 + (2 * V + 2) * 4;  // alreadyMatched, overflow lists (prepare_memo)
+  if (tables) {  // first-pass tables: 2048 waves x 2^log2 entries x 20 bytes
     int bits = 0;
     while (bits < 63 && (1ull << bits) < V) ++bits;
     const int log2 = std::min(18, std::max(14, bits - 6));
-    bytes += 2048ull * (1ull << log2) * 20 + 2 * V * 4;
+    bytes += 2048ull * (1ull << log2) * 20;
   }
+  if (memo) bytes += 2 * (V + 1) * 8 + (E + 1) * 4;  // active view: counts, offsets, at most every slot
   return bytes;
+}
+
+int pjh_query_uses_memo(const pjh_constraints* c) {
+  const auto& cs = c->c;
+  if (!cs.tds.empty()) return 1;
+  for (uint32_t i = 0; i < cs.cycles.size(); ++i)
+    if (pjh_constraints_memo_inert(c, PJ_STEP_CC, i) == 0 || cs.cycles[i].vertex.size() >= 4) return 1;
+  for (uint32_t i = 0; i < cs.paths.size(); ++i)
+    if (pjh_constraints_memo_inert(c, PJ_STEP_PC, i) == 0 || cs.paths[i].vertex.size() >= 4) return 1;
+  return 0;
 }
 
 int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int n, int workers, int flags,
@@ -264,7 +288,14 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
   uint64_t vc = 0, ec = 0;
   pj_get_counters(ctxs[0], &vc, &ec);  // a fresh context: V and E
   std::vector<uint64_t> projected(n);
-  for (int i = 0; i < n; ++i) projected[i] = pjh_projected_bytes(templates[i], static_cast<uint32_t>(vc), ec);
+  bool any_memo = false;
+  for (int i = 0; i < n; ++i) {
+    projected[i] = pjh_projected_bytes(templates[i], static_cast<uint32_t>(vc), ec);
+    any_memo = any_memo || pjh_query_uses_memo(templates[i]);
+  }
+  // The retry tables are one set per device shared by all contexts
+  // (PJ_RETRY_POOL_BYTES): held back from the budget once.
+  if (budget_bytes && any_memo) budget_bytes = budget_bytes > PJ_RETRY_POOL_BYTES ? budget_bytes - PJ_RETRY_POOL_BYTES : 1;
   std::mutex mu;
   std::condition_variable cv;
   int next = 0;
@@ -312,6 +343,7 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
       } else {
         pj::MultipleLabelDeviceState st;
         st.ctx = ctx;
+        st.managed = true;
         const int rc = pj::run_step_machine(
             &st, templates[i]->c,
             [&](const pj::StepRecord& rec) {

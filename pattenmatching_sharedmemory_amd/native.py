@@ -141,6 +141,7 @@ HOST_SIGNATURES = {
     "pjh_run_queries": (C.c_int, [P, C.POINTER(P), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_uint64,
                                   C.POINTER(QueryResult), C.POINTER(C.c_double), C.c_char_p, C.c_size_t]),
     "pjh_projected_bytes": (C.c_uint64, [P, C.c_uint32, C.c_uint64]),
+    "pjh_query_uses_memo": (C.c_int, [P]),
     "pjh_main": (C.c_int, [C.c_int, C.POINTER(C.c_char_p)]),
 }
 

@@ -68,3 +68,42 @@ def test_checksum_tracks_the_state(pj):
     assert a.checksum() == b.checksum()
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("kind", ["inert triangle", "memoised triangle", "tds diamond", "long cycle"])
+def test_projection_covers_what_a_query_context_allocates(pj, kind):
+    """Device memory taken by a worker context (sharing the owner's graph)
+    over a whole solve stays within pjh_projected_bytes of its template, up
+    to the allocator's rounding (2 MiB per allocation)."""
+    import ctypes as C
+
+    import torch
+
+    from pattenmatching_sharedmemory_amd import native
+
+    shapes = {
+        "inert triangle": (3, [(0, 1), (1, 2), (2, 0)], [0, 1, 2]),
+        "memoised triangle": (5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 2)], [0, 3, 1, 2, 1]),
+        "tds diamond": (4, [(0, 1), (0, 2), (0, 3), (1, 2), (2, 3)], [0, 1, 2, 3]),
+        "long cycle": (5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0)], [0, 1, 2, 3, 4]),
+    }
+    g = pj.Graph.rmat(16, 16, seed=3).labels_zipf(8, 1.0, seed=4)
+    c = pj.Constraints(pj.Graph.pattern(*shapes[kind]))
+    owner = pj.Pruner(g)
+    hip, host = native.hip(), native.host()
+    projected = host.pjh_projected_bytes(c.handle, g.vertex_count, g.edge_count)
+    torch.cuda.init()
+    ctx = C.c_void_p()
+    assert hip.pj_create(C.byref(ctx), 0) == 0
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    assert hip.pj_share_graph(ctx, owner.ctx) == 0
+    assert hip.pj_set_template(ctx, c.desc) == 0
+    total = C.c_double()
+    assert host.pjh_run_pipeline(ctx, c.handle, 1, None, 0, C.byref(total)) > 0
+    torch.cuda.synchronize()
+    used = free0 - torch.cuda.mem_get_info()[0]
+    print(f"[{kind}] used {used} bytes, projected {projected}", flush=True)
+    assert used <= projected + 48 * (2 << 20), kind
+    hip.pj_destroy(ctx)
+    owner.close()

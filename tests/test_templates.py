@@ -45,3 +45,25 @@ def test_projected_bytes_grow_with_the_graph(pj):
     small = native.host().pjh_projected_bytes(c.handle, 1 << 10, 1 << 14)
     large = native.host().pjh_projected_bytes(c.handle, 1 << 20, 1 << 24)
     assert 0 < small < large
+
+
+def test_projected_bytes_count_memo_tables_and_view(pj):
+    """A 3-vertex cycle whose memo may fire (repeated label at the tested
+    position) and a TDS template project the memo tables, alreadyMatched and
+    the active view; a memo-inert triangle projects the state alone."""
+    from pattenmatching_sharedmemory_amd import native
+
+    h = native.host()
+    plain = pj.Constraints(pj.Graph.pattern(3, [(0, 1), (1, 2), (2, 0)], [0, 1, 2]))
+    memo = pj.Constraints(pj.Graph.pattern(5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 2)], [0, 3, 1, 2, 1]))
+    tds = pj.Constraints(pj.Graph.pattern(4, [(0, 1), (0, 2), (0, 3), (1, 2), (2, 3)], [0, 1, 2, 3]))
+    V, E = 1 << 20, 1 << 24
+    assert h.pjh_query_uses_memo(plain.handle) == 0
+    assert h.pjh_query_uses_memo(memo.handle) == 1 and h.pjh_query_uses_memo(tds.handle) == 1
+    base = h.pjh_projected_bytes(plain.handle, V, E)
+    assert base == V * 57 + E // 8
+    tables = 2048 * (1 << 14) * 20  # 2^20 vertices: 2^14 entries per wave
+    view = 2 * (V + 1) * 8 + (E + 1) * 4
+    extra = V * 8 + 8 + (2 * V + 2) * 4 + tables + view
+    assert h.pjh_projected_bytes(memo.handle, V, E) == base + extra
+    assert h.pjh_projected_bytes(tds.handle, V, E) == base + extra
