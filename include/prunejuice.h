@@ -219,6 +219,25 @@ int pj_upload_state(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags,
                     const uint32_t* ea_words, uint64_t active_vertices,
                     uint64_t active_edges);
 
+/* The surviving graph, compacted on the device: the vertices whose candidate
+ * set is not empty (ascending, with their candidate sets as 64-bit words) and
+ * the slots v->u whose edge-active bit is set between two surviving vertices
+ * (CSR order, as src / dst pairs).  These are what the reference's final
+ * dumps print, printActiveVertex and printActiveGraph
+ * (src/patternmatching/multiple_label_cpu.cc:270-297), which read the whole
+ * state on the host.  Outputs are device memory of the context's device with
+ * room for cap_vertices / cap_slots entries (cand may be NULL); the counts
+ * come back in *n_vertices / *n_slots.  When a count exceeds its capacity the
+ * call returns -1 after writing what fits, with the counts set.  The
+ * library keeps per-block counters for the context (allocated on first use). */
+int pj_compact_active(pj_ctx* ctx, uint32_t* vertices, uint64_t* cand, uint64_t cap_vertices, uint32_t* src,
+                      uint32_t* dst, uint64_t cap_slots, uint64_t* n_vertices, uint64_t* n_slots);
+
+/* pj_compact_active into host arrays (the context holds the device copies).
+ * With every output pointer NULL only the counts are returned. */
+int pj_download_active(pj_ctx* ctx, uint32_t* vertices, uint64_t* cand, uint64_t cap_vertices, uint32_t* src,
+                       uint32_t* dst, uint64_t cap_slots, uint64_t* n_vertices, uint64_t* n_slots);
+
 /* Last error message of the context (never NULL). */
 const char* pj_last_error(pj_ctx* ctx);
 

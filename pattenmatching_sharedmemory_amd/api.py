@@ -310,6 +310,21 @@ class Pruner:
         av, ae = self.counters()
         return State(cand, vf, ea, av, ae)
 
+    def active_graph(self):
+        """The surviving graph compacted on the device (pj_download_active):
+        (vertices, candidate words, slot sources, slot destinations), vertices
+        ascending, slots in CSR order."""
+        lib = native.hip()
+        nv, ne = C.c_uint64(), C.c_uint64()
+        self._check(lib.pj_download_active(self._ctx, None, None, 0, None, None, 0, C.byref(nv), C.byref(ne)))
+        v = np.zeros(nv.value, dtype=np.uint32)
+        c = np.zeros(nv.value, dtype=np.uint64)
+        s = np.zeros(ne.value, dtype=np.uint32)
+        d = np.zeros(ne.value, dtype=np.uint32)
+        self._check(lib.pj_download_active(self._ctx, v.ctypes.data, c.ctypes.data, nv.value, s.ctypes.data,
+                                           d.ctypes.data, ne.value, C.byref(nv), C.byref(ne)))
+        return v, c, s, d
+
     def checksum(self) -> int:
         """pj_state_checksum of the current state."""
         h = C.c_uint64()

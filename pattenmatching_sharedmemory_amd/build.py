@@ -51,7 +51,8 @@ def build_hip() -> Path:
     LIB.mkdir(exist_ok=True)
     out = LIB / "libprunejuice_hip.so"
     tmp = LIB / "libprunejuice_hip.so.tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", str(CSRC / "hip" / "pj_device.hip"), "-o", str(tmp)])
+    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", str(CSRC / "hip" / "pj_device.hip"),
+          str(CSRC / "hip" / "pj_compact.hip"), "-o", str(tmp)])
     os.replace(tmp, out)
     return out
 

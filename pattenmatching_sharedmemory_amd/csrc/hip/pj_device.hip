@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../../include/prunejuice.h"
+#include "pj_compact.h"
 
 namespace {
 
@@ -2754,6 +2755,12 @@ struct pj_ctx {
   uint64_t last_slots = 0;
   unsigned long long* h_sums = nullptr;   // pinned: per counter block, kCounterStride sums
   unsigned long long* d_sums = nullptr;
+  // Active-graph compaction: the counters of pj_compact_active and the
+  // device outputs of pj_download_active (grown on demand).
+  unsigned long long* d_compact_scratch = nullptr;
+  uint32_t *d_out_v = nullptr, *d_out_src = nullptr, *d_out_dst = nullptr;
+  uint64_t* d_out_cand = nullptr;
+  uint64_t out_cap_v = 0, out_cap_e = 0;
 };
 
 namespace {
@@ -2802,7 +2809,8 @@ void free_graph(pj_ctx* ctx) {
                   ctx->d_chunk_hub, ctx->d_chunk_start, ctx->d_hub_count, ctx->d_hub_elim, ctx->d_nhubs, ctx->d_hub_hit,
                   ctx->d_last_scan, ctx->d_hub_new, ctx->memo1.keys, ctx->memo1.vals, ctx->memo1.epochs,
                   ctx->memo1.wave_epoch, ctx->d_already, ctx->d_ovf, ctx->d_view_count, ctx->d_view_off,
-                  ctx->d_view_edges, ctx->d_scan_tmp, ctx->d_rev, ctx->d_live[0], ctx->d_live[1]};
+                  ctx->d_view_edges, ctx->d_scan_tmp, ctx->d_rev, ctx->d_live[0], ctx->d_live[1],
+                  ctx->d_compact_scratch, ctx->d_out_v, ctx->d_out_src, ctx->d_out_dst, ctx->d_out_cand};
   for (void* b : bufs)
     if (b) hipFree(b);
   ctx->d_off = nullptr;
@@ -2814,6 +2822,10 @@ void free_graph(pj_ctx* ctx) {
   ctx->d_ea = nullptr;
   ctx->d_rev = nullptr;
   ctx->d_live[0] = ctx->d_live[1] = nullptr;
+  ctx->d_compact_scratch = nullptr;
+  ctx->d_out_v = ctx->d_out_src = ctx->d_out_dst = nullptr;
+  ctx->d_out_cand = nullptr;
+  ctx->out_cap_v = ctx->out_cap_e = 0;
   ctx->live = nullptr;
   ctx->n_live = 0;
   ctx->d_bins = ctx->d_bin_count = nullptr;
@@ -4181,6 +4193,75 @@ int pj_upload_state(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags, co
 const char* pj_last_error(pj_ctx* ctx) {
   if (!ctx) return "null context";
   return ctx->err.c_str();
+}
+
+// Active-graph compaction (pj_compact.hip).
+static int compact_common(pj_ctx* ctx, unsigned long long* scratch, uint32_t* vertices, uint64_t* cand,
+                          uint64_t cap_v, uint32_t* src, uint32_t* dst, uint64_t cap_e, uint64_t* n_v,
+                          uint64_t* n_e) {
+  pjc::CompactArgs a{ctx->d_off, ctx->d_edges, ctx->d_ea, ctx->d_cand, ctx->word_bytes, ctx->V, ctx->E,
+                     vertices, cand, src, dst, cap_v, cap_e, ctx->stream};
+  uint64_t nv = 0, ne = 0;
+  const int rc = pjc::compact_active(a, scratch, &nv, &ne, &ctx->err);
+  if (n_v) *n_v = nv;
+  if (n_e) *n_e = ne;
+  return rc;
+}
+
+static int compact_ready(pj_ctx* ctx) {
+  if (!ctx) return -1;
+  if (!ctx->have_template) {
+    ctx->err = "no template installed";
+    return -1;
+  }
+  PJ_HIP(hipSetDevice(ctx->device));
+  return 0;
+}
+
+int pj_compact_active(pj_ctx* ctx, uint32_t* vertices, uint64_t* cand, uint64_t cap_vertices, uint32_t* src,
+                      uint32_t* dst, uint64_t cap_slots, uint64_t* n_vertices, uint64_t* n_slots) {
+  if (compact_ready(ctx)) return -1;
+  // counters sized for this graph, kept by the context
+  if (!ctx->d_compact_scratch && dev_alloc(ctx, &ctx->d_compact_scratch, pjc::scratch_words(ctx->V, ctx->E)))
+    return -1;
+  return compact_common(ctx, ctx->d_compact_scratch, vertices, cand, cap_vertices, src, dst, cap_slots, n_vertices,
+                        n_slots);
+}
+
+int pj_download_active(pj_ctx* ctx, uint32_t* vertices, uint64_t* cand, uint64_t cap_vertices, uint32_t* src,
+                       uint32_t* dst, uint64_t cap_slots, uint64_t* n_vertices, uint64_t* n_slots) {
+  if (compact_ready(ctx)) return -1;
+  uint64_t nv = 0, ne = 0;
+  // counts first (outputs of capacity 0), then outputs of exactly that size
+  if (!ctx->d_compact_scratch && dev_alloc(ctx, &ctx->d_compact_scratch, pjc::scratch_words(ctx->V, ctx->E)))
+    return -1;
+  if (compact_common(ctx, ctx->d_compact_scratch, nullptr, nullptr, 0, nullptr, nullptr, 0, &nv, &ne) &&
+      (nv == 0 && ne == 0))
+    return -1;
+  if (n_vertices) *n_vertices = nv;
+  if (n_slots) *n_slots = ne;
+  if (!vertices && !cand && !src && !dst) return 0;  // counts only
+  if (nv > cap_vertices || ne > cap_slots) {
+    ctx->err = "pj_download_active: output capacity too small";
+    return -1;
+  }
+  if (nv > ctx->out_cap_v) {
+    if (dev_alloc(ctx, &ctx->d_out_v, nv) || dev_alloc(ctx, &ctx->d_out_cand, nv)) return -1;
+    ctx->out_cap_v = nv;
+  }
+  if (ne > ctx->out_cap_e) {
+    if (dev_alloc(ctx, &ctx->d_out_src, ne) || dev_alloc(ctx, &ctx->d_out_dst, ne)) return -1;
+    ctx->out_cap_e = ne;
+  }
+  if (compact_common(ctx, ctx->d_compact_scratch, ctx->d_out_v, ctx->d_out_cand, ctx->out_cap_v, ctx->d_out_src,
+                     ctx->d_out_dst, ctx->out_cap_e, &nv, &ne))
+    return -1;
+  if (vertices && nv) PJ_HIP(hipMemcpyAsync(vertices, ctx->d_out_v, nv * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (cand && nv) PJ_HIP(hipMemcpyAsync(cand, ctx->d_out_cand, nv * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (src && ne) PJ_HIP(hipMemcpyAsync(src, ctx->d_out_src, ne * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (dst && ne) PJ_HIP(hipMemcpyAsync(dst, ctx->d_out_dst, ne * 4, hipMemcpyDeviceToHost, ctx->stream));
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
 }
 
 void pj_destroy(pj_ctx* ctx) {

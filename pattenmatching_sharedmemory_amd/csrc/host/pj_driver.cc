@@ -435,26 +435,25 @@ void MultipleLabelHip::logResults(int iteration, bool logGraph) {
             static_cast<size_t>(state_.graphEdgeCount), results_.matchEliminated, results_.enumeration)
      << std::endl;
   if (!logGraph) return;
-  const uint32_t V = graph_.vertex_count;
-  std::vector<uint64_t> cand(V);
-  std::vector<uint8_t> vf(V);
-  std::vector<uint32_t> ea((graph_.edge_count + 31) / 32);
-  if (pj_download_state(state_.ctx, cand.data(), vf.data(), ea.data())) {
+  // The surviving vertices and slots, compacted on the device
+  // (pj_download_active); the host only formats them.
+  uint64_t nv = 0, ne = 0;
+  if (pj_download_active(state_.ctx, nullptr, nullptr, 0, nullptr, nullptr, 0, &nv, &ne)) {
+    error_ = pj_last_error(state_.ctx);
+    return;
+  }
+  std::vector<uint32_t> vid(nv), src(ne), dst(ne);
+  std::vector<uint64_t> cand(nv);
+  if (pj_download_active(state_.ctx, vid.data(), cand.data(), nv, src.data(), dst.data(), ne, &nv, &ne)) {
     error_ = pj_last_error(state_.ctx);
     return;
   }
   std::ostringstream g, v;
-  for (uint32_t x = 0; x < V; ++x) {
-    if (!(vf[x] & 1)) continue;
-    for (uint64_t e = graph_.vertices[x]; e < graph_.vertices[x + 1]; ++e) {
-      if (!((ea[e >> 5] >> (e & 31)) & 1)) continue;
-      const uint32_t u = graph_.edges[e];
-      if (!(vf[u] & 1)) continue;
-      g << x << " " << u << " " << graph_.values[x] << " " << graph_.values[u] << "\n";
-    }
-    v << x << " ";
-    for (uint32_t p = 0; p < 64; ++p)
-      if ((cand[x] >> p) & 1) v << p << " ";
+  for (uint64_t i = 0; i < ne; ++i)
+    g << src[i] << " " << dst[i] << " " << graph_.values[src[i]] << " " << graph_.values[dst[i]] << "\n";
+  for (uint64_t i = 0; i < nv; ++i) {
+    v << vid[i] << " ";
+    for (uint64_t m = cand[i]; m; m &= m - 1) v << __builtin_ctzll(m) << " ";
     v << "\n";
   }
   std::ofstream(options_.outputDir + fmt("graph_%04d.totem", iteration)) << g.str() << std::endl;

@@ -74,7 +74,11 @@ def test_checksum_tracks_the_state(pj):
 def test_projection_covers_what_a_query_context_allocates(pj, kind):
     """Device memory taken by a worker context (sharing the owner's graph)
     over a whole solve stays within pjh_projected_bytes of its template, up
-    to the allocator's rounding (2 MiB per allocation)."""
+    to the allocator's rounding (2 MiB per allocation).  The retry tables are
+    not the context's: one set per device, allocated when a source first
+    outgrows the first-pass tables and shared afterwards; so the first
+    context's solve may add up to PJ_RETRY_POOL_BYTES, a second context's
+    solve of the same template stays within its projection."""
     import ctypes as C
 
     import torch
@@ -87,23 +91,26 @@ def test_projection_covers_what_a_query_context_allocates(pj, kind):
         "tds diamond": (4, [(0, 1), (0, 2), (0, 3), (1, 2), (2, 3)], [0, 1, 2, 3]),
         "long cycle": (5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0)], [0, 1, 2, 3, 4]),
     }
+    pool = (32 << 22) * 20 + (4 << 26) * 20 + (1 << 29) * 20  # PJ_RETRY_POOL_BYTES
     g = pj.Graph.rmat(16, 16, seed=3).labels_zipf(8, 1.0, seed=4)
     c = pj.Constraints(pj.Graph.pattern(*shapes[kind]))
     owner = pj.Pruner(g)
     hip, host = native.hip(), native.host()
     projected = host.pjh_projected_bytes(c.handle, g.vertex_count, g.edge_count)
+    slack = 48 * (2 << 20)
     torch.cuda.init()
-    ctx = C.c_void_p()
-    assert hip.pj_create(C.byref(ctx), 0) == 0
-    torch.cuda.synchronize()
-    free0 = torch.cuda.mem_get_info()[0]
-    assert hip.pj_share_graph(ctx, owner.ctx) == 0
-    assert hip.pj_set_template(ctx, c.desc) == 0
-    total = C.c_double()
-    assert host.pjh_run_pipeline(ctx, c.handle, 1, None, 0, C.byref(total)) > 0
-    torch.cuda.synchronize()
-    used = free0 - torch.cuda.mem_get_info()[0]
-    print(f"[{kind}] used {used} bytes, projected {projected}", flush=True)
-    assert used <= projected + 48 * (2 << 20), kind
-    hip.pj_destroy(ctx)
+    for attempt in range(2):
+        ctx = C.c_void_p()
+        assert hip.pj_create(C.byref(ctx), 0) == 0
+        torch.cuda.synchronize()
+        free0 = torch.cuda.mem_get_info()[0]
+        assert hip.pj_share_graph(ctx, owner.ctx) == 0
+        assert hip.pj_set_template(ctx, c.desc) == 0
+        total = C.c_double()
+        assert host.pjh_run_pipeline(ctx, c.handle, 1, None, 0, C.byref(total)) > 0
+        torch.cuda.synchronize()
+        used = free0 - torch.cuda.mem_get_info()[0]
+        print(f"[{kind} context {attempt}] used {used} bytes, projected {projected}", flush=True)
+        assert used <= projected + slack + (pool if attempt == 0 else 0), (kind, attempt)
+        hip.pj_destroy(ctx)
     owner.close()
