@@ -233,6 +233,19 @@ int pj_upload_state(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags,
 int pj_compact_active(pj_ctx* ctx, uint32_t* vertices, uint64_t* cand, uint64_t cap_vertices, uint32_t* src,
                       uint32_t* dst, uint64_t cap_slots, uint64_t* n_vertices, uint64_t* n_slots);
 
+/* pj_compact_active without allocating: no reference counterpart; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Its per-block
+ * counters live in a static device array of the library (256 KiB per device,
+ * held until the process ends), so it takes graphs of at most 2^30 vertices
+ * and 2^30 slots and returns -1 with an error for larger ones.  Calls on one
+ * device share the array: this process serialises them with a lock; calls
+ * from another copy of the library in the same process would race.  Same
+ * outputs, counts and errors as pj_compact_active. */
+int pj_compact_active_noalloc(pj_ctx* ctx, uint32_t* vertices, uint64_t* cand, uint64_t cap_vertices,
+                              uint32_t* src, uint32_t* dst, uint64_t cap_slots, uint64_t* n_vertices,
+                              uint64_t* n_slots);
+
 /* pj_compact_active into host arrays (the context holds the device copies).
  * With every output pointer NULL only the counts are returned. */
 int pj_download_active(pj_ctx* ctx, uint32_t* vertices, uint64_t* cand, uint64_t cap_vertices, uint32_t* src,

@@ -17,6 +17,11 @@ namespace pjc {
 // one block per 2^16 slots and one per 2^16 vertices, and one 64-bit counter
 // of scratch per block.
 constexpr uint64_t kSpan = 1ull << 16;
+// Scratch of the no-allocation entry point: counters for 2^14 blocks of each
+// kind (256 KiB), so graphs of at most 2^30 vertices and 2^30 slots.
+constexpr uint64_t kStaticBlocks = 1ull << 14;
+constexpr uint64_t kStaticMaxItems = kStaticBlocks * kSpan;
+
 struct CompactArgs {
   const uint64_t* off;     // [V+1]
   const uint32_t* edges;   // [E]
@@ -42,6 +47,11 @@ uint64_t scratch_words(uint32_t V, uint64_t E);
 // totals are still returned; nothing is written past a capacity) or on a HIP
 // error.  Allocates nothing.
 int compact_active(const CompactArgs& a, unsigned long long* scratch, uint64_t* n_v, uint64_t* n_e, std::string* err);
+
+// Device address of this library's static scratch on `device`, which must be
+// the current device (hipGetSymbolAddress, looked up once per device and
+// kept: each device holds its own copy of the array).
+int static_scratch(int device, unsigned long long** ptr, std::string* err);
 
 }  // namespace pjc
 

@@ -15,7 +15,17 @@
 // accumulated, so the scratch needs no clearing between calls.
 #include "pj_compact.h"
 
+#include <mutex>
+#include <vector>
+
 namespace pjc {
+
+// The no-allocation entry point's counters: one array per device, used
+// through the address hipGetSymbolAddress returns for it (in host code the
+// name is only a stand-in); held until the process ends.  Calls that
+// overlap on one device would share it: pj_compact_active_noalloc takes a
+// lock for the whole call.
+__device__ __align__(256) unsigned long long scratch[kStaticBlocks * 2 + 2];
 
 namespace {
 
@@ -200,6 +210,28 @@ int compact_active(const CompactArgs& a, unsigned long long* sc, uint64_t* n_v, 
     default:
       return run<uint64_t>(a, sc, n_v, n_e, err);
   }
+}
+
+int static_scratch(int device, unsigned long long** ptr, std::string* err) {
+  static std::mutex mu;
+  static std::vector<unsigned long long*> by_device;
+  std::lock_guard<std::mutex> lk(mu);
+  if (device < 0) {
+    *err = "bad device";
+    return -1;
+  }
+  if (by_device.size() <= static_cast<size_t>(device)) by_device.resize(device + 1, nullptr);
+  if (!by_device[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));  // the current device's copy
+    if (e != hipSuccess || !p) {
+      *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return -1;
+    }
+    by_device[device] = static_cast<unsigned long long*>(p);
+  }
+  *ptr = by_device[device];
+  return 0;
 }
 
 }  // namespace pjc

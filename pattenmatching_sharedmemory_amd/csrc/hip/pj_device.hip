@@ -4228,6 +4228,21 @@ int pj_compact_active(pj_ctx* ctx, uint32_t* vertices, uint64_t* cand, uint64_t 
                         n_slots);
 }
 
+int pj_compact_active_noalloc(pj_ctx* ctx, uint32_t* vertices, uint64_t* cand, uint64_t cap_vertices, uint32_t* src,
+                              uint32_t* dst, uint64_t cap_slots, uint64_t* n_vertices, uint64_t* n_slots) {
+  if (compact_ready(ctx)) return -1;
+  if (ctx->V > pjc::kStaticMaxItems || ctx->E > pjc::kStaticMaxItems) {
+    ctx->err = "pj_compact_active_noalloc: graphs of at most 2^30 vertices and 2^30 slots (its static scratch)";
+    return -1;
+  }
+  // one call at a time in this process: every call on a device shares its scratch
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  unsigned long long* scratch = nullptr;
+  if (pjc::static_scratch(ctx->device, &scratch, &ctx->err)) return -1;
+  return compact_common(ctx, scratch, vertices, cand, cap_vertices, src, dst, cap_slots, n_vertices, n_slots);
+}
+
 int pj_download_active(pj_ctx* ctx, uint32_t* vertices, uint64_t* cand, uint64_t cap_vertices, uint32_t* src,
                        uint32_t* dst, uint64_t cap_slots, uint64_t* n_vertices, uint64_t* n_slots) {
   if (compact_ready(ctx)) return -1;
