@@ -202,6 +202,16 @@ int pj_download_state(pj_ctx* ctx, uint64_t* cand, uint8_t* vflags, uint32_t* ea
  * multiple_label_cpu.cc:252-297). */
 int pj_state_checksum(pj_ctx* ctx, uint64_t* checksum);
 
+/* Streams a binary TOTEM graph file (graph_store_binary's format, offsets of
+ * 32 or 64 bits) into the context: the neighbour ids go to HBM in 64 MiB
+ * chunks through two pinned host buffers, reading one while the other is
+ * copied, so no host array holds the edge list; offsets and values are read
+ * whole.  `labels` [V] replaces the file's vertex values (NULL: use them; the
+ * file must then be valued).  Then the same checks and set-up as
+ * pj_upload_graph.  Replaces graph_initialize on a binary file followed by
+ * pj_upload_graph (src/totem/totem_graph.cu:253-312, 388-409). */
+int pj_upload_totem_file(pj_ctx* ctx, const char* path, const uint64_t* labels);
+
 /* Makes ctx use the graph uploaded into `owner` (same device) without a
  * copy: CSR, labels and hub lists are shared read-only, the pruning state is
  * ctx's own.  `owner` must outlive ctx and keep its graph.  Many contexts

@@ -224,17 +224,28 @@ class State:
 class Pruner:
     """One device context (pj_ctx) with a resident graph and template."""
 
-    def __init__(self, graph: Graph, constraints: Constraints | None = None, device: int = 0):
+    def __init__(self, graph: Graph | None, constraints: Constraints | None = None, device: int = 0,
+                 path: str | None = None, labels=None):
+        """A context over `graph` (uploaded from host memory), or over the
+        binary TOTEM file `path` streamed into HBM (pj_upload_totem_file;
+        `labels` replaces the file's vertex values)."""
         lib = native.hip()
         ctx = C.c_void_p()
         if lib.pj_create(C.byref(ctx), device):
             raise RuntimeError("pj_create failed (no HIP device?)")
         self._ctx = ctx
         self.graph = graph
-        off, ed, lab = graph.raw_arrays()
-        if not lab:
-            raise ValueError("graph has no labels")
-        self._check(lib.pj_upload_graph(ctx, graph.vertex_count, graph.edge_count, off, ed, lab))
+        if path is not None:
+            lab = None if labels is None else np.ascontiguousarray(labels, dtype=np.uint64)
+            self._check(lib.pj_upload_totem_file(ctx, str(path).encode(),
+                                                 None if lab is None else _ptr(lab, C.c_uint64)))
+            self.vertex_count, self.edge_count = self.counters()
+        else:
+            off, ed, lab = graph.raw_arrays()
+            if not lab:
+                raise ValueError("graph has no labels")
+            self._check(lib.pj_upload_graph(ctx, graph.vertex_count, graph.edge_count, off, ed, lab))
+            self.vertex_count, self.edge_count = graph.vertex_count, graph.edge_count
         self.constraints = None
         if constraints is not None:
             self.set_template(constraints)
@@ -301,7 +312,7 @@ class Pruner:
         return av.value, ae.value
 
     def state(self) -> State:
-        V, E = self.graph.vertex_count, self.graph.edge_count
+        V, E = self.vertex_count, self.edge_count
         cand = np.zeros(V, dtype=np.uint64)
         vf = np.zeros(V, dtype=np.uint8)
         ea = np.zeros((E + 31) // 32, dtype=np.uint32)

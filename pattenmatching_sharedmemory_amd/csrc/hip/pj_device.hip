@@ -484,10 +484,37 @@ __device__ __forceinline__ int bin_of(uint64_t v, uint32_t V, const Word* __rest
 // classifies them (kept in registers) and counts per bin with ballots, one
 // global atomic per block and bin reserves space, pass 2 writes the vertex
 // ids (each wave's run stays contiguous in its bin).
+// Phase 1 of a scheduled vertex without slots (mode 0): every count is 0, so
+// candidate p fails iff its label differs or it requires any neighbour
+// (multiple_label_lcc_cpu.tpp:79-101 over an empty row); the vertex turns
+// modified when one fails.  No slot, no edge elimination, no neighbour to
+// mark for a rescan.
+template <typename Word>
+__device__ __forceinline__ void phase1_isolated(uint32_t v, const Word* __restrict__ cand,
+                                                const uint8_t* __restrict__ code, uint8_t* __restrict__ flags,
+                                                uint8_t* __restrict__ marks, Word* __restrict__ unmatch,
+                                                const DevTemplate* __restrict__ t) {
+  const Word cv = cand[v];
+  const uint8_t vcode = code[v];
+  Word fail = 0;
+  for (Word m = cv; m; m &= m - 1) {
+    const int p = ctz<Word>(m);
+    bool bad = t->pcode[p] != vcode;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) bad |= t->need[p][w] != 0;
+    if (bad) fail |= Word(1) << p;
+  }
+  if (fail) {
+    unmatch[v] = fail;
+    flags[v] = static_cast<uint8_t>(flags[v] | kModified);
+  }
+  if (marks) marks[v] = 0;
+}
+
 template <typename Word>
 __global__ void __launch_bounds__(kBlock) k_build_bins(
-    const uint32_t* __restrict__ live, uint32_t n_live, const Word* __restrict__ cand, uint8_t* __restrict__ flags, const uint8_t* __restrict__ code, const uint8_t* __restrict__ marks,
-    const Word* __restrict__ unmatch, const uint64_t* __restrict__ off, uint32_t V, int mode,
+    const uint32_t* __restrict__ live, uint32_t n_live, const Word* __restrict__ cand, uint8_t* __restrict__ flags, const uint8_t* __restrict__ code, uint8_t* __restrict__ marks,
+    Word* __restrict__ unmatch, const uint64_t* __restrict__ off, uint32_t V, int mode,
     const DevTemplate* __restrict__ t, uint32_t* __restrict__ bins, uint32_t* __restrict__ bin_count,
     uint32_t* __restrict__ last_scan, const uint32_t* __restrict__ first, unsigned long long* counters) {
   __shared__ uint32_t s_wave[kBlock / 64][kBins];
@@ -507,6 +534,14 @@ __global__ void __launch_bounds__(kBlock) k_build_bins(
 #pragma unroll
   for (int i = 0; i < kBinIters; ++i)
     bin[i] = bin_of<Word>(vid[i], V, cand, flags, code, marks, unmatch, off, mode, path_code, deg[i]);
+  if (mode == 0) {  // LCC phase 1 of vertices without slots, here instead of in a bin
+#pragma unroll
+    for (int i = 0; i < kBinIters; ++i)
+      if (bin[i] == 0 && deg[i] == 0) {
+        phase1_isolated<Word>(vid[i], cand, code, flags, marks, unmatch, t);
+        bin[i] = -1;
+      }
+  }
   uint32_t cnt[kBins] = {};
   uint64_t skipped = 0;
 #pragma unroll
@@ -651,101 +686,104 @@ __device__ __forceinline__ void load_lcc_tables(LccTables<KW>& s, const DevTempl
   for (int i = threadIdx.x; i < 64 * KW; i += blockDim.x) s.need[i] = t->need[i / KW][i % KW];
 }
 
-template <int KW>
-__device__ __forceinline__ void count_labels(uint64_t (&cnt)[KW], bool counted, uint32_t lab, uint64_t gmask,
-                                             uint32_t K) {
-#pragma unroll
-  for (int w = 0; w < KW; ++w) {
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint32_t L = w * 8 + b;
-      if (L >= K) break;
-      uint64_t m = __ballot(counted && lab == L) & gmask;
-      uint32_t c = __popcll(m);
-      if (c) {
-        uint32_t cur = static_cast<uint32_t>(cnt[w] >> (8 * b)) & 0xFFu;
-        cur = min(cur + c, 127u);
-        cnt[w] = (cnt[w] & ~(0xFFull << (8 * b))) | (static_cast<uint64_t>(cur) << (8 * b));
-      }
-    }
-  }
+// Saturating sum of packed byte counters that are each at most 127: the
+// byte sums stay below 256, so no carry crosses a byte; bytes that reach
+// 128 become 127.
+__device__ __forceinline__ uint64_t sat_add_bytes(uint64_t a, uint64_t b) {
+  const uint64_t H = 0x8080808080808080ull;
+  const uint64_t x = a + b;
+  const uint64_t hi = x & H;
+  return (x & ~H) | (hi - (hi >> 7));
 }
 
-// With `derive` set, every active vertex's candidates carry its own label
-// (true once LCC0 has run: later steps only remove candidates), so the code
-// of a neighbour u is that of any template vertex in C(u) and is read from a
-// table in LDS instead of gathering code[u].
-template <typename Word, int G, int KW>
-__device__ void lcc_phase1_bin(uint32_t block, uint32_t nblocks, const LccTables<KW>& tab,
-                               const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
-                               const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
-                               const uint8_t* __restrict__ code, bool derive, const Word* __restrict__ cand,
-                               Word* __restrict__ unmatch, uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty,
-                               uint8_t* __restrict__ rescan, uint32_t* __restrict__ ea,
-                               const DevTemplate* __restrict__ t, unsigned long long* counters) {
+// LCC phase 1 for one vertex per group of G lanes (G = 1, 4, 32 or 64), each
+// lane owning S slots of a pass: slot base + k*G + gl, k < S, so that slice
+// k of the pass is G consecutive slots.  The ids and edge-active words of a
+// whole pass are requested before any neighbour gather, and the gathers
+// before any use, so a pass waits on two rounds of memory latency and rows
+// of at most G*S slots take one pass.  The vertex's own loads (candidates,
+// row bounds, code, flags) are issued together before the row.  Each lane
+// counts neighbour label codes in packed bytes (at most 2048 / G slots per
+// lane, below 128); the group sums them with shuffles and saturating byte
+// adds (cap 127, above every requirement).  With `derive` set, every active
+// vertex's candidates carry its own label (true once LCC0 has run: later
+// steps only remove candidates), so a neighbour's code is that of any of its
+// candidates, read from the LDS table instead of gathering code[u].
+template <typename Word, int G, int S, int KW>
+__device__ void lcc_phase1_lanes(uint32_t block, uint32_t nblocks, const LccTables<KW>& tab,
+                                 const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
+                                 const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+                                 const uint8_t* __restrict__ code, bool derive, const Word* __restrict__ cand,
+                                 Word* __restrict__ unmatch, uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty,
+                                 uint8_t* __restrict__ rescan, uint32_t* __restrict__ ea,
+                                 const DevTemplate* __restrict__ t, unsigned long long* counters) {
   const int lane = lane_id();
   const int gl = lane & (G - 1);
   const int gbase = lane & ~(G - 1);
   const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
   const uint32_t n = *list_count;
-  const uint32_t K = t->K;
   const uint64_t groups_per_grid = (uint64_t)nblocks * (blockDim.x / G);
+  // every lane runs the same number of iterations (shuffles and ballots)
   const uint64_t span = ((n + groups_per_grid - 1) / groups_per_grid) * groups_per_grid;
   uint64_t elim_total = 0, slots_total = 0, xslots = 0;
   for (uint64_t g = (block * (uint64_t)blockDim.x + threadIdx.x) / G; g < span; g += groups_per_grid) {
     if (g < n) {
       const uint32_t v = list[g];
       const Word cv = cand[v];
+      const uint64_t beg = off[v], end = off[v + 1];
+      const uint8_t vcode = code[v];
+      const uint8_t f = flags[v];
       Word U = 0;
       for (Word m = cv; m; m &= m - 1) U |= static_cast<Word>(tab.nbr[ctz<Word>(m)]);
-      const uint64_t beg = off[v], end = off[v + 1];
       uint64_t cnt[KW];
 #pragma unroll
       for (int w = 0; w < KW; ++w) cnt[w] = 0;
       uint32_t elim = 0;
-      // kUnroll slices of G slots per pass: the edge-active words and ids of
-      // all slices are loaded first, then all neighbour gathers, so each pass
-      // waits on two rounds of memory latency.
-      constexpr int kUnroll = G >= 32 ? 4 : G == 16 ? 2 : 1;
-      for (uint64_t base = beg; base < end; base += static_cast<uint64_t>(G) * kUnroll) {
-        uint32_t word[kUnroll], u[kUnroll];
+      for (uint64_t base = beg; base < end; base += static_cast<uint64_t>(G) * S) {
+        uint32_t u[S];
+        uint32_t act = 0;  // bit k: slot k of this lane exists and is active
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k) {
+        for (int k = 0; k < S; ++k) {
           const uint64_t e = base + k * G + gl;
-          word[k] = e < end ? ea[e >> 5] : 0u;
           u[k] = e < end ? edges[e] : 0u;
+          const uint32_t word = e < end ? ea[e >> 5] : 0u;
+          act |= ((word >> (e & 31)) & 1u) << k;
         }
-        Word cu[kUnroll];
-        uint8_t lb[kUnroll];
+        Word cu[S];
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k) {
-          const uint64_t e = base + k * G + gl;
-          const bool act = (word[k] >> (e & 31)) & 1u;
-          cu[k] = act ? cand[u[k]] : Word(0);
-          lb[k] = act && !derive ? code[u[k]] : kNoCode;
-        }
+        for (int k = 0; k < S; ++k) cu[k] = (act >> k) & 1u ? cand[u[k]] : Word(0);
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k) {
-          bool counted = false, el = false;
-          uint32_t lab = kNoCode;
+        for (int k = 0; k < S; ++k) {
+          bool el = false;
           if (cu[k]) {
             if ((cu[k] & U) == 0) {
               el = true;
               ++elim;
             } else {
-              lab = derive ? tab.pcode[ctz<Word>(cu[k])] : lb[k];
-              counted = lab != kNoCode;
+              const uint32_t lab = derive ? tab.pcode[ctz<Word>(cu[k])] : code[u[k]];
+              if (lab != kNoCode) {
+#pragma unroll
+                for (int w = 0; w < KW; ++w)
+                  if (static_cast<int>(lab >> 3) == w) cnt[w] += 1ull << (8 * (lab & 7));
+              }
             }
           }
-          clear_slots<G>(ea, base + k * G, el, gl, gbase);
-          count_labels<KW>(cnt, counted, lab, gmask, K);
+          if (G == 1) {
+            const uint64_t e = base + k;
+            if (el) atomicAnd(&ea[e >> 5], ~(1u << (e & 31)));
+          } else {
+            clear_slots<G>(ea, base + k * G, el, gl, gbase);
+          }
         }
       }
-      const bool any_elim = (__ballot(elim != 0) & gmask) != 0;
+#pragma unroll
+      for (int o = 1; o < G; o <<= 1)
+#pragma unroll
+        for (int w = 0; w < KW; ++w) cnt[w] = sat_add_bytes(cnt[w], __shfl_xor(cnt[w], o, 64));
+      const bool any_elim = G == 1 ? elim != 0 : (__ballot(elim != 0) & gmask) != 0;
       elim_total += elim;
       // every lane of the group holds the same counts, so each can evaluate
-      const Word fail = lcc_requirements<Word, KW, KW>(cv, code[v], cnt, tab.pcode, tab.need);
-      const uint8_t f = flags[v];
+      const Word fail = lcc_requirements<Word, KW, KW>(cv, vcode, cnt, tab.pcode, tab.need);
       if (gl == 0) {
         slots_total += end - beg;
         if (fail) unmatch[v] = fail;
@@ -753,8 +791,19 @@ __device__ void lcc_phase1_bin(uint32_t block, uint32_t nblocks, const LccTables
         if (dirty) dirty[v] = 0;
       }
       if (rescan && (fail || any_elim) && !(f & kModified)) {
-        // v turns modified: the phase 2 scans of its neighbours can change
-        for (uint64_t e = beg + gl; e < end; e += G) rescan[edges[e]] = 1;
+        // v turns modified: the phase 2 scans of its neighbours can change.
+        // Eight ids per lane are requested before their marks are stored.
+        for (uint64_t e0 = beg + gl; e0 < end; e0 += static_cast<uint64_t>(G) * 8) {
+          uint32_t x[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const uint64_t e = e0 + static_cast<uint64_t>(k) * G;
+            x[k] = e < end ? edges[e] : 0u;
+          }
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            if (e0 + static_cast<uint64_t>(k) * G < end) rescan[x[k]] = 1;
+        }
         if (gl == 0) xslots += end - beg;
       }
     }
@@ -769,7 +818,7 @@ __device__ void lcc_phase1_bin(uint32_t block, uint32_t nblocks, const LccTables
 // Label counts and eliminations are summed in hub_count[h][code] and
 // hub_elim[h]; k_lcc_hub_finish evaluates the requirements and clears them.
 
-// `derive`: neighbour codes from candidates, as in lcc_phase1_bin.
+// `derive`: neighbour codes from candidates, as in lcc_phase1_lanes.
 template <typename Word>
 __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* s_pcode,
                                const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start,
@@ -903,8 +952,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
                        cand, flags, dirty, ea, hub_count, hub_elim, t, counters);
 }
 
-// LCC phase 1 of the degree bins in one launch: G = 64, 16 and 4 lanes per
-// vertex, each bin grid-strided over the whole grid.
+// LCC phase 1 of the degree bins in one launch, each bin grid-strided over
+// the whole grid.
 template <typename Word, int KW>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW <= 2 ? 8 : 1))) k_lcc_phase1_all(
     const uint32_t* __restrict__ bins, const uint32_t* __restrict__ bin_count, uint32_t V,
@@ -919,14 +968,19 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW 
     atomicAdd(&counters[C_ITEMS],
               static_cast<unsigned long long>(bin_count[0]) + bin_count[1] + bin_count[2] + bin_count[3]);
   const uint64_t V64 = V;
-  lcc_phase1_bin<Word, 64, KW>(blockIdx.x, gridDim.x, tab, bins + 3 * V64, bin_count + 3, off, edges, code,
-                               derive, cand, unmatch, flags, dirty, rescan, ea, t, counters);
-  lcc_phase1_bin<Word, 32, KW>(blockIdx.x, gridDim.x, tab, bins + 2 * V64, bin_count + 2, off, edges, code,
-                               derive, cand, unmatch, flags, dirty, rescan, ea, t, counters);
-  lcc_phase1_bin<Word, 16, KW>(blockIdx.x, gridDim.x, tab, bins + V64, bin_count + 1, off, edges, code, derive,
-                               cand, unmatch, flags, dirty, rescan, ea, t, counters);
-  lcc_phase1_bin<Word, 4, KW>(blockIdx.x, gridDim.x, tab, bins, bin_count + 0, off, edges, code, derive, cand,
-                              unmatch, flags, dirty, rescan, ea, t, counters);
+  // bins: 257-2048 slots (64 lanes x 4 slots per pass), 33-256 (32 x 4),
+  // 5-32 (4 x 4), 1-4 (a lane per vertex).  Four slots per lane keep the
+  // kernel within 64 VGPRs (8 waves per SIMD) without spills; eight per lane
+  // needed 82 (5 waves) and measured slower (1.05 against 1.01 ms per
+  // scale-22 solve).
+  lcc_phase1_lanes<Word, 64, 4, KW>(blockIdx.x, gridDim.x, tab, bins + 3 * V64, bin_count + 3, off, edges, code,
+                                    derive, cand, unmatch, flags, dirty, rescan, ea, t, counters);
+  lcc_phase1_lanes<Word, 32, 4, KW>(blockIdx.x, gridDim.x, tab, bins + 2 * V64, bin_count + 2, off, edges, code,
+                                    derive, cand, unmatch, flags, dirty, rescan, ea, t, counters);
+  lcc_phase1_lanes<Word, 4, 4, KW>(blockIdx.x, gridDim.x, tab, bins + V64, bin_count + 1, off, edges, code, derive,
+                                   cand, unmatch, flags, dirty, rescan, ea, t, counters);
+  lcc_phase1_lanes<Word, 1, 4, KW>(blockIdx.x, gridDim.x, tab, bins, bin_count + 0, off, edges, code, derive, cand,
+                                   unmatch, flags, dirty, rescan, ea, t, counters);
 }
 
 template <typename Word, int KW>
@@ -2999,13 +3053,13 @@ void launch_bins(pj_ctx* ctx, int mode) {
   zero_bins(ctx);
   unsigned long long* c = begin_launch(ctx, K_BINS);
   const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(ctx->n_live) + kBinVerts - 1) / kBinVerts);
-  const uint8_t* marks = !ctx->skip_clean  ? nullptr
-                         : mode == 0       ? ctx->d_dirty
-                         : mode == 3 || mode == 4 ? ctx->d_rescan
-                                                  : nullptr;
+  uint8_t* marks = !ctx->skip_clean  ? nullptr
+                   : mode == 0       ? ctx->d_dirty
+                   : mode == 3 || mode == 4 ? ctx->d_rescan
+                                            : nullptr;
   hipLaunchKernelGGL(k_build_bins<Word>, dim3(std::max<uint32_t>(blocks, 1)), dim3(kBlock), 0, ctx->stream,
                      ctx->live, ctx->n_live, static_cast<const Word*>(ctx->d_cand), ctx->d_flags, ctx->d_code, marks,
-                     static_cast<const Word*>(ctx->d_unmatch), ctx->d_off, ctx->V, mode, ctx->d_t, ctx->d_bins,
+                     static_cast<Word*>(ctx->d_unmatch), ctx->d_off, ctx->V, mode, ctx->d_t, ctx->d_bins,
                      ctx->d_bin_count, ctx->d_last_scan, mode == 6 ? static_cast<const uint32_t*>(ctx->d_u64) : nullptr,
                      c);
   end_launch(ctx);
@@ -3707,8 +3761,17 @@ __global__ void k_state_checksum(const Word* __restrict__ cand, const uint8_t* _
   }
   for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < words; w += stride)
     acc += mix64((w + 0xD1B54A32D192ED03ull) * 0x94D049BB133111EBull ^ ea[w]);
+  // wave sums, then one atomic per block (sums mod 2^64 do not depend on the
+  // order; one atomic per wave on a single address serialised ~16 K of them)
+  __shared__ uint64_t s_acc[kBlock / 64];
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-  if (lane_id() == 0) atomicAdd(out, static_cast<unsigned long long>(acc));
+  if (lane_id() == 0) s_acc[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t b = 0;
+    for (int w = 0; w < kBlock / 64; ++w) b += s_acc[w];
+    atomicAdd(out, static_cast<unsigned long long>(b));
+  }
 }
 
 template <typename Word>
@@ -3788,8 +3851,12 @@ int pj_create(pj_ctx** out, int device) {
   return 0;
 }
 
-static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* offsets, const uint32_t* edges,
-                         const uint64_t* labels, hipMemcpyKind kind) {
+// A graph upload: upload_begin allocates the CSR and the state of V vertices
+// and E slots, the caller fills d_off, d_edges and d_labels on ctx->stream,
+// and upload_finish checks the CSR (offsets on the host, neighbour ids in
+// the row-order pass), builds the hub chunk list and runs the order and
+// symmetry checks.
+static int upload_begin(pj_ctx* ctx, uint32_t V, uint64_t E) {
   PJ_HIP(hipSetDevice(ctx->device));
   free_graph(ctx);
   ctx->V = V;
@@ -3798,6 +3865,14 @@ static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* of
   if (dev_alloc(ctx, &ctx->d_off, V + 1ull) || dev_alloc(ctx, &ctx->d_edges, E) || dev_alloc(ctx, &ctx->d_labels, V) ||
       alloc_state(ctx))
     return -1;
+  return 0;
+}
+
+static int upload_finish(pj_ctx* ctx, const uint64_t* host_off);
+
+static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* offsets, const uint32_t* edges,
+                         const uint64_t* labels, hipMemcpyKind kind) {
+  if (upload_begin(ctx, V, E)) return -1;
   PJ_HIP(hipMemcpyAsync(ctx->d_off, offsets, (V + 1ull) * 8, kind, ctx->stream));
   if (E) PJ_HIP(hipMemcpyAsync(ctx->d_edges, edges, E * 4, kind, ctx->stream));
   if (V) PJ_HIP(hipMemcpyAsync(ctx->d_labels, labels, V * 8ull, kind, ctx->stream));
@@ -3809,6 +3884,12 @@ static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* of
     PJ_HIP(hipStreamSynchronize(ctx->stream));
     host_off = h_off.data();
   }
+  return upload_finish(ctx, host_off);
+}
+
+static int upload_finish(pj_ctx* ctx, const uint64_t* host_off) {
+  const uint32_t V = ctx->V;
+  const uint64_t E = ctx->E;
   // Every kernel indexes off[v + 1] and cand[edges[e]] unchecked: refuse a
   // CSR whose offsets decrease or leave [0, E] (here) or whose ids are not
   // vertices (bit 2 of the row-order check below).
@@ -3881,6 +3962,118 @@ int pj_upload_graph_device(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* 
                            const uint64_t* labels) {
   if (!ctx || !offsets || (E && !edges) || (V && !labels)) return -1;
   return upload_common(ctx, V, E, offsets, edges, labels, hipMemcpyDeviceToDevice);
+}
+
+// Binary TOTEM file (graph_store_binary, src/totem/totem_graph.cu:883-923;
+// read back by :253-312): magic, id widths, counts, flags, offsets of 32 or
+// 64 bits, vertex values when valued, neighbour ids.  The neighbour ids go
+// to HBM in chunks through two pinned host buffers: the read of one chunk
+// overlaps the copy of the other, and no host array holds the edge list.
+// Offsets (widened to 64 bits) and values are read whole: 8 bytes per vertex.
+int pj_upload_totem_file(pj_ctx* ctx, const char* path, const uint64_t* labels) {
+  if (!ctx || !path) return -1;
+  struct FileCloser {
+    FILE* f;
+    ~FileCloser() {
+      if (f) fclose(f);
+    }
+  } fc{fopen(path, "rb")};
+  FILE* f = fc.f;
+  if (!f) {
+    ctx->err = std::string("cannot open ") + path;
+    return -1;
+  }
+  uint32_t head[4] = {0, 0, 0, 0};  // magic, vid bytes, eid bytes, vertex count
+  uint64_t E = 0;
+  uint8_t fl[3] = {0, 0, 0};        // valued, weighted, directed
+  bool ok = fread(head, 4, 4, f) == 4 && head[0] == 0x10102048u && head[1] == 4 && (head[2] == 4 || head[2] == 8);
+  if (ok) {
+    if (head[2] == 4) {
+      uint32_t e32 = 0;
+      ok = fread(&e32, 4, 1, f) == 1;
+      E = e32;
+    } else {
+      ok = fread(&E, 8, 1, f) == 1;
+    }
+  }
+  ok = ok && fread(fl, 1, 3, f) == 3;
+  if (!ok) {
+    ctx->err = std::string("not a binary TOTEM graph: ") + path;
+    return -1;
+  }
+  const uint32_t V = head[3];
+  if (!labels && !fl[0]) {
+    ctx->err = std::string("graph file has no vertex values and no labels were given: ") + path;
+    return -1;
+  }
+  std::vector<uint64_t> off(V + 1ull);
+  if (head[2] == 8) {
+    ok = fread(off.data(), 8, V + 1ull, f) == V + 1ull;
+  } else {
+    std::vector<uint32_t> o32(V + 1ull);
+    ok = fread(o32.data(), 4, V + 1ull, f) == V + 1ull;
+    for (uint64_t i = 0; ok && i <= V; ++i) off[i] = o32[i];
+  }
+  std::vector<uint64_t> values;
+  if (ok && fl[0]) {
+    if (labels) {
+      ok = fseeko(f, static_cast<off_t>(V) * 8, SEEK_CUR) == 0;
+    } else {
+      values.resize(V);
+      ok = fread(values.data(), 8, V, f) == V;
+    }
+  }
+  if (!ok) {
+    ctx->err = std::string("truncated binary graph: ") + path;
+    return -1;
+  }
+  if (upload_begin(ctx, V, E)) return -1;
+  PJ_HIP(hipMemcpyAsync(ctx->d_off, off.data(), (V + 1ull) * 8, hipMemcpyHostToDevice, ctx->stream));
+  if (V)
+    PJ_HIP(hipMemcpyAsync(ctx->d_labels, labels ? labels : values.data(), V * 8ull, hipMemcpyHostToDevice, ctx->stream));
+  constexpr uint64_t kChunk = 16ull << 20;  // ids per chunk (64 MiB)
+  uint32_t* buf[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};
+  int rc = 0;
+  for (int b = 0; b < 2 && !rc; ++b) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&buf[b]), kChunk * 4, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&done[b], hipEventDisableTiming) != hipSuccess) {
+      ctx->err = "pinned staging buffers for the edge list";
+      rc = -1;
+    }
+  }
+  for (uint64_t pos = 0, k = 0; !rc && pos < E; pos += kChunk, ++k) {
+    const int b = static_cast<int>(k & 1);
+    const uint64_t n = std::min<uint64_t>(kChunk, E - pos);
+    if (k >= 2 && hipEventSynchronize(done[b]) != hipSuccess) {  // its previous copy has left the buffer
+      ctx->err = "edge list copy";
+      rc = -1;
+      break;
+    }
+    if (fread(buf[b], 4, n, f) != n) {
+      ctx->err = std::string("truncated binary graph: ") + path;
+      rc = -1;
+      break;
+    }
+    if (hipMemcpyAsync(ctx->d_edges + pos, buf[b], n * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipEventRecord(done[b], ctx->stream) != hipSuccess) {
+      ctx->err = "edge list copy";
+      rc = -1;
+    }
+  }
+  hipStreamSynchronize(ctx->stream);
+  for (int b = 0; b < 2; ++b) {
+    if (buf[b]) hipHostFree(buf[b]);
+    if (done[b]) hipEventDestroy(done[b]);
+  }
+  if (rc) {
+    free_graph(ctx);
+    ctx->V = 0;
+    ctx->E = 0;
+    ctx->ea_words = 0;
+    return -1;
+  }
+  return upload_finish(ctx, off.data());
 }
 
 int pj_share_graph(pj_ctx* ctx, const pj_ctx* owner) {

@@ -352,7 +352,17 @@ int MultipleLabelHip::allocate(const Options& options) {
   std::ofstream(options_.outputDir + "log.totem", std::ios::trunc).close();
   std::string err;
   log("Loading graph", true);
-  if (is_file(options_.graphBin)) {
+  // A binary cache streams into HBM (pj_upload_totem_file); the host keeps
+  // its counts and vertex values only.
+  bool streamed = false;
+  int kind = is_file(options_.graphBin) ? load_totem_binary_values(options_.graphBin, &graph_, &err) : 1;
+  if (kind < 0) {
+    error_ = err;
+    return -1;
+  }
+  if (kind == 0) {
+    streamed = true;
+  } else if (is_file(options_.graphBin)) {
     if (load_totem(options_.graphBin, &graph_, &err)) {
       error_ = err;
       return -1;
@@ -399,8 +409,10 @@ int MultipleLabelHip::allocate(const Options& options) {
     error_ = "pj_create failed";
     return -1;
   }
-  if (pj_upload_graph(state_.ctx, graph_.vertex_count, graph_.edge_count, graph_.vertices.data(),
-                      graph_.edges.data(), graph_.values.data()) ||
+  const int up = streamed ? pj_upload_totem_file(state_.ctx, options_.graphBin.c_str(), graph_.values.data())
+                          : pj_upload_graph(state_.ctx, graph_.vertex_count, graph_.edge_count,
+                                            graph_.vertices.data(), graph_.edges.data(), graph_.values.data());
+  if (up ||
       pj_set_template(state_.ctx, constraints_.desc())) {
     error_ = pj_last_error(state_.ctx);
     return -1;

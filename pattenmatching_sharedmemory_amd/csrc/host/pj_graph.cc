@@ -228,6 +228,43 @@ bad:
   return -1;
 }
 
+int load_totem_binary_values(const std::string& path, Graph* g, std::string* err) {
+  FilePtr f(fopen(path.c_str(), "rb"));
+  if (!f) {
+    *err = "cannot open " + path;
+    return -1;
+  }
+  uint32_t head[4] = {0, 0, 0, 0};  // magic, vid bytes, eid bytes, vertex count
+  if (fread(head, 4, 1, f.get()) != 1 || head[0] != kBinaryMagic) return 1;
+  Graph out;
+  uint64_t e = 0;
+  uint8_t flags[3];
+  bool ok = fread(head + 1, 4, 3, f.get()) == 3 && head[1] == 4 && (head[2] == 4 || head[2] == 8);
+  if (ok && head[2] == 4) {
+    uint32_t e32 = 0;
+    ok = fread(&e32, 4, 1, f.get()) == 1;
+    e = e32;
+  } else if (ok) {
+    ok = fread(&e, 8, 1, f.get()) == 1;
+  }
+  ok = ok && fread(flags, 1, 3, f.get()) == 3;
+  out.vertex_count = head[3];
+  out.edge_count = e;
+  out.valued = ok && flags[0] != 0;
+  out.directed = ok && flags[2] != 0;
+  if (ok && out.valued) {
+    ok = fseeko(f.get(), static_cast<off_t>(head[2]) * (out.vertex_count + 1ull), SEEK_CUR) == 0;
+    out.values.resize(out.vertex_count);
+    ok = ok && fread(out.values.data(), 8, out.vertex_count, f.get()) == out.vertex_count;
+  }
+  if (!ok) {
+    *err = "truncated binary graph " + path;
+    return -1;
+  }
+  *g = std::move(out);
+  return 0;
+}
+
 int validate_csr(const Graph& g, std::string* err) {
   const uint64_t V = g.vertex_count, E = g.edge_count;
   if (g.vertices.size() != V + 1 || g.edges.size() != E || g.vertices[0] != 0 || g.vertices[V] != E) {

@@ -30,6 +30,12 @@ struct Graph {
 // Edge weights are skipped.  Returns 0 / -1 (message in *err).
 int load_totem(const std::string& path, Graph* g, std::string* err);
 
+// Header and vertex values of a binary TOTEM file, without its offsets and
+// edge list (for pj_upload_totem_file, which streams those to the device):
+// vertex_count, edge_count, valued and values are set, vertices and edges
+// stay empty.  Returns 1 when `path` is not a binary TOTEM file, 0 / -1.
+int load_totem_binary_values(const std::string& path, Graph* g, std::string* err);
+
 // CSR well-formedness: offsets start at 0, never decrease, end at the edge
 // count, and every neighbour id is a vertex.  Every graph the loaders and
 // pjh_graph_from_csr return has passed it (kernels index cand[edges[e]] and

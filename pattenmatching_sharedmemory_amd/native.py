@@ -112,6 +112,7 @@ HIP_SIGNATURES = {
     "pj_share_graph": (C.c_int, [P, P]),
     "pj_get_device": (C.c_int, [P, C.POINTER(C.c_int)]),
     "pj_last_error": (C.c_char_p, [P]),
+    "pj_upload_totem_file": (C.c_int, [P, C.c_char_p, u64p]),
     "pj_compact_active": (C.c_int, [P, P, P, C.c_uint64, P, P, C.c_uint64, u64p, u64p]),
     "pj_compact_active_noalloc": (C.c_int, [P, P, P, C.c_uint64, P, P, C.c_uint64, u64p, u64p]),
     "pj_download_active": (C.c_int, [P, P, P, C.c_uint64, P, P, C.c_uint64, u64p, u64p]),

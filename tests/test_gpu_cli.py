@@ -55,3 +55,27 @@ def test_cli_result_files(pj, orc, name, tmp_path):
     gl = (out / f"graph_{last:04d}.totem").read_text().split("\n")
     assert gl[: len(expect_g)] == expect_g
     assert "Circular constraint number" in (out / "log.totem").read_text()
+
+
+@pytest.mark.parametrize("name", ["test_3", "test_4"])
+def test_cli_second_run_streams_the_binary_cache(pj, name, tmp_path):
+    """A second run finds the binary cache the first wrote and streams it
+    into HBM (pj_upload_totem_file); its result files equal the first's
+    apart from the timing columns."""
+    f = fixture_paths(name)
+    graph = tmp_path / "graph.totem"
+    shutil.copy(f["graph"], graph)
+    outs = []
+    for run in range(2):
+        out = tmp_path / f"out{run}"
+        res = subprocess.run([str(pj.native.CLI), "-i", str(graph), "-v", f["vertex"], "-p", f["pattern_dir"] + "/",
+                              "-o", str(out)], capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, res.stderr[-2000:]
+        assert ("Saved graph as" in res.stdout) == (run == 0)
+        outs.append(out)
+    strip = lambda l: [x.strip() for i, x in enumerate(l.split(";")) if i not in (2, 3)]
+    a = [strip(l) for l in (outs[0] / "results.totem").read_text().splitlines()]
+    b = [strip(l) for l in (outs[1] / "results.totem").read_text().splitlines()]
+    assert a == b
+    for dump in sorted(p.name for p in outs[0].iterdir() if p.name.startswith(("graph_", "vertex_"))):
+        assert (outs[0] / dump).read_text() == (outs[1] / dump).read_text(), dump
