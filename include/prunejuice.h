@@ -261,6 +261,46 @@ int pj_compact_active_noalloc(pj_ctx* ctx, uint32_t* vertices, uint64_t* cand, u
 int pj_download_active(pj_ctx* ctx, uint32_t* vertices, uint64_t* cand, uint64_t cap_vertices, uint32_t* src,
                        uint32_t* dst, uint64_t cap_slots, uint64_t* n_vertices, uint64_t* n_slots);
 
+/* Label statistics of the active graph: the reference's GraphStat
+ * (src/patternmatching/algorithms/graph_stat.h:14-45) as GraphStatCpu::compute
+ * fills it (src/patternmatching/algorithms/graph_stat_cpu.tpp:40-82), for the
+ * effectiveness estimate of the dynamic constraint order
+ * (src/patternmatching/test_article/test_order_multiple_label_cpu.cc:170-290).
+ * Labels are the installed template's distinct labels in template vertex
+ * order (codes 0..label_count-1); tables over label pairs are indexed
+ * [from * 64 + to].  Counted over active vertices and over active slots whose
+ * both ends are active:
+ *   vertex[l]     vertexLabelTotalNumberMap
+ *   edge[f][t]    edgeLabelTotalNumberMap
+ *   outbound[f]   edgeOutboundLabelTotalNumberMap (sum over t of edge[f][t])
+ *   vertex_total, edge_total   vertexTotalNumber, edgeTotalNumber
+ *   at_least_max[f][t]  accumulationMaximumMap (GraphStatCpu::preprocessPatern,
+ *                 graph_stat_cpu.tpp:14-38): the most neighbours of label t
+ *                 that a template vertex of label f has (0: no entry)
+ *   at_least[at_least_offset[f][t] + k - 1], k = 1..at_least_max[f][t]:
+ *                 vertexLabelWithAtLeastNeighborLabelMap[f][t][k], the active
+ *                 vertices of label f with at least k such slots to label t.
+ * The last counts each vertex's own slots; the reference's per-thread tally
+ * is never cleared between vertices (graph_stat_cpu.tpp:51-70), so its
+ * counts also depend on the thread schedule.  pj_last_step_ms reports the
+ * device time of the pass.  Needs the pruning state of the template
+ * (pj_set_template). */
+#define PJ_STAT_MAX_AT_LEAST 4096
+typedef struct pj_label_stat {
+  uint32_t label_count;
+  uint32_t at_least_count;
+  uint64_t vertex_total;
+  uint64_t edge_total;
+  uint64_t label[64];
+  uint64_t vertex[64];
+  uint64_t outbound[64];
+  uint64_t edge[64 * 64];
+  uint32_t at_least_max[64 * 64];
+  uint32_t at_least_offset[64 * 64];
+  uint64_t at_least[PJ_STAT_MAX_AT_LEAST];
+} pj_label_stat;
+int pj_graph_stat(pj_ctx* ctx, pj_label_stat* out);
+
 /* Last error message of the context (never NULL). */
 const char* pj_last_error(pj_ctx* ctx);
 

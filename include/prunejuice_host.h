@@ -73,6 +73,10 @@ typedef struct pjh_row {
   uint64_t active_vertices, active_edges;
   uint64_t slots; /* CSR slots examined (pj_last_step_slots) */
   double ms;      /* device time of the step (HIP events) */
+  int32_t constraint;   /* cycle / path / TDS constraint the step ran, else -1 */
+  int32_t reserved;
+  double effectiveness; /* its estimate, when the dynamic order chose it */
+  double stat_ms;       /* device time of the label statistics taken after the step (dynamic order) */
 } pjh_row;
 
 /* Runs the step machine on a context that holds the graph and the
@@ -85,6 +89,13 @@ typedef struct pjh_row {
  * measure the other phases on graphs where the memoised TDS search of the
  * reference does not finish in reasonable time). */
 #define PJH_STOP_BEFORE_TDS 2
+/* With PJH_ORDER_DYNAMIC the constraints run in the order of the reference's
+ * effectiveness estimate (test_article/test_order_multiple_label_cpu.cc:
+ * 170-290): at each LCC fixed point the label statistics of the active graph
+ * (pj_graph_stat) score every cycle, path and TDS constraint not yet run and
+ * the best runs next, once.  The cost divisor of the estimate (the
+ * reference's omp_get_max_threads()) is 1; it does not change the order. */
+#define PJH_ORDER_DYNAMIC 4
 int pjh_run_pipeline(pj_ctx* ctx, const pjh_constraints* c, int flags, pjh_row* rows, int max_rows,
                      double* total_ms);
 
@@ -126,6 +137,27 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
 uint64_t pjh_projected_bytes(const pjh_constraints* c, uint32_t vertex_count, uint64_t edge_count);
 /* 1 when a query of these constraints allocates memo tables (see above). */
 int pjh_query_uses_memo(const pjh_constraints* c);
+
+/* Effectiveness estimate ----------------------------------------------------
+ * MultipleLabelConstraintEffectiveness::computeBacktrack
+ * (src/patternmatching/multiple_label/multiple_label_constraint_effectiveness.cc:
+ * 226-340 cycle, 570-696 path, 972-1118 TDS) of constraint `index` of kind
+ * `step` (PJ_STEP_CC / PJ_STEP_PC / PJ_STEP_TDS) from label statistics of
+ * the active graph (pj_graph_stat): out[0] approximateMatchEliminated,
+ * out[1] approximateCostEliminated, out[2] approximateEffectiveness.
+ * `threads` is the reference's omp_get_max_threads() cost divisor.  A label
+ * or label pair without statistics reads 0 (the reference's map lookups
+ * throw there).  Host arithmetic only. */
+int pjh_constraint_effectiveness(const pj_label_stat* stat, const pjh_constraints* c, int step, uint32_t index,
+                                 int threads, double out[3]);
+/* The dynamic order's choice (src/patternmatching/test_article/
+ * test_order_multiple_label_cpu.cc:184-274) among the constraints whose
+ * done_* flag is 0 (NULL: none done): the greatest effectiveness, ties to
+ * the earlier constraint and to cycles, then paths, then TDS; *step is
+ * PJ_STEP_ENUMERATION when none is left. */
+int pjh_next_constraint(const pj_label_stat* stat, const pjh_constraints* c, const uint8_t* done_cycles,
+                        const uint8_t* done_paths, const uint8_t* done_tds, int threads, int* step, uint32_t* index,
+                        double* effectiveness);
 
 /* patternmatching_binary: -i graph -b binary-cache -v vertex-data -p pattern-dir
  * -o output-dir [-d device]; writes results.totem, graph_%04d.totem,

@@ -207,6 +207,82 @@ class Constraints:
     def memo_inert(self, step: int, index: int) -> int:
         return native.host().pjh_constraints_memo_inert(self._h, step, index)
 
+    def effectiveness(self, stat: "LabelStat", step: int, index: int, threads: int = 1) -> tuple[float, float, float]:
+        """(approximateMatchEliminated, approximateCostEliminated,
+        approximateEffectiveness) of one cycle / path / TDS constraint
+        (pjh_constraint_effectiveness)."""
+        out = (C.c_double * 3)()
+        if native.host().pjh_constraint_effectiveness(C.byref(stat.desc), self._h, step, index, threads, out):
+            raise ValueError(f"no constraint {index} of step {step}")
+        return out[0], out[1], out[2]
+
+    def next_constraint(self, stat: "LabelStat", done_cycles=None, done_paths=None, done_tds=None,
+                        threads: int = 1) -> tuple[int, int, float]:
+        """The dynamic order's choice among the constraints not done
+        (pjh_next_constraint): (step, index, effectiveness)."""
+        def flags(done, n):
+            if done is None:
+                return None
+            a = np.zeros(max(n, 1), dtype=np.uint8)
+            for i in done:
+                a[i] = 1
+            return _ptr(a, C.c_uint8), a
+
+        keep = [flags(done_cycles, len(self.cycles())), flags(done_paths, len(self.paths())),
+                flags(done_tds, len(self.tds()))]
+        step, index, eff = C.c_int(), C.c_uint32(), C.c_double()
+        if native.host().pjh_next_constraint(C.byref(stat.desc), self._h, *[k[0] if k else None for k in keep],
+                                             threads, C.byref(step), C.byref(index), C.byref(eff)):
+            raise ValueError("pjh_next_constraint failed")
+        return step.value, index.value, eff.value
+
+
+class LabelStat:
+    """pj_label_stat as arrays: labels (template label per code), vertex[K],
+    edge[K, K], outbound[K], at_least_max[K, K] and at_least[(f, t)] =
+    counts for k = 1..at_least_max[f, t] (codes f, t)."""
+
+    def __init__(self, desc: native.LabelStatDesc):
+        self.desc = desc
+        K = desc.label_count
+        self.labels = [desc.label[k] for k in range(K)]
+        self.vertex = np.array(desc.vertex[:K], dtype=np.uint64)
+        self.outbound = np.array(desc.outbound[:K], dtype=np.uint64)
+        e = np.ctypeslib.as_array(desc.edge).reshape(64, 64)
+        self.edge = e[:K, :K].astype(np.uint64)
+        mx = np.ctypeslib.as_array(desc.at_least_max).reshape(64, 64)
+        self.at_least_max = mx[:K, :K].astype(np.uint32)
+        off = np.ctypeslib.as_array(desc.at_least_offset).reshape(64, 64)
+        al = np.ctypeslib.as_array(desc.at_least)
+        self.at_least = {(f, t): al[off[f, t]: off[f, t] + mx[f, t]].astype(np.uint64).copy()
+                         for f in range(K) for t in range(K) if mx[f, t]}
+        self.vertex_total = desc.vertex_total
+        self.edge_total = desc.edge_total
+
+    @classmethod
+    def from_arrays(cls, labels, vertex, edge, at_least_max, at_least: dict) -> "LabelStat":
+        """A pj_label_stat built from host arrays (codes in `labels` order)."""
+        d = native.LabelStatDesc()
+        K = len(labels)
+        d.label_count = K
+        off = 0
+        for f in range(K):
+            d.label[f] = int(labels[f])
+            d.vertex[f] = int(vertex[f])
+            for t in range(K):
+                d.edge[f * 64 + t] = int(edge[f][t])
+                d.outbound[f] += int(edge[f][t])
+                m = int(at_least_max[f][t])
+                d.at_least_max[f * 64 + t] = m
+                d.at_least_offset[f * 64 + t] = off
+                for k in range(m):
+                    d.at_least[off + k] = int(at_least[(f, t)][k])
+                off += m
+            d.vertex_total += int(vertex[f])
+            d.edge_total += d.outbound[f]
+        d.at_least_count = off
+        return cls(d)
+
 
 @dataclass
 class State:
@@ -336,6 +412,12 @@ class Pruner:
                                            d.ctypes.data, ne.value, C.byref(nv), C.byref(ne)))
         return v, c, s, d
 
+    def graph_stat(self) -> LabelStat:
+        """Label statistics of the active graph (pj_graph_stat)."""
+        d = native.LabelStatDesc()
+        self._check(native.hip().pj_graph_stat(self._ctx, C.byref(d)))
+        return LabelStat(d)
+
     def checksum(self) -> int:
         """pj_state_checksum of the current state."""
         h = C.c_uint64()
@@ -370,13 +452,18 @@ class Pruner:
                                                  _ptr(ea, C.c_uint32), st.active_vertices, st.active_edges))
 
     def run(self, max_rows: int = 1000, enumerate_matches: bool = True,
-            run_tds: bool = True) -> tuple[list[Row], float]:
+            run_tds: bool = True, order: str = "static") -> tuple[list[Row], float]:
         """Whole step machine; returns (rows, total device ms).  Without
         enumerate_matches it stops at the pruning fixed point; without run_tds
-        it stops when the first TDS step is due."""
+        it stops when the first TDS step is due.  order="dynamic" runs the
+        constraints in the order of the effectiveness estimate (a statistics
+        pass at every LCC fixed point; rows carry the chosen constraint, its
+        estimate and the pass's device time)."""
+        if order not in ("static", "dynamic"):
+            raise ValueError(f"unknown order {order!r}")
         rows = (Row * max_rows)()
         total = C.c_double()
-        flags = (0 if enumerate_matches else 1) | (0 if run_tds else 2)
+        flags = (0 if enumerate_matches else 1) | (0 if run_tds else 2) | (4 if order == "dynamic" else 0)
         n = native.host().pjh_run_pipeline(self._ctx, self.constraints.handle, flags, rows, max_rows,
                                            C.byref(total))
         if n < 0:
@@ -390,5 +477,5 @@ def run_cli(args: list[str]) -> int:
     return native.host().pjh_main(len(args) + 1, argv)
 
 
-__all__ = ["Graph", "Constraints", "Pruner", "State", "run_cli", "STEP_LCC0", "STEP_LCC", "STEP_CC", "STEP_PC",
+__all__ = ["Graph", "Constraints", "LabelStat", "Pruner", "State", "run_cli", "STEP_LCC0", "STEP_LCC", "STEP_CC", "STEP_PC",
            "STEP_TDS", "STEP_ENUMERATION"]

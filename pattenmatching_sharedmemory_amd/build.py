@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-result"]
 
-HOST_SOURCES = ["pj_graph.cc", "pj_constraints.cc", "pj_driver.cc", "pj_host_capi.cc"]
+HOST_SOURCES = ["pj_graph.cc", "pj_constraints.cc", "pj_effectiveness.cc", "pj_driver.cc", "pj_host_capi.cc"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -52,7 +52,7 @@ def build_hip() -> Path:
     out = LIB / "libprunejuice_hip.so"
     tmp = LIB / "libprunejuice_hip.so.tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", str(CSRC / "hip" / "pj_device.hip"),
-          str(CSRC / "hip" / "pj_compact.hip"), "-o", str(tmp)])
+          str(CSRC / "hip" / "pj_compact.hip"), str(CSRC / "hip" / "pj_stat.hip"), "-o", str(tmp)])
     os.replace(tmp, out)
     return out
 

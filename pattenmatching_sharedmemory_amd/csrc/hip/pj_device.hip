@@ -27,6 +27,7 @@
 
 #include "../../../include/prunejuice.h"
 #include "pj_compact.h"
+#include "pj_stat.h"
 
 namespace {
 
@@ -63,7 +64,7 @@ constexpr int kWalkBlocksPerCu = 16;
 // Kernel kinds reported by pj_kernel_stats.
 enum KernelId {
   K_LCC0, K_BINS, K_LCC_P1, K_LCC_HUB, K_LCC_HUB_FINISH, K_P2, K_P2_HUB_FINISH, K_NLCC, K_NLCC_HUB, K_TRIANGLES,
-  K_TRIANGLES_MEMO, K_MEMO_WALK, K_HUB_PUSH, K_VIEW, K_COUNT
+  K_TRIANGLES_MEMO, K_MEMO_WALK, K_HUB_PUSH, K_VIEW, K_LABEL_STAT, K_COUNT
 };
 // nlcc_walk: k_nlcc (cycle / path / TDS / enumeration walks over the bins,
 // and the memo retry passes); triangles: k_triangles (3-vertex cycles from
@@ -72,7 +73,7 @@ enum KernelId {
 const char* const kKernelNames[K_COUNT] = {
     "lcc0",          "build_bins",        "lcc_phase1", "lcc_phase1_hub_chunks", "lcc_phase1_hub_finish",
     "phase2",        "phase2_hub_finish", "nlcc_walk",  "nlcc_walk_hubs",        "triangles",
-    "triangles_memo", "memo_walk",        "hub_rescan_push", "active_view"};
+    "triangles_memo", "memo_walk",        "hub_rescan_push", "active_view",    "label_stat"};
 
 // Template tables in device memory.
 struct DevTemplate {
@@ -2815,6 +2816,8 @@ struct pj_ctx {
   uint32_t *d_out_v = nullptr, *d_out_src = nullptr, *d_out_dst = nullptr;
   uint64_t* d_out_cand = nullptr;
   uint64_t out_cap_v = 0, out_cap_e = 0;
+  // Label statistics (pj_graph_stat): counts and pair table, pjs::kBufferBytes.
+  unsigned long long* d_stat = nullptr;
 };
 
 namespace {
@@ -2864,7 +2867,8 @@ void free_graph(pj_ctx* ctx) {
                   ctx->d_last_scan, ctx->d_hub_new, ctx->memo1.keys, ctx->memo1.vals, ctx->memo1.epochs,
                   ctx->memo1.wave_epoch, ctx->d_already, ctx->d_ovf, ctx->d_view_count, ctx->d_view_off,
                   ctx->d_view_edges, ctx->d_scan_tmp, ctx->d_rev, ctx->d_live[0], ctx->d_live[1],
-                  ctx->d_compact_scratch, ctx->d_out_v, ctx->d_out_src, ctx->d_out_dst, ctx->d_out_cand};
+                  ctx->d_compact_scratch, ctx->d_out_v, ctx->d_out_src, ctx->d_out_dst, ctx->d_out_cand,
+                  ctx->d_stat};
   for (void* b : bufs)
     if (b) hipFree(b);
   ctx->d_off = nullptr;
@@ -2877,6 +2881,7 @@ void free_graph(pj_ctx* ctx) {
   ctx->d_rev = nullptr;
   ctx->d_live[0] = ctx->d_live[1] = nullptr;
   ctx->d_compact_scratch = nullptr;
+  ctx->d_stat = nullptr;
   ctx->d_out_v = ctx->d_out_src = ctx->d_out_dst = nullptr;
   ctx->d_out_cand = nullptr;
   ctx->out_cap_v = ctx->out_cap_e = 0;
@@ -3807,6 +3812,106 @@ int with_word(const pj_ctx* ctx, F&& f) {
 
 }  // namespace
 
+namespace {
+
+// Label statistics (pjs::label_stat): the pair table from the template's
+// local constraints, the bins of every active vertex, one pass, and the
+// counts turned into the reference's maps.
+template <typename Word>
+int graph_stat_t(pj_ctx* ctx, pj_label_stat* out) {
+  const DevTemplate& t = ctx->h_t;
+  const uint32_t K = t.K;
+  // accumulationMaximumMap (graph_stat_cpu.tpp:14-38): per template vertex
+  // the count of its neighbours of each label, maximised over the vertices
+  // of each label; the local constraints hold those counts.
+  std::vector<uint32_t> mx(pjs::kLabels * pjs::kLabels, 0), tab(pjs::kLabels * pjs::kLabels, 0);
+  for (uint32_t p = 0; p < t.P; ++p)
+    for (uint32_t k = 0; k < K; ++k) {
+      const uint32_t need = static_cast<uint32_t>((t.need[p][k >> 3] >> (8 * (k & 7))) & 0xFFu);
+      uint32_t& m = mx[t.pcode[p] * pjs::kLabels + k];
+      m = std::max(m, need);
+    }
+  uint32_t words = 0;
+  for (uint32_t f = 0; f < K; ++f)
+    for (uint32_t l = 0; l < K; ++l) {
+      const uint32_t m = mx[f * pjs::kLabels + l];
+      if (!m) continue;
+      tab[f * pjs::kLabels + l] = words << 8 | m;
+      words += m;
+    }
+  if (words > pjs::kExactMax) {
+    ctx->err = "pj_graph_stat: template neighbour counts exceed the statistics table";
+    return -1;
+  }
+  if (!ctx->d_stat && dev_alloc(ctx, &ctx->d_stat, pjs::kBufferBytes / 8)) return -1;
+  PJ_HIP(hipMemcpyAsync(ctx->d_stat + pjs::kCountWords, tab.data(), tab.size() * 4, hipMemcpyHostToDevice,
+                        ctx->stream));
+  PJ_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  launch_bins<Word>(ctx, 1);
+  pjs::StatArgs a{};
+  a.off = ctx->d_off;
+  a.edges = ctx->d_edges;
+  a.ea = ctx->d_ea;
+  a.cand = ctx->d_cand;
+  a.word_bytes = static_cast<int>(sizeof(Word));
+  a.code = ctx->d_code;
+  a.V = ctx->V;
+  a.bins = ctx->d_bins;
+  a.bin_count = ctx->d_bin_count;
+  a.hubs = ctx->d_hubs;
+  a.nhubs = ctx->nhubs;
+  a.chunk_hub = ctx->d_chunk_hub;
+  a.chunk_start = ctx->d_chunk_start;
+  a.nchunks = ctx->nhubs ? ctx->nchunks : 0u;
+  a.hub_count = ctx->d_hub_count;
+  a.K = K;
+  a.exact_words = words;
+  a.buffer = ctx->d_stat;
+  a.cus = ctx->cus;
+  a.stream = ctx->stream;
+  begin_launch(ctx, K_LABEL_STAT);
+  if (pjs::label_stat(a, &ctx->err)) return -1;
+  end_launch(ctx);
+  PJ_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  std::vector<unsigned long long> counts(pjs::kCountWords);
+  PJ_HIP(hipMemcpyAsync(counts.data(), ctx->d_stat, counts.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  PJ_HIP(hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
+  ctx->last_slots = 0;
+  for (int b = 0; b < ctx->nlaunch; ++b) {  // kernel timing: the bins and the pass
+    float ms = 0.f;
+    PJ_HIP(hipEventElapsedTime(&ms, ctx->lev[2 * b], ctx->lev[2 * b + 1]));
+    ctx->k_launches[ctx->launch_kid[b]] += 1;
+    ctx->k_ms[ctx->launch_kid[b]] += ms;
+  }
+  std::memset(out, 0, sizeof(*out));
+  out->label_count = K;
+  out->at_least_count = words;
+  for (uint32_t f = 0; f < K; ++f) {
+    out->label[f] = t.label_raw[f];
+    out->vertex[f] = counts[pjs::kVertexAt + f];
+    out->vertex_total += out->vertex[f];
+    for (uint32_t l = 0; l < K; ++l) {
+      const uint32_t i = f * pjs::kLabels + l;
+      out->edge[i] = counts[pjs::kEdgeAt + i];
+      out->outbound[f] += out->edge[i];
+      const uint32_t m = tab[i] & 0xFFu, at = tab[i] >> 8;
+      out->at_least_max[i] = m;
+      out->at_least_offset[i] = at;
+      // at least k = the vertices whose capped tally is k..m
+      uint64_t run = 0;
+      for (uint32_t k = m; k >= 1; --k) {
+        run += counts[pjs::kExactAt + at + k - 1];
+        out->at_least[at + k - 1] = run;
+      }
+    }
+    out->edge_total += out->outbound[f];
+  }
+  return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 int pj_create(pj_ctx** out, int device) {
@@ -4470,6 +4575,21 @@ int pj_download_active(pj_ctx* ctx, uint32_t* vertices, uint64_t* cand, uint64_t
   if (dst && ne) PJ_HIP(hipMemcpyAsync(dst, ctx->d_out_dst, ne * 4, hipMemcpyDeviceToHost, ctx->stream));
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
+}
+
+int pj_graph_stat(pj_ctx* ctx, pj_label_stat* out) {
+  if (!ctx) return -1;
+  if (!out) {
+    ctx->err = "pj_graph_stat: no output";
+    return -1;
+  }
+  if (!ctx->have_template) {
+    ctx->err = "no template installed";
+    return -1;
+  }
+  PJ_HIP(hipSetDevice(ctx->device));
+  if (zero_counters(ctx)) return -1;
+  return with_word(ctx, [&](auto w) { return graph_stat_t<decltype(w)>(ctx, out); });
 }
 
 void pj_destroy(pj_ctx* ctx) {

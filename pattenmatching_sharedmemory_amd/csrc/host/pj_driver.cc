@@ -1,6 +1,8 @@
 // Host pipeline driver (see pj_driver.h).
 #include "pj_driver.h"
 
+#include "pj_effectiveness.h"
+
 #include <sys/stat.h>
 
 #include <cstdarg>
@@ -241,6 +243,7 @@ int run_step_machine(MultipleLabelDeviceState* state, const ConstraintSet& cs,
   int iteration = 0;
   double total = 0.0;
   const bool trace = std::getenv("PJ_TRACE") != nullptr;  // per-step progress on stderr
+  int cc_runs = 0, pc_runs = 0;
   machine.getNextStep(results, &step, &name);
   while (!finished && iteration < 1000) {
     if (stop_before_enumeration && step == MultipleLabelStep::E_ENUMERATION) break;
@@ -292,12 +295,149 @@ int run_step_machine(MultipleLabelDeviceState* state, const ConstraintSet& cs,
       std::fprintf(stderr, "[pj] step %d %s %.3f ms device, %llu slots, active %llu\n", iteration,
                    MultipleLabelStep::name(step), dev, static_cast<unsigned long long>(slots),
                    static_cast<unsigned long long>(state->graphActiveVertexCount));
-    if (on_step)
-      on_step(StepRecord{iteration, step, results, state->graphActiveVertexCount, state->graphActiveEdgeCount, slots,
-                         wall, dev});
+    if (on_step) {
+      StepRecord rec{iteration, step, results, state->graphActiveVertexCount, state->graphActiveEdgeCount, slots,
+                     wall, dev};
+      // the iterators: cycles and paths advance per step, TDS stays at 0
+      if (step == MultipleLabelStep::E_CC) rec.constraint = cc_runs;
+      if (step == MultipleLabelStep::E_PC) rec.constraint = pc_runs;
+      if (step == MultipleLabelStep::E_TDS) rec.constraint = 0;
+      on_step(rec);
+    }
+    if (step == MultipleLabelStep::E_CC) ++cc_runs;
+    if (step == MultipleLabelStep::E_PC) ++pc_runs;
     finished = machine.getNextStep(results, &step, &name);
     ++iteration;
   }
+  if (total_ms) *total_ms = total;
+  return iteration;
+}
+
+int run_dynamic_order(MultipleLabelDeviceState* state, const ConstraintSet& cs,
+                      const std::function<void(const StepRecord&)>& on_step, double* total_ms,
+                      bool stop_before_enumeration, int threads) {
+  MultipleLabelHip::Lcc0Type lcc0;
+  MultipleLabelHip::LccType lcc;
+  MultipleLabelHip::CcType cc;
+  MultipleLabelHip::PcType pc;
+  MultipleLabelHip::TdsType tds;
+  MultipleLabelHip::EnumerationType en;
+  lcc0.setConstraintVector(cs.local);
+  lcc.setConstraintVector(cs.local);
+  cc.setConstraintVector(cs.cycles);
+  pc.setConstraintVector(cs.paths);
+  tds.setConstraintVector(cs.tds);
+  en.setConstraint(cs.enumeration);
+  // the constraints not yet run, by original index in list order
+  std::vector<uint32_t> cycles_left, paths_left, tds_left;
+  for (uint32_t i = 0; i < cs.cycles.size(); ++i) cycles_left.push_back(i);
+  for (uint32_t i = 0; i < cs.paths.size(); ++i) paths_left.push_back(i);
+  for (uint32_t i = 0; i < cs.tds.size(); ++i) tds_left.push_back(i);
+  auto drop = [](std::vector<uint32_t>* v, uint32_t index) {
+    for (auto it = v->begin(); it != v->end(); ++it)
+      if (*it == index) {
+        v->erase(it);
+        return;
+      }
+  };
+
+  if (pj_state_reset(state->ctx) != 0) {
+    state->error = 1;
+    state->message = pj_last_error(state->ctx);
+    return -1;
+  }
+  pj_get_counters(state->ctx, &state->graphActiveVertexCount, &state->graphActiveEdgeCount);
+  Graph unused;
+  AlgoResults results;
+  MultipleLabelStep::Step step = MultipleLabelStep::E_LCC0;
+  NextConstraint chosen;
+  pj_label_stat* stat = new pj_label_stat;
+  bool finished = false;
+  int iteration = 0;
+  double total = 0.0;
+  int rc = 0;
+  while (!finished && iteration < 1000) {
+    if (stop_before_enumeration && step == MultipleLabelStep::E_ENUMERATION) break;
+    auto t0 = std::chrono::steady_clock::now();
+    switch (step) {
+      case MultipleLabelStep::E_LCC0:
+        results = lcc0.compute(unused, state);
+        break;
+      case MultipleLabelStep::E_LCC:
+        results = lcc.compute(unused, state);
+        break;
+      case MultipleLabelStep::E_CC:
+        cc.resetState(state);
+        cc.setConstraintIterator(chosen.index);
+        results = cc.compute(unused, state);
+        break;
+      case MultipleLabelStep::E_PC:
+        pc.resetState(state);
+        pc.setConstraintIterator(chosen.index);
+        results = pc.compute(unused, state);
+        break;
+      case MultipleLabelStep::E_TDS:
+        tds.resetState(state);
+        tds.setConstraintIterator(chosen.index);
+        results = tds.compute(unused, state);
+        break;
+      case MultipleLabelStep::E_ENUMERATION:
+        results = en.compute(unused, state);
+        break;
+      default:
+        break;
+    }
+    const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (state->error) {
+      rc = -1;
+      break;
+    }
+    double dev = 0.0;
+    uint64_t slots = 0;
+    pj_last_step_ms(state->ctx, &dev);
+    pj_last_step_slots(state->ctx, &slots);
+    total += dev;
+    StepRecord rec{iteration, step, results, state->graphActiveVertexCount, state->graphActiveEdgeCount, slots,
+                   wall, dev};
+    const bool nlcc = step == MultipleLabelStep::E_CC || step == MultipleLabelStep::E_PC ||
+                      step == MultipleLabelStep::E_TDS;
+    if (nlcc) {
+      rec.constraint = static_cast<int>(chosen.index);
+      rec.effectiveness = chosen.effectiveness;
+    }
+    // next step (test_order_multiple_label_cpu.cc:171-287)
+    if (step == MultipleLabelStep::E_LCC && results.isEmpty()) {
+      if (pj_graph_stat(state->ctx, stat) != 0) {
+        state->error = 1;
+        state->message = pj_last_error(state->ctx);
+        rc = -1;
+        break;
+      }
+      pj_last_step_ms(state->ctx, &rec.statMs);
+      total += rec.statMs;
+      chosen = choose_next(LabelStat(*stat), cs, cycles_left, paths_left, tds_left, threads);
+      if (chosen.step == PJ_STEP_CC) {
+        drop(&cycles_left, chosen.index);
+        step = MultipleLabelStep::E_CC;
+      } else if (chosen.step == PJ_STEP_PC) {
+        drop(&paths_left, chosen.index);
+        step = MultipleLabelStep::E_PC;
+      } else if (chosen.step == PJ_STEP_TDS) {
+        drop(&tds_left, chosen.index);
+        step = MultipleLabelStep::E_TDS;
+      } else {
+        step = MultipleLabelStep::E_ENUMERATION;
+      }
+    } else if (step == MultipleLabelStep::E_ENUMERATION) {
+      finished = true;
+    } else {
+      step = MultipleLabelStep::E_LCC;
+    }
+    if (on_step) on_step(rec);
+    ++iteration;
+  }
+  delete stat;
+  if (rc) return -1;
   if (total_ms) *total_ms = total;
   return iteration;
 }

@@ -414,6 +414,9 @@ struct StepRecord {
   uint64_t activeVertices, activeEdges;
   uint64_t slots;
   double wallMs, deviceMs;
+  int constraint = -1;         // cycle / path / TDS constraint the step ran
+  double effectiveness = 0.0;  // its estimate when the dynamic order chose it
+  double statMs = 0.0;         // device time of the label statistics taken after the step
 };
 
 // Runs the step machine (multiple_label_cpu.cc:113-185) on a state whose
@@ -423,6 +426,20 @@ struct StepRecord {
 int run_step_machine(MultipleLabelDeviceState* state, const ConstraintSet& cs,
                      const std::function<void(const StepRecord&)>& on_step, double* total_ms,
                      bool stop_before_enumeration = false, bool stop_before_tds = false);
+
+// The dynamic constraint order of the reference's article driver
+// (test_article/test_order_multiple_label_cpu.cc:127-290, ORDER_USE_DYNAMIC):
+// LCC0, then LCC until a step eliminates nothing; there the label statistics
+// of the active graph (pj_graph_stat) and the effectiveness estimate
+// (choose_next) pick the next cycle, path or TDS constraint, which runs once
+// and leaves its list; LCC follows every such step.  With every list empty
+// the enumeration runs (USE_ENUMERATE), or the machine stops there with
+// stop_before_enumeration.  That driver enters its loop with no step set
+// and reaches LCC without LCC0; LCC0 runs first here, as in the plain
+// machine: LCC repeats the label test, so the fixed point is the same.
+int run_dynamic_order(MultipleLabelDeviceState* state, const ConstraintSet& cs,
+                      const std::function<void(const StepRecord&)>& on_step, double* total_ms,
+                      bool stop_before_enumeration, int threads);
 
 struct Options {
   std::string graph, graphBin, vertexData, edgeData, patternDir, outputDir;

@@ -14,6 +14,7 @@
 #include "../../../include/prunejuice_host.h"
 #include "pj_constraints.h"
 #include "pj_driver.h"
+#include "pj_effectiveness.h"
 #include "pj_graph.h"
 
 struct pjh_graph {
@@ -206,24 +207,68 @@ int pjh_run_pipeline(pj_ctx* ctx, const pjh_constraints* c, int flags, pjh_row* 
   st.ctx = ctx;
   st.managed = true;
   int n = 0;
-  int rc = pj::run_step_machine(
-      &st, c->c,
-      [&](const pj::StepRecord& r) {
-        if (rows && n < max_rows)
-          rows[n] = pjh_row{r.iteration,
-                            static_cast<int32_t>(r.step),
-                            r.results.vertexEliminated,
-                            r.results.edgeEliminated,
-                            r.results.matchEliminated,
-                            r.results.enumeration,
-                            r.activeVertices,
-                            r.activeEdges,
-                            r.slots,
-                            r.deviceMs};
-        ++n;
-      },
-      total_ms, (flags & PJH_STOP_BEFORE_ENUMERATION) != 0, (flags & PJH_STOP_BEFORE_TDS) != 0);
+  auto on_step = [&](const pj::StepRecord& r) {
+    if (rows && n < max_rows)
+      rows[n] = pjh_row{r.iteration,
+                        static_cast<int32_t>(r.step),
+                        r.results.vertexEliminated,
+                        r.results.edgeEliminated,
+                        r.results.matchEliminated,
+                        r.results.enumeration,
+                        r.activeVertices,
+                        r.activeEdges,
+                        r.slots,
+                        r.deviceMs,
+                        r.constraint,
+                        0,
+                        r.effectiveness,
+                        r.statMs};
+    ++n;
+  };
+  const bool stop_enum = (flags & PJH_STOP_BEFORE_ENUMERATION) != 0;
+  const int rc = (flags & PJH_ORDER_DYNAMIC)
+                     ? pj::run_dynamic_order(&st, c->c, on_step, total_ms, stop_enum, 1)
+                     : pj::run_step_machine(&st, c->c, on_step, total_ms, stop_enum, (flags & PJH_STOP_BEFORE_TDS) != 0);
   return rc < 0 ? -1 : n;
+}
+
+int pjh_constraint_effectiveness(const pj_label_stat* stat, const pjh_constraints* c, int step, uint32_t index,
+                                 int threads, double out[3]) {
+  if (!stat || !c || !out || threads < 1) return -1;
+  const auto& cs = c->c;
+  const pj::LabelStat st(*stat);
+  pj::Effectiveness e;
+  if (step == PJ_STEP_CC && index < cs.cycles.size())
+    e = pj::effectiveness(st, cs.cycles[index], threads);
+  else if (step == PJ_STEP_PC && index < cs.paths.size())
+    e = pj::effectiveness(st, cs.paths[index], threads);
+  else if (step == PJ_STEP_TDS && index < cs.tds.size())
+    e = pj::effectiveness(st, cs.tds[index], threads);
+  else
+    return -1;
+  out[0] = e.match_eliminated;
+  out[1] = e.cost_eliminated;
+  out[2] = e.effectiveness;
+  return 0;
+}
+
+int pjh_next_constraint(const pj_label_stat* stat, const pjh_constraints* c, const uint8_t* done_cycles,
+                        const uint8_t* done_paths, const uint8_t* done_tds, int threads, int* step, uint32_t* index,
+                        double* effectiveness) {
+  if (!stat || !c || !step || !index || threads < 1) return -1;
+  const auto& cs = c->c;
+  std::vector<uint32_t> cl, pl, tl;
+  for (uint32_t i = 0; i < cs.cycles.size(); ++i)
+    if (!done_cycles || !done_cycles[i]) cl.push_back(i);
+  for (uint32_t i = 0; i < cs.paths.size(); ++i)
+    if (!done_paths || !done_paths[i]) pl.push_back(i);
+  for (uint32_t i = 0; i < cs.tds.size(); ++i)
+    if (!done_tds || !done_tds[i]) tl.push_back(i);
+  const pj::NextConstraint n = pj::choose_next(pj::LabelStat(*stat), cs, cl, pl, tl, threads);
+  *step = n.step;
+  *index = n.index;
+  if (effectiveness) *effectiveness = n.effectiveness;
+  return 0;
 }
 
 uint64_t pjh_projected_bytes(const pjh_constraints* c, uint32_t vertex_count, uint64_t edge_count) {

@@ -64,7 +64,8 @@ class Row(C.Structure):
     _fields_ = [("iteration", C.c_int32), ("step", C.c_int32), ("vertex_eliminated", C.c_uint64),
                 ("edge_eliminated", C.c_uint64), ("match_eliminated", C.c_uint64), ("enumeration", C.c_uint64),
                 ("active_vertices", C.c_uint64), ("active_edges", C.c_uint64), ("slots", C.c_uint64),
-                ("ms", C.c_double)]
+                ("ms", C.c_double), ("constraint", C.c_int32), ("reserved", C.c_int32),
+                ("effectiveness", C.c_double), ("stat_ms", C.c_double)]
 
     def key(self) -> tuple:
         """The step's results (what the reference's results.totem row holds,
@@ -72,6 +73,17 @@ class Row(C.Structure):
         measure how much a given algorithm read, not what it decided."""
         return (self.iteration, self.step, self.vertex_eliminated, self.edge_eliminated, self.match_eliminated,
                 self.enumeration, self.active_vertices, self.active_edges)
+
+
+STAT_MAX_AT_LEAST = 4096
+
+
+class LabelStatDesc(C.Structure):
+    """pj_label_stat: label statistics of the active graph (pj_graph_stat)."""
+    _fields_ = [("label_count", C.c_uint32), ("at_least_count", C.c_uint32), ("vertex_total", C.c_uint64),
+                ("edge_total", C.c_uint64), ("label", C.c_uint64 * 64), ("vertex", C.c_uint64 * 64),
+                ("outbound", C.c_uint64 * 64), ("edge", C.c_uint64 * 4096), ("at_least_max", C.c_uint32 * 4096),
+                ("at_least_offset", C.c_uint32 * 4096), ("at_least", C.c_uint64 * STAT_MAX_AT_LEAST)]
 
 
 class QueryResult(C.Structure):
@@ -116,6 +128,7 @@ HIP_SIGNATURES = {
     "pj_compact_active": (C.c_int, [P, P, P, C.c_uint64, P, P, C.c_uint64, u64p, u64p]),
     "pj_compact_active_noalloc": (C.c_int, [P, P, P, C.c_uint64, P, P, C.c_uint64, u64p, u64p]),
     "pj_download_active": (C.c_int, [P, P, P, C.c_uint64, P, P, C.c_uint64, u64p, u64p]),
+    "pj_graph_stat": (C.c_int, [P, C.POINTER(LabelStatDesc)]),
     "pj_destroy": (None, [P]),
 }
 
@@ -146,6 +159,10 @@ HOST_SIGNATURES = {
                                   C.POINTER(QueryResult), C.POINTER(C.c_double), C.c_char_p, C.c_size_t]),
     "pjh_projected_bytes": (C.c_uint64, [P, C.c_uint32, C.c_uint64]),
     "pjh_query_uses_memo": (C.c_int, [P]),
+    "pjh_constraint_effectiveness": (C.c_int, [C.POINTER(LabelStatDesc), P, C.c_int, C.c_uint32, C.c_int,
+                                               C.POINTER(C.c_double)]),
+    "pjh_next_constraint": (C.c_int, [C.POINTER(LabelStatDesc), P, u8p, u8p, u8p, C.c_int, C.POINTER(C.c_int),
+                                      u32p, C.POINTER(C.c_double)]),
     "pjh_main": (C.c_int, [C.c_int, C.POINTER(C.c_char_p)]),
 }
 
