@@ -25,6 +25,11 @@ the oracle (CPU restatement of the reference) solving the same scale-22
 graph and template once, on every CPU the process may use (OMP_NUM_THREADS
 when set), with the host's model and core counts.
 
+The plain workload also reports the dynamic constraint order (the
+reference's effectiveness-driven order, ORDER_USE_DYNAMIC): its solve time
+and choices, and the label statistics pass it takes at each LCC fixed point
+against the HBM roofline (label_stat).
+
 --copy-calibration MIB runs only a device-to-device copy of MIB MiB three
 times (the plain copy kernel the PMC byte counters are scaled by).
 
@@ -171,6 +176,65 @@ def phases(rows, edge_count: int) -> dict:
     for ph in out.values():
         ph["ms"] = round(ph["ms"], 4)
         ph["TEPS"] = ph["edge_sweeps"] / (ph["ms"] * 1e-3) if ph["ms"] else None
+    return out
+
+
+def label_stat_pass(pr, g, reps: int = 20) -> dict:
+    """The label statistics pass (pj_graph_stat) of the dynamic constraint
+    order on the state of its first decision (LCC0, then LCC to the fixed
+    point): device time of the pass (bins of every active vertex, the
+    counting launch and the hub finish) and its algorithmic bytes, counted
+    once at natural width: per active vertex its bin entry 4, offsets 16,
+    label code 1 and candidate word W; per slot of an active vertex its id
+    4, edge-active bit 1/8, the neighbour's candidate word W and code 1."""
+    import numpy as np
+
+    from pattenmatching_sharedmemory_amd import STEP_LCC, STEP_LCC0
+
+    pr.reset()
+    pr.step(STEP_LCC0)
+    while not pr.step(STEP_LCC).is_empty():
+        pass
+    st = pr.state()
+    active = (st.vflags & 1).astype(bool)
+    deg = np.diff(g.offsets().astype(np.int64))
+    n_v, slots = int(active.sum()), int(deg[active].sum())
+    pr.kernel_timing(True, only="label_stat")
+    for _ in range(reps):
+        pr.graph_stat()
+    k = [x for x in pr.kernel_stats() if x["name"] == "label_stat"][0]
+    pr.kernel_timing(False)
+    W = k["word_bytes"] or 1
+    nbytes = n_v * (4 + 16 + 1 + W) + slots * (4 + 0.125 + W + 1)
+    avg = k["ms"] / k["launches"]
+    gbs = nbytes / (avg * 1e-3) / 1e9
+    log(f"[bench] label statistics: {n_v} active vertices, {slots} slots, {avg:.4f} ms per pass, "
+        f"{gbs:.0f} GB/s algorithmic")
+    return {"kernel": "label_stat", "active_vertices": n_v, "slots": slots, "avg_launch_ms": avg,
+            "bytes_per_launch": nbytes, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS, "launches": k["launches"]}
+
+
+def dynamic_order(pr, rows_static, steps: int) -> dict:
+    """The dynamic constraint order (PJH_ORDER_DYNAMIC) on the same graph and
+    template: solve time, the constraints it chose, its statistics passes,
+    and whether its pruning fixed point equals the static order's."""
+    pr.run(enumerate_matches=False)
+    static_sum = pr.checksum()
+    times, dev = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        rows, dev_ms = pr.run(enumerate_matches=False, order="dynamic")
+        times.append(time.perf_counter() - t0)
+        dev.append(dev_ms)
+    chosen = [(STEP_NAMES.get(r.step, str(r.step)), r.constraint, r.effectiveness) for r in rows
+              if r.step in (4, 5, 6)]
+    out = {"ms_per_solve": 1e3 * sum(times) / len(times), "device_ms_per_solve": sum(dev) / len(dev),
+           "steps": len(rows), "static_steps": len(rows_static),
+           "choices": [{"step": s, "constraint": i, "effectiveness": e} for s, i, e in chosen],
+           "stat_passes": sum(r.stat_ms > 0 for r in rows), "stat_ms_per_solve": sum(r.stat_ms for r in rows),
+           "same_fixed_point_as_static": pr.checksum() == static_sum}
+    log(f"[bench] dynamic order: {out['ms_per_solve']:.2f} ms/solve, {len(rows)} steps, choices {chosen}, "
+        f"statistics {out['stat_ms_per_solve']:.3f} ms/solve, same fixed point: {out['same_fixed_point_as_static']}")
     return out
 
 
@@ -463,6 +527,9 @@ def main() -> int:
                      "algorithmic_GBs": round(algorithmic_bytes(k, g.vertex_count) / (k["ms"] * 1e-3) / 1e9, 1)
                      if k["ms"] else 0.0} for k in sorted(kstats, key=lambda k: -k["ms"])],
     }
+    if args.workload == "plain":
+        result["dynamic_order"] = dynamic_order(pr, rows, max(1, min(args.steps, 5)))
+        result["label_stat"] = label_stat_pass(pr, g)
     pr.close()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(g, c, rows, sec, baseline_threads(), not args.stop_before_tds)

@@ -18,6 +18,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -3864,6 +3865,8 @@ int graph_stat_t(pj_ctx* ctx, pj_label_stat* out) {
   a.chunk_start = ctx->d_chunk_start;
   a.nchunks = ctx->nhubs ? ctx->nchunks : 0u;
   a.hub_count = ctx->d_hub_count;
+  a.pcode = reinterpret_cast<const uint8_t*>(ctx->d_t) + offsetof(DevTemplate, pcode);  // device address
+  a.derive = ctx->labels_consistent;
   a.K = K;
   a.exact_words = words;
   a.buffer = ctx->d_stat;

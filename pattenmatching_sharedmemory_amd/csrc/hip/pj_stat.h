@@ -54,6 +54,8 @@ struct StatArgs {
   const uint64_t* chunk_start; // first slot of each chunk
   uint32_t nchunks;
   uint32_t* hub_count;         // [nhubs x 64] per-hub label counts: zero on entry, zero again on return
+  const uint8_t* pcode;        // [64] label code of each template vertex (device)
+  bool derive;                 // every candidate carries its vertex's label: codes from candidate words
   uint32_t K;                  // label codes in use
   uint32_t exact_words;        // sum of max over the pairs
   unsigned long long* buffer;  // kBufferBytes, pair table filled by the caller

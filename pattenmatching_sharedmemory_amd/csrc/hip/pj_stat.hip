@@ -11,12 +11,12 @@
 // thread visited before it; the count of one vertex is what the statistic's
 // name and its use in the effectiveness estimate describe).
 //
-// One launch walks the frontier bins of every active vertex (a group of 8,
-// 32 or 64 lanes per row by degree) and then the 2048-slot chunks of the hub
-// rows (a wave per chunk).  Counts go to LDS with LDS atomics (the vertex
-// and label-pair tables, the "exact" table of capped per-vertex tallies, and
-// one tally of K counters per lane group) and leave the block with one
-// global atomic per non-zero entry.  A hub's tally is summed over its chunks
+// One launch walks the frontier bins of every active vertex (a group of 2,
+// 8 or 64 lanes per row by degree, 4 slots per lane per pass) and then the
+// 2048-slot chunks of the hub rows (a wave per chunk).  Counts go to LDS (the
+// vertex and label-pair tables, the "exact" table of capped per-vertex
+// tallies, and one tally of K counters per lane group) and leave the block
+// with one global atomic per non-zero entry.  A hub's tally is summed over its chunks
 // in hub_count and capped by a second, small launch.  The pass is bound by
 // the gathers of the neighbours' candidate words and label codes.
 #include "pj_stat.h"
@@ -27,41 +27,68 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr uint32_t kChunk = 2048;
-constexpr int kRowsPerWave = 64 / 8;      // groups of the narrowest (8-lane) rows in a wave
+constexpr int kSlots = 4;                   // slots per lane per pass, loaded before any is counted
+constexpr int kRowsPerWave = 64 / 2;        // groups of the narrowest (2-lane) rows in a wave
 constexpr int kMaxGroups = kWaves * kRowsPerWave;
 
 __device__ __forceinline__ bool ea_on(const uint32_t* __restrict__ ea, uint64_t e) {
   return (ea[e >> 5] >> (e & 31)) & 1u;
 }
 
+template <typename Word>
+__device__ __forceinline__ int lowest(Word w) {
+  return sizeof(Word) == 8 ? __builtin_ctzll(static_cast<uint64_t>(w)) : __builtin_ctz(static_cast<uint32_t>(w));
+}
+
 struct Lds {
   uint32_t* vtx;    // [K]
   uint32_t* edge;   // [K x K]
   uint32_t* exact;  // [exact_words]
-  uint32_t* tab;    // [K x K]: offset << 8 | max
-  uint32_t* tally;  // [kMaxGroups x K]
+  uint16_t* tally;  // [kMaxGroups x K]: slots of one row (<= 2048 per row or chunk)
+  uint8_t* pcode;   // [64]: label code of each template vertex
 };
 
-// The G lanes of one group count the slots [beg, end) of row v into the
-// label-pair table and the group's tally (lane gl of the group).
+// One pass of the G lanes of a group over the slots [b, b + G * kSlots) of
+// row v (lane gl takes b + gl + j * G): edge ids and edge-active words first,
+// then the neighbours' candidate words (and label codes unless they are
+// derived from the candidates: once every candidate of every vertex carries
+// the vertex's label, the lowest candidate's template vertex gives it), then
+// the counts.  Counts are aggregated across the wave by (group, label): one
+// LDS update per distinct pair instead of one per slot.
 template <typename Word, int G>
-__device__ __forceinline__ void count_slots(const StatArgs& a, const Lds& s, uint32_t K, uint32_t lf, uint64_t beg,
-                                            uint64_t end, int gl, uint32_t* tally) {
+__device__ __forceinline__ void count_pass(const StatArgs& a, const Lds& s, uint32_t K, uint32_t lf, uint64_t b,
+                                           uint64_t end, int gl, uint16_t* tally) {
   const Word* __restrict__ cand = static_cast<const Word*>(a.cand);
-  for (uint64_t b = beg; b < end; b += G) {
-    const uint64_t e = b + gl;
-    if (e < end && ea_on(a.ea, e)) {
-      const uint32_t u = a.edges[e];
-      const Word cu = cand[u];
-      const uint32_t lt = a.code[u];
-      if (cu != Word(0) && lt < K) {
-        atomicAdd(&s.edge[lf * K + lt], 1u);
-        atomicAdd(&tally[lt], 1u);
+  const int lane = __lane_id();
+  uint32_t u[kSlots], on[kSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const uint64_t e = b + gl + static_cast<uint64_t>(j) * G;
+    u[j] = e < end ? a.edges[e] : 0u;
+    on[j] = e < end ? ea_on(a.ea, e) : 0u;
+  }
+  uint32_t lt[kSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const Word cu = on[j] ? cand[u[j]] : Word(0);
+    lt[j] = cu == Word(0) ? kLabels : a.derive ? s.pcode[lowest<Word>(cu)] : a.code[u[j]];
+  }
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const uint32_t key = static_cast<uint32_t>(lane / G) * kLabels + lt[j];
+    uint64_t pending = __ballot(lt[j] < K);
+    while (pending) {
+      const int leader = __builtin_ctzll(pending);
+      const uint32_t k = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(key), leader));
+      const uint64_t m = __ballot(lt[j] < K && key == k);
+      if (lane == leader) {
+        const uint32_t n = static_cast<uint32_t>(__popcll(m));
+        atomicAdd(&s.edge[lf * K + lt[j]], n);
+        tally[lt[j]] = static_cast<uint16_t>(tally[lt[j]] + n);  // the group's own entry, one writer
       }
+      pending &= ~m;
     }
   }
-  // The group's lanes read the tally next: their LDS atomics come first.
-  __threadfence_block();
 }
 
 // Rows of one bin: a group of G lanes per vertex, grid-strided.
@@ -71,21 +98,24 @@ __device__ void stat_bin(const StatArgs& a, const Lds& s, uint32_t K, const uint
   constexpr uint32_t kGroups = kThreads / G;
   const int gl = threadIdx.x & (G - 1);
   const uint32_t gi = threadIdx.x / G;
-  // Tally rows are private to a wave (8 per wave, one per group): a wave may
-  // start the next bin while another still counts rows of this one.
-  uint32_t* tally = s.tally + ((threadIdx.x >> 6) * kRowsPerWave + (__lane_id() / G)) * K;
+  const uint32_t* gtab = reinterpret_cast<const uint32_t*>(a.buffer + kCountWords);
+  // Tally rows are private to a wave (one per group): a wave may start the
+  // next bin while another still counts rows of this one.
+  uint16_t* tally = s.tally + ((threadIdx.x >> 6) * kRowsPerWave + (__lane_id() / G)) * K;
   for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * kGroups + gi; g < n;
        g += static_cast<uint64_t>(gridDim.x) * kGroups) {
     const uint32_t v = list[g];
     const uint32_t lf = a.code[v];
     if (lf >= K) continue;  // group-uniform; an active vertex carries a template label
     if (gl == 0) atomicAdd(&s.vtx[lf], 1u);
-    count_slots<Word, G>(a, s, K, lf, a.off[v], a.off[v + 1], gl, tally);
+    const uint64_t beg = a.off[v], end = a.off[v + 1];
+    for (uint64_t b = beg; b < end; b += G * kSlots) count_pass<Word, G>(a, s, K, lf, b, end, gl, tally);
+    __threadfence_block();  // the leaders' tally writes before the group reads them
     for (uint32_t lt = gl; lt < K; lt += G) {
       const uint32_t c = tally[lt];
       if (!c) continue;
       tally[lt] = 0;
-      const uint32_t tb = s.tab[lf * K + lt];
+      const uint32_t tb = gtab[lf * kLabels + lt];
       const uint32_t m = tb & 0xFFu;
       if (m) atomicAdd(&s.exact[(tb >> 8) + min(c, m) - 1], 1u);
     }
@@ -100,7 +130,7 @@ __device__ void stat_hub_chunks(const StatArgs& a, const Lds& s, uint32_t K) {
   const Word* __restrict__ cand = static_cast<const Word*>(a.cand);
   const int lane = __lane_id();
   const uint32_t w = threadIdx.x >> 6;
-  uint32_t* tally = s.tally + w * kRowsPerWave * K;
+  uint16_t* tally = s.tally + w * kRowsPerWave * K;
   for (uint32_t c = blockIdx.x * kWaves + w; c < a.nchunks; c += gridDim.x * kWaves) {
     const uint32_t h = a.chunk_hub[c];
     const uint32_t v = a.hubs[h];
@@ -110,7 +140,8 @@ __device__ void stat_hub_chunks(const StatArgs& a, const Lds& s, uint32_t K) {
     const uint64_t beg = a.chunk_start[c];
     const uint64_t end = beg + kChunk < row_end ? beg + kChunk : row_end;
     if (beg == row && lane == 0) atomicAdd(&s.vtx[lf], 1u);
-    count_slots<Word, 64>(a, s, K, lf, beg, end, lane, tally);
+    for (uint64_t b = beg; b < end; b += 64 * kSlots) count_pass<Word, 64>(a, s, K, lf, b, end, lane, tally);
+    __threadfence_block();
     for (uint32_t lt = lane; lt < K; lt += 64) {
       const uint32_t n = tally[lt];
       if (!n) continue;
@@ -129,20 +160,18 @@ __global__ void __launch_bounds__(kThreads) k_label_stat(StatArgs a) {
   s.vtx = lds;
   s.edge = s.vtx + K;
   s.exact = s.edge + K * K;
-  s.tab = s.exact + X;
-  s.tally = s.tab + K * K;
-  const uint32_t zero_words = K + K * K + X;
+  s.tally = reinterpret_cast<uint16_t*>(s.exact + X);
+  s.pcode = reinterpret_cast<uint8_t*>(s.tally + kMaxGroups * K);
+  const uint32_t zero_words = K + K * K + X + (kMaxGroups * K + 1) / 2;
   for (uint32_t i = threadIdx.x; i < zero_words; i += kThreads) lds[i] = 0;
-  for (uint32_t i = threadIdx.x; i < kMaxGroups * K; i += kThreads) s.tally[i] = 0;
-  const uint32_t* gtab = reinterpret_cast<const uint32_t*>(a.buffer + kCountWords);
-  for (uint32_t i = threadIdx.x; i < K * K; i += kThreads) s.tab[i] = gtab[(i / K) * kLabels + i % K];
+  if (threadIdx.x < 64) s.pcode[threadIdx.x] = a.pcode[threadIdx.x];
   __syncthreads();
   const uint64_t V = a.V;
-  // longest rows first
+  // longest rows first; 2 / 8 / 64 / 64 lanes of kSlots slots per row
   stat_bin<Word, 64>(a, s, K, a.bins + 3 * V, a.bin_count[3]);
   stat_bin<Word, 64>(a, s, K, a.bins + 2 * V, a.bin_count[2]);
-  stat_bin<Word, 32>(a, s, K, a.bins + 1 * V, a.bin_count[1]);
-  stat_bin<Word, 8>(a, s, K, a.bins, a.bin_count[0]);
+  stat_bin<Word, 8>(a, s, K, a.bins + 1 * V, a.bin_count[1]);
+  stat_bin<Word, 2>(a, s, K, a.bins, a.bin_count[0]);
   stat_hub_chunks<Word>(a, s, K);
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < K; i += kThreads)
@@ -184,9 +213,9 @@ __global__ void k_zero_counts(unsigned long long* p) {
 
 template <typename Word>
 int run(const StatArgs& a, std::string* err) {
-  const size_t lds = 4ull * (a.K + 2ull * a.K * a.K + a.exact_words + static_cast<uint64_t>(kMaxGroups) * a.K);
+  const size_t lds = 4ull * (a.K + 1ull * a.K * a.K + a.exact_words) + 2ull * kMaxGroups * a.K + 64;
   hipLaunchKernelGGL(k_zero_counts, dim3(16), dim3(kThreads), 0, a.stream, a.buffer);
-  const uint32_t blocks = static_cast<uint32_t>(a.cus > 0 ? a.cus * 4 : 256);
+  const uint32_t blocks = static_cast<uint32_t>(a.cus > 0 ? a.cus * 16 : 1024);
   hipLaunchKernelGGL(k_label_stat<Word>, dim3(blocks), dim3(kThreads), lds, a.stream, a);
   if (a.nhubs)
     hipLaunchKernelGGL(k_stat_hub_finish<Word>, dim3((a.nhubs + kThreads - 1) / kThreads), dim3(kThreads), 0,
