@@ -307,6 +307,49 @@ const char* pj_last_error(pj_ctx* ctx);
 /* Releases every device buffer and the stream. */
 void pj_destroy(pj_ctx* ctx);
 
+/* ---- Batched LCC over one resident graph --------------------------------
+ * No counterpart in the reference, which solves one template per run
+ * (MultipleLabelCpu::runPatternMatching, multiple_label_cpu.cc:113-185); the
+ * batch serves the batched and served forms.  It runs, for many templates at
+ * once, the steps the reference runs first for each: LCC0
+ * (multiple_label_lcc0_cpu.tpp:13-147) and LCC steps
+ * (multiple_label_lcc_cpu.tpp:18-152) until a step eliminates nothing
+ * (AlgoResults::isEmpty, algo_results.cc:18-23).  Up to 64 templates share
+ * each sweep of the graph (one lane of a wavefront each); every template's
+ * per-step results, candidate sets, flags and edge-active bits are those of
+ * the same steps run alone.  A template that reaches that fixed point is
+ * parked; pj_batch_export installs its state into a context (with the same
+ * template installed, sharing the graph) that continues the template's
+ * pipeline, and pj_batch_release frees its lane for another template.
+ * Calls on one batch may come from several threads. */
+typedef struct pj_batch pj_batch;
+
+/* A batch of `lanes` template lanes (rounded up to a multiple of 64) over
+ * the graph resident in `graph_owner` (which must outlive the batch).  On
+ * failure *batch may still be set for pj_batch_last_error; destroy it. */
+int pj_batch_create(pj_batch** batch, const pj_ctx* graph_owner, uint32_t lanes);
+/* Admits a template of at most 16 vertices into a free lane (its LCC0 runs
+ * in the next step); -1 when no lane is free or the template does not fit. */
+int pj_batch_admit(pj_batch* batch, const pj_template_desc* desc, uint32_t* lane);
+/* Runs up to `max_steps` steps (each: LCC0 of the lanes admitted since the
+ * last step and one LCC step of every running lane), returning after the
+ * first step in which some lanes reach their fixed point; those lanes are
+ * parked and listed in done[0 .. *n_done) (at most max_done are written). */
+int pj_batch_run(pj_batch* batch, uint32_t max_steps, uint32_t* done, uint32_t max_done, uint32_t* n_done);
+/* Per-step counters of a parked lane: LCC0, then each LCC step (the last one
+ * eliminated nothing). */
+int pj_batch_trace(pj_batch* batch, uint32_t lane, pj_algo_results* rows, uint32_t max_rows, uint32_t* n_rows);
+/* Installs a parked lane's state into `ctx`: candidate sets, modified and
+ * scheduled flags, edge-active bits and the active counters, as
+ * pj_run_step would have left them after the same steps. */
+int pj_batch_export(pj_batch* batch, uint32_t lane, pj_ctx* ctx);
+int pj_batch_release(pj_batch* batch, uint32_t lane);
+/* Lanes, running and parked lanes, device time and steps run so far. */
+int pj_batch_info(pj_batch* batch, uint32_t* lanes, uint32_t* running, uint32_t* parked, double* device_ms,
+                  uint64_t* steps);
+const char* pj_batch_last_error(pj_batch* batch);
+void pj_batch_destroy(pj_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

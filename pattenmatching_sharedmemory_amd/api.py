@@ -471,6 +471,79 @@ class Pruner:
         return list(rows[: min(n, max_rows)]), total.value
 
 
+class Batch:
+    """Batched LCC (pj_batch): LCC0 and the LCC steps to the first fixed
+    point of many templates in sweeps they share, over the graph resident in
+    `owner`.  A template that reaches that fixed point is parked; its state
+    can be exported into a context with the same template installed."""
+
+    def __init__(self, owner: Pruner, lanes: int = 64):
+        lib = native.hip()
+        b = C.c_void_p()
+        rc = lib.pj_batch_create(C.byref(b), owner.ctx, lanes)
+        self._b = b
+        self.owner = owner
+        if rc:
+            msg = lib.pj_batch_last_error(b).decode() if b.value else "pj_batch_create failed"
+            self.close()
+            raise RuntimeError(msg)
+        self.constraints: dict[int, Constraints] = {}
+
+    def _check(self, rc: int) -> None:
+        if rc:
+            raise RuntimeError(native.hip().pj_batch_last_error(self._b).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_b", None) is not None and self._b.value:
+            native.hip().pj_batch_destroy(self._b)
+            self._b = C.c_void_p(0)
+
+    def __del__(self):
+        self.close()
+
+    def admit(self, constraints: Constraints) -> int:
+        lane = C.c_uint32()
+        self._check(native.hip().pj_batch_admit(self._b, constraints.desc, C.byref(lane)))
+        self.constraints[lane.value] = constraints
+        return lane.value
+
+    def run(self, max_steps: int = 1000) -> list[int]:
+        """Steps until some lanes reach their fixed point; returns them (parked)."""
+        done = (C.c_uint32 * 4096)()
+        n = C.c_uint32()
+        self._check(native.hip().pj_batch_run(self._b, max_steps, done, 4096, C.byref(n)))
+        return list(done[: n.value])
+
+    def run_all(self) -> list[int]:
+        """Steps until no lane runs; the parked lanes in the order they finished."""
+        out = []
+        while self.info()["running"]:
+            out += self.run()
+        return out
+
+    def trace(self, lane: int) -> list[tuple[int, int, int, int]]:
+        n = C.c_uint32()
+        self._check(native.hip().pj_batch_trace(self._b, lane, None, 0, C.byref(n)))
+        rows = (AlgoResults * max(n.value, 1))()
+        self._check(native.hip().pj_batch_trace(self._b, lane, rows, n.value, C.byref(n)))
+        return [r.as_tuple() for r in rows[: n.value]]
+
+    def export(self, lane: int, pruner: Pruner) -> None:
+        self._check(native.hip().pj_batch_export(self._b, lane, pruner.ctx))
+
+    def release(self, lane: int) -> None:
+        self._check(native.hip().pj_batch_release(self._b, lane))
+        self.constraints.pop(lane, None)
+
+    def info(self) -> dict:
+        lanes, running, parked = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        ms, steps = C.c_double(), C.c_uint64()
+        self._check(native.hip().pj_batch_info(self._b, C.byref(lanes), C.byref(running), C.byref(parked),
+                                               C.byref(ms), C.byref(steps)))
+        return {"lanes": lanes.value, "running": running.value, "parked": parked.value, "device_ms": ms.value,
+                "steps": steps.value}
+
+
 def run_cli(args: list[str]) -> int:
     """patternmatching_binary in-process (writes results.totem etc.)."""
     argv = (C.c_char_p * (len(args) + 1))(b"patternmatching_hip", *[a.encode() for a in args])

@@ -52,7 +52,8 @@ def build_hip() -> Path:
     out = LIB / "libprunejuice_hip.so"
     tmp = LIB / "libprunejuice_hip.so.tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", str(CSRC / "hip" / "pj_device.hip"),
-          str(CSRC / "hip" / "pj_compact.hip"), str(CSRC / "hip" / "pj_stat.hip"), "-o", str(tmp)])
+          str(CSRC / "hip" / "pj_compact.hip"), str(CSRC / "hip" / "pj_stat.hip"), str(CSRC / "hip" / "pj_batch.hip"),
+          "-o", str(tmp)])
     os.replace(tmp, out)
     return out
 

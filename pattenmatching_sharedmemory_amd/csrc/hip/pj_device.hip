@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../../include/prunejuice.h"
+#include "pj_batch.h"
 #include "pj_compact.h"
 #include "pj_stat.h"
 
@@ -4613,3 +4614,60 @@ void pj_destroy(pj_ctx* ctx) {
 }
 
 }  // extern "C"
+
+// Internal interface of the batched LCC (pj_batch.h).
+namespace pjb {
+
+int graph_view(const pj_ctx* ctx, GraphView* out) {
+  if (!ctx || !out || !ctx->d_off) return -1;
+  out->device = ctx->device;
+  out->cus = ctx->cus;
+  out->V = ctx->V;
+  out->E = ctx->E;
+  out->off = ctx->d_off;
+  out->edges = ctx->d_edges;
+  out->labels = ctx->d_labels;
+  out->hubs = ctx->d_hubs;
+  out->nhubs = ctx->nhubs;
+  out->chunk_hub = ctx->d_chunk_hub;
+  out->chunk_start = ctx->d_chunk_start;
+  out->nchunks = ctx->nhubs ? ctx->nchunks : 0u;
+  out->hub_degree = kHubDegree;
+  out->hub_chunk = kHubChunk;
+  return 0;
+}
+
+int import_lcc_state(pj_ctx* ctx, const void* cand, int word_bytes, const uint8_t* flags, const uint32_t* ea_words,
+                     uint64_t active_vertices, uint64_t active_edges, hipEvent_t ready) {
+  if (!ctx) return -1;
+  if (!ctx->have_template) {
+    ctx->err = "no template installed";
+    return -1;
+  }
+  if (word_bytes != ctx->word_bytes) {
+    ctx->err = "imported candidate words do not match the installed template";
+    return -1;
+  }
+  PJ_HIP(hipSetDevice(ctx->device));
+  if (ready) PJ_HIP(hipStreamWaitEvent(ctx->stream, ready, 0));
+  PJ_HIP(hipMemcpyAsync(ctx->d_cand, cand, static_cast<uint64_t>(ctx->V) * word_bytes, hipMemcpyDeviceToDevice,
+                        ctx->stream));
+  PJ_HIP(hipMemcpyAsync(ctx->d_flags, flags, ctx->V, hipMemcpyDeviceToDevice, ctx->stream));
+  PJ_HIP(hipMemcpyAsync(ctx->d_ea, ea_words, ctx->ea_words * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  // toUnmatch is empty after every phase 2
+  const uint64_t words = (static_cast<uint64_t>(ctx->V) * word_bytes + 3) / 4;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, words)), dim3(kBlock), 0, ctx->stream,
+                     static_cast<uint32_t*>(ctx->d_unmatch), words, 0u);
+  mark_all_dirty(ctx);
+  ctx->active_vertices = active_vertices;
+  ctx->active_edges = active_edges;
+  ctx->labels_consistent = true;  // LCC0 ran: every candidate carries its vertex's label
+  ctx->modified_cleared = false;
+  ctx->live = nullptr;
+  ctx->n_live = ctx->V;
+  PJ_HIP(hipGetLastError());
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+}  // namespace pjb

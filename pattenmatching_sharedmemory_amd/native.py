@@ -130,6 +130,15 @@ HIP_SIGNATURES = {
     "pj_download_active": (C.c_int, [P, P, P, C.c_uint64, P, P, C.c_uint64, u64p, u64p]),
     "pj_graph_stat": (C.c_int, [P, C.POINTER(LabelStatDesc)]),
     "pj_destroy": (None, [P]),
+    "pj_batch_create": (C.c_int, [C.POINTER(P), P, C.c_uint32]),
+    "pj_batch_admit": (C.c_int, [P, C.POINTER(TemplateDesc), u32p]),
+    "pj_batch_run": (C.c_int, [P, C.c_uint32, u32p, C.c_uint32, u32p]),
+    "pj_batch_trace": (C.c_int, [P, C.c_uint32, C.POINTER(AlgoResults), C.c_uint32, u32p]),
+    "pj_batch_export": (C.c_int, [P, C.c_uint32, P]),
+    "pj_batch_release": (C.c_int, [P, C.c_uint32]),
+    "pj_batch_info": (C.c_int, [P, u32p, u32p, u32p, C.POINTER(C.c_double), u64p]),
+    "pj_batch_last_error": (C.c_char_p, [P]),
+    "pj_batch_destroy": (None, [P]),
 }
 
 HOST_SIGNATURES = {
