@@ -324,7 +324,7 @@ void pj_destroy(pj_ctx* ctx);
  * Calls on one batch may come from several threads. */
 typedef struct pj_batch pj_batch;
 
-/* A batch of `lanes` template lanes (rounded up to a multiple of 64) over
+/* A batch of `lanes` template lanes (groups of 64 share a sweep) over
  * the graph resident in `graph_owner` (which must outlive the batch).  On
  * failure *batch may still be set for pj_batch_last_error; destroy it. */
 int pj_batch_create(pj_batch** batch, const pj_ctx* graph_owner, uint32_t lanes);

@@ -120,6 +120,13 @@ typedef struct pjh_query_result {
   double arrival_ms, start_ms, end_ms; /* on the batch clock */
   double device_ms;         /* summed step device time */
 } pjh_query_result;
+/* With PJH_BATCH_LCC in `flags` the LCC0 step and the LCC steps to the first
+ * fixed point of every template of at most 16 vertices run in a batch
+ * (pj_batch, include/prunejuice.h) whose lanes take queries as they are
+ * admitted; a query that reaches that fixed point is handed to a worker,
+ * which installs its state and continues the step machine from there, and
+ * its lane takes the next query.  Results are those of a context of its own. */
+#define PJH_BATCH_LCC 8
 int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int n, int workers, int flags,
                     const double* arrival_ms, uint64_t budget_bytes, pjh_query_result* out, double* total_ms,
                     char* err, size_t errlen);

@@ -425,9 +425,12 @@ class Pruner:
         return h.value
 
     def run_queries(self, templates: list, workers: int = 4, enumerate_matches: bool = False,
-                    run_tds: bool = True, arrivals_ms=None, budget_bytes: int = 0) -> tuple[list, float]:
+                    run_tds: bool = True, arrivals_ms=None, budget_bytes: int = 0,
+                    batch_lcc: bool = False) -> tuple[list, float]:
         """Batched / served form (pjh_run_queries): every template is solved
-        over this context's resident graph by `workers` contexts sharing it.
+        over this context's resident graph by `workers` contexts sharing it;
+        with batch_lcc the LCC0 and first LCC fixed point of templates of at
+        most 16 vertices run in shared sweeps (PJH_BATCH_LCC).
         Returns (QueryResult per template, total wall ms)."""
         n = len(templates)
         handles = (C.c_void_p * max(n, 1))(*[t.handle.value for t in templates])
@@ -437,7 +440,7 @@ class Pruner:
             arr = (C.c_double * max(n, 1))(*[float(a) for a in arrivals_ms])
         total = C.c_double()
         err = C.create_string_buffer(512)
-        flags = (0 if enumerate_matches else 1) | (0 if run_tds else 2)
+        flags = (0 if enumerate_matches else 1) | (0 if run_tds else 2) | (8 if batch_lcc else 0)
         rc = native.host().pjh_run_queries(self._ctx, handles, n, workers, flags, arr, budget_bytes, out,
                                            C.byref(total), err, 512)
         if rc:

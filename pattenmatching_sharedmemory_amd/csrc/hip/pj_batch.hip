@@ -658,11 +658,12 @@ enum LaneStatus { kFree = 0, kRunning = 1, kParked = 2 };
 struct pj_batch {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, parked = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
   std::mutex mu;
   pjb::GraphView gv{};
   uint32_t G = 0;
+  uint32_t nlanes = 0;  // lanes that take templates (the first nlanes of G * 64)
   BArgs a{};
   // device buffers
   LaneTab* d_tab = nullptr;
@@ -782,12 +783,12 @@ int pj_batch_create(pj_batch** out, const pj_ctx* owner, uint32_t lanes) {
     return -1;
   };
   if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
-      hipEventCreateWithFlags(&b->parked, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
     b->err = "pj_batch_create: stream or event creation failed";
     return fail();
   }
   b->G = (lanes + 63) / 64;
+  b->nlanes = lanes;
   const uint64_t V = b->gv.V, E = b->gv.E, G = b->G, L = G * 64;
   BArgs& a = b->a;
   a.V = b->gv.V;
@@ -854,8 +855,8 @@ int pj_batch_admit(pj_batch* b, const pj_template_desc* d, uint32_t* lane) {
     return -1;
   }
   uint32_t L = 0;
-  while (L < b->status.size() && b->status[L] != kFree) ++L;
-  if (L == b->status.size()) {
+  while (L < b->nlanes && b->status[L] != kFree) ++L;
+  if (L == b->nlanes) {
     b->err = "pj_batch_admit: no free lane";
     return -1;
   }
@@ -920,7 +921,10 @@ int pj_batch_admit(pj_batch* b, const pj_template_desc* d, uint32_t* lane) {
 
 int pj_batch_run(pj_batch* b, uint32_t max_steps, uint32_t* done, uint32_t max_done, uint32_t* n_done) {
   if (!b) return -1;
-  std::lock_guard<std::mutex> lk(b->mu);
+  // The lock guards the lane table; it is let go while the device works, so
+  // that other threads can export and release parked lanes meanwhile.
+  // Admission and runs come from one thread.
+  std::unique_lock<std::mutex> lk(b->mu);
   if (n_done) *n_done = 0;
   PJB_HIP(hipSetDevice(b->device));
   BArgs& a = b->a;
@@ -948,17 +952,20 @@ int pj_batch_run(pj_batch* b, uint32_t max_steps, uint32_t* done, uint32_t max_d
         maxneed = std::max(maxneed, b->maxneed[L]);
       }
     if (!any) break;
+    lk.unlock();
     PJB_HIP(hipEventRecord(b->ev0, b->stream));
     if (launch_step(b, maxneed > 3 ? 4 : 2)) return -1;
     PJB_HIP(hipEventRecord(b->ev1, b->stream));
     PJB_HIP(hipMemcpyAsync(b->h_ctl, a.ctl, b->G * kCtl * 8, hipMemcpyDeviceToHost, b->stream));
     PJB_HIP(hipStreamSynchronize(b->stream));
+    lk.lock();
     float ms = 0.f;
     PJB_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     b->ms += ms;
     ++b->steps;
     // lanes that reached their fixed point in this step
     bool parked_any = false;
+    std::vector<uint64_t> parked_now(b->G, 0);
     for (uint32_t g = 0; g < b->G; ++g) {
       uint64_t newly = 0;
       for (int t = 0; t < 64; ++t) {
@@ -990,13 +997,21 @@ int pj_batch_run(pj_batch* b, uint32_t max_steps, uint32_t* done, uint32_t max_d
       }
       if (park(b, g, newly)) return -1;
       parked_any = true;
-      for (int t = 0; t < 64; ++t)
-        if ((newly >> t) & 1) {
-          const uint32_t L = g * 64 + t;
-          b->status[L] = kParked;
-          if (done && nd < max_done) done[nd] = L;
-          ++nd;
-        }
+      parked_now[g] = newly;
+    }
+    if (parked_any) {
+      // the parked states are complete before any lane is handed out
+      lk.unlock();
+      PJB_HIP(hipStreamSynchronize(b->stream));
+      lk.lock();
+      for (uint32_t g = 0; g < b->G; ++g)
+        for (int t = 0; t < 64; ++t)
+          if ((parked_now[g] >> t) & 1) {
+            const uint32_t L = g * 64 + t;
+            b->status[L] = kParked;
+            if (done && nd < max_done) done[nd] = L;
+            ++nd;
+          }
     }
     // a lane at the step cap without a fixed point
     if (b->steps % 64 == 0) {
@@ -1008,11 +1023,7 @@ int pj_batch_run(pj_batch* b, uint32_t max_steps, uint32_t* done, uint32_t max_d
           return -1;
         }
     }
-    if (parked_any) {
-      PJB_HIP(hipEventRecord(b->parked, b->stream));
-      PJB_HIP(hipStreamSynchronize(b->stream));
-      break;
-    }
+    if (parked_any) break;
   }
   if (n_done) *n_done = nd;
   return 0;
@@ -1047,7 +1058,7 @@ int pj_batch_export(pj_batch* b, uint32_t lane, pj_ctx* ctx) {
   }
   // The park arrays of a parked lane are not written until it is released.
   if (pjb::import_lcc_state(ctx, b->park_cand + lane * 2 * b->vpad, wb, b->park_flags + lane * b->vpad,
-                            b->park_ea + lane * b->ea_stride, av, ae, b->parked) != 0) {
+                            b->park_ea + lane * b->ea_stride, av, ae, nullptr) != 0) {
     std::lock_guard<std::mutex> lk(b->mu);
     b->err = std::string("pj_batch_export: ") + pj_last_error(ctx);
     return -1;
@@ -1076,7 +1087,7 @@ int pj_batch_info(pj_batch* b, uint32_t* lanes, uint32_t* running, uint32_t* par
     r += s == kRunning;
     p += s == kParked;
   }
-  if (lanes) *lanes = static_cast<uint32_t>(b->status.size());
+  if (lanes) *lanes = b->nlanes;
   if (running) *running = r;
   if (parked) *parked = p;
   if (device_ms) *device_ms = b->ms;
@@ -1096,7 +1107,6 @@ void pj_batch_destroy(pj_batch* b) {
   bfree(b);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
-  if (b->parked) hipEventDestroy(b->parked);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
 }

@@ -211,7 +211,7 @@ void reset_modified(MultipleLabelDeviceState* s) {
 
 int run_step_machine(MultipleLabelDeviceState* state, const ConstraintSet& cs,
                      const std::function<void(const StepRecord&)>& on_step, double* total_ms,
-                     bool stop_before_enumeration, bool stop_before_tds) {
+                     bool stop_before_enumeration, bool stop_before_tds, const std::vector<AlgoResults>* prefix) {
   MultipleLabelHip::Lcc0Type lcc0;
   MultipleLabelHip::LccType lcc;
   MultipleLabelHip::CcType cc;
@@ -229,12 +229,6 @@ int run_step_machine(MultipleLabelDeviceState* state, const ConstraintSet& cs,
   machine.initStepPc(static_cast<int>(cs.paths.size()));
   machine.initStepTds(static_cast<int>(cs.tds.size()));
 
-  if (pj_state_reset(state->ctx) != 0) {
-    state->error = 1;
-    state->message = pj_last_error(state->ctx);
-    return -1;
-  }
-  pj_get_counters(state->ctx, &state->graphActiveVertexCount, &state->graphActiveEdgeCount);
   Graph unused;
   AlgoResults results;
   MultipleLabelStep::Step step;
@@ -245,6 +239,42 @@ int run_step_machine(MultipleLabelDeviceState* state, const ConstraintSet& cs,
   const bool trace = std::getenv("PJ_TRACE") != nullptr;  // per-step progress on stderr
   int cc_runs = 0, pc_runs = 0;
   machine.getNextStep(results, &step, &name);
+  if (prefix && !prefix->empty()) {
+    // The batch ran LCC0 and the LCC steps to the first fixed point; the
+    // active counts of each row follow from the eliminations.
+    uint64_t V = 0, E = 0;
+    pj_get_counters(state->ctx, &state->graphActiveVertexCount, &state->graphActiveEdgeCount);
+    for (const AlgoResults& r : *prefix) {
+      V += r.vertexEliminated;
+      E += r.edgeEliminated;
+    }
+    V += state->graphActiveVertexCount;
+    E += state->graphActiveEdgeCount;
+    for (const AlgoResults& r : *prefix) {
+      const bool lcc = step == MultipleLabelStep::E_LCC0 || step == MultipleLabelStep::E_LCC;
+      if (!lcc) {
+        state->error = 1;
+        state->message = "batch rows do not follow the step machine";
+        return -1;
+      }
+      V -= r.vertexEliminated;
+      E -= r.edgeEliminated;
+      if (on_step) on_step(StepRecord{iteration, step, r, V, E, 0, 0.0, 0.0});
+      results = r;
+      finished = machine.getNextStep(results, &step, &name);
+      ++iteration;
+    }
+    if (step == MultipleLabelStep::E_LCC) {
+      state->error = 1;
+      state->message = "batch rows end before the LCC fixed point";
+      return -1;
+    }
+  } else if (pj_state_reset(state->ctx) != 0) {
+    state->error = 1;
+    state->message = pj_last_error(state->ctx);
+    return -1;
+  }
+  pj_get_counters(state->ctx, &state->graphActiveVertexCount, &state->graphActiveEdgeCount);
   while (!finished && iteration < 1000) {
     if (stop_before_enumeration && step == MultipleLabelStep::E_ENUMERATION) break;
     if (stop_before_tds && step == MultipleLabelStep::E_TDS) break;

@@ -423,9 +423,14 @@ struct StepRecord {
 // context holds the graph and template.  Calls `on_step` after every step.
 // With stop_before_enumeration the machine ends at the pruning fixed point;
 // with stop_before_tds it ends when the first TDS step is due.
+// With `prefix` (the LCC0 row and the LCC rows up to the first one that
+// eliminated nothing, as a batch ran them: pj_batch_trace) the state already
+// holds the result of those steps (pj_batch_export): the machine replays the
+// rows without running them and continues from there.
 int run_step_machine(MultipleLabelDeviceState* state, const ConstraintSet& cs,
                      const std::function<void(const StepRecord&)>& on_step, double* total_ms,
-                     bool stop_before_enumeration = false, bool stop_before_tds = false);
+                     bool stop_before_enumeration = false, bool stop_before_tds = false,
+                     const std::vector<AlgoResults>* prefix = nullptr);
 
 // The dynamic constraint order of the reference's article driver
 // (test_article/test_order_multiple_label_cpu.cc:127-290, ORDER_USE_DYNAMIC):

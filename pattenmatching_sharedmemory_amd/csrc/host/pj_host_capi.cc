@@ -4,6 +4,8 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
+#include <deque>
 #include <cstring>
 #include <iostream>
 #include <mutex>
@@ -314,21 +316,104 @@ int pjh_query_uses_memo(const pjh_constraints* c) {
   return 0;
 }
 
+namespace {
+
+// One query on a worker context: installs the template, imports the batch's
+// parked state when `lane` is given, runs the step machine (from the batch's
+// rows when given) and takes the state checksum.
+void solve_query(pj_ctx* ctx, const pjh_constraints* c, int flags, pj_batch* batch, int64_t lane,
+                 pjh_query_result* r, std::string* e) {
+  uint64_t h = 1469598103934665603ull;
+  auto mixin = [&h](uint64_t x) {
+    h ^= x + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    h *= 1099511628211ull;
+  };
+  int steps = 0;
+  double dev = 0.0;
+  if (pj_set_template(ctx, c->c.desc()) != 0) {
+    *e = pj_last_error(ctx);
+    return;
+  }
+  std::vector<pj::AlgoResults> prefix;
+  if (lane >= 0) {
+    uint32_t nrows = 0;
+    std::vector<pj_algo_results> rows(1000);
+    if (pj_batch_trace(batch, static_cast<uint32_t>(lane), rows.data(), static_cast<uint32_t>(rows.size()), &nrows) ||
+        pj_batch_export(batch, static_cast<uint32_t>(lane), ctx) ||
+        pj_batch_release(batch, static_cast<uint32_t>(lane))) {
+      *e = pj_batch_last_error(batch);
+      return;
+    }
+    for (uint32_t k = 0; k < nrows && k < rows.size(); ++k) {
+      pj::AlgoResults a;
+      a.vertexEliminated = rows[k].vertex_eliminated;
+      a.edgeEliminated = rows[k].edge_eliminated;
+      a.matchEliminated = rows[k].match_eliminated;
+      a.enumeration = rows[k].enumeration;
+      prefix.push_back(a);
+    }
+  }
+  pj::MultipleLabelDeviceState st;
+  st.ctx = ctx;
+  st.managed = true;
+  const int rc = pj::run_step_machine(
+      &st, c->c,
+      [&](const pj::StepRecord& rec) {
+        ++steps;
+        mixin(static_cast<uint64_t>(rec.iteration));
+        mixin(static_cast<uint64_t>(rec.step));
+        mixin(rec.results.vertexEliminated);
+        mixin(rec.results.edgeEliminated);
+        mixin(rec.results.matchEliminated);
+        mixin(rec.results.enumeration);
+        mixin(rec.activeVertices);
+        mixin(rec.activeEdges);
+      },
+      &dev, (flags & PJH_STOP_BEFORE_ENUMERATION) != 0, (flags & PJH_STOP_BEFORE_TDS) != 0,
+      lane >= 0 ? &prefix : nullptr);
+  if (rc < 0) *e = st.message.empty() ? std::string(pj_last_error(ctx)) : st.message;
+  if (e->empty() && pj_state_checksum(ctx, &r->state_checksum) != 0) *e = pj_last_error(ctx);
+  pj_get_counters(ctx, &r->active_vertices, &r->active_edges);
+  r->steps = steps;
+  r->rows_hash = h;
+  r->device_ms = dev;
+}
+
+}  // namespace
+
 int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int n, int workers, int flags,
                     const double* arrival_ms, uint64_t budget_bytes, pjh_query_result* out, double* total_ms,
                     char* err, size_t errlen) {
   if (!owner || (n > 0 && (!templates || !out)) || workers < 1) return -1;
-  // every worker owns a context sharing the graph; created before the clock
+  const bool batched = (flags & PJH_BATCH_LCC) != 0;
+  // every worker owns a context sharing the graph; created before the clock,
+  // as is the batch
   std::vector<pj_ctx*> ctxs(workers, nullptr);
+  pj_batch* batch = nullptr;
   int device = 0;
   if (pj_get_device(owner, &device) != 0) return -1;
+  auto cleanup = [&]() {
+    if (batch) pj_batch_destroy(batch);
+    for (auto* c : ctxs)
+      if (c) pj_destroy(c);
+  };
   for (int w = 0; w < workers; ++w) {
     if (pj_create(&ctxs[w], device) || pj_share_graph(ctxs[w], owner)) {
       put_err(ctxs[w] ? pj_last_error(ctxs[w]) : "pj_create failed", err, errlen);
-      for (auto* c : ctxs)
-        if (c) pj_destroy(c);
+      cleanup();
       return -1;
     }
+  }
+  uint32_t lanes = 0;
+  if (batched) {
+    lanes = static_cast<uint32_t>(std::min(256, std::max(64, (n + 63) / 64 * 64)));
+    if (const char* env = std::getenv("PJ_BATCH_LANES")) lanes = static_cast<uint32_t>(std::max(1, std::atoi(env)));
+    if (pj_batch_create(&batch, owner, lanes) != 0) {
+      put_err(batch ? pj_batch_last_error(batch) : "pj_batch_create failed", err, errlen);
+      cleanup();
+      return -1;
+    }
+    pj_batch_info(batch, &lanes, nullptr, nullptr, nullptr, nullptr);
   }
   uint64_t vc = 0, ec = 0;
   pj_get_counters(ctxs[0], &vc, &ec);  // a fresh context: V and E
@@ -343,87 +428,135 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
   if (budget_bytes && any_memo) budget_bytes = budget_bytes > PJ_RETRY_POOL_BYTES ? budget_bytes - PJ_RETRY_POOL_BYTES : 1;
   std::mutex mu;
   std::condition_variable cv;
-  int next = 0;
+  int next = 0;  // next query to admit
   uint64_t reserved = 0;
   std::string first_error;
+  // batched: queries ready for a worker (query, parked lane or -1)
+  std::deque<std::pair<int, int64_t>> ready;
+  bool admission_done = false;
   const auto t0 = std::chrono::steady_clock::now();
   auto now_ms = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
+  auto fits = [&](int i) { return budget_bytes == 0 || reserved == 0 || reserved + projected[i] <= budget_bytes; };
+  auto begin = [&](int i) {
+    pjh_query_result& r = out[i];
+    r = pjh_query_result{};
+    r.projected_bytes = projected[i];
+    r.arrival_ms = arrival_ms ? arrival_ms[i] : 0.0;
+    r.start_ms = now_ms();
+    reserved += projected[i];
+  };
+  auto finish = [&](int i, int w, const std::string& e) {
+    out[i].worker = w;
+    out[i].end_ms = now_ms();
+    std::lock_guard<std::mutex> lk(mu);
+    reserved -= projected[i];
+    if (!e.empty() && first_error.empty()) first_error = "query " + std::to_string(i) + ": " + e;
+  };
 
   auto worker = [&](int w) {
     pj_ctx* ctx = ctxs[w];
     while (true) {
       int i;
+      int64_t lane = -1;
       {
         std::unique_lock<std::mutex> lk(mu);
-        while (true) {
-          if (next >= n || !first_error.empty()) return;
-          const double due = arrival_ms ? arrival_ms[next] : 0.0;
-          const double t = now_ms();
-          const bool fits = budget_bytes == 0 || reserved == 0 || reserved + projected[next] <= budget_bytes;
-          if (t >= due && fits) break;
-          if (t < due)
-            cv.wait_for(lk, std::chrono::duration<double, std::milli>(due - t));
-          else
-            cv.wait(lk);
+        if (batched) {
+          cv.wait(lk, [&] { return !ready.empty() || admission_done || !first_error.empty(); });
+          if (!first_error.empty() || ready.empty()) return;
+          i = ready.front().first;
+          lane = ready.front().second;
+          ready.pop_front();
+        } else {
+          while (true) {
+            if (next >= n || !first_error.empty()) return;
+            const double due = arrival_ms ? arrival_ms[next] : 0.0;
+            const double t = now_ms();
+            if (t >= due && fits(next)) break;
+            if (t < due)
+              cv.wait_for(lk, std::chrono::duration<double, std::milli>(due - t));
+            else
+              cv.wait(lk);
+          }
+          i = next++;
+          begin(i);
         }
-        i = next++;
-        reserved += projected[i];
       }
-      pjh_query_result& r = out[i];
-      r = pjh_query_result{};
-      r.worker = w;
-      r.projected_bytes = projected[i];
-      r.arrival_ms = arrival_ms ? arrival_ms[i] : 0.0;
-      r.start_ms = now_ms();
       std::string e;
-      uint64_t h = 1469598103934665603ull;
-      auto mixin = [&h](uint64_t x) {
-        h ^= x + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
-        h *= 1099511628211ull;
-      };
-      int steps = 0;
-      double dev = 0.0;
-      if (pj_set_template(ctx, templates[i]->c.desc()) != 0) {
-        e = pj_last_error(ctx);
-      } else {
-        pj::MultipleLabelDeviceState st;
-        st.ctx = ctx;
-        st.managed = true;
-        const int rc = pj::run_step_machine(
-            &st, templates[i]->c,
-            [&](const pj::StepRecord& rec) {
-              ++steps;
-              mixin(static_cast<uint64_t>(rec.iteration));
-              mixin(static_cast<uint64_t>(rec.step));
-              mixin(rec.results.vertexEliminated);
-              mixin(rec.results.edgeEliminated);
-              mixin(rec.results.matchEliminated);
-              mixin(rec.results.enumeration);
-              mixin(rec.activeVertices);
-              mixin(rec.activeEdges);
-            },
-            &dev, (flags & PJH_STOP_BEFORE_ENUMERATION) != 0, (flags & PJH_STOP_BEFORE_TDS) != 0);
-        if (rc < 0) e = st.message.empty() ? std::string(pj_last_error(ctx)) : st.message;
-        if (e.empty() && pj_state_checksum(ctx, &r.state_checksum) != 0) e = pj_last_error(ctx);
-        pj_get_counters(ctx, &r.active_vertices, &r.active_edges);
-      }
-      r.steps = steps;
-      r.rows_hash = h;
-      r.device_ms = dev;
-      r.end_ms = now_ms();
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        reserved -= projected[i];
-        if (!e.empty() && first_error.empty()) first_error = "query " + std::to_string(i) + ": " + e;
-      }
+      solve_query(ctx, templates[i], flags, batch, lane, &out[i], &e);
+      finish(i, w, e);
       cv.notify_all();
     }
   };
   std::vector<std::thread> threads;
   for (int w = 0; w < workers; ++w) threads.emplace_back(worker, w);
+
+  if (batched) {
+    // Admission into the batch's lanes (templates of more than 16 vertices
+    // go to the workers as they are); every step hands the templates that
+    // reached their LCC fixed point to the workers and frees their lanes.
+    std::vector<int> lane_query(lanes, -1);
+    std::vector<uint32_t> done(lanes);
+    while (true) {
+      uint32_t running = 0, parked = 0;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        if (!first_error.empty()) break;
+        pj_batch_info(batch, nullptr, &running, &parked, nullptr, nullptr);
+        while (next < n) {
+          const double due = arrival_ms ? arrival_ms[next] : 0.0;
+          if (now_ms() < due || !fits(next)) break;
+          if (templates[next]->c.vertex_count > 16) {
+            begin(next);
+            ready.emplace_back(next, -1);
+            ++next;
+            cv.notify_all();
+            continue;
+          }
+          if (running + parked >= lanes) break;
+          uint32_t lane = 0;
+          if (pj_batch_admit(batch, templates[next]->c.desc(), &lane) != 0) {
+            first_error = std::string("query ") + std::to_string(next) + ": " + pj_batch_last_error(batch);
+            break;
+          }
+          begin(next);
+          lane_query[lane] = next;
+          ++running;
+          ++next;
+        }
+        if (!first_error.empty()) break;
+        if (!running) {
+          if (next >= n) break;
+          // nothing to sweep: wait for the next arrival, a lane or budget
+          const double due = arrival_ms ? arrival_ms[next] : 0.0;
+          const double t = now_ms();
+          if (t < due)
+            cv.wait_for(lk, std::chrono::duration<double, std::milli>(due - t));
+          else
+            cv.wait_for(lk, std::chrono::milliseconds(1));
+          continue;
+        }
+      }
+      uint32_t nd = 0;
+      if (pj_batch_run(batch, 1, done.data(), lanes, &nd) != 0) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (first_error.empty()) first_error = std::string("batch: ") + pj_batch_last_error(batch);
+        break;
+      }
+      if (nd) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (uint32_t k = 0; k < nd; ++k) ready.emplace_back(lane_query[done[k]], static_cast<int64_t>(done[k]));
+      }
+      cv.notify_all();
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      admission_done = true;
+    }
+    cv.notify_all();
+  }
   for (auto& t : threads) t.join();
   if (total_ms) *total_ms = now_ms();
-  for (auto* c : ctxs) pj_destroy(c);
+  cleanup();
   if (!first_error.empty()) {
     put_err(first_error, err, errlen);
     return -1;

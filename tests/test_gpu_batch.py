@@ -118,6 +118,8 @@ def test_batch_rejects_what_it_cannot_hold(pj):
         batch.admit(big)
     small = _templates(pj, 2, seed=3)
     batch.admit(small[0])
-    assert batch.info()["lanes"] == 64
+    assert batch.info()["lanes"] == 1
+    with pytest.raises(RuntimeError, match="no free lane"):
+        batch.admit(small[1])
     batch.close()
     owner.close()

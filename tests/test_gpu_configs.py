@@ -109,8 +109,11 @@ def test_batched_configuration_templates_against_the_oracle(pj, orc):
     alone, _ = owner.run_queries(cs, workers=1, run_tds=True)
     batch, _ = owner.run_queries(cs, workers=4, run_tds=True)
     assert len({r.worker for r in batch}) > 1
+    # the LCC0 and first LCC fixed point of every template in shared sweeps
+    shared, _ = owner.run_queries(cs, workers=4, run_tds=True, batch_lcc=True)
     for i in determined:
         assert alone[i].result() == batch[i].result(), i
+        assert alone[i].result() == shared[i].result(), i
         assert batch[i].state_checksum == oracle_checksum(pj, g, cs[i], finals[i]), i
     owner.close()
 
@@ -131,12 +134,17 @@ def test_served_configuration_templates_against_the_oracle(pj, orc):
     arrivals = np.cumsum(rng.exponential(1.0, len(cs)))
     budget = int(max(r.projected_bytes for r in alone) * 3)
     served, _ = owner.run_queries(cs, workers=6, run_tds=True, arrivals_ms=arrivals, budget_bytes=budget)
-    for r, a in zip(served, arrivals):
-        assert r.start_ms + 1e-6 >= a
-    for r in served:
-        live = sum(q.projected_bytes for q in served if q.start_ms <= r.start_ms < q.end_ms)
-        assert live <= budget
+    # continuous batching: arrivals join the shared LCC sweeps as lanes free up
+    shared, _ = owner.run_queries(cs, workers=6, run_tds=True, arrivals_ms=arrivals, budget_bytes=budget,
+                                  batch_lcc=True)
+    for form in (served, shared):
+        for r, a in zip(form, arrivals):
+            assert r.start_ms + 1e-6 >= a
+        for r in form:
+            live = sum(q.projected_bytes for q in form if q.start_ms <= r.start_ms < q.end_ms)
+            assert live <= budget
     for i in determined:
         assert alone[i].result() == served[i].result(), i
+        assert alone[i].result() == shared[i].result(), i
         assert served[i].state_checksum == oracle_checksum(pj, g, cs[i], finals[i]), i
     owner.close()

@@ -39,6 +39,27 @@ def test_batched_results_equal_separate_solves(pj):
     owner.close()
 
 
+def test_shared_lcc_sweeps_give_every_template_its_own_result(pj):
+    """PJH_BATCH_LCC: templates of at most 16 vertices run LCC0 and their
+    first LCC fixed point in a batch, with fewer lanes than templates (lanes
+    are taken again as templates leave), a template of 18 vertices goes to
+    the workers as it is; every result equals the template's own solve."""
+    import os
+
+    g, cs = _setup(pj, 40, seed=7)
+    k = 18
+    cs.append(pj.Constraints(pj.Graph.pattern(k, [(i, i + 1) for i in range(k - 1)], [i % 6 for i in range(k)])))
+    owner = pj.Pruner(g)
+    seq, _ = owner.run_queries(cs, workers=1, run_tds=False)
+    os.environ["PJ_BATCH_LANES"] = "16"
+    try:
+        shared, _ = owner.run_queries(cs, workers=3, run_tds=False, batch_lcc=True)
+    finally:
+        del os.environ["PJ_BATCH_LANES"]
+    assert [r.result() for r in seq] == [r.result() for r in shared]
+    owner.close()
+
+
 def test_served_results_equal_alone_and_admission_respects_budget(pj):
     g, cs = _setup(pj, 24, seed=9)
     owner = pj.Pruner(g)
