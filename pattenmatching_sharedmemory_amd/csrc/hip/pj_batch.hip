@@ -13,9 +13,10 @@
 //
 // HBM layout of a batch of G groups (V vertices, E slots):
 //   cand, unm  u16[G][V][64]  candidate set / toUnmatch of each template
-//   mod, sched, act, pend  u64[G][V]  per-template bits: modified (sticky
-//              until the state leaves the batch), scheduled, active
-//              (cand != 0), toUnmatch pending for phase 2
+//   mod, sched, act, pend, dirty  u64[G][V]  per-template bits: modified
+//              (sticky until the state leaves the batch), scheduled, active
+//              (cand != 0), toUnmatch pending for phase 2, candidates of v or
+//              of a neighbour changed since v's last evaluation
 //   ea         u64[G][E]      per-template edge-active bits of each slot
 //   blab       u8[V]          index of the vertex label in the union of the
 //              batch's template labels (0xFF: no template has it)
@@ -55,7 +56,7 @@ constexpr uint8_t kNoLab = 0xFF;
 constexpr uint32_t kMaxLabels = 255;  // labels of the union (index 0xFF means none)
 constexpr uint8_t kFlagModified = 2, kFlagScheduled = 4;  // flag bytes of a context
 // control words per group
-constexpr int kRun = 0, kInit = 1, kFirst = 2, kDone = 3, kCtl = 4;
+constexpr int kRun = 0, kInit = 1, kFirst = 2, kDone = 3, kSecond = 4, kCtl = 8;
 
 // Tables of one template lane.  ent[ent_off[p] .. ent_off[p + 1]) are the
 // requirements of template vertex p (multiple_label_generate_constraint_
@@ -87,6 +88,7 @@ struct BArgs {
   uint64_t* sched;
   uint64_t* act;
   uint64_t* pend;
+  uint64_t* dirty;              // [G][V] templates whose candidates at v or at a neighbour changed
   uint64_t* ea;
   const uint8_t* blab;
   const LaneTab* tab;
@@ -211,6 +213,7 @@ __global__ void kb_admit(BArgs a, uint32_t g, uint64_t mask) {
     c[kInit] |= mask;
     c[kRun] &= ~mask;
     c[kFirst] &= ~mask;
+    c[kSecond] &= ~mask;
     c[kDone] &= ~mask;
   }
   if (blockIdx.x == 0 && threadIdx.x < 64 && ((mask >> threadIdx.x) & 1)) a.lane_step[g * 64 + threadIdx.x] = 0;
@@ -276,39 +279,73 @@ __global__ void __launch_bounds__(kBlockB) kb_lcc0(BArgs a) {
 }
 
 // Frontiers.  mode 1 (phase 1): vertices scheduled and active in a running
-// template, rows of at most hub_degree slots (hub rows go by chunks).
+// template that must be evaluated, rows of at most hub_degree slots (hub rows
+// go by chunks).  A template in its first or second LCC step evaluates every
+// scheduled vertex; later, only those whose candidates or whose neighbours'
+// candidates changed since their last evaluation (dirty): a scheduled vertex
+// without such a change would remove nothing and eliminate no slot (its
+// removals were applied, its eliminated slots stay eliminated, its counts are
+// the same), so skipping it changes no result, flag or counter.
 // mode 2 (phase 2): vertices with phase 2 work in a running template: a
 // modified template with removals pending or not yet scheduled, or an
 // unmodified active template whose scheduled bit must be recomputed (every
 // one in its first LCC step; later only those not scheduled, see kb_phase2).
+// Each block compacts tiles of kTile consecutive vertices with one atomic.
+constexpr int kFrontierPer = 8;  // vertices per thread
+constexpr uint32_t kTile = kBlockB * kFrontierPer;
+
 __global__ void __launch_bounds__(kBlockB) kb_frontier(BArgs a, int mode) {
   const uint32_t g = blockIdx.y;
-  const uint64_t run = a.ctl[g * kCtl + kRun];
+  const uint64_t* ctl = a.ctl + g * kCtl;
+  const uint64_t run = ctl[kRun];
   if (!run) return;
-  const uint64_t first = a.ctl[g * kCtl + kFirst];
+  const uint64_t first = ctl[kFirst], full = first | ctl[kSecond];
   const uint64_t gV = static_cast<uint64_t>(g) * a.V;
   uint32_t* list = (mode == 1 ? a.list1 : a.list2) + gV;
   uint32_t* cnt = a.count + g * 4 + mode;
-  const int t = lane_id();
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  const uint64_t span = (a.V + stride - 1) / stride * stride;
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < span; v += stride) {
-    bool in = false;
-    if (v < a.V) {
+  __shared__ uint32_t s_wave[kWaves];
+  __shared__ uint32_t s_base;
+  const int t = lane_id(), wv = threadIdx.x >> 6;
+  for (uint64_t tile = blockIdx.x; tile * kTile < a.V; tile += gridDim.x) {
+    const uint64_t v0 = tile * kTile + static_cast<uint64_t>(threadIdx.x) * kFrontierPer;
+    uint32_t bits = 0;
+#pragma unroll
+    for (int k = 0; k < kFrontierPer; ++k) {
+      const uint64_t v = v0 + k;
+      if (v >= a.V) break;
       const uint64_t gv = gV + v;
+      bool in;
       if (mode == 1) {
-        in = (a.sched[gv] & a.act[gv] & run) != 0 && a.off[v + 1] - a.off[v] <= a.hub_degree;
+        in = (a.sched[gv] & a.act[gv] & run & (a.dirty[gv] | full)) != 0 && a.off[v + 1] - a.off[v] <= a.hub_degree;
       } else {
-        const uint64_t ac = a.act[gv] & run, m = a.mod[gv], s = a.sched[gv], p = a.pend[gv];
-        in = ((ac & m & (p | ~s)) | (ac & ~m & (first | ~s))) != 0;
+        const uint64_t ac = a.act[gv] & run, m = a.mod[gv], sc = a.sched[gv], p = a.pend[gv];
+        in = ((ac & m & (p | ~sc)) | (ac & ~m & (first | ~sc))) != 0;
       }
+      bits |= static_cast<uint32_t>(in) << k;
     }
-    const uint64_t b = __ballot(in);
-    if (!b) continue;
-    uint32_t base = 0;
-    if (t == 0) base = atomicAdd(cnt, static_cast<uint32_t>(__builtin_popcountll(b)));
-    base = __shfl(base, 0, 64);
-    if (in) list[base + __builtin_popcountll(b & ((1ull << t) - 1))] = static_cast<uint32_t>(v);
+    const uint32_t n = __builtin_popcount(bits);
+    uint32_t x = n;  // inclusive prefix over the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (t >= o) x += y;
+    }
+    if (t == 63) s_wave[wv] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t tot = 0;
+      for (int w = 0; w < kWaves; ++w) {
+        const uint32_t c = s_wave[w];
+        s_wave[w] = tot;
+        tot += c;
+      }
+      s_base = tot ? atomicAdd(cnt, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t pos = s_base + s_wave[wv] + x - n;
+    for (int k = 0; k < kFrontierPer; ++k)
+      if ((bits >> k) & 1) list[pos++] = static_cast<uint32_t>(v0 + k);
+    __syncthreads();
   }
 }
 
@@ -328,6 +365,7 @@ __global__ void __launch_bounds__(kBlockB) kb_phase1(BArgs a) {
   const uint32_t g = blockIdx.y;
   const uint64_t run = a.ctl[g * kCtl + kRun];
   if (!run) return;
+  const uint64_t full = a.ctl[g * kCtl + kFirst] | a.ctl[g * kCtl + kSecond];
   __shared__ Word s_nbr[64 * kMaxP];
   __shared__ uint8_t s_off[64 * (kMaxP + 1)];
   for (uint32_t i = threadIdx.x; i < 64 * kMaxP; i += blockDim.x) s_nbr[i] = a.tab[g * 64 + i / kMaxP].nbr[i % kMaxP];
@@ -360,7 +398,8 @@ __global__ void __launch_bounds__(kBlockB) kb_phase1(BArgs a) {
       const uint64_t row_end = a.off[v + 1];
       end = row_end < beg + a.hub_chunk ? row_end : beg + a.hub_chunk;
     }
-    const uint64_t w = a.sched[gV + v] & a.act[gV + v] & run;
+    const uint64_t dv = a.dirty[gV + v];
+    const uint64_t w = a.sched[gV + v] & a.act[gV + v] & run & (dv | full);
     if (!w) continue;
     const Word C = ((w >> t) & 1) ? cand[static_cast<uint64_t>(v) * 64 + t] : Word(0);
     Word U = 0;
@@ -401,6 +440,7 @@ __global__ void __launch_bounds__(kBlockB) kb_phase1(BArgs a) {
       if (fail) a.unm[(gV + v) * 64 + t] = fail;
       if (t == 0 && M) a.mod[gV + v] |= M;
       if (t == 1 && F) a.pend[gV + v] |= F;
+      if (t == 2 && (dv & w)) a.dirty[gV + v] = dv & ~w;
     } else {
       if (C) {
         uint32_t* hc = a.hub_cnt + ((static_cast<uint64_t>(g) * a.nhubs + h) * 64 + t) * 16;
@@ -430,6 +470,7 @@ __global__ void __launch_bounds__(kBlockB) kb_hub_finish(BArgs a) {
   const uint32_t g = blockIdx.y;
   const uint64_t run = a.ctl[g * kCtl + kRun];
   if (!run || !a.nhubs) return;
+  const uint64_t full = a.ctl[g * kCtl + kFirst] | a.ctl[g * kCtl + kSecond];
   const int t = lane_id();
   const LaneTab& tb = a.tab[g * 64 + t];
   const uint64_t gV = static_cast<uint64_t>(g) * a.V;
@@ -437,7 +478,8 @@ __global__ void __launch_bounds__(kBlockB) kb_hub_finish(BArgs a) {
   const uint32_t nw = gridDim.x * kWaves;
   for (uint32_t h = wave; h < a.nhubs; h += nw) {
     const uint32_t v = a.hubs[h];
-    const uint64_t w = a.sched[gV + v] & a.act[gV + v] & run;
+    const uint64_t dv = a.dirty[gV + v];
+    const uint64_t w = a.sched[gV + v] & a.act[gV + v] & run & (dv | full);
     if (!w) continue;
     Word fail = 0;
     if ((w >> t) & 1) {
@@ -470,6 +512,7 @@ __global__ void __launch_bounds__(kBlockB) kb_hub_finish(BArgs a) {
     if (t == 0 && M) a.mod[gV + v] |= M;
     if (t == 1 && F) a.pend[gV + v] |= F;
     if (t == 2 && EL) *he = 0;
+    if (t == 3 && (dv & w)) a.dirty[gV + v] = dv & ~w;
   }
 }
 
@@ -501,13 +544,14 @@ __global__ void __launch_bounds__(kBlockB) kb_phase2(BArgs a) {
     const uint64_t ac = a.act[gv], m = a.mod[gv], s = a.sched[gv], p = a.pend[gv];
     const uint64_t A = ac & run & m & (p | ~s);
     const uint64_t N = ac & run & ~m & (first | ~s);
-    bool died = false;
+    bool died = false, changed = false;
     if (((A & p) >> t) & 1) {
       const uint64_t idx = gv * 64 + t;
       const Word c = a.cand[idx], T = a.unm[idx];
       const Word nc = c & ~T;
       if (T) {
         a.cand[idx] = nc;
+        changed = true;
         me += __builtin_popcount(T);
         if (!nc) {
           ++ve;
@@ -516,6 +560,28 @@ __global__ void __launch_bounds__(kBlockB) kb_phase2(BArgs a) {
       }
     }
     const uint64_t D = __ballot(died);
+    // Templates past their first LCC step whose candidates changed at v mark
+    // v and its neighbours dirty (their next evaluation can change).
+    const uint64_t push = __ballot(changed) & ~first;
+    if (push) {
+      if (t == 0) atomicOr(reinterpret_cast<unsigned long long*>(&a.dirty[gv]), static_cast<unsigned long long>(push));
+      const uint64_t beg = a.off[v], end = a.off[v + 1];
+      for (uint64_t base = beg; base < end; base += 4 * 64) {
+        uint32_t u[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const uint64_t e = base + k * 64 + t;
+          u[k] = e < end ? a.edges[e] : 0xFFFFFFFFu;
+        }
+        uint64_t d[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) d[k] = u[k] != 0xFFFFFFFFu ? a.dirty[gV + u[k]] : push;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if ((d[k] & push) != push)
+            atomicOr(reinterpret_cast<unsigned long long*>(&a.dirty[gV + u[k]]), static_cast<unsigned long long>(push));
+      }
+    }
     uint64_t H = 0;
     if (N) {
       const uint64_t beg = a.off[v], end = a.off[v + 1];
@@ -567,6 +633,7 @@ __global__ void kb_update(BArgs a) {
   if (t == 0) {
     c[kRun] = (run & ~C) | init;
     c[kInit] = 0;
+    c[kSecond] = c[kFirst] & ~C;
     c[kFirst] = init;
     c[kDone] = done | C;
   }
@@ -706,7 +773,7 @@ int balloc(pj_batch* b, T** p, uint64_t count) {
 }
 
 void bfree(pj_batch* b) {
-  void* bufs[] = {b->a.cand, b->a.unm, b->a.mod, b->a.sched, b->a.act, b->a.pend, b->a.ea, b->a.ctl, b->a.lane_step,
+  void* bufs[] = {b->a.cand, b->a.unm, b->a.mod, b->a.sched, b->a.act, b->a.pend, b->a.dirty, b->a.ea, b->a.ctl, b->a.lane_step,
                   b->a.trace, b->a.list1, b->a.list2, b->a.count, b->a.hub_cnt, b->a.hub_elim, b->d_tab, b->d_union,
                   b->d_blab, b->park_cand, b->park_flags, b->park_ea};
   for (void* p : bufs)
@@ -727,7 +794,8 @@ int launch_step(pj_batch* b, int nb) {
   const int cus = b->gv.cus;
   hipLaunchKernelGGL(kb_fill, dim3(1), dim3(64), 0, b->stream, a.count, b->G * 4, 0u);
   hipLaunchKernelGGL(kb_lcc0, dim3(cus * 4, b->G), dim3(kBlockB), 64 * a.nlab * sizeof(uint16_t), b->stream, a);
-  hipLaunchKernelGGL(kb_frontier, dim3(grid_cap(b, a.V, 16), b->G), dim3(kBlockB), 0, b->stream, a, 1);
+  const uint32_t tiles = static_cast<uint32_t>(std::min<uint64_t>((a.V + kTile - 1) / kTile, cus * 4));
+  hipLaunchKernelGGL(kb_frontier, dim3(tiles, b->G), dim3(kBlockB), 0, b->stream, a, 1);
   if (nb == 4)
     hipLaunchKernelGGL(kb_phase1<4>, dim3(cus * 8, b->G), dim3(kBlockB), 0, b->stream, a);
   else
@@ -737,7 +805,7 @@ int launch_step(pj_batch* b, int nb) {
                                                                                   cus * 4)),
                                            b->G),
                        dim3(kBlockB), 0, b->stream, a);
-  hipLaunchKernelGGL(kb_frontier, dim3(grid_cap(b, a.V, 16), b->G), dim3(kBlockB), 0, b->stream, a, 2);
+  hipLaunchKernelGGL(kb_frontier, dim3(tiles, b->G), dim3(kBlockB), 0, b->stream, a, 2);
   hipLaunchKernelGGL(kb_phase2, dim3(cus * 8, b->G), dim3(kBlockB), 0, b->stream, a);
   hipLaunchKernelGGL(kb_update, dim3(b->G), dim3(64), 0, b->stream, a);
   PJB_HIP(hipGetLastError());
@@ -807,6 +875,7 @@ int pj_batch_create(pj_batch** out, const pj_ctx* owner, uint32_t lanes) {
   b->ea_stride = (E + 63) / 64 * 2;
   if (balloc(b, &a.cand, G * V * 64) || balloc(b, &a.unm, G * V * 64) || balloc(b, &a.mod, G * V) ||
       balloc(b, &a.sched, G * V) || balloc(b, &a.act, G * V) || balloc(b, &a.pend, G * V) ||
+      balloc(b, &a.dirty, G * V) ||
       balloc(b, &a.ea, G * E) || balloc(b, &a.ctl, G * kCtl) || balloc(b, &a.lane_step, L) ||
       balloc(b, &a.trace, L * kMaxSteps * 4) || balloc(b, &a.list1, G * V) || balloc(b, &a.list2, G * V) ||
       balloc(b, &a.count, G * 4) || balloc(b, &a.hub_cnt, G * std::max<uint64_t>(1, a.nhubs) * 64 * 16) ||
@@ -829,6 +898,7 @@ int pj_batch_create(pj_batch** out, const pj_ctx* owner, uint32_t lanes) {
       hipMemsetAsync(a.mod, 0, G * V * 8, b->stream) != hipSuccess ||
       hipMemsetAsync(a.sched, 0, G * V * 8, b->stream) != hipSuccess ||
       hipMemsetAsync(a.pend, 0, G * V * 8, b->stream) != hipSuccess ||
+      hipMemsetAsync(a.dirty, 0, G * V * 8, b->stream) != hipSuccess ||
       hipMemsetAsync(a.ea, 0, G * E * 8, b->stream) != hipSuccess ||
       hipMemsetAsync(b->d_tab, 0, L * sizeof(LaneTab), b->stream) != hipSuccess ||
       hipStreamSynchronize(b->stream) != hipSuccess) {
