@@ -328,8 +328,10 @@ typedef struct pj_batch pj_batch;
  * the graph resident in `graph_owner` (which must outlive the batch).  On
  * failure *batch may still be set for pj_batch_last_error; destroy it. */
 int pj_batch_create(pj_batch** batch, const pj_ctx* graph_owner, uint32_t lanes);
-/* Admits a template of at most 16 vertices into a free lane (its LCC0 runs
- * in the next step); -1 when no lane is free or the template does not fit. */
+/* Admits a template into a free lane (its LCC0 runs in the next step).
+ * Returns -2 when the batch cannot hold the template (more than 16 vertices,
+ * or more than 255 labels over the batch's templates): solve it in a context
+ * of its own; -1 on other errors, such as no free lane. */
 int pj_batch_admit(pj_batch* batch, const pj_template_desc* desc, uint32_t* lane);
 /* Runs up to `max_steps` steps (each: LCC0 of the lanes admitted since the
  * last step and one LCC step of every running lane), returning after the

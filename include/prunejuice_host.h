@@ -121,7 +121,7 @@ typedef struct pjh_query_result {
   double device_ms;         /* summed step device time */
 } pjh_query_result;
 /* With PJH_BATCH_LCC in `flags` the LCC0 step and the LCC steps to the first
- * fixed point of every template of at most 16 vertices run in a batch
+ * fixed point of every template the batch can hold (pj_batch_admit) run in a batch
  * (pj_batch, include/prunejuice.h) whose lanes take queries as they are
  * admitted; a query that reaches that fixed point is handed to a worker,
  * which installs its state and continues the step machine from there, and

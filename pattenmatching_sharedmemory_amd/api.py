@@ -504,9 +504,14 @@ class Batch:
     def __del__(self):
         self.close()
 
-    def admit(self, constraints: Constraints) -> int:
+    def admit(self, constraints: Constraints, strict: bool = True) -> int | None:
+        """The lane the template takes.  A template the batch cannot hold
+        (pj_batch_admit returns -2) raises, or gives None when not strict."""
         lane = C.c_uint32()
-        self._check(native.hip().pj_batch_admit(self._b, constraints.desc, C.byref(lane)))
+        rc = native.hip().pj_batch_admit(self._b, constraints.desc, C.byref(lane))
+        if rc == -2 and not strict:
+            return None
+        self._check(rc)
         self.constraints[lane.value] = constraints
         return lane.value
 

@@ -922,7 +922,7 @@ int pj_batch_admit(pj_batch* b, const pj_template_desc* d, uint32_t* lane) {
   const uint32_t P = d->vertex_count;
   if (P == 0 || P > kMaxP) {
     b->err = "pj_batch_admit: the batch takes templates of 1..16 vertices";
-    return -1;
+    return -2;
   }
   uint32_t L = 0;
   while (L < b->nlanes && b->status[L] != kFree) ++L;
@@ -946,7 +946,7 @@ int pj_batch_admit(pj_batch* b, const pj_template_desc* d, uint32_t* lane) {
     const int li = index_of(d->vertex_label[p]);
     if (li < 0) {
       b->err = "pj_batch_admit: the batch's templates carry more than 255 labels";
-      return -1;
+      return -2;
     }
     tb.labmask[li] |= static_cast<Word>(1u << p);
     tb.nbr[p] = static_cast<Word>(d->neighbor_mask[p] & 0xFFFFu);
@@ -964,7 +964,7 @@ int pj_batch_admit(pj_batch* b, const pj_template_desc* d, uint32_t* lane) {
       }
       if (nent >= static_cast<uint32_t>(kMaxEnt)) {
         b->err = "pj_batch_admit: too many requirement entries";
-        return -1;
+        return -2;
       }
       const uint32_t need = std::min<uint32_t>(d->need_count[i], 127u);
       maxneed = std::max(maxneed, need);

@@ -491,9 +491,9 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
   for (int w = 0; w < workers; ++w) threads.emplace_back(worker, w);
 
   if (batched) {
-    // Admission into the batch's lanes (templates of more than 16 vertices
-    // go to the workers as they are); every step hands the templates that
-    // reached their LCC fixed point to the workers and frees their lanes.
+    // Admission into the batch's lanes (templates the batch cannot hold go to
+    // the workers as they are); every step hands the templates that reached
+    // their LCC fixed point to the workers and frees their lanes.
     std::vector<int> lane_query(lanes, -1);
     std::vector<uint32_t> done(lanes);
     while (true) {
@@ -505,16 +505,17 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
         while (next < n) {
           const double due = arrival_ms ? arrival_ms[next] : 0.0;
           if (now_ms() < due || !fits(next)) break;
-          if (templates[next]->c.vertex_count > 16) {
+          if (running + parked >= lanes) break;
+          uint32_t lane = 0;
+          const int rc = pj_batch_admit(batch, templates[next]->c.desc(), &lane);
+          if (rc == -2) {  // the batch cannot hold it: a worker solves it whole
             begin(next);
             ready.emplace_back(next, -1);
             ++next;
             cv.notify_all();
             continue;
           }
-          if (running + parked >= lanes) break;
-          uint32_t lane = 0;
-          if (pj_batch_admit(batch, templates[next]->c.desc(), &lane) != 0) {
+          if (rc != 0) {
             first_error = std::string("query ") + std::to_string(next) + ": " + pj_batch_last_error(batch);
             break;
           }

@@ -64,10 +64,14 @@ def test_batch_lcc_matches_the_oracle(pj, orc):
     g = pj.Graph.rmat(15, 16, seed=3).labels_zipf(32, 1.2, seed=4)
     deg = np.diff(g.offsets().astype(np.int64))
     assert (deg > 2048).sum() > 0  # hub rows go by chunks
-    cs = _special(pj) + _templates(pj, 70, seed=11)
+    cs = _special(pj) + _templates(pj, 80, seed=11)
     owner = pj.Pruner(g)
     batch = pj.Batch(owner, lanes=len(cs))
-    lanes = [batch.admit(c) for c in cs]
+    lanes = [batch.admit(c, strict=False) for c in cs]
+    held = [(lane, c) for lane, c in zip(lanes, cs) if lane is not None]
+    assert len(held) >= 70 and all(lane is not None for lane in lanes[:4])
+    cs = [c for _, c in held]
+    lanes = [lane for lane, _ in held]
     assert len(set(lanes)) == len(cs)
     done = batch.run_all()
     assert sorted(done) == sorted(lanes)
@@ -84,7 +88,7 @@ def test_lanes_admitted_while_others_run(pj, orc):
     """Continuous admission: templates join while others are mid-way, parked
     lanes are released and taken by new templates."""
     g = pj.Graph.rmat(14, 16, seed=5).labels_zipf(32, 1.2, seed=6)
-    cs = _templates(pj, 40, seed=23)
+    cs = _templates(pj, 40, seed=23, max_vertices=8)
     owner = pj.Pruner(g)
     batch = pj.Batch(owner, lanes=16)
     pr = pj.Pruner(g)
@@ -94,7 +98,11 @@ def test_lanes_admitted_while_others_run(pj, orc):
     while pending or batch.info()["running"]:
         while pending and batch.info()["running"] + batch.info()["parked"] < 16:
             i = pending.pop(0)
-            lane_of[batch.admit(cs[i])] = i
+            lane = batch.admit(cs[i], strict=False)
+            if lane is None:  # not for the batch
+                checked += 1
+                continue
+            lane_of[lane] = i
             if len(pending) % 3 == 0:
                 break  # the rest joins after another step
         for lane in batch.run(max_steps=1):
@@ -113,7 +121,7 @@ def test_batch_rejects_what_it_cannot_hold(pj):
     g = pj.Graph.rmat(10, 16, seed=1).labels_zipf(8, 1.2, seed=2)
     owner = pj.Pruner(g)
     batch = pj.Batch(owner, lanes=1)
-    big = pj.Constraints(pj.Graph.pattern(17, [(i, i + 1) for i in range(16)], [0] * 17))
+    big = pj.Constraints(pj.Graph.pattern(17, [(i, i + 1) for i in range(16)], [i % 8 for i in range(17)]))
     with pytest.raises(RuntimeError, match="1..16"):
         batch.admit(big)
     small = _templates(pj, 2, seed=3)
