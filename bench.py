@@ -341,71 +341,93 @@ def pct(xs, q):
     return xs[min(len(xs) - 1, int(round(q / 100.0 * (len(xs) - 1))))] if xs else None
 
 
-def bench_queries(args) -> int:
+def query_form(name: str, g=None, queries: int = 0, workers: int = 8, rates=(50.0, 200.0, 1000.0, 2000.0),
+               budget_gb: float = 64.0, scale: int | None = None) -> dict:
     """Batched and served forms: many templates over one resident graph,
-    solved to the pruning fixed point before TDS by worker contexts that
-    share the graph (one stream and host thread each)."""
+    each solved to the pruning fixed point before TDS.  Three ways are
+    timed on the same templates: one at a time (one context), worker
+    contexts sharing the graph (one stream and host thread each), and the
+    shared sweeps (PJH_BATCH_LCC: the LCC0 and first LCC fixed point of up
+    to 256 templates in one batch whose lanes take queries as they arrive,
+    workers continuing each template from its parked state).  Every result
+    is compared with the one-at-a-time result."""
     import numpy as np
 
     from pattenmatching_sharedmemory_amd import Graph, Pruner
 
-    form = QUERY_FORMS[args.workload]
-    scale = args.scale or form["scale"]
-    n = args.queries or form["queries"]
-    t0 = time.time()
-    g = Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=GRAPH_SEED).labels_zipf(32, 1.2, seed=LABEL_SEED)
-    log(f"[bench] graph scale {scale}: V={g.vertex_count} E={g.edge_count} built in {time.time() - t0:.1f}s")
+    form = QUERY_FORMS[name]
+    scale = scale or form["scale"]
+    n = queries or form["queries"]
+    if g is None:
+        t0 = time.time()
+        g = Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=GRAPH_SEED).labels_zipf(32, 1.2, seed=LABEL_SEED)
+        log(f"[bench] graph scale {scale}: V={g.vertex_count} E={g.edge_count} built in {time.time() - t0:.1f}s")
     cs, specs = query_templates(form, n)
     owner = Pruner(g, device=0)
     owner.run_queries(cs[: min(8, n)], workers=1, run_tds=False)  # warm the contexts' allocations
     seq, seq_ms = owner.run_queries(cs, workers=1, run_tds=False)
-    log(f"[bench] one at a time: {n} queries in {seq_ms:.0f} ms ({n / seq_ms * 1e3:.1f} q/s)")
-    result = {"metric": METRIC, "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-              "dtype": "u8", "data": "synthetic", "steps": n, "warmup": 1}
+    log(f"[bench] {name}: one at a time {n} queries in {seq_ms:.0f} ms ({n / seq_ms * 1e3:.1f} q/s)")
     service = [r.end_ms - r.start_ms for r in seq]
-    cfg = {"workload": f"{args.workload} form: {n} random connected templates ({form['min_vertices']}-"
+    out = {"workload": f"{name} form: {n} random connected templates ({form['min_vertices']}-"
                        f"{form['max_vertices']} vertices, spanning tree + 0-{form['max_extra']} extra edges, "
                        f"distinct labels drawn from Zipf(1.2) over 32, cycles of at most {MAX_CYCLE} vertices) on "
                        f"R-MAT scale {scale} EF16, each solved to the pruning fixed point before TDS",
-           "vertices": g.vertex_count, "directed_slots": g.edge_count, "workers": args.workers,
-           "service_ms_p50": pct(service, 50), "service_ms_p99": pct(service, 99),
-           "template_vertices_mean": float(np.mean([k for k, _ in specs]))}
-    if args.workload == "batch":
-        bat, bat_ms = owner.run_queries(cs, workers=args.workers, run_tds=False)
-        equal = sum(a.result() == b.result() for a, b in zip(seq, bat))
-        log(f"[bench] batched ({args.workers} workers): {n} queries in {bat_ms:.0f} ms "
-            f"({n / bat_ms * 1e3:.1f} q/s); {equal}/{n} results equal to one-at-a-time")
-        result.update({"value": n / bat_ms * 1e3, "unit": "queries/s", "ms_per_step": bat_ms / n,
-                       "config": cfg,
-                       "batched": {"queries_per_s": n / bat_ms * 1e3, "wall_ms": bat_ms},
-                       "plain_sequential": {"queries_per_s": n / seq_ms * 1e3, "wall_ms": seq_ms},
-                       "speedup": seq_ms / bat_ms, "results_equal": equal})
+           "vertices": g.vertex_count, "directed_slots": g.edge_count, "queries": n, "workers": workers,
+           "template_vertices_mean": float(np.mean([k for k, _ in specs])),
+           "one_at_a_time": {"queries_per_s": n / seq_ms * 1e3, "wall_ms": seq_ms,
+                             "service_ms_p50": pct(service, 50), "service_ms_p99": pct(service, 99)}}
+
+    def equal(res):
+        return sum(a.result() == b.result() for a, b in zip(seq, res))
+
+    if name == "batch":
+        con, con_ms = owner.run_queries(cs, workers=workers, run_tds=False)
+        sh, sh_ms = owner.run_queries(cs, workers=workers, run_tds=False, batch_lcc=True)
+        log(f"[bench] batch: worker contexts {n / con_ms * 1e3:.1f} q/s ({equal(con)}/{n} equal), shared sweeps "
+            f"{n / sh_ms * 1e3:.1f} q/s ({equal(sh)}/{n} equal), one at a time {n / seq_ms * 1e3:.1f} q/s")
+        out.update({"worker_contexts": {"queries_per_s": n / con_ms * 1e3, "wall_ms": con_ms,
+                                        "results_equal": equal(con)},
+                    "shared_sweeps": {"queries_per_s": n / sh_ms * 1e3, "wall_ms": sh_ms,
+                                      "results_equal": equal(sh)},
+                    "queries_per_s": n / sh_ms * 1e3, "speedup": seq_ms / sh_ms})
     else:
         rng = np.random.default_rng(form["seed"])
-        rates = [float(x) for x in args.rates.split(",")]
+        budget = int(budget_gb * (1 << 30))
         table = []
-        budget = int(args.budget_gb * (1 << 30))
         for rate in rates:
             arrivals = np.cumsum(rng.exponential(1e3 / rate, n))
             arrivals -= arrivals[0]
-            srv, srv_ms = owner.run_queries(cs, workers=args.workers, run_tds=False, arrivals_ms=arrivals,
-                                            budget_bytes=budget)
-            one, one_ms = owner.run_queries(cs, workers=1, run_tds=False, arrivals_ms=arrivals)
-            lat_s = [r.end_ms - r.arrival_ms for r in srv]
-            lat_1 = [r.end_ms - r.arrival_ms for r in one]
-            equal = sum(a.result() == b.result() for a, b in zip(seq, srv))
-            row = {"rate_qps": rate, "served_qps": n / srv_ms * 1e3, "served_p50_ms": pct(lat_s, 50),
-                   "served_p99_ms": pct(lat_s, 99), "plain_qps": n / one_ms * 1e3, "plain_p50_ms": pct(lat_1, 50),
-                   "plain_p99_ms": pct(lat_1, 99), "results_equal_alone": equal}
-            log(f"[bench] rate {rate:.0f} q/s: served {row['served_qps']:.1f} q/s p50 {row['served_p50_ms']:.1f} "
-                f"p99 {row['served_p99_ms']:.1f} ms | one at a time {row['plain_qps']:.1f} q/s p50 "
-                f"{row['plain_p50_ms']:.1f} p99 {row['plain_p99_ms']:.1f} ms | equal {equal}/{n}")
+            row = {"rate_qps": rate}
+            for key, kw in (("served", {"workers": workers, "batch_lcc": True, "budget_bytes": budget}),
+                            ("worker_contexts", {"workers": workers, "budget_bytes": budget}),
+                            ("one_at_a_time", {"workers": 1})):
+                res, ms = owner.run_queries(cs, run_tds=False, arrivals_ms=arrivals, **kw)
+                lat = [r.end_ms - r.arrival_ms for r in res]
+                row[key] = {"queries_per_s": n / ms * 1e3, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                            "results_equal_alone": equal(res)}
+            log(f"[bench] served rate {rate:.0f} q/s: " + " | ".join(
+                f"{k} {row[k]['queries_per_s']:.1f} q/s p50 {row[k]['p50_ms']:.1f} p99 {row[k]['p99_ms']:.1f} ms "
+                f"equal {row[k]['results_equal_alone']}/{n}" for k in ("served", "worker_contexts", "one_at_a_time")))
             table.append(row)
         top = table[-1]
-        cfg["budget_bytes"] = budget
-        result.update({"value": top["served_qps"], "unit": "queries/s", "ms_per_step": 1e3 / top["served_qps"],
-                       "config": cfg, "rates": table, "plain_solve_ms_p99": pct(service, 99)})
+        out.update({"budget_bytes": budget, "rates": table, "queries_per_s": top["served"]["queries_per_s"],
+                    "speedup": top["served"]["queries_per_s"] / top["one_at_a_time"]["queries_per_s"],
+                    "p99_ms": top["served"]["p99_ms"], "plain_solve_ms_p99": pct(service, 99)})
     owner.close()
+    return out
+
+
+def bench_queries(args) -> int:
+    """The batched or served form as the bench line itself."""
+    rates = tuple(float(x) for x in args.rates.split(","))
+    q = query_form(args.workload, queries=args.queries, workers=args.workers, rates=rates, budget_gb=args.budget_gb,
+                   scale=args.scale)
+    result = {"metric": METRIC, "value": q["queries_per_s"], "unit": "queries/s", "n_gpus": 1, "steps": q["queries"],
+              "warmup": 1, "ms_per_step": 1e3 / q["queries_per_s"], "higher_is_better": True, "scaling": "weak",
+              "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+              "config": {k: v for k, v in q.items() if k in ("workload", "vertices", "directed_slots", "workers",
+                                                              "template_vertices_mean", "budget_bytes")},
+              "form": q}
     print(json.dumps(result), flush=True)
     return 0
 
@@ -426,6 +448,7 @@ def main() -> int:
                     help="end each solve when the first TDS step is due (phases up to TDS only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="plain workload: skip the batched and served forms")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -533,6 +556,12 @@ def main() -> int:
     pr.close()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(g, c, rows, sec, baseline_threads(), not args.stop_before_tds)
+    if args.workload == "plain" and args.scale == 22 and not args.no_forms:
+        # the batched form runs on this same graph; the served form on its own
+        # scale-24 graph at the highest arrival rate
+        result["batched_form"] = query_form("batch", g=g)
+        del g
+        result["served_form"] = query_form("served", rates=(2000.0,))
     print(json.dumps(result), flush=True)
     return 0
 

@@ -105,6 +105,7 @@ struct DevTemplate {
   uint32_t walk_hyp;                // memoised walks use the hypothesis memo (TDS)
   uint32_t path_code;               // code of the path's initial label
   uint32_t rows_sorted;             // every CSR row ascending (binary search allowed)
+  uint8_t slot_code[256];           // code of each graph label index (slot_lab), kNoCode when absent
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -352,6 +353,12 @@ __global__ void k_sum_counters(const unsigned long long* __restrict__ counters, 
 __global__ void k_clear_modified(uint8_t* flags, uint32_t V) {
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
     flags[v] &= static_cast<uint8_t>(~kModified);
+}
+
+__global__ void k_slot_labels(const uint32_t* __restrict__ edges, uint64_t E, const uint8_t* __restrict__ idx,
+                              uint8_t* __restrict__ slot_lab) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x)
+    slot_lab[e] = idx[edges[e]];
 }
 
 __global__ void k_label_codes(const uint64_t* __restrict__ labels, uint8_t* __restrict__ code, uint32_t V,
@@ -676,7 +683,9 @@ __device__ __forceinline__ Word lcc_requirements(Word cv, uint8_t vcode, const u
 template <int KW>
 struct LccTables {
   uint8_t pcode[64];
+  uint8_t slot_code[256];  // graph label index -> code (first LCC step, slot_lab)
   uint64_t nbr[64];
+  uint64_t cv[kMaxLabels];  // template vertices of each code
   uint64_t need[64 * KW];
 };
 
@@ -685,7 +694,9 @@ __device__ __forceinline__ void load_lcc_tables(LccTables<KW>& s, const DevTempl
   for (int i = threadIdx.x; i < 64; i += blockDim.x) {
     s.pcode[i] = t->pcode[i];
     s.nbr[i] = t->nbr[i];
+    s.cv[i] = t->code_vertices[i];
   }
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) s.slot_code[i] = t->slot_code[i];
   for (int i = threadIdx.x; i < 64 * KW; i += blockDim.x) s.need[i] = t->need[i / KW][i % KW];
 }
 
@@ -716,7 +727,8 @@ template <typename Word, int G, int S, int KW>
 __device__ void lcc_phase1_lanes(uint32_t block, uint32_t nblocks, const LccTables<KW>& tab,
                                  const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
                                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
-                                 const uint8_t* __restrict__ code, bool derive, const Word* __restrict__ cand,
+                                 const uint8_t* __restrict__ code, bool derive,
+                                 const uint8_t* __restrict__ slot_lab, const Word* __restrict__ cand,
                                  Word* __restrict__ unmatch, uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty,
                                  uint8_t* __restrict__ rescan, uint32_t* __restrict__ ea,
                                  const DevTemplate* __restrict__ t, unsigned long long* counters) {
@@ -745,16 +757,36 @@ __device__ void lcc_phase1_lanes(uint32_t block, uint32_t nblocks, const LccTabl
       for (uint64_t base = beg; base < end; base += static_cast<uint64_t>(G) * S) {
         uint32_t u[S];
         uint32_t act = 0;  // bit k: slot k of this lane exists and is active
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-          const uint64_t e = base + k * G + gl;
-          u[k] = e < end ? edges[e] : 0u;
-          const uint32_t word = e < end ? ea[e >> 5] : 0u;
-          act |= ((word >> (e & 31)) & 1u) << k;
-        }
         Word cu[S];
+        if (slot_lab) {
+          // First LCC step after LCC0 on a fresh state: a neighbour's
+          // candidates are exactly the template vertices of its label, so
+          // the slot's label index replaces the gather of cand[u].
+          uint8_t sl[S];
 #pragma unroll
-        for (int k = 0; k < S; ++k) cu[k] = (act >> k) & 1u ? cand[u[k]] : Word(0);
+          for (int k = 0; k < S; ++k) {
+            const uint64_t e = base + k * G + gl;
+            sl[k] = e < end ? slot_lab[e] : uint8_t(0xFF);
+            const uint32_t word = e < end ? ea[e >> 5] : 0u;
+            act |= ((word >> (e & 31)) & 1u) << k;
+          }
+#pragma unroll
+          for (int k = 0; k < S; ++k) {
+            const uint8_t c = (act >> k) & 1u ? tab.slot_code[sl[k]] : kNoCode;
+            cu[k] = c == kNoCode ? Word(0) : static_cast<Word>(tab.cv[c]);
+            u[k] = 0;
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < S; ++k) {
+            const uint64_t e = base + k * G + gl;
+            u[k] = e < end ? edges[e] : 0u;
+            const uint32_t word = e < end ? ea[e >> 5] : 0u;
+            act |= ((word >> (e & 31)) & 1u) << k;
+          }
+#pragma unroll
+          for (int k = 0; k < S; ++k) cu[k] = (act >> k) & 1u ? cand[u[k]] : Word(0);
+        }
 #pragma unroll
         for (int k = 0; k < S; ++k) {
           bool el = false;
@@ -763,7 +795,7 @@ __device__ void lcc_phase1_lanes(uint32_t block, uint32_t nblocks, const LccTabl
               el = true;
               ++elim;
             } else {
-              const uint32_t lab = derive ? tab.pcode[ctz<Word>(cu[k])] : code[u[k]];
+              const uint32_t lab = derive || slot_lab ? tab.pcode[ctz<Word>(cu[k])] : code[u[k]];
               if (lab != kNoCode) {
 #pragma unroll
                 for (int w = 0; w < KW; ++w)
@@ -823,10 +855,12 @@ __device__ void lcc_phase1_lanes(uint32_t block, uint32_t nblocks, const LccTabl
 
 // `derive`: neighbour codes from candidates, as in lcc_phase1_lanes.
 template <typename Word>
-__device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* s_pcode,
+__device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* s_pcode, const uint8_t* s_scode,
+                               const uint64_t* s_cv,
                                const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start,
                                uint32_t nchunks, const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off,
                                const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code, bool derive,
+                               const uint8_t* __restrict__ slot_lab,
                                const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
                                const uint8_t* __restrict__ dirty, uint32_t* __restrict__ ea,
                                uint32_t* __restrict__ hub_count, uint32_t* __restrict__ hub_elim,
@@ -879,22 +913,39 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
         const uint64_t beg = s_beg[slot];
         const uint64_t end = s_end[slot];
         uint32_t word[kPer], u[kPer];
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-          const uint64_t e = beg + threadIdx.x + k * kBlock;
-          word[k] = e < end ? ea[e >> 5] : 0u;
-          u[k] = e < end ? edges[e] : 0u;
-        }
         Word cu[kPer];
         uint8_t lb[kPer];
+        if (slot_lab) {  // first LCC step after LCC0: labels of the slots, as in lcc_phase1_lanes
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-          const uint64_t e = beg + threadIdx.x + k * kBlock;
-          const bool act = (word[k] >> (e & 31)) & 1u;
-          cu[k] = act ? cand[u[k]] : Word(0);
-          lb[k] = act && !derive ? code[u[k]] : kNoCode;
+          for (int k = 0; k < kPer; ++k) {
+            const uint64_t e = beg + threadIdx.x + k * kBlock;
+            word[k] = e < end ? ea[e >> 5] : 0u;
+            lb[k] = e < end ? slot_lab[e] : uint8_t(0xFF);
+          }
+#pragma unroll
+          for (int k = 0; k < kPer; ++k) {
+            const uint64_t e = beg + threadIdx.x + k * kBlock;
+            const bool act = (word[k] >> (e & 31)) & 1u;
+            const uint8_t c = act ? s_scode[lb[k]] : kNoCode;
+            cu[k] = c == kNoCode ? Word(0) : static_cast<Word>(s_cv[c]);
+            lb[k] = c;
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < kPer; ++k) {
+            const uint64_t e = beg + threadIdx.x + k * kBlock;
+            word[k] = e < end ? ea[e >> 5] : 0u;
+            u[k] = e < end ? edges[e] : 0u;
+          }
+#pragma unroll
+          for (int k = 0; k < kPer; ++k) {
+            const uint64_t e = beg + threadIdx.x + k * kBlock;
+            const bool act = (word[k] >> (e & 31)) & 1u;
+            cu[k] = act ? cand[u[k]] : Word(0);
+            lb[k] = act && !derive ? code[u[k]] : kNoCode;
+          }
         }
-        if (derive) {
+        if (derive && !slot_lab) {
 #pragma unroll
           for (int k = 0; k < kPer; ++k)
             if (cu[k]) lb[k] = s_pcode[ctz<Word>(cu[k])];
@@ -945,14 +996,21 @@ template <typename Word>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_lcc_hub_chunks(
     const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
     const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
-    const uint8_t* __restrict__ code, bool derive, const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
+    const uint8_t* __restrict__ code, bool derive, const uint8_t* __restrict__ slot_lab,
+    const Word* __restrict__ cand, const uint8_t* __restrict__ flags,
     const uint8_t* __restrict__ dirty, uint32_t* __restrict__ ea, uint32_t* __restrict__ hub_count,
     uint32_t* __restrict__ hub_elim, const DevTemplate* __restrict__ t, unsigned long long* counters) {
   __shared__ uint8_t s_pcode[64];
-  if (threadIdx.x < 64) s_pcode[threadIdx.x] = t->pcode[threadIdx.x];
+  __shared__ uint8_t s_scode[256];
+  __shared__ uint64_t s_cv[kMaxLabels];
+  if (threadIdx.x < 64) {
+    s_pcode[threadIdx.x] = t->pcode[threadIdx.x];
+    s_cv[threadIdx.x] = t->code_vertices[threadIdx.x];
+  }
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) s_scode[i] = t->slot_code[i];
   __syncthreads();
-  lcc_hub_chunks<Word>(blockIdx.x, gridDim.x, s_pcode, chunk_hub, chunk_start, nchunks, hubs, off, edges, code, derive,
-                       cand, flags, dirty, ea, hub_count, hub_elim, t, counters);
+  lcc_hub_chunks<Word>(blockIdx.x, gridDim.x, s_pcode, s_scode, s_cv, chunk_hub, chunk_start, nchunks, hubs, off, edges,
+                       code, derive, slot_lab, cand, flags, dirty, ea, hub_count, hub_elim, t, counters);
 }
 
 // LCC phase 1 of the degree bins in one launch, each bin grid-strided over
@@ -961,8 +1019,8 @@ template <typename Word, int KW>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW <= 2 ? 8 : 1))) k_lcc_phase1_all(
     const uint32_t* __restrict__ bins, const uint32_t* __restrict__ bin_count, uint32_t V,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code,
-    bool derive, const Word* __restrict__ cand, Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
-    uint8_t* __restrict__ dirty, uint8_t* __restrict__ rescan, uint32_t* __restrict__ ea,
+    bool derive, const uint8_t* __restrict__ slot_lab, const Word* __restrict__ cand, Word* __restrict__ unmatch,
+    uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty, uint8_t* __restrict__ rescan, uint32_t* __restrict__ ea,
     const DevTemplate* __restrict__ t, unsigned long long* counters) {
   __shared__ LccTables<KW> tab;
   load_lcc_tables<KW>(tab, t);
@@ -977,13 +1035,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW 
   // needed 82 (5 waves) and measured slower (1.05 against 1.01 ms per
   // scale-22 solve).
   lcc_phase1_lanes<Word, 64, 4, KW>(blockIdx.x, gridDim.x, tab, bins + 3 * V64, bin_count + 3, off, edges, code,
-                                    derive, cand, unmatch, flags, dirty, rescan, ea, t, counters);
+                                    derive, slot_lab, cand, unmatch, flags, dirty, rescan, ea, t, counters);
   lcc_phase1_lanes<Word, 32, 4, KW>(blockIdx.x, gridDim.x, tab, bins + 2 * V64, bin_count + 2, off, edges, code,
-                                    derive, cand, unmatch, flags, dirty, rescan, ea, t, counters);
+                                    derive, slot_lab, cand, unmatch, flags, dirty, rescan, ea, t, counters);
   lcc_phase1_lanes<Word, 4, 4, KW>(blockIdx.x, gridDim.x, tab, bins + V64, bin_count + 1, off, edges, code, derive,
-                                   cand, unmatch, flags, dirty, rescan, ea, t, counters);
-  lcc_phase1_lanes<Word, 1, 4, KW>(blockIdx.x, gridDim.x, tab, bins, bin_count + 0, off, edges, code, derive, cand,
-                                   unmatch, flags, dirty, rescan, ea, t, counters);
+                                   slot_lab, cand, unmatch, flags, dirty, rescan, ea, t, counters);
+  lcc_phase1_lanes<Word, 1, 4, KW>(blockIdx.x, gridDim.x, tab, bins, bin_count + 0, off, edges, code, derive,
+                                   slot_lab, cand, unmatch, flags, dirty, rescan, ea, t, counters);
 }
 
 template <typename Word, int KW>
@@ -2736,6 +2794,16 @@ struct pj_ctx {
   uint32_t* d_hub_new = nullptr;     // per hub: turned modified in the last phase 1
   uint32_t* d_ea = nullptr;
   uint32_t* d_rev = nullptr;  // per slot u->v: position of u in row(v) (symmetric graphs only)
+  // Per slot: index of the neighbour's label among the graph's distinct
+  // labels (graphs of at most 255 labels; null otherwise), and those labels.
+  uint8_t* d_slot_lab = nullptr;
+  std::vector<uint64_t> graph_labels;
+  // 0 after a state reset, 1 after LCC0 on such a state (every active
+  // vertex's candidates are then the template vertices of its label), 2 once
+  // anything else ran: the first LCC step reads slot labels instead of
+  // neighbour candidates only at 1.
+  int fresh = 2;
+  bool slot_labels_enabled = true;  // PJ_SLOT_LABELS=0: never
   // Live list: the vertices that may still be active (a superset of the
   // active ones; candidates only shrink until the state is replaced).
   // live == nullptr: every vertex.  Two buffers for the compaction.
@@ -2858,6 +2926,7 @@ void free_graph(pj_ctx* ctx) {
     ctx->d_edges = nullptr;
     ctx->d_labels = nullptr;
     ctx->d_rev = nullptr;
+    ctx->d_slot_lab = nullptr;
     ctx->d_hubs = ctx->d_chunk_hub = nullptr;
     ctx->d_chunk_start = nullptr;
     ctx->graph_shared = false;
@@ -2870,7 +2939,7 @@ void free_graph(pj_ctx* ctx) {
                   ctx->memo1.wave_epoch, ctx->d_already, ctx->d_ovf, ctx->d_view_count, ctx->d_view_off,
                   ctx->d_view_edges, ctx->d_scan_tmp, ctx->d_rev, ctx->d_live[0], ctx->d_live[1],
                   ctx->d_compact_scratch, ctx->d_out_v, ctx->d_out_src, ctx->d_out_dst, ctx->d_out_cand,
-                  ctx->d_stat};
+                  ctx->d_stat, ctx->d_slot_lab};
   for (void* b : bufs)
     if (b) hipFree(b);
   ctx->d_off = nullptr;
@@ -2881,6 +2950,8 @@ void free_graph(pj_ctx* ctx) {
   ctx->d_flags = ctx->d_vflags = ctx->d_dirty = ctx->d_rescan = nullptr;
   ctx->d_ea = nullptr;
   ctx->d_rev = nullptr;
+  ctx->d_slot_lab = nullptr;
+  ctx->graph_labels.clear();
   ctx->d_live[0] = ctx->d_live[1] = nullptr;
   ctx->d_compact_scratch = nullptr;
   ctx->d_stat = nullptr;
@@ -3015,6 +3086,7 @@ int state_reset_t(pj_ctx* ctx) {
   ctx->active_vertices = ctx->V;
   ctx->active_edges = ctx->E;
   ctx->labels_consistent = false;  // every template vertex is a candidate again
+  ctx->fresh = 0;
   ctx->modified_cleared = false;
   ctx->live = nullptr;              // every vertex is active again
   ctx->n_live = ctx->V;
@@ -3145,22 +3217,23 @@ template <typename Word, int KW>
 void launch_lcc_phase1(pj_ctx* ctx) {
   launch_bins<Word>(ctx, 0);
   const bool derive = ctx->labels_consistent;
+  const uint8_t* slot_lab = ctx->fresh == 1 && ctx->slot_labels_enabled ? ctx->d_slot_lab : nullptr;
   uint8_t* dirty = ctx->skip_clean ? ctx->d_dirty : nullptr;
   uint8_t* rescan = ctx->skip_clean && !ctx->rescan_all ? ctx->d_rescan : nullptr;  // push target
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   unsigned long long* c = begin_launch(ctx, K_LCC_P1);
   hipLaunchKernelGGL((k_lcc_phase1_all<Word, KW>), dim3(ctx->cus * kPhase1BlocksPerCu), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
-                     ctx->d_bin_count, ctx->V, ctx->d_off, ctx->d_edges, ctx->d_code, derive, cand, unm, ctx->d_flags,
-                     dirty, rescan, ctx->d_ea, ctx->d_t, c);
+                     ctx->d_bin_count, ctx->V, ctx->d_off, ctx->d_edges, ctx->d_code, derive, slot_lab, cand, unm,
+                     ctx->d_flags, dirty, rescan, ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
   if (ctx->nhubs) {
     c = begin_launch(ctx, K_LCC_HUB);
     hipLaunchKernelGGL(k_lcc_hub_chunks<Word>, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * kPhase1BlocksPerCu)),
                        dim3(kBlock), 0,
                        ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
-                       ctx->d_edges, ctx->d_code, derive, cand, ctx->d_flags, dirty, ctx->d_ea, ctx->d_hub_count,
-                       ctx->d_hub_elim, ctx->d_t, c);
+                       ctx->d_edges, ctx->d_code, derive, slot_lab, cand, ctx->d_flags, dirty, ctx->d_ea,
+                       ctx->d_hub_count, ctx->d_hub_elim, ctx->d_t, c);
     end_launch(ctx);
     begin_launch(ctx, K_LCC_HUB_FINISH);
     hipLaunchKernelGGL((k_lcc_hub_finish<Word, KW>), dim3(grid_for(ctx, ctx->nhubs)), dim3(kBlock), 0, ctx->stream,
@@ -3668,16 +3741,19 @@ int run_step_t(pj_ctx* ctx, int step, uint32_t index) {
       end_launch(ctx);
       mark_all_dirty(ctx);  // every candidate set was rewritten
       ctx->labels_consistent = true;
+      ctx->fresh = ctx->fresh == 0 ? 1 : 2;
       break;
     }
     case PJ_STEP_LCC:
       launch_lcc<Word>(ctx);
+      ctx->fresh = 2;
       break;
     case PJ_STEP_CC:
     case PJ_STEP_PC:
     case PJ_STEP_TDS:
     case PJ_STEP_ENUMERATION: {
       const int kind = step == PJ_STEP_CC ? 0 : step == PJ_STEP_PC ? 1 : step == PJ_STEP_TDS ? 2 : 3;
+      ctx->fresh = 2;
       if (launch_nlcc<Word>(ctx, kind)) return -1;
       launch_nlcc_hubs<Word>(ctx, kind, ctx->d_nhubs);
       if (ctx->h_t.nlcc_memo && kind != 3 && !triangle_memo(ctx, kind) && launch_nlcc_retry<Word>(ctx, kind))
@@ -3743,6 +3819,7 @@ int upload_state_t(pj_ctx* ctx, const uint64_t* cand, const uint8_t* vflags, con
                      static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->V);
   mark_all_dirty(ctx);
   ctx->labels_consistent = false;  // an uploaded state may hold candidates of other labels
+  ctx->fresh = 2;
   ctx->modified_cleared = false;
   ctx->live = nullptr;              // and any vertex may be active
   ctx->n_live = ctx->V;
@@ -3944,6 +4021,9 @@ int pj_create(pj_ctx** out, int device) {
   // PJ_SKIP_CLEAN=0 evaluates every scheduled vertex in LCC phase 1 (the
   // results are the same either way; the switch exists for comparisons).
   if (const char* sc = getenv("PJ_SKIP_CLEAN")) ctx->dirty_enabled = strcmp(sc, "0") != 0;
+  // PJ_SLOT_LABELS=0 keeps the neighbour-candidate gathers in the first LCC
+  // step (same results; the switch exists for comparisons).
+  if (const char* sl = getenv("PJ_SLOT_LABELS")) ctx->slot_labels_enabled = strcmp(sl, "0") != 0;
   // PJ_MEMO_WALKS=1 runs cycle and path steps as the reference's memoised
   // search even where the complete search is known to give the same result.
   if (const char* mw = getenv("PJ_MEMO_WALKS")) ctx->memo_walks = strcmp(mw, "0") != 0;
@@ -3994,6 +4074,47 @@ static int upload_common(pj_ctx* ctx, uint32_t V, uint64_t E, const uint64_t* of
     host_off = h_off.data();
   }
   return upload_finish(ctx, host_off);
+}
+
+// Per-slot label indices (d_slot_lab) for graphs of at most 255 distinct
+// labels: the labels are indexed on the host, then every slot takes its
+// neighbour's index.  Graphs with more labels go without (the first LCC step
+// then gathers neighbour candidates as every other step does).
+static int slot_labels(pj_ctx* ctx) {
+  const uint32_t V = ctx->V;
+  const uint64_t E = ctx->E;
+  ctx->graph_labels.clear();
+  if (!V || !E) return 0;
+  std::vector<uint64_t> lab(V);
+  PJ_HIP(hipMemcpyAsync(lab.data(), ctx->d_labels, V * 8ull, hipMemcpyDeviceToHost, ctx->stream));
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<uint8_t> idx(V);
+  std::vector<uint64_t> keys(512);
+  std::vector<int> slot(512, -1);  // open addressing over the distinct labels
+  for (uint32_t v = 0; v < V; ++v) {
+    const uint64_t l = lab[v];
+    uint64_t h = (l * 0x9E3779B97F4A7C15ull) >> 55;
+    while (slot[h] >= 0 && keys[h] != l) h = (h + 1) & 511;
+    if (slot[h] < 0) {
+      if (ctx->graph_labels.size() == 255) {  // too many labels for a byte per slot
+        ctx->graph_labels.clear();
+        return 0;
+      }
+      keys[h] = l;
+      slot[h] = static_cast<int>(ctx->graph_labels.size());
+      ctx->graph_labels.push_back(l);
+    }
+    idx[v] = static_cast<uint8_t>(slot[h]);
+  }
+  uint8_t* d_idx = nullptr;
+  if (dev_alloc(ctx, &ctx->d_slot_lab, E) || dev_alloc(ctx, &d_idx, V)) return -1;
+  PJ_HIP(hipMemcpyAsync(d_idx, idx.data(), V, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_slot_labels, dim3(grid_for(ctx, E)), dim3(kBlock), 0, ctx->stream, ctx->d_edges, E, d_idx,
+                     ctx->d_slot_lab);
+  PJ_HIP(hipGetLastError());
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  PJ_HIP(hipFree(d_idx));
+  return 0;
 }
 
 static int upload_finish(pj_ctx* ctx, const uint64_t* host_off) {
@@ -4052,6 +4173,7 @@ static int upload_finish(pj_ctx* ctx, const uint64_t* host_off) {
     }
     ctx->skip_clean = ctx->rows_sorted && ctx->symmetric && ctx->dirty_enabled;
   }
+  if (slot_labels(ctx)) return -1;
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   ctx->have_template = false;
   ctx->active_vertices = V;
@@ -4205,6 +4327,8 @@ int pj_share_graph(pj_ctx* ctx, const pj_ctx* owner) {
   ctx->d_edges = owner->d_edges;
   ctx->d_labels = owner->d_labels;
   ctx->d_rev = owner->d_rev;
+  ctx->d_slot_lab = owner->d_slot_lab;
+  ctx->graph_labels = owner->graph_labels;
   ctx->self_loops = owner->self_loops;
   ctx->rows_sorted = owner->rows_sorted;
   ctx->symmetric = owner->symmetric;
@@ -4289,6 +4413,10 @@ int pj_set_template(pj_ctx* ctx, const pj_template_desc* d) {
       t.need[p][k >> 3] |= static_cast<uint64_t>(need) << (8 * (k & 7));
     }
   }
+  std::memset(t.slot_code, kNoCode, sizeof(t.slot_code));
+  for (size_t i = 0; i < ctx->graph_labels.size() && i < 255; ++i)
+    for (uint32_t k = 0; k < t.K; ++k)
+      if (t.label_raw[k] == ctx->graph_labels[i]) t.slot_code[i] = static_cast<uint8_t>(k);
   ctx->kw = t.K <= 8 ? 1 : t.K <= 16 ? 2 : t.K <= 32 ? 4 : 8;
   ctx->word_bytes = P <= 8 ? 1 : P <= 16 ? 2 : P <= 32 ? 4 : 8;
   ctx->cycles.clear();
@@ -4344,6 +4472,7 @@ int pj_reset_modified(pj_ctx* ctx) {
   mark_all_rescan(ctx);
   PJ_HIP(hipGetLastError());
   ctx->modified_cleared = true;
+  ctx->fresh = 2;
   return 0;
 }
 

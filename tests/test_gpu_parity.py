@@ -213,6 +213,43 @@ def test_clean_vertex_skipping_changes_nothing(pj, orc, case, monkeypatch):
     assert [r.key() for r in r1] == [r.key() for r in orows]
 
 
+@pytest.mark.parametrize("case", ["plain s16", "hub", "many labels"])
+def test_first_lcc_step_from_slot_labels_changes_nothing(pj, orc, case, monkeypatch):
+    """The first LCC step after LCC0 reads each slot's neighbour label
+    (built at upload for graphs of at most 255 labels) instead of the
+    neighbour's candidates; the step trace, candidate sets, flags and
+    edge-active bits equal those with the gathers and the oracle's.  A graph
+    of more than 255 labels has no slot labels and takes the gathers."""
+    if case == "hub":
+        g = pj.Graph.rmat(14, 32, seed=3).labels_uniform(4, seed=4)  # R-MAT rows above 2048 slots
+        p = pj.Graph.pattern(4, [(0, 1), (1, 2), (2, 0), (2, 3)], [0, 1, 2, 3])
+    elif case == "many labels":
+        g = pj.Graph.rmat(13, 16, seed=9).labels_uniform(300, seed=2)
+        p = pj.Graph.pattern(4, [(0, 1), (1, 2), (2, 3), (3, 0)], [0, 1, 2, 1])
+    else:
+        g = pj.Graph.rmat(16, 16, seed=20221).labels_zipf(32, 1.2, seed=7)
+        p = pj.Graph.pattern(6, TRIANGLE_TAIL, [0, 1, 2, 3, 4, 3])
+    c = pj.Constraints(p)
+    runs = []
+    for on in ("0", "1"):
+        monkeypatch.setenv("PJ_SLOT_LABELS", on)
+        pr = pj.Pruner(g, c)
+        rows, _ = pr.run(enumerate_matches=False)
+        runs.append((rows, pr.state()))
+        rows2, _ = pr.run(enumerate_matches=False)  # a second solve on the same context
+        assert [r.key() for r in rows2] == [r.key() for r in rows]
+        pr.close()
+    (r0, s0), (r1, s1) = runs
+    print(f"[{case}] steps={len(r0)}", flush=True)
+    assert [r.key() for r in r0] == [r.key() for r in r1]
+    assert [r.slots for r in r0 if r.step == 3] == [r.slots for r in r1 if r.step == 3]
+    assert np.array_equal(s0.cand, s1.cand) and np.array_equal(s0.vflags, s1.vflags)
+    assert np.array_equal(s0.ea_words, s1.ea_words)
+    o = orc.Oracle(g, c, threads=8)
+    orows, _ = o.run(orc.FAITHFUL, enumerate_matches=False)
+    assert [r.key() for r in r1] == [r.key() for r in orows]
+
+
 # Templates on which the reference's memos change the result (the oracle's
 # faithful and complete modes differ; the faithful one is the same at 1 and 8
 # threads): repeated labels on a cycle, which also yield paths and TDS steps.
