@@ -119,13 +119,17 @@ typedef struct pjh_query_result {
   uint64_t projected_bytes;
   double arrival_ms, start_ms, end_ms; /* on the batch clock */
   double device_ms;         /* summed step device time */
+  double worker_ms;         /* when a worker context took it (start_ms: admission) */
 } pjh_query_result;
 /* With PJH_BATCH_LCC in `flags` the LCC0 step and the LCC steps to the first
  * fixed point of every template the batch can hold (pj_batch_admit) run in a batch
  * (pj_batch, include/prunejuice.h) whose lanes take queries as they are
  * admitted; a query that reaches that fixed point is handed to a worker,
  * which installs its state and continues the step machine from there, and
- * its lane takes the next query.  Results are those of a context of its own. */
+ * its lane takes the next query.  Results are those of a context of its own.
+ * Lanes bound the batch; the budget (projected bytes of the worker contexts'
+ * state and memo tables) then holds from the moment a worker takes a query
+ * (worker_ms) to its end. */
 #define PJH_BATCH_LCC 8
 int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int n, int workers, int flags,
                     const double* arrival_ms, uint64_t budget_bytes, pjh_query_result* out, double* total_ms,

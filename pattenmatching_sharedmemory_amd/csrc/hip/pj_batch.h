@@ -32,6 +32,11 @@ struct GraphView {
   uint32_t nchunks;
   uint32_t hub_degree;
   uint32_t hub_chunk;
+  // per slot, the index of the neighbour's label in graph_labels (null when
+  // the graph has more than 255 labels)
+  const uint8_t* slot_lab;
+  const uint64_t* graph_labels;  // host memory, owned by the context
+  uint32_t ngraph_labels;
 };
 
 // Fills `out` from a context holding a graph; -1 when it has none.

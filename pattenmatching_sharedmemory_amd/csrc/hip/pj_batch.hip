@@ -66,6 +66,7 @@ constexpr int kRun = 0, kInit = 1, kFirst = 2, kDone = 3, kSecond = 4, kCtl = 8;
 struct LaneTab {
   uint32_t ent[kMaxEnt];
   uint16_t labmask[256];  // template vertices carrying each union label
+  uint16_t glabmask[256]; // template vertices carrying each graph label (index into the graph's labels)
   uint16_t nbr[kMaxP];    // template neighbours of each template vertex
   uint8_t ent_off[kMaxP + 1];
   uint8_t P;
@@ -76,7 +77,9 @@ struct BArgs {
   uint32_t V;
   uint32_t nhubs, nchunks, hub_chunk, hub_degree;
   uint32_t nlab;
+  uint32_t nglab;               // graph labels indexed by slot_lab (0: no slot labels)
   uint64_t E;
+  const uint8_t* slot_lab;      // [E] index of each slot's neighbour label among the graph's labels
   const uint64_t* off;
   const uint32_t* edges;
   const uint32_t* hubs;
@@ -368,6 +371,17 @@ __global__ void __launch_bounds__(kBlockB) kb_phase1(BArgs a) {
   const uint64_t full = a.ctl[g * kCtl + kFirst] | a.ctl[g * kCtl + kSecond];
   __shared__ Word s_nbr[64 * kMaxP];
   __shared__ uint8_t s_off[64 * (kMaxP + 1)];
+  // Every running template in its first LCC step (right after LCC0): a
+  // neighbour's candidates are the template vertices of its label, read from
+  // the slot's label index instead of gathering the neighbour's words.
+  const bool fresh = a.slot_lab && a.nglab && (run & ~a.ctl[g * kCtl + kFirst]) == 0;
+  // [64][gstride] when fresh: 32-bit entries and an odd row stride, so that
+  // the 64 lanes reading one label index hit distinct banks
+  extern __shared__ uint32_t s_gmask[];
+  const uint32_t gstride = a.nglab | 1u;
+  if (fresh)
+    for (uint32_t i = threadIdx.x; i < 64 * a.nglab; i += blockDim.x)
+      s_gmask[(i / a.nglab) * gstride + i % a.nglab] = a.tab[g * 64 + i / a.nglab].glabmask[i % a.nglab];
   for (uint32_t i = threadIdx.x; i < 64 * kMaxP; i += blockDim.x) s_nbr[i] = a.tab[g * 64 + i / kMaxP].nbr[i % kMaxP];
   for (uint32_t i = threadIdx.x; i < 64 * (kMaxP + 1); i += blockDim.x)
     s_off[i] = a.tab[g * 64 + i / (kMaxP + 1)].ent_off[i % (kMaxP + 1)];
@@ -411,14 +425,24 @@ __global__ void __launch_bounds__(kBlockB) kb_phase1(BArgs a) {
     for (uint64_t base = beg; base < end; base += kS) {
       const uint64_t e = base + (t & (kS - 1));
       const bool own = t < kS && e < end;
-      const uint32_t uid = own ? a.edges[e] : 0u;
       const uint64_t mk = own ? eam[e] : 0ull;
       Word cu[kS];
+      if (fresh) {
+        const uint32_t sl = own ? a.slot_lab[e] : 0u;
 #pragma unroll
-      for (int k = 0; k < kS; ++k) {
-        const uint32_t uk = __shfl(uid, k, 64);
-        const uint64_t mkk = shfl64(mk, k);
-        cu[k] = (C && ((mkk >> t) & 1)) ? cand[static_cast<uint64_t>(uk) * 64 + t] : Word(0);
+        for (int k = 0; k < kS; ++k) {
+          const uint32_t lk = static_cast<uint32_t>(__shfl(static_cast<int>(sl), k, 64));
+          const uint64_t mkk = shfl64(mk, k);
+          cu[k] = (C && ((mkk >> t) & 1) && lk < a.nglab) ? static_cast<Word>(s_gmask[t * gstride + lk]) : Word(0);
+        }
+      } else {
+        const uint32_t uid = own ? a.edges[e] : 0u;
+#pragma unroll
+        for (int k = 0; k < kS; ++k) {
+          const uint32_t uk = __shfl(uid, k, 64);
+          const uint64_t mkk = shfl64(mk, k);
+          cu[k] = (C && ((mkk >> t) & 1)) ? cand[static_cast<uint64_t>(uk) * 64 + t] : Word(0);
+        }
       }
       uint64_t clr = 0;  // lane k: templates that eliminated slot base + k
 #pragma unroll
@@ -749,6 +773,7 @@ struct pj_batch {
   std::vector<uint64_t> av, ae;
   std::vector<uint64_t> pending;  // per group: lanes admitted since the last step
   std::vector<uint64_t> uni;      // label union
+  std::vector<uint64_t> graph_labels;  // labels indexed by slot_lab
   bool labels_dirty = false;
   double ms = 0.0;
   uint64_t steps = 0;
@@ -796,10 +821,11 @@ int launch_step(pj_batch* b, int nb) {
   hipLaunchKernelGGL(kb_lcc0, dim3(cus * 4, b->G), dim3(kBlockB), 64 * a.nlab * sizeof(uint16_t), b->stream, a);
   const uint32_t tiles = static_cast<uint32_t>(std::min<uint64_t>((a.V + kTile - 1) / kTile, cus * 4));
   hipLaunchKernelGGL(kb_frontier, dim3(tiles, b->G), dim3(kBlockB), 0, b->stream, a, 1);
+  const size_t gm = 64 * (a.nglab | 1u) * sizeof(uint32_t);
   if (nb == 4)
-    hipLaunchKernelGGL(kb_phase1<4>, dim3(cus * 8, b->G), dim3(kBlockB), 0, b->stream, a);
+    hipLaunchKernelGGL(kb_phase1<4>, dim3(cus * 8, b->G), dim3(kBlockB), gm, b->stream, a);
   else
-    hipLaunchKernelGGL(kb_phase1<2>, dim3(cus * 8, b->G), dim3(kBlockB), 0, b->stream, a);
+    hipLaunchKernelGGL(kb_phase1<2>, dim3(cus * 8, b->G), dim3(kBlockB), gm, b->stream, a);
   if (a.nhubs)
     hipLaunchKernelGGL(kb_hub_finish, dim3(std::max<uint32_t>(1, std::min<uint32_t>((a.nhubs + kWaves - 1) / kWaves,
                                                                                   cus * 4)),
@@ -871,6 +897,9 @@ int pj_batch_create(pj_batch** out, const pj_ctx* owner, uint32_t lanes) {
   a.hub_chunk = b->gv.hub_chunk;
   a.hub_degree = b->gv.hub_degree;
   a.nlab = 1;
+  a.slot_lab = b->gv.slot_lab;
+  a.nglab = b->gv.slot_lab ? b->gv.ngraph_labels : 0u;
+  b->graph_labels.assign(b->gv.graph_labels, b->gv.graph_labels + b->gv.ngraph_labels);
   b->vpad = (V + 63) / 64 * 64;
   b->ea_stride = (E + 63) / 64 * 2;
   if (balloc(b, &a.cand, G * V * 64) || balloc(b, &a.unm, G * V * 64) || balloc(b, &a.mod, G * V) ||
@@ -949,6 +978,8 @@ int pj_batch_admit(pj_batch* b, const pj_template_desc* d, uint32_t* lane) {
       return -2;
     }
     tb.labmask[li] |= static_cast<Word>(1u << p);
+    for (size_t gi = 0; gi < b->graph_labels.size(); ++gi)
+      if (b->graph_labels[gi] == d->vertex_label[p]) tb.glabmask[gi] |= static_cast<Word>(1u << p);
     tb.nbr[p] = static_cast<Word>(d->neighbor_mask[p] & 0xFFFFu);
   }
   uint32_t nent = 0, maxneed = 0;

@@ -4763,6 +4763,9 @@ int graph_view(const pj_ctx* ctx, GraphView* out) {
   out->nchunks = ctx->nhubs ? ctx->nchunks : 0u;
   out->hub_degree = kHubDegree;
   out->hub_chunk = kHubChunk;
+  out->slot_lab = ctx->d_slot_lab;
+  out->graph_labels = ctx->graph_labels.data();
+  out->ngraph_labels = static_cast<uint32_t>(ctx->graph_labels.size());
   return 0;
 }
 

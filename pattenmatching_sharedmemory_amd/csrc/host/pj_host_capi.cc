@@ -442,8 +442,8 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
     r = pjh_query_result{};
     r.projected_bytes = projected[i];
     r.arrival_ms = arrival_ms ? arrival_ms[i] : 0.0;
-    r.start_ms = now_ms();
-    reserved += projected[i];
+    r.start_ms = r.worker_ms = now_ms();
+    if (!batched) reserved += projected[i];
   };
   auto finish = [&](int i, int w, const std::string& e) {
     out[i].worker = w;
@@ -461,11 +461,16 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
       {
         std::unique_lock<std::mutex> lk(mu);
         if (batched) {
-          cv.wait(lk, [&] { return !ready.empty() || admission_done || !first_error.empty(); });
+          // the next ready query, once its bytes fit the budget
+          cv.wait(lk, [&] {
+            return !first_error.empty() || (ready.empty() ? admission_done : fits(ready.front().first));
+          });
           if (!first_error.empty() || ready.empty()) return;
           i = ready.front().first;
           lane = ready.front().second;
           ready.pop_front();
+          reserved += projected[i];
+          out[i].worker_ms = now_ms();
         } else {
           while (true) {
             if (next >= n || !first_error.empty()) return;
@@ -504,7 +509,7 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
         pj_batch_info(batch, nullptr, &running, &parked, nullptr, nullptr);
         while (next < n) {
           const double due = arrival_ms ? arrival_ms[next] : 0.0;
-          if (now_ms() < due || !fits(next)) break;
+          if (now_ms() < due) break;
           if (running + parked >= lanes) break;
           uint32_t lane = 0;
           const int rc = pj_batch_admit(batch, templates[next]->c.desc(), &lane);

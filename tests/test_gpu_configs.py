@@ -139,9 +139,9 @@ def test_served_configuration_templates_against_the_oracle(pj, orc):
                                   batch_lcc=True)
     for form in (served, shared):
         for r, a in zip(form, arrivals):
-            assert r.start_ms + 1e-6 >= a
-        for r in form:
-            live = sum(q.projected_bytes for q in form if q.start_ms <= r.start_ms < q.end_ms)
+            assert r.start_ms + 1e-6 >= a and r.worker_ms + 1e-6 >= r.start_ms
+        for r in form:  # bytes of the queries on workers when r reached one
+            live = sum(q.projected_bytes for q in form if q.worker_ms <= r.worker_ms < q.end_ms)
             assert live <= budget
     for i in determined:
         assert alone[i].result() == served[i].result(), i
