@@ -52,6 +52,7 @@ constexpr uint32_t kMaxP = 16;
 constexpr uint32_t kMaxSteps = 1000;  // the reference's step cap (multiple_label_cpu.cc:113)
 constexpr int kMaxEnt = 240;          // requirement entries per template (<= sum of template degrees)
 constexpr int kS = 16;                // slots per pass of LCC phase 1
+constexpr int kEntLds = 32;           // requirement entries per lane kept in LDS by phase 1
 constexpr uint8_t kNoLab = 0xFF;
 constexpr uint32_t kMaxLabels = 255;  // labels of the union (index 0xFF means none)
 constexpr uint8_t kFlagModified = 2, kFlagScheduled = 4;  // flag bytes of a context
@@ -113,6 +114,20 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t x, int src) {
   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+// A wave-uniform 64-bit value in scalar registers.
+__device__ __forceinline__ uint64_t uni64(uint64_t x) {
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x >> 32)));
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x)));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Lane k's 64-bit value (split in halves hi, lo), wave-uniform.
+__device__ __forceinline__ uint64_t lane64(uint32_t hi, uint32_t lo, int k) {
+  const uint32_t h = static_cast<uint32_t>(__builtin_amdgcn_readlane(hi, k));
+  const uint32_t l = static_cast<uint32_t>(__builtin_amdgcn_readlane(lo, k));
+  return (static_cast<uint64_t>(h) << 32) | l;
+}
+
 __device__ __forceinline__ uint64_t wave_or64(uint64_t x) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -144,7 +159,9 @@ __device__ void flush_counters(const BArgs& a, uint32_t g, uint64_t ve, uint64_t
 }
 
 // Saturating per-template-vertex counters, bit-sliced over NB planes (cap
-// 2^NB - 1): x is one bit, the template vertex counted.
+// 2^NB - 1): x is one bit, the template vertex counted.  NB = 1 keeps
+// whether a count is non-zero (every requirement of the running templates
+// is one neighbour).
 template <int NB>
 __device__ __forceinline__ void count_one(Word (&c)[NB], Word x) {
   Word sat = c[0];
@@ -166,12 +183,12 @@ __device__ __forceinline__ void count_one(Word (&c)[NB], Word x) {
 // the count of a label is the sum over the template vertices of that label;
 // counts saturate at 2^NB - 1, which is at least every count needed.
 template <int NB>
-__device__ __forceinline__ Word requirements(Word cv, const Word (&c)[NB], const LaneTab& tb, const uint8_t* off) {
+__device__ __forceinline__ Word requirements(Word cv, const Word (&c)[NB], const uint32_t* ent, const uint8_t* off) {
   Word fail = 0;
   for (Word m = cv; m; m &= m - 1) {
     const int p = __builtin_ctz(m);
     for (int j = off[p]; j < off[p + 1]; ++j) {
-      const uint32_t e = tb.ent[j];
+      const uint32_t e = ent[j];
       const Word mask = static_cast<Word>(e & 0xFFFFu);
       uint32_t cnt = 0;
 #pragma unroll
@@ -371,6 +388,8 @@ __global__ void __launch_bounds__(kBlockB) kb_phase1(BArgs a) {
   const uint64_t full = a.ctl[g * kCtl + kFirst] | a.ctl[g * kCtl + kSecond];
   __shared__ Word s_nbr[64 * kMaxP];
   __shared__ uint8_t s_off[64 * (kMaxP + 1)];
+  __shared__ uint32_t s_ent[64 * kEntLds];  // the first kEntLds requirement entries of each lane
+  for (uint32_t i = threadIdx.x; i < 64 * kEntLds; i += blockDim.x) s_ent[i] = a.tab[g * 64 + i / kEntLds].ent[i % kEntLds];
   // Every running template in its first LCC step (right after LCC0): a
   // neighbour's candidates are the template vertices of its label, read from
   // the slot's label index instead of gathering the neighbour's words.
@@ -396,28 +415,48 @@ __global__ void __launch_bounds__(kBlockB) kb_phase1(BArgs a) {
   const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
   const uint32_t nw = gridDim.x * kWaves;
   uint64_t ee = 0;
-  for (uint32_t i = wave; i < items; i += nw) {
-    uint32_t v, h = 0;
-    uint64_t beg, end;
-    const bool chunk = i >= n1;
-    if (!chunk) {
-      v = a.list1[gV + i];
-      beg = a.off[v];
-      end = a.off[v + 1];
-    } else {
-      const uint32_t c = i - n1;
-      h = a.chunk_hub[c];
-      v = a.hubs[h];
-      beg = a.chunk_start[c];
-      const uint64_t row_end = a.off[v + 1];
-      end = row_end < beg + a.hub_chunk ? row_end : beg + a.hub_chunk;
+  // Items are taken 64 at a time per wave: lane i loads item i's vertex,
+  // slot range, dirty mask and evaluation mask in two rounds of loads; the
+  // wave then works through them with their values broadcast from the lanes
+  // (one item's chain of dependent loads is then the own words and the row).
+  // (item r of a wave's k-th round is wave + (64 k + r) * nw: the rows are
+  // spread over the waves as in a grid-strided loop)
+  for (uint64_t k0 = 0; wave + k0 * nw < items; k0 += 64) {
+    const uint64_t ii64 = wave + (k0 + t) * nw;
+    const uint32_t ii = ii64 < items ? static_cast<uint32_t>(ii64) : items;
+    uint32_t iv = 0, ih = 0;
+    uint64_t ib = 0, ie = 0;
+    if (ii < n1) {
+      iv = a.list1[gV + ii];
+      ib = a.off[iv];
+      ie = a.off[iv + 1];
+    } else if (ii < items) {
+      const uint32_t c = ii - n1;
+      ih = a.chunk_hub[c];
+      iv = a.hubs[ih];
+      ib = a.chunk_start[c];
+      const uint64_t row_end = a.off[iv + 1];
+      ie = row_end < ib + a.hub_chunk ? row_end : ib + a.hub_chunk;
     }
-    const uint64_t dv = a.dirty[gV + v];
-    const uint64_t w = a.sched[gV + v] & a.act[gV + v] & run & (dv | full);
-    if (!w) continue;
-    const Word C = ((w >> t) & 1) ? cand[static_cast<uint64_t>(v) * 64 + t] : Word(0);
-    Word U = 0;
-    for (Word m = C; m; m &= m - 1) U |= s_nbr[t * kMaxP + __builtin_ctz(m)];
+    uint64_t idv = 0, iw = 0;
+    if (ii < items) {
+      idv = a.dirty[gV + iv];
+      iw = a.sched[gV + iv] & a.act[gV + iv] & run & (idv | full);
+    }
+    const uint64_t todo = __ballot(iw != 0);
+    for (uint64_t tm = todo; tm; tm &= tm - 1) {
+    const int j = __builtin_ctzll(tm);
+    const uint32_t i = static_cast<uint32_t>(wave + (k0 + j) * nw);
+    const bool chunk = i >= n1;
+    const uint32_t v = static_cast<uint32_t>(__builtin_amdgcn_readlane(iv, j));
+    const uint32_t h = static_cast<uint32_t>(__builtin_amdgcn_readlane(ih, j));
+    const uint64_t beg = lane64(static_cast<uint32_t>(ib >> 32), static_cast<uint32_t>(ib), j);
+    const uint64_t end = lane64(static_cast<uint32_t>(ie >> 32), static_cast<uint32_t>(ie), j);
+    const uint64_t dv = lane64(static_cast<uint32_t>(idv >> 32), static_cast<uint32_t>(idv), j);
+    const uint64_t w = lane64(static_cast<uint32_t>(iw >> 32), static_cast<uint32_t>(iw), j);
+    // v's own words are read while the first slots' reads are in flight
+    Word C = 0, U = 0;
+    bool own_read = false;
     Word cp[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k) cp[k] = 0;
@@ -426,44 +465,62 @@ __global__ void __launch_bounds__(kBlockB) kb_phase1(BArgs a) {
       const uint64_t e = base + (t & (kS - 1));
       const bool own = t < kS && e < end;
       const uint64_t mk = own ? eam[e] : 0ull;
+      const uint32_t src = own ? (fresh ? static_cast<uint32_t>(a.slot_lab[e]) : a.edges[e]) : 0u;
+      const uint32_t mlo = static_cast<uint32_t>(mk), mhi = static_cast<uint32_t>(mk >> 32);
+      // Slot k's mask and neighbour (or label index) are uniform, in scalar
+      // registers; Lk = templates for which slot k is evaluated.  Every
+      // lane reads its template's word of each neighbour (one line per
+      // slot for the wave), all reads issued before any is used; lanes
+      // outside Lk are masked out of the results.
+      uint64_t Lk[kS];
       Word cu[kS];
       if (fresh) {
-        const uint32_t sl = own ? a.slot_lab[e] : 0u;
 #pragma unroll
         for (int k = 0; k < kS; ++k) {
-          const uint32_t lk = static_cast<uint32_t>(__shfl(static_cast<int>(sl), k, 64));
-          const uint64_t mkk = shfl64(mk, k);
-          cu[k] = (C && ((mkk >> t) & 1) && lk < a.nglab) ? static_cast<Word>(s_gmask[t * gstride + lk]) : Word(0);
+          Lk[k] = lane64(mhi, mlo, k) & w;
+          const uint32_t sk = static_cast<uint32_t>(__builtin_amdgcn_readlane(src, k));
+          cu[k] = Lk[k] && sk < a.nglab ? static_cast<Word>(s_gmask[t * gstride + sk]) : Word(0);
         }
       } else {
-        const uint32_t uid = own ? a.edges[e] : 0u;
 #pragma unroll
         for (int k = 0; k < kS; ++k) {
-          const uint32_t uk = __shfl(uid, k, 64);
-          const uint64_t mkk = shfl64(mk, k);
-          cu[k] = (C && ((mkk >> t) & 1)) ? cand[static_cast<uint64_t>(uk) * 64 + t] : Word(0);
+          Lk[k] = lane64(mhi, mlo, k) & w;
+          const uint32_t sk = static_cast<uint32_t>(__builtin_amdgcn_readlane(src, k));
+          cu[k] = Lk[k] ? cand[static_cast<uint64_t>(sk) * 64 + t] : Word(0);
         }
+      }
+      if (!own_read) {
+        C = ((w >> t) & 1) ? cand[static_cast<uint64_t>(v) * 64 + t] : Word(0);
+        for (Word m = C; m; m &= m - 1) U |= s_nbr[t * kMaxP + __builtin_ctz(m)];
+        own_read = true;
       }
       uint64_t clr = 0;  // lane k: templates that eliminated slot base + k
 #pragma unroll
       for (int k = 0; k < kS; ++k) {
-        const bool ok = (cu[k] & U) != 0;
-        const bool elim = cu[k] != 0 && !ok;
-        const uint64_t em = __ballot(elim);
-        clr = t == k ? em : clr;
-        el += elim;
-        count_one<NB>(cp, ok ? static_cast<Word>(cu[k] & static_cast<Word>(-cu[k])) : Word(0));
+        const uint64_t L = Lk[k];
+        if (!L) continue;
+        const uint64_t ok = __ballot((cu[k] & U) != 0) & L;
+        const uint64_t em = __ballot(cu[k] != 0) & L & ~ok;
+        el += (em >> t) & 1;
+        if (em) clr = t == k ? em : clr;
+        count_one<NB>(cp, ((ok >> t) & 1) ? static_cast<Word>(cu[k] & static_cast<Word>(-cu[k])) : Word(0));
       }
       if (own && clr) eam[e] = mk & ~clr;
     }
+    if (!own_read) C = ((w >> t) & 1) ? cand[static_cast<uint64_t>(v) * 64 + t] : Word(0);  // a row without slots
     ee += el;
     if (!chunk) {
-      const Word fail = C ? requirements<NB>(C, cp, tb, &s_off[t * (kMaxP + 1)]) : Word(0);
+      const uint8_t* ro = &s_off[t * (kMaxP + 1)];
+      const Word fail = !C                        ? Word(0)
+                        : ro[kMaxP] <= kEntLds ? requirements<NB>(C, cp, &s_ent[t * kEntLds], ro)
+                                               : requirements<NB>(C, cp, tb.ent, ro);
       const uint64_t M = __ballot(fail != 0 || el != 0);
       const uint64_t F = __ballot(fail != 0);
       if (fail) a.unm[(gV + v) * 64 + t] = fail;
-      if (t == 0 && M) a.mod[gV + v] |= M;
-      if (t == 1 && F) a.pend[gV + v] |= F;
+      // no other wave writes these words of v in phase 1; atomics without a
+      // return value leave no load to wait for
+      if (t == 0 && M) atomicOr(reinterpret_cast<unsigned long long*>(&a.mod[gV + v]), static_cast<unsigned long long>(M));
+      if (t == 1 && F) atomicOr(reinterpret_cast<unsigned long long*>(&a.pend[gV + v]), static_cast<unsigned long long>(F));
       if (t == 2 && (dv & w)) a.dirty[gV + v] = dv & ~w;
     } else {
       if (C) {
@@ -481,6 +538,7 @@ __global__ void __launch_bounds__(kBlockB) kb_phase1(BArgs a) {
       }
       const uint64_t EL = __ballot(el != 0);
       if (t == 0 && EL) atomicOr(&a.hub_elim[static_cast<uint64_t>(g) * a.nhubs + h], static_cast<unsigned long long>(EL));
+    }
     }
   }
   flush_counters(a, g, 0, ee, 0);
@@ -824,8 +882,10 @@ int launch_step(pj_batch* b, int nb) {
   const size_t gm = 64 * (a.nglab | 1u) * sizeof(uint32_t);
   if (nb == 4)
     hipLaunchKernelGGL(kb_phase1<4>, dim3(cus * 8, b->G), dim3(kBlockB), gm, b->stream, a);
-  else
+  else if (nb == 2)
     hipLaunchKernelGGL(kb_phase1<2>, dim3(cus * 8, b->G), dim3(kBlockB), gm, b->stream, a);
+  else
+    hipLaunchKernelGGL(kb_phase1<1>, dim3(cus * 8, b->G), dim3(kBlockB), gm, b->stream, a);
   if (a.nhubs)
     hipLaunchKernelGGL(kb_hub_finish, dim3(std::max<uint32_t>(1, std::min<uint32_t>((a.nhubs + kWaves - 1) / kWaves,
                                                                                   cus * 4)),
@@ -1055,7 +1115,7 @@ int pj_batch_run(pj_batch* b, uint32_t max_steps, uint32_t* done, uint32_t max_d
     if (!any) break;
     lk.unlock();
     PJB_HIP(hipEventRecord(b->ev0, b->stream));
-    if (launch_step(b, maxneed > 3 ? 4 : 2)) return -1;
+    if (launch_step(b, maxneed > 3 ? 4 : maxneed > 1 ? 2 : 1)) return -1;
     PJB_HIP(hipEventRecord(b->ev1, b->stream));
     PJB_HIP(hipMemcpyAsync(b->h_ctl, a.ctl, b->G * kCtl * 8, hipMemcpyDeviceToHost, b->stream));
     PJB_HIP(hipStreamSynchronize(b->stream));
