@@ -15,7 +15,6 @@
 // exactly when its candidate set becomes empty (lcc0_cpu.tpp:42-45,
 // lcc_cpu.tpp:114-117, cc_cpu.tpp:141-147).
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstddef>
@@ -2251,6 +2250,102 @@ __global__ void __launch_bounds__(kBlock) k_view_hubs(const uint32_t* __restrict
   }
 }
 
+// Exclusive prefix sum of the view's row counts, in three launches: per
+// span of kScanSpan counts a block total (k_scan_spans), the exclusive scan
+// of those totals in one block (k_scan_totals), then each span rescanned from
+// LDS with its total's offset (k_scan_apply).  Each count is read twice and
+// each offset written once.
+constexpr int kScanPer = 16;
+constexpr uint64_t kScanSpan = static_cast<uint64_t>(kBlock) * kScanPer;
+constexpr int kTotalsBlock = 1024;
+
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d) {
+  const uint32_t lo = __shfl_up(static_cast<uint32_t>(v), d);
+  const uint32_t hi = __shfl_up(static_cast<uint32_t>(v >> 32), d);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Exclusive prefix of x over the block's threads (blockDim.x a multiple of
+// 64, at most kTotalsBlock); *total receives the block's sum.  Every thread
+// of the block must call it.
+__device__ __forceinline__ uint64_t block_exclusive(uint64_t x, uint64_t* total) {
+  __shared__ uint64_t s_wave[kTotalsBlock / 64];
+  const int lane = lane_id(), w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  uint64_t inc = x;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t y = shfl_up64(inc, d);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) s_wave[w] = inc;
+  __syncthreads();
+  uint64_t before = 0, all = 0;
+  for (int i = 0; i < nw; ++i) {
+    const uint64_t t = s_wave[i];
+    before += i < w ? t : 0;
+    all += t;
+  }
+  __syncthreads();
+  *total = all;
+  return before + inc - x;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_spans(const uint64_t* __restrict__ in, uint64_t n,
+                                                      uint64_t* __restrict__ totals) {
+  const uint64_t a = blockIdx.x * kScanSpan;
+  const uint64_t z = a + kScanSpan < n ? a + kScanSpan : n;
+  uint64_t sum = 0;
+  for (uint64_t i = a + threadIdx.x; i < z; i += kBlock) sum += in[i];
+  uint64_t all;
+  block_exclusive(sum, &all);
+  if (threadIdx.x == 0) totals[blockIdx.x] = all;
+}
+
+__global__ void __launch_bounds__(kTotalsBlock) k_scan_totals(uint64_t* __restrict__ totals, uint64_t n) {
+  const uint64_t seg = (n + kTotalsBlock - 1) / kTotalsBlock;
+  const uint64_t a = threadIdx.x * seg < n ? threadIdx.x * seg : n;
+  const uint64_t z = a + seg < n ? a + seg : n;
+  uint64_t sum = 0;
+  for (uint64_t i = a; i < z; ++i) sum += totals[i];
+  uint64_t all;
+  uint64_t run = block_exclusive(sum, &all);
+  for (uint64_t i = a; i < z; ++i) {
+    const uint64_t x = totals[i];
+    totals[i] = run;
+    run += x;
+  }
+}
+
+// One span: counts into LDS (coalesced), kScanPer consecutive counts per
+// thread scanned in registers, offsets back through LDS.  The LDS index is
+// padded by one word per kScanPer so that a wave's threads, each at the
+// start of its own run, fall on distinct banks.
+__global__ void __launch_bounds__(kBlock) k_scan_apply(const uint64_t* __restrict__ in, uint64_t n,
+                                                      const uint64_t* __restrict__ totals,
+                                                      uint64_t* __restrict__ out) {
+  __shared__ uint64_t s[kScanSpan + kScanSpan / kScanPer];
+  const uint64_t a = blockIdx.x * kScanSpan;
+  const uint32_t len = static_cast<uint32_t>(a + kScanSpan < n ? kScanSpan : n - a);
+  for (uint32_t i = threadIdx.x; i < kScanSpan; i += kBlock) s[i + i / kScanPer] = i < len ? in[a + i] : 0;
+  __syncthreads();
+  uint64_t* mine = s + threadIdx.x * (kScanPer + 1);
+  uint64_t v[kScanPer];
+  uint64_t sum = 0;
+#pragma unroll
+  for (int j = 0; j < kScanPer; ++j) {
+    v[j] = mine[j];
+    sum += v[j];
+  }
+  uint64_t all;
+  uint64_t run = totals[blockIdx.x] + block_exclusive(sum, &all);
+#pragma unroll
+  for (int j = 0; j < kScanPer; ++j) {
+    mine[j] = run;
+    run += v[j];
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < len; i += kBlock) out[a + i] = s[i + i / kScanPer];
+}
+
 // The memoised walks of one source: plans in walk order; a root already
 // matched by another source's walk is taken as matched (cc_cpu.tpp:102-108,
 // pc_cpu.tpp:104-142, tds_cpu.tpp:160-188).  Returns the failing roots;
@@ -2835,8 +2930,7 @@ struct pj_ctx {
   uint64_t* d_view_off = nullptr;    // V + 1
   uint32_t* d_view_edges = nullptr;
   uint64_t view_cap = 0;
-  void* d_scan_tmp = nullptr;
-  size_t scan_tmp_bytes = 0;
+  uint64_t* d_scan_tmp = nullptr;  // per-span totals of the view-count scan
   uint32_t* d_ovf = nullptr;  // [0] first-pass overflows, [1] second-pass overflows, then the list
   uint32_t* d_bins = nullptr;        // kBins * V
   uint32_t* d_bin_count = nullptr;   // kBins
@@ -2974,7 +3068,6 @@ void free_graph(pj_ctx* ctx) {
   ctx->d_view_edges = nullptr;
   ctx->d_scan_tmp = nullptr;
   ctx->view_cap = 0;
-  ctx->scan_tmp_bytes = 0;
   ctx->V = 0;
   ctx->E = 0;
 }
@@ -3328,9 +3421,8 @@ int build_view(pj_ctx* ctx) {
   const uint64_t V = ctx->V;
   if (!ctx->d_view_off) {
     if (dev_alloc(ctx, &ctx->d_view_count, V + 1) || dev_alloc(ctx, &ctx->d_view_off, V + 1)) return -1;
-    PJ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, ctx->scan_tmp_bytes, ctx->d_view_count, ctx->d_view_off, V + 1,
-                                            ctx->stream));
-    PJ_HIP(hipMalloc(&ctx->d_scan_tmp, ctx->scan_tmp_bytes));
+    const uint64_t spans = (V + 1 + kScanSpan - 1) / kScanSpan;
+    if (dev_alloc(ctx, &ctx->d_scan_tmp, spans)) return -1;
   }
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
   const int g = ctx->cus * 8;
@@ -3349,8 +3441,13 @@ int build_view(pj_ctx* ctx) {
   if (ctx->nhubs)
     hipLaunchKernelGGL(k_view_hubs<Word>, dim3(hub_blocks), dim3(kBlock), 0, ctx->stream, ctx->d_hubs, ctx->nhubs,
                        ctx->d_off, ctx->d_edges, ctx->d_ea, cand, ctx->d_view_count, nullptr, nullptr);
-  PJ_HIP(hipcub::DeviceScan::ExclusiveSum(ctx->d_scan_tmp, ctx->scan_tmp_bytes, ctx->d_view_count, ctx->d_view_off,
-                                          V + 1, ctx->stream));
+  const uint64_t spans = (V + 1 + kScanSpan - 1) / kScanSpan;
+  hipLaunchKernelGGL(k_scan_spans, dim3(spans), dim3(kBlock), 0, ctx->stream, ctx->d_view_count, V + 1,
+                     ctx->d_scan_tmp);
+  hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kTotalsBlock), 0, ctx->stream, ctx->d_scan_tmp, spans);
+  hipLaunchKernelGGL(k_scan_apply, dim3(spans), dim3(kBlock), 0, ctx->stream, ctx->d_view_count, V + 1,
+                     ctx->d_scan_tmp, ctx->d_view_off);
+  PJ_HIP(hipGetLastError());
   uint64_t total = 0;
   PJ_HIP(hipMemcpyAsync(&total, ctx->d_view_off + V, 8, hipMemcpyDeviceToHost, ctx->stream));
   PJ_HIP(hipStreamSynchronize(ctx->stream));

@@ -33,6 +33,12 @@ _TEMPLATE_NAMES = {
     "k_triangles_mid": "triangles",
     "k_triangles_memo": "triangles_memo",
     "k_memo_lds": "memo_walk",
+    "k_view_count": "active_view",
+    "k_view_hubs": "active_view",
+    "k_view_fill": "active_view",
+    "k_scan_spans": "active_view",
+    "k_scan_totals": "active_view",
+    "k_scan_apply": "active_view",
 }
 
 
