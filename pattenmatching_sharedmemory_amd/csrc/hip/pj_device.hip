@@ -1691,14 +1691,58 @@ __device__ __forceinline__ bool active_slot(uint32_t a, uint32_t b, const uint64
   return false;
 }
 
+// Slot positions between heavy vertices (rows above the index's degree
+// threshold), built at upload for symmetric graphs with sorted rows:
+// doff[v] = off[v] | (heavy id of v + 1) << kPairShift (no id: 0), and
+// pos[ha * H + hb] = first slot of hb in row(ha) minus that row's start, or
+// kNoPos.  A search between two heavy rows (each binary-search probe of a
+// long row fetches its own cache line for four useful bytes) becomes one
+// load.  pos == nullptr: no index (searches only).
+constexpr int kPairShift = 40;
+constexpr uint64_t kPairOffMask = (1ull << kPairShift) - 1;
+constexpr uint32_t kNoPos = 0xFFFFFFFFu;
+// The pair rounds of a 3-vertex cycle source start with this many pairs and
+// double up to the group width: most sources that succeed do so in their
+// first pairs, and every pair tested in the round after a success is read
+// for nothing (modelled on the scale-22 plain workload's cycle step: rounds
+// of 32, 64 read 76 % of the lines that flat rounds of 64 read, in as many
+// rounds; starting at 8 reads 44 % but takes 20 % more rounds, and measured
+// slower: the rounds' latency, not the lines, bounds the kernels once the
+// pair index decides the searches between heavy rows).
+constexpr int kFirstRound = 32;
+struct PairIndex {
+  const uint64_t* doff;
+  const uint32_t* pos;
+  uint64_t H;
+};
+
 // active_slot on a symmetric graph with reverse positions: the shorter of
 // row(a) and row(b) is searched (rows hold no duplicate slots), a hit in
-// row(b) being mapped to the slot a->b through rev.
+// row(b) being mapped to the slot a->b through rev; two heavy rows are
+// decided by the pair index (its position is the slot the search finds).
 __device__ __forceinline__ bool active_slot_rev(uint32_t a, uint32_t b, const uint64_t* __restrict__ off,
                                                 const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
-                                                const uint32_t* __restrict__ rev, uint64_t& probes) {
-  const uint64_t ab = off[a], bb = off[b];
-  const uint64_t da = off[a + 1] - ab, db = off[b + 1] - bb;
+                                                const uint32_t* __restrict__ rev, const PairIndex& pi,
+                                                uint64_t& probes) {
+  uint64_t ab, bb, da, db;
+  if (pi.pos) {
+    const uint64_t xa = pi.doff[a], xb = pi.doff[b];
+    ab = xa & kPairOffMask;
+    bb = xb & kPairOffMask;
+    const uint64_t ha = xa >> kPairShift, hb = xb >> kPairShift;
+    if (ha && hb) {
+      ++probes;
+      const uint32_t r = pi.pos[(ha - 1) * pi.H + (hb - 1)];
+      return r != kNoPos && ea_test(ea, ab + r);
+    }
+    da = (pi.doff[a + 1] & kPairOffMask) - ab;
+    db = (pi.doff[b + 1] & kPairOffMask) - bb;
+  } else {
+    ab = off[a];
+    bb = off[b];
+    da = off[a + 1] - ab;
+    db = off[b + 1] - bb;
+  }
   const bool in_a = da <= db;
   uint64_t lo = in_a ? ab : bb, hi = lo + (in_a ? da : db);
   const uint64_t end = hi;
@@ -1715,6 +1759,26 @@ __device__ __forceinline__ bool active_slot_rev(uint32_t a, uint32_t b, const ui
   return ea_test(ea, in_a ? lo : ab + rev[lo]);
 }
 
+// Pair index positions: one wave per heavy row; each slot to a heavy
+// neighbour records its offset in the row (the smallest, which is where the
+// binary search's lower bound lands, should a row repeat a neighbour).
+__global__ void __launch_bounds__(kBlock) k_pair_positions(const uint32_t* __restrict__ heavy, uint64_t H,
+                                                          const uint64_t* __restrict__ doff,
+                                                          const uint32_t* __restrict__ edges,
+                                                          uint32_t* __restrict__ pos) {
+  const int lane = lane_id();
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64);
+  for (uint64_t h = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6); h < H; h += waves) {
+    const uint32_t a = heavy[h];
+    const uint64_t beg = doff[a] & kPairOffMask, end = doff[a + 1] & kPairOffMask;
+    uint32_t* row = pos + h * H;
+    for (uint64_t e = beg + lane; e < end; e += 64) {
+      const uint64_t hb = doff[edges[e]] >> kPairShift;
+      if (hb) atomicMin(&row[hb - 1], static_cast<uint32_t>(e - beg));
+    }
+  }
+}
+
 // Last two hops of a cycle on sorted rows: is there a u with an active slot
 // v->u, template vertex q among u's candidates, and an active slot u->s?
 // The shorter of row(v) and row(s) is read 64 slots at a time, each lane
@@ -1725,7 +1789,8 @@ __device__ __forceinline__ bool active_slot_rev(uint32_t a, uint32_t b, const ui
 template <typename Word>
 __device__ bool close_cycle(uint32_t v, uint32_t s, int q, const uint64_t* __restrict__ off,
                             const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
-                            const uint32_t* __restrict__ rev, const Word* __restrict__ cand, uint64_t& slots) {
+                            const uint32_t* __restrict__ rev, const PairIndex& pi, const Word* __restrict__ cand,
+                            uint64_t& slots) {
   const int lane = lane_id();
   const uint64_t vb = off[v], ve = off[v + 1], sb = off[s], se = off[s + 1];
   const bool from_v = (ve - vb) <= (se - sb);
@@ -1740,9 +1805,9 @@ __device__ bool close_cycle(uint32_t v, uint32_t s, int q, const uint64_t* __res
       if ((cand[u] >> q) & 1) {
         if (rev) {
           if (from_v)
-            ok = ea_test(ea, e) && active_slot_rev(u, s, off, edges, ea, rev, probes);
+            ok = ea_test(ea, e) && active_slot_rev(u, s, off, edges, ea, rev, pi, probes);
           else
-            ok = ea_test(ea, off[u] + rev[e]) && active_slot_rev(v, u, off, edges, ea, rev, probes);
+            ok = ea_test(ea, off[u] + rev[e]) && active_slot_rev(v, u, off, edges, ea, rev, pi, probes);
         } else if (from_v) {
           ok = ea_test(ea, e) && active_slot(u, s, off, edges, ea, probes);
         } else {
@@ -1774,7 +1839,8 @@ __device__ bool close_cycle(uint32_t v, uint32_t s, int q, const uint64_t* __res
 template <typename Word>
 __device__ bool triangle_small(uint32_t s, int q1, int q2, const uint64_t* __restrict__ off,
                                const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
-                               const uint32_t* __restrict__ rev, const Word* __restrict__ cand, uint64_t& slots) {
+                               const uint32_t* __restrict__ rev, const PairIndex& pi, const Word* __restrict__ cand,
+                               uint64_t& slots) {
   const int lane = lane_id();
   const uint64_t beg = off[s];
   const int deg = static_cast<int>(off[s + 1] - beg);
@@ -1803,11 +1869,11 @@ __device__ bool triangle_small(uint32_t s, int q1, int q2, const uint64_t* __res
     const uint32_t a_list = __shfl(w, a_from, 64), u_list = __shfl(w, u_from, 64);
     const int na = __popcll(ma), nu = __popcll(mu);
     const int pairs = na * nu;
-    for (int p0 = 0; p0 < pairs && !found; p0 += 64) {
+    for (int p0 = 0, sz = kFirstRound; p0 < pairs && !found; p0 += sz, sz = 2 * sz < 64 ? 2 * sz : 64) {
       const int p = p0 + lane;
       const int ia = p / nu, iu = p - ia * nu;
       const uint32_t a = __shfl(a_list, ia & 63, 64), u = __shfl(u_list, iu & 63, 64);
-      const bool ok = p < pairs && active_slot_rev(a, u, off, edges, ea, rev, probes);
+      const bool ok = lane < sz && p < pairs && active_slot_rev(a, u, off, edges, ea, rev, pi, probes);
       found = __ballot(ok) != 0;
     }
   }
@@ -1821,7 +1887,7 @@ __device__ bool triangle_small(uint32_t s, int q1, int q2, const uint64_t* __res
 template <typename Word>
 __device__ bool cycle_walk(uint32_t s, uint32_t j, uint32_t L, bool sorted, const uint64_t* __restrict__ off,
                            const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
-                           const uint32_t* __restrict__ rev, const Word* __restrict__ cand,
+                           const uint32_t* __restrict__ rev, const PairIndex& pi, const Word* __restrict__ cand,
                            const DevTemplate* __restrict__ t, MemoTable* fm, uint64_t& slots) {
   const int lane = lane_id();
   uint32_t st_v = s;          // lane i: vertex at level i
@@ -1836,7 +1902,7 @@ __device__ bool cycle_walk(uint32_t s, uint32_t j, uint32_t L, bool sorted, cons
   while (true) {
     if (sorted && depth == static_cast<int>(L) - 2) {
       const int q = static_cast<int>(t->nlcc_vertex[(j + L - 1) % L]);
-      if (close_cycle<Word>(v, s, q, off, edges, ea, rev, cand, slots)) return true;
+      if (close_cycle<Word>(v, s, q, off, edges, ea, rev, pi, cand, slots)) return true;
       if (fm && depth >= 2) state_fail(*fm, v, depth);
       depth = uniform(depth - 1);
       v = rl32(st_v, depth);
@@ -2444,8 +2510,9 @@ __global__ void __launch_bounds__(kBlock) k_triangles(const uint32_t* __restrict
                                                       const uint64_t* __restrict__ off,
                                                       const uint32_t* __restrict__ edges,
                                                       const uint32_t* __restrict__ ea, const uint32_t* __restrict__ rev,
-                                                      const Word* __restrict__ cand, Word* __restrict__ unmatch,
-                                                      uint8_t* __restrict__ flags, const DevTemplate* __restrict__ t,
+                                                      PairIndex pi, const Word* __restrict__ cand,
+                                                      Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
+                                                      const DevTemplate* __restrict__ t,
                                                       unsigned long long* counters) {
   const int lane = lane_id();
   const int gl = lane & (G - 1);
@@ -2493,12 +2560,13 @@ __global__ void __launch_bounds__(kBlock) k_triangles(const uint32_t* __restrict
         const uint32_t a_list = __shfl(w, gbase + a_from, 64), u_list = __shfl(w, gbase + u_from, 64);
         const int na = __popcll(ma), nu = __popcll(mu);
         const int pairs = na * nu;
-        for (int p0 = 0; p0 < pairs && !found; p0 += G) {
+        for (int p0 = 0, sz = kFirstRound < G ? kFirstRound : G; p0 < pairs && !found;
+             p0 += sz, sz = 2 * sz < G ? 2 * sz : G) {
           const int p = p0 + gl;
           const int ia = p / nu, iu = p - ia * nu;
           const uint32_t a = __shfl(a_list, gbase + (ia & (G - 1)), 64);
           const uint32_t u = __shfl(u_list, gbase + (iu & (G - 1)), 64);
-          const bool ok = p < pairs && active_slot_rev(a, u, off, edges, ea, rev, probes);
+          const bool ok = gl < sz && p < pairs && active_slot_rev(a, u, off, edges, ea, rev, pi, probes);
           found = (__ballot(ok) & gmask) != 0;
         }
       }
@@ -2525,7 +2593,8 @@ __global__ void __launch_bounds__(kBlock) k_triangles(const uint32_t* __restrict
 template <typename Word, int KC>
 __device__ bool triangle_chunks(uint32_t s, int q1, int q2, const uint64_t* __restrict__ off,
                                 const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
-                                const uint32_t* __restrict__ rev, const Word* __restrict__ cand, uint64_t& slots) {
+                                const uint32_t* __restrict__ rev, const PairIndex& pi, const Word* __restrict__ cand,
+                                uint64_t& slots) {
   const int lane = lane_id();
   const uint64_t beg = off[s];
   const int deg = static_cast<int>(off[s + 1] - beg);
@@ -2574,6 +2643,7 @@ __device__ bool triangle_chunks(uint32_t s, int q1, int q2, const uint64_t* __re
       a_list[k] = __shfl(w[k], a_from, 64);
       u_list[k] = __shfl(w[k], u_from, 64);
     }
+    int sz = kFirstRound;  // grows over the source's rounds, whichever chunk pair they test
 #pragma unroll
     for (int ka = 0; ka < KC; ++ka) {
 #pragma unroll
@@ -2581,11 +2651,11 @@ __device__ bool triangle_chunks(uint32_t s, int q1, int q2, const uint64_t* __re
         if (found || !ma[ka] || !mu[ku]) continue;
         const int na = __popcll(ma[ka]), nu = __popcll(mu[ku]);
         const int pairs = na * nu;
-        for (int p0 = 0; p0 < pairs && !found; p0 += 64) {
+        for (int p0 = 0; p0 < pairs && !found; p0 += sz, sz = 2 * sz < 64 ? 2 * sz : 64) {
           const int p = p0 + lane;
           const int ia = p / nu, iu = p - ia * nu;
           const uint32_t a = __shfl(a_list[ka], ia & 63, 64), u = __shfl(u_list[ku], iu & 63, 64);
-          const bool ok = p < pairs && active_slot_rev(a, u, off, edges, ea, rev, probes);
+          const bool ok = lane < sz && p < pairs && active_slot_rev(a, u, off, edges, ea, rev, pi, probes);
           found = __ballot(ok) != 0;
         }
       }
@@ -2606,9 +2676,9 @@ template <typename Word>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
 k_triangles_mid(const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
                 const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
-                const uint32_t* __restrict__ ea, const uint32_t* __restrict__ rev, const Word* __restrict__ cand,
-                Word* __restrict__ unmatch, uint8_t* __restrict__ flags, const DevTemplate* __restrict__ t,
-                unsigned long long* counters) {
+                const uint32_t* __restrict__ ea, const uint32_t* __restrict__ rev, PairIndex pi,
+                const Word* __restrict__ cand, Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
+                const DevTemplate* __restrict__ t, unsigned long long* counters) {
   const uint32_t n = *list_count;
   if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_ITEMS] = n;
   const uint32_t waves = gridDim.x * (blockDim.x / 64);
@@ -2624,8 +2694,8 @@ k_triangles_mid(const uint32_t* __restrict__ list, const uint32_t* __restrict__ 
       const uint32_t q = qv[j];
       if (!((cs >> q) & 1)) continue;
       const int q1 = static_cast<int>(qv[(j + 1) % 3]), q2 = static_cast<int>(qv[(j + 2) % 3]);
-      const bool ok = deg <= 64 ? triangle_small<Word>(s, q1, q2, off, edges, ea, rev, cand, slots)
-                                : triangle_chunks<Word, kMidChunks>(s, q1, q2, off, edges, ea, rev, cand, slots);
+      const bool ok = deg <= 64 ? triangle_small<Word>(s, q1, q2, off, edges, ea, rev, pi, cand, slots)
+                                : triangle_chunks<Word, kMidChunks>(s, q1, q2, off, edges, ea, rev, pi, cand, slots);
       if (!ok) fail |= Word(1) << q;
     }
     if (fail && lane_id() == 0) {  // s is in one task list once: this wave is its only writer
@@ -2661,7 +2731,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
                                                            const uint64_t* __restrict__ off,
                                                            const uint32_t* __restrict__ edges,
                                                            const uint32_t* __restrict__ ea,
-                                                           const uint32_t* __restrict__ rev,
+                                                           const uint32_t* __restrict__ rev, PairIndex pi,
                                                            const Word* __restrict__ cand, Word* __restrict__ unmatch,
                                                            uint8_t* __restrict__ flags, Word* already,
                                                            const DevTemplate* __restrict__ t,
@@ -2715,7 +2785,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
               }
               if (lo < se && edges[lo] == u && lo - sb < pa && ea_test(ea, lo)) ok = false;
             }
-            if (ok) ok = active_slot_rev(u, s, off, edges, ea, rev, probes);
+            if (ok) ok = active_slot_rev(u, s, off, edges, ea, rev, pi, probes);
           }
           const uint64_t m = __ballot(ok);
           slots += m ? __builtin_ctzll(m) + 1 : min<uint64_t>(64, ae - base);
@@ -2758,9 +2828,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kOc
                                                  const uint32_t* __restrict__ list2_count,
                                                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
                                                  const uint32_t* __restrict__ ea, const uint32_t* __restrict__ rev,
-                                                 const Word* __restrict__ cand, Word* __restrict__ unmatch,
-                                                 uint8_t* __restrict__ flags, const DevTemplate* __restrict__ t,
-                                                 MemoArgs ma, unsigned long long* counters) {
+                                                 PairIndex pi, const Word* __restrict__ cand,
+                                                 Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
+                                                 const DevTemplate* __restrict__ t, MemoArgs ma,
+                                                 unsigned long long* counters) {
   // list2 (may be null) follows list: sources w >= n1 are list2[w - n1]
   const uint32_t n1 = *list_count;
   const uint32_t n = n1 + (list2 ? *list2_count : 0u);
@@ -2798,8 +2869,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kOc
         const bool ok = L == 3 && rev && off[s + 1] - off[s] <= 64
                             ? triangle_small<Word>(s, static_cast<int>(t->nlcc_vertex[(j + 1) % 3]),
                                                    static_cast<int>(t->nlcc_vertex[(j + 2) % 3]), off, edges, ea,
-                                                   rev, cand, slots)
-                            : cycle_walk<Word>(s, j, L, t->rows_sorted != 0, off, edges, ea, rev, cand, t, fm, slots);
+                                                   rev, pi, cand, slots)
+                            : cycle_walk<Word>(s, j, L, t->rows_sorted != 0, off, edges, ea, rev, pi, cand, t, fm,
+                                               slots);
         if (!ok) fail |= Word(1) << q;
       }
     } else if (kind == 1) {
@@ -2893,6 +2965,15 @@ struct pj_ctx {
   // labels (graphs of at most 255 labels; null otherwise), and those labels.
   uint8_t* d_slot_lab = nullptr;
   std::vector<uint64_t> graph_labels;
+  // Pair index of the heavy rows (PairIndex): decorated offsets [V + 1] and
+  // positions [pair_heavy^2]; null when the graph has fewer than two rows
+  // above the threshold that fits the budget, or no reverse positions.
+  uint64_t* d_pair_doff = nullptr;
+  uint32_t* d_pair_pos = nullptr;
+  uint64_t pair_heavy = 0;
+  uint64_t pair_degree = 0;            // rows of more slots than this are heavy
+  uint64_t pair_min_degree = 512;      // PJ_PAIR_DEGREE: the smallest threshold tried
+  uint64_t pair_budget = 8ull << 30;   // PJ_PAIR_INDEX_MB: bytes the positions may take (0: no index)
   // 0 after a state reset, 1 after LCC0 on such a state (every active
   // vertex's candidates are then the template vertices of its label), 2 once
   // anything else ran: the first LCC step reads slot labels instead of
@@ -3021,6 +3102,8 @@ void free_graph(pj_ctx* ctx) {
     ctx->d_labels = nullptr;
     ctx->d_rev = nullptr;
     ctx->d_slot_lab = nullptr;
+    ctx->d_pair_doff = nullptr;
+    ctx->d_pair_pos = nullptr;
     ctx->d_hubs = ctx->d_chunk_hub = nullptr;
     ctx->d_chunk_start = nullptr;
     ctx->graph_shared = false;
@@ -3033,9 +3116,12 @@ void free_graph(pj_ctx* ctx) {
                   ctx->memo1.wave_epoch, ctx->d_already, ctx->d_ovf, ctx->d_view_count, ctx->d_view_off,
                   ctx->d_view_edges, ctx->d_scan_tmp, ctx->d_rev, ctx->d_live[0], ctx->d_live[1],
                   ctx->d_compact_scratch, ctx->d_out_v, ctx->d_out_src, ctx->d_out_dst, ctx->d_out_cand,
-                  ctx->d_stat, ctx->d_slot_lab};
+                  ctx->d_stat, ctx->d_slot_lab, ctx->d_pair_doff, ctx->d_pair_pos};
   for (void* b : bufs)
     if (b) hipFree(b);
+  ctx->d_pair_doff = nullptr;
+  ctx->d_pair_pos = nullptr;
+  ctx->pair_heavy = ctx->pair_degree = 0;
   ctx->d_off = nullptr;
   ctx->d_edges = nullptr;
   ctx->d_labels = nullptr;
@@ -3267,6 +3353,11 @@ void launch_phase2(pj_ctx* ctx, int mode = 3) {
 // switched off).
 const uint32_t* reverse_positions(const pj_ctx* ctx) {
   return ctx->rows_sorted && ctx->symmetric ? ctx->d_rev : nullptr;
+}
+
+PairIndex pair_index(const pj_ctx* ctx) {
+  if (!ctx->d_pair_pos || !reverse_positions(ctx)) return PairIndex{nullptr, nullptr, 0};
+  return PairIndex{ctx->d_pair_doff, ctx->d_pair_pos, ctx->pair_heavy};
 }
 
 // The push of an NLCC phase 2: rescan marks cleared, then every modified
@@ -3502,7 +3593,8 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
       unsigned long long* c = begin_launch(ctx, K_TRIANGLES_MEMO);
       hipLaunchKernelGGL(k_triangles_memo<Word>, dim3(ctx->cus * kWalkBlocksPerCu), dim3(kBlock), 0, ctx->stream,
                          ctx->d_bins + b * static_cast<uint64_t>(ctx->V), ctx->d_bin_count + b, ctx->d_off,
-                         ctx->d_edges, ctx->d_ea, reverse_positions(ctx), static_cast<const Word*>(ctx->d_cand),
+                         ctx->d_edges, ctx->d_ea, reverse_positions(ctx), pair_index(ctx),
+                         static_cast<const Word*>(ctx->d_cand),
                          static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, static_cast<Word*>(ctx->d_already),
                          ctx->d_t, c);
       end_launch(ctx);
@@ -3529,6 +3621,7 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
   const int g = tables ? kMemoWaves / (kBlock / 64) : ctx->cus * kWalkBlocksPerCu;
   // 3-vertex cycles from low-degree sources: several sources per wave
   const uint32_t* rev = reverse_positions(ctx);
+  const PairIndex pi = memo ? PairIndex{nullptr, nullptr, 0} : pair_index(ctx);  // positions refer to the CSR
   const bool triangles = kind == 0 && !tables && rev && ctx->h_t.nlcc_len == 3;
   // Every bin holds active vertices; hubs are run from their own list.
   for (int b = 0; b < kBins; ++b) {
@@ -3537,13 +3630,13 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
         begin_launch(ctx, triangles && b <= 2 ? K_TRIANGLES : memo ? K_MEMO_WALK : K_NLCC);
     if (triangles && b == 0)
       hipLaunchKernelGGL((k_triangles<Word, kWalkBin0>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
-                         ctx->d_bin_count + b, off, edges, ea, rev, cand, unm, ctx->d_flags, ctx->d_t, c);
+                         ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
     else if (triangles && b == 1)
       hipLaunchKernelGGL((k_triangles<Word, 32>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
-                         ctx->d_bin_count + b, off, edges, ea, rev, cand, unm, ctx->d_flags, ctx->d_t, c);
+                         ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
     else if (triangles && b == 2)
       hipLaunchKernelGGL(k_triangles_mid<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
-                         ctx->d_bin_count + b, off, edges, ea, rev, cand, unm, ctx->d_flags, ctx->d_t, c);
+                         ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
     else if (memo)  // first pass with LDS tables; overflows go to the list of d_ovf[0]
       hipLaunchKernelGGL(k_memo_lds<Word>, dim3(kMemoWaves), dim3(64), 0, ctx->stream, ctx->d_bins + b * V,
                          ctx->d_bin_count + b, off, edges, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
@@ -3552,7 +3645,7 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
                          kind, ctx->d_bins + b * V,
                          ctx->d_bin_count + b, b == 3 && !triangles ? ctx->d_bins + 2 * V : nullptr,
                          b == 3 && !triangles ? ctx->d_bin_count + 2 : nullptr, off, edges, ea,
-                         reverse_positions(ctx), cand, unm, ctx->d_flags, ctx->d_t, ma, c);
+                         reverse_positions(ctx), pi, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
     end_launch(ctx);
   }
   return 0;
@@ -3612,7 +3705,8 @@ int launch_nlcc_retry(pj_ctx* ctx, int kind) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, out_count, static_cast<uint64_t>(1), 0u);
     unsigned long long* c = begin_launch(ctx, K_NLCC);
     hipLaunchKernelGGL((k_nlcc<Word, false>), dim3(waves), dim3(64), 0, ctx->stream, kind, in_list, in_count, nullptr,
-                       nullptr, ctx->d_view_off, ctx->d_view_edges, nullptr, nullptr, static_cast<const Word*>(ctx->d_cand),
+                       nullptr, ctx->d_view_off, ctx->d_view_edges, nullptr, nullptr, PairIndex{nullptr, nullptr, 0},
+                       static_cast<const Word*>(ctx->d_cand),
                        static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->d_t, m, c);
     end_launch(ctx);
   }
@@ -3642,7 +3736,7 @@ void launch_nlcc_hubs(pj_ctx* ctx, int kind, uint32_t* d_hub_n) {
   if (triangle_memo(ctx, kind))
     hipLaunchKernelGGL(k_triangles_memo<Word>, dim3(std::min<uint32_t>(ctx->nhubs, ctx->cus * 8)), dim3(64), 0,
                        ctx->stream, ctx->d_hubs, d_hub_n, ctx->d_off, ctx->d_edges, ctx->d_ea, reverse_positions(ctx),
-                       cand, unm, ctx->d_flags, static_cast<Word*>(ctx->d_already), ctx->d_t, c);
+                       pair_index(ctx), cand, unm, ctx->d_flags, static_cast<Word*>(ctx->d_already), ctx->d_t, c);
   else if (memo)
     hipLaunchKernelGGL(k_memo_lds<Word>, dim3(g), dim3(64), 0, ctx->stream, ctx->d_hubs, d_hub_n, ctx->d_view_off,
                        ctx->d_view_edges, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
@@ -3650,7 +3744,8 @@ void launch_nlcc_hubs(pj_ctx* ctx, int kind, uint32_t* d_hub_n) {
     hipLaunchKernelGGL((tables ? k_nlcc<Word, false> : k_nlcc<Word, true>), dim3(g), dim3(64), 0, ctx->stream, kind,
                        ctx->d_hubs, d_hub_n, nullptr, nullptr,
                        ctx->d_off,
-                       ctx->d_edges, ctx->d_ea, reverse_positions(ctx), cand, unm, ctx->d_flags, ctx->d_t, ma, c);
+                       ctx->d_edges, ctx->d_ea, reverse_positions(ctx), pair_index(ctx), cand, unm, ctx->d_flags,
+                       ctx->d_t, ma, c);
   end_launch(ctx);
 }
 
@@ -4121,6 +4216,12 @@ int pj_create(pj_ctx** out, int device) {
   // PJ_SLOT_LABELS=0 keeps the neighbour-candidate gathers in the first LCC
   // step (same results; the switch exists for comparisons).
   if (const char* sl = getenv("PJ_SLOT_LABELS")) ctx->slot_labels_enabled = strcmp(sl, "0") != 0;
+  // PJ_PAIR_INDEX_MB bounds the pair index of heavy rows (0: none, every
+  // pair search by binary search); PJ_PAIR_DEGREE is the smallest row length
+  // above which a row is heavy (doubled until the index fits).  Same results
+  // either way.
+  if (const char* pb = getenv("PJ_PAIR_INDEX_MB")) ctx->pair_budget = static_cast<uint64_t>(std::max(0, atoi(pb))) << 20;
+  if (const char* pd = getenv("PJ_PAIR_DEGREE")) ctx->pair_min_degree = static_cast<uint64_t>(std::max(1, atoi(pd)));
   // PJ_MEMO_WALKS=1 runs cycle and path steps as the reference's memoised
   // search even where the complete search is known to give the same result.
   if (const char* mw = getenv("PJ_MEMO_WALKS")) ctx->memo_walks = strcmp(mw, "0") != 0;
@@ -4214,6 +4315,50 @@ static int slot_labels(pj_ctx* ctx) {
   return 0;
 }
 
+// The pair index (PairIndex) of a symmetric graph with sorted rows and
+// reverse positions: the threshold starts at pair_min_degree and doubles
+// until the positions of the rows above it fit pair_budget; heavy ids follow
+// vertex order.  Fewer than two heavy rows: no index.
+static int pair_index_build(pj_ctx* ctx, const uint64_t* host_off) {
+  const uint32_t V = ctx->V;
+  const uint64_t E = ctx->E;
+  if (!reverse_positions(ctx) || !ctx->pair_budget || E > kPairOffMask) return 0;
+  uint64_t T = ctx->pair_min_degree, H = 0;
+  for (;; T *= 2) {
+    H = 0;
+    for (uint64_t v = 0; v < V; ++v) H += host_off[v + 1] - host_off[v] > T;
+    if (H < 2) return 0;
+    if (H * H * 4 <= ctx->pair_budget && H < (1ull << (64 - kPairShift))) break;
+  }
+  std::vector<uint64_t> doff(static_cast<uint64_t>(V) + 1);
+  std::vector<uint32_t> heavy;
+  heavy.reserve(H);
+  for (uint64_t v = 0; v < V; ++v) {
+    doff[v] = host_off[v];
+    if (host_off[v + 1] - host_off[v] > T) {
+      heavy.push_back(static_cast<uint32_t>(v));
+      doff[v] |= static_cast<uint64_t>(heavy.size()) << kPairShift;
+    }
+  }
+  doff[V] = host_off[V];
+  uint32_t* d_heavy = nullptr;
+  if (dev_alloc(ctx, &ctx->d_pair_doff, static_cast<uint64_t>(V) + 1) || dev_alloc(ctx, &ctx->d_pair_pos, H * H) ||
+      dev_alloc(ctx, &d_heavy, H))
+    return -1;
+  PJ_HIP(hipMemcpyAsync(ctx->d_pair_doff, doff.data(), doff.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  PJ_HIP(hipMemcpyAsync(d_heavy, heavy.data(), H * 4, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, H * H)), dim3(kBlock), 0, ctx->stream, ctx->d_pair_pos, H * H,
+                     kNoPos);
+  hipLaunchKernelGGL(k_pair_positions, dim3(std::min<uint64_t>((H + 3) / 4, ctx->cus * 16ull)), dim3(kBlock), 0,
+                     ctx->stream, d_heavy, H, ctx->d_pair_doff, ctx->d_edges, ctx->d_pair_pos);
+  PJ_HIP(hipGetLastError());
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  PJ_HIP(hipFree(d_heavy));
+  ctx->pair_heavy = H;
+  ctx->pair_degree = T;
+  return 0;
+}
+
 static int upload_finish(pj_ctx* ctx, const uint64_t* host_off) {
   const uint32_t V = ctx->V;
   const uint64_t E = ctx->E;
@@ -4271,6 +4416,7 @@ static int upload_finish(pj_ctx* ctx, const uint64_t* host_off) {
     ctx->skip_clean = ctx->rows_sorted && ctx->symmetric && ctx->dirty_enabled;
   }
   if (slot_labels(ctx)) return -1;
+  if (pair_index_build(ctx, host_off)) return -1;
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   ctx->have_template = false;
   ctx->active_vertices = V;
@@ -4426,6 +4572,10 @@ int pj_share_graph(pj_ctx* ctx, const pj_ctx* owner) {
   ctx->d_rev = owner->d_rev;
   ctx->d_slot_lab = owner->d_slot_lab;
   ctx->graph_labels = owner->graph_labels;
+  ctx->d_pair_doff = owner->d_pair_doff;
+  ctx->d_pair_pos = owner->d_pair_pos;
+  ctx->pair_heavy = owner->pair_heavy;
+  ctx->pair_degree = owner->pair_degree;
   ctx->self_loops = owner->self_loops;
   ctx->rows_sorted = owner->rows_sorted;
   ctx->symmetric = owner->symmetric;

@@ -453,15 +453,62 @@ def test_reverse_positions_change_nothing(pj, orc, case, monkeypatch):
     assert [r.key() for r in r1] == [r.key() for r in orows]
 
 
+@pytest.mark.parametrize("case", ["C2-shape s16", "hub", "four-cycle"])
+def test_pair_index_changes_nothing(pj, orc, case, monkeypatch):
+    """Cycle steps with the pair index of heavy rows (rows above 16 or 32
+    slots, so that most searches between a source's neighbours read it) and
+    with binary searches only (PJ_PAIR_INDEX_MB=0) give identical traces and
+    states, and the reference's; the index replaces searches, so the cycle
+    steps read fewer slots with it."""
+    from pattenmatching_sharedmemory_amd import STEP_CC
+
+    if case == "hub":
+        g = pj.Graph.rmat(14, 32, seed=3).labels_uniform(4, seed=4)
+        p = pj.Graph.pattern(4, [(0, 1), (1, 2), (2, 0), (2, 3)], [0, 1, 2, 3])
+        degree = "16"
+    elif case == "four-cycle":
+        g = pj.Graph.rmat(14, 16, seed=1).labels_uniform(8, seed=2)
+        p = pj.Graph.pattern(4, [(0, 1), (1, 2), (2, 3), (3, 0)], [0, 1, 2, 3])
+        degree = "16"
+    else:
+        g = pj.Graph.rmat(16, 16, seed=20221).labels_zipf(32, 1.2, seed=7)
+        p = pj.Graph.pattern(6, TRIANGLE_TAIL, [0, 1, 2, 3, 4, 3])
+        degree = "32"
+    c = pj.Constraints(p)
+    monkeypatch.setenv("PJ_PAIR_DEGREE", degree)
+    runs = []
+    for budget in ("0", "1024"):
+        monkeypatch.setenv("PJ_PAIR_INDEX_MB", budget)
+        pr = pj.Pruner(g, c)
+        rows, _ = pr.run(enumerate_matches=False)
+        runs.append((rows, pr.state()))
+        pr.close()
+    (r0, s0), (r1, s1) = runs
+    assert [r.key() for r in r0] == [r.key() for r in r1], case
+    assert np.array_equal(s0.cand, s1.cand) and np.array_equal(s0.vflags, s1.vflags)
+    assert np.array_equal(s0.ea_words, s1.ea_words)
+    o = orc.Oracle(g, c, threads=8)
+    orows, _ = o.run(orc.FAITHFUL, enumerate_matches=False)
+    assert [r.key() for r in r1] == [r.key() for r in orows], case
+    cc0 = sum(r.slots for r in r0 if r.step == STEP_CC)
+    cc1 = sum(r.slots for r in r1 if r.step == STEP_CC)
+    print(f"[{case}] cycle-step slots: searches {cc0}, with the pair index {cc1}", flush=True)
+    assert cc1 < cc0, case
+
+
+@pytest.mark.parametrize("pair_degree", ["512", "16"])
 @pytest.mark.parametrize("seed", [31, 32])
-def test_triangle_steps_from_random_states_in_every_degree_bin(pj, orc, seed):
+def test_triangle_steps_from_random_states_in_every_degree_bin(pj, orc, seed, pair_degree, monkeypatch):
     """3-vertex cycle steps from random states (candidates, flags and
     edge-active bits) on a graph with sources in every degree bin, so that
     k_triangles (1-8 / 9-32 slots), k_triangles_mid (33-64 slots in one
     chunk, 65-256 in up to four), the walk (257-2048) and the hub walk all
-    decide sources, many of them failing through cleared slots."""
+    decide sources, many of them failing through cleared slots.  Rows above
+    `pair_degree` slots are heavy: searches between two of them read the
+    pair index."""
     from pattenmatching_sharedmemory_amd import STEP_CC
 
+    monkeypatch.setenv("PJ_PAIR_DEGREE", pair_degree)
     rng = np.random.default_rng(seed)
     g = pj.Graph.rmat(14, 32, seed=3).labels_uniform(3, seed=4)
     deg = np.diff(g.offsets().astype(np.int64))
