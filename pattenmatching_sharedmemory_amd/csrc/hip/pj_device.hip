@@ -722,7 +722,7 @@ __device__ __forceinline__ uint64_t sat_add_bytes(uint64_t a, uint64_t b) {
 // vertex's candidates carry its own label (true once LCC0 has run: later
 // steps only remove candidates), so a neighbour's code is that of any of its
 // candidates, read from the LDS table instead of gathering code[u].
-template <typename Word, int G, int S, int KW>
+template <typename Word, int G, int S, int KW, bool kSlot>
 __device__ void lcc_phase1_lanes(uint32_t block, uint32_t nblocks, const LccTables<KW>& tab,
                                  const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
                                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
@@ -757,7 +757,7 @@ __device__ void lcc_phase1_lanes(uint32_t block, uint32_t nblocks, const LccTabl
         uint32_t u[S];
         uint32_t act = 0;  // bit k: slot k of this lane exists and is active
         Word cu[S];
-        if (slot_lab) {
+        if (kSlot) {
           // First LCC step after LCC0 on a fresh state: a neighbour's
           // candidates are exactly the template vertices of its label, so
           // the slot's label index replaces the gather of cand[u].
@@ -794,7 +794,7 @@ __device__ void lcc_phase1_lanes(uint32_t block, uint32_t nblocks, const LccTabl
               el = true;
               ++elim;
             } else {
-              const uint32_t lab = derive || slot_lab ? tab.pcode[ctz<Word>(cu[k])] : code[u[k]];
+              const uint32_t lab = derive || kSlot ? tab.pcode[ctz<Word>(cu[k])] : code[u[k]];
               if (lab != kNoCode) {
 #pragma unroll
                 for (int w = 0; w < KW; ++w)
@@ -1014,8 +1014,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
 
 // LCC phase 1 of the degree bins in one launch, each bin grid-strided over
 // the whole grid.
-template <typename Word, int KW>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW <= 2 ? 8 : 1))) k_lcc_phase1_all(
+template <typename Word, int KW, bool kSlot>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW == 1 && !kSlot ? 8 : KW <= 2 ? 7 : 1))) k_lcc_phase1_all(
     const uint32_t* __restrict__ bins, const uint32_t* __restrict__ bin_count, uint32_t V,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code,
     bool derive, const uint8_t* __restrict__ slot_lab, const Word* __restrict__ cand, Word* __restrict__ unmatch,
@@ -1033,13 +1033,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW 
   // kernel within 64 VGPRs (8 waves per SIMD) without spills; eight per lane
   // needed 82 (5 waves) and measured slower (1.05 against 1.01 ms per
   // scale-22 solve).
-  lcc_phase1_lanes<Word, 64, 4, KW>(blockIdx.x, gridDim.x, tab, bins + 3 * V64, bin_count + 3, off, edges, code,
+  lcc_phase1_lanes<Word, 64, 4, KW, kSlot>(blockIdx.x, gridDim.x, tab, bins + 3 * V64, bin_count + 3, off, edges, code,
                                     derive, slot_lab, cand, unmatch, flags, dirty, rescan, ea, t, counters);
-  lcc_phase1_lanes<Word, 32, 4, KW>(blockIdx.x, gridDim.x, tab, bins + 2 * V64, bin_count + 2, off, edges, code,
+  lcc_phase1_lanes<Word, 32, 4, KW, kSlot>(blockIdx.x, gridDim.x, tab, bins + 2 * V64, bin_count + 2, off, edges, code,
                                     derive, slot_lab, cand, unmatch, flags, dirty, rescan, ea, t, counters);
-  lcc_phase1_lanes<Word, 4, 4, KW>(blockIdx.x, gridDim.x, tab, bins + V64, bin_count + 1, off, edges, code, derive,
+  lcc_phase1_lanes<Word, 4, 4, KW, kSlot>(blockIdx.x, gridDim.x, tab, bins + V64, bin_count + 1, off, edges, code, derive,
                                    slot_lab, cand, unmatch, flags, dirty, rescan, ea, t, counters);
-  lcc_phase1_lanes<Word, 1, 4, KW>(blockIdx.x, gridDim.x, tab, bins, bin_count + 0, off, edges, code, derive,
+  lcc_phase1_lanes<Word, 1, 4, KW, kSlot>(blockIdx.x, gridDim.x, tab, bins, bin_count + 0, off, edges, code, derive,
                                    slot_lab, cand, unmatch, flags, dirty, rescan, ea, t, counters);
 }
 
@@ -3407,7 +3407,10 @@ void launch_lcc_phase1(pj_ctx* ctx) {
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
   unsigned long long* c = begin_launch(ctx, K_LCC_P1);
-  hipLaunchKernelGGL((k_lcc_phase1_all<Word, KW>), dim3(ctx->cus * kPhase1BlocksPerCu), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
+  // the first step after LCC0 (slot labels) has a build of its own: one
+  // body holding both ways of reading neighbours spills registers
+  hipLaunchKernelGGL((slot_lab ? k_lcc_phase1_all<Word, KW, true> : k_lcc_phase1_all<Word, KW, false>),
+                     dim3(ctx->cus * kPhase1BlocksPerCu), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
                      ctx->d_bin_count, ctx->V, ctx->d_off, ctx->d_edges, ctx->d_code, derive, slot_lab, cand, unm,
                      ctx->d_flags, dirty, rescan, ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
@@ -4323,13 +4326,27 @@ static int pair_index_build(pj_ctx* ctx, const uint64_t* host_off) {
   const uint32_t V = ctx->V;
   const uint64_t E = ctx->E;
   if (!reverse_positions(ctx) || !ctx->pair_budget || E > kPairOffMask) return 0;
-  uint64_t T = ctx->pair_min_degree, H = 0;
-  for (;; T *= 2) {
-    H = 0;
-    for (uint64_t v = 0; v < V; ++v) H += host_off[v + 1] - host_off[v] > T;
-    if (H < 2) return 0;
-    if (H * H * 4 <= ctx->pair_budget && H < (1ull << (64 - kPairShift))) break;
+  // above[b]: rows of more than T0 << b slots (one pass; b < 40 as E < 2^40)
+  const uint64_t T0 = ctx->pair_min_degree;
+  uint64_t above[kPairShift + 1] = {};
+  for (uint64_t v = 0; v < V; ++v) {
+    const uint64_t d = host_off[v + 1] - host_off[v];
+    if (d <= T0) continue;
+    int b = 0;
+    while (b < kPairShift && ((d - 1) >> (b + 1)) >= T0) ++b;  // d > T0 << (b + 1), without overflow
+    ++above[b];
   }
+  for (int b = kPairShift - 1; b >= 0; --b) above[b] += above[b + 1];
+  uint64_t T = 0, H = 0;
+  for (int b = 0; b <= kPairShift; ++b) {
+    H = above[b];
+    if (H < 2) return 0;
+    if (H * H * 4 <= ctx->pair_budget && H < (1ull << (64 - kPairShift))) {
+      T = T0 << b;
+      break;
+    }
+  }
+  if (!T) return 0;
   std::vector<uint64_t> doff(static_cast<uint64_t>(V) + 1);
   std::vector<uint32_t> heavy;
   heavy.reserve(H);
