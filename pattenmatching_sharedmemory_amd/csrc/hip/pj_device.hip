@@ -853,7 +853,7 @@ __device__ void lcc_phase1_lanes(uint32_t block, uint32_t nblocks, const LccTabl
 // hub_elim[h]; k_lcc_hub_finish evaluates the requirements and clears them.
 
 // `derive`: neighbour codes from candidates, as in lcc_phase1_lanes.
-template <typename Word>
+template <typename Word, int KW>
 __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* s_pcode, const uint8_t* s_scode,
                                const uint64_t* s_cv,
                                const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start,
@@ -949,27 +949,37 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
           for (int k = 0; k < kPer; ++k)
             if (cu[k]) lb[k] = s_pcode[ctz<Word>(cu[k])];
         }
-        uint32_t elim = 0, mine = 0;  // lane L counts label code L
+        // label counts in packed bytes (at most kPer per lane), summed over
+        // the wave with saturating byte adds (capped partials keep the
+        // finishing kernel's min(count, 127) exact); lane L then holds the
+        // count of label code L
+        uint64_t cnt[KW];
+#pragma unroll
+        for (int w = 0; w < KW; ++w) cnt[w] = 0;
+        uint32_t elim = 0;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-          bool counted = false, el = false;
+          bool el = false;
           if (cu[k]) {
             if ((cu[k] & U) == 0) {
               el = true;
               ++elim;
-            } else {
-              counted = lb[k] != kNoCode;
+            } else if (lb[k] != kNoCode) {
+#pragma unroll
+              for (int w = 0; w < KW; ++w)
+                if (static_cast<int>(lb[k] >> 3) == w) cnt[w] += 1ull << (8 * (lb[k] & 7));
             }
           }
           clear_slots<64>(ea, beg + k * kBlock + (threadIdx.x & ~63u), el, lane, 0);
-          // one ballot per distinct label code among the counted slots
-          for (uint64_t m = __ballot(counted); m; m = __ballot(counted)) {
-            const uint32_t L = rl32(lb[k], __builtin_ctzll(m));
-            const uint64_t same = __ballot(counted && lb[k] == L);
-            if (lane == static_cast<int>(L)) mine += __popcll(same);
-            counted = counted && lb[k] != L;
-          }
         }
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+#pragma unroll
+          for (int w = 0; w < KW; ++w) cnt[w] = sat_add_bytes(cnt[w], __shfl_xor(cnt[w], o, 64));
+        uint32_t mine = 0;
+#pragma unroll
+        for (int w = 0; w < KW; ++w)
+          if ((lane >> 3) == w) mine = static_cast<uint32_t>(cnt[w] >> (8 * (lane & 7))) & 0xFFu;
         if (mine) atomicAdd(&hub_count[h * kMaxLabels + lane], mine);
         if (__ballot(elim != 0)) {
           elim = __ockl_wfred_add_u32(elim);
@@ -991,8 +1001,8 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
 
 // Hub rows of LCC phase 1 in a launch of their own (fused with the bins,
 // the unrolled chunk loop raises the register count of the whole kernel).
-template <typename Word>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_lcc_hub_chunks(
+template <typename Word, int KW>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW == 1 ? 7 : 6))) k_lcc_hub_chunks(
     const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
     const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
     const uint8_t* __restrict__ code, bool derive, const uint8_t* __restrict__ slot_lab,
@@ -1008,7 +1018,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
   }
   for (int i = threadIdx.x; i < 256; i += blockDim.x) s_scode[i] = t->slot_code[i];
   __syncthreads();
-  lcc_hub_chunks<Word>(blockIdx.x, gridDim.x, s_pcode, s_scode, s_cv, chunk_hub, chunk_start, nchunks, hubs, off, edges,
+  lcc_hub_chunks<Word, KW>(blockIdx.x, gridDim.x, s_pcode, s_scode, s_cv, chunk_hub, chunk_start, nchunks, hubs, off, edges,
                        code, derive, slot_lab, cand, flags, dirty, ea, hub_count, hub_elim, t, counters);
 }
 
@@ -3416,7 +3426,7 @@ void launch_lcc_phase1(pj_ctx* ctx) {
   end_launch(ctx);
   if (ctx->nhubs) {
     c = begin_launch(ctx, K_LCC_HUB);
-    hipLaunchKernelGGL(k_lcc_hub_chunks<Word>, dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * kPhase1BlocksPerCu)),
+    hipLaunchKernelGGL((k_lcc_hub_chunks<Word, KW>), dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * kPhase1BlocksPerCu)),
                        dim3(kBlock), 0,
                        ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
                        ctx->d_edges, ctx->d_code, derive, slot_lab, cand, ctx->d_flags, dirty, ctx->d_ea,
