@@ -1040,9 +1040,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW 
   const uint64_t V64 = V;
   // bins: 257-2048 slots (64 lanes x 4 slots per pass), 33-256 (32 x 4),
   // 5-32 (4 x 4), 1-4 (a lane per vertex).  Four slots per lane keep the
-  // kernel within 64 VGPRs (8 waves per SIMD) without spills; eight per lane
-  // needed 82 (5 waves) and measured slower (1.05 against 1.01 ms per
-  // scale-22 solve).
+  // one-word build within 64 VGPRs (8 waves per SIMD) without spills; eight
+  // per lane needed 82 (5 waves) and measured slower (1.05 against 1.01 ms
+  // per scale-22 solve).  The slot-label and two-word builds need up to 72
+  // (7 waves): held to 64 they spilled 40 bytes per lane to scratch, written
+  // and read by every lane of every launch.
   lcc_phase1_lanes<Word, 64, 4, KW, kSlot>(blockIdx.x, gridDim.x, tab, bins + 3 * V64, bin_count + 3, off, edges, code,
                                     derive, slot_lab, cand, unmatch, flags, dirty, rescan, ea, t, counters);
   lcc_phase1_lanes<Word, 32, 4, KW, kSlot>(blockIdx.x, gridDim.x, tab, bins + 2 * V64, bin_count + 2, off, edges, code,
@@ -1726,14 +1728,16 @@ struct PairIndex {
   uint64_t H;
 };
 
-// active_slot on a symmetric graph with reverse positions: the shorter of
-// row(a) and row(b) is searched (rows hold no duplicate slots), a hit in
-// row(b) being mapped to the slot a->b through rev; two heavy rows are
-// decided by the pair index (its position is the slot the search finds).
-__device__ __forceinline__ bool active_slot_rev(uint32_t a, uint32_t b, const uint64_t* __restrict__ off,
-                                                const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
-                                                const uint32_t* __restrict__ rev, const PairIndex& pi,
-                                                uint64_t& probes) {
+constexpr uint64_t kNoSlot = ~0ull;
+
+// The slot a->b of a symmetric graph with reverse positions (rows hold no
+// duplicate slots), or kNoSlot: the shorter of row(a) and row(b) is
+// searched, a hit in row(b) being mapped to the slot a->b through rev; two
+// heavy rows are decided by the pair index (its position is the slot the
+// search finds).
+__device__ __forceinline__ uint64_t slot_rev(uint32_t a, uint32_t b, const uint64_t* __restrict__ off,
+                                             const uint32_t* __restrict__ edges, const uint32_t* __restrict__ rev,
+                                             const PairIndex& pi, uint64_t& probes) {
   uint64_t ab, bb, da, db;
   if (pi.pos) {
     const uint64_t xa = pi.doff[a], xb = pi.doff[b];
@@ -1743,7 +1747,7 @@ __device__ __forceinline__ bool active_slot_rev(uint32_t a, uint32_t b, const ui
     if (ha && hb) {
       ++probes;
       const uint32_t r = pi.pos[(ha - 1) * pi.H + (hb - 1)];
-      return r != kNoPos && ea_test(ea, ab + r);
+      return r == kNoPos ? kNoSlot : ab + r;
     }
     da = (pi.doff[a + 1] & kPairOffMask) - ab;
     db = (pi.doff[b + 1] & kPairOffMask) - bb;
@@ -1765,8 +1769,17 @@ __device__ __forceinline__ bool active_slot_rev(uint32_t a, uint32_t b, const ui
     else
       hi = mid;
   }
-  if (lo == end || edges[lo] != key) return false;
-  return ea_test(ea, in_a ? lo : ab + rev[lo]);
+  if (lo == end || edges[lo] != key) return kNoSlot;
+  return in_a ? lo : ab + rev[lo];
+}
+
+// Whether row(a) holds an active slot to b (slot_rev, then its edge-active bit).
+__device__ __forceinline__ bool active_slot_rev(uint32_t a, uint32_t b, const uint64_t* __restrict__ off,
+                                                const uint32_t* __restrict__ edges, const uint32_t* __restrict__ ea,
+                                                const uint32_t* __restrict__ rev, const PairIndex& pi,
+                                                uint64_t& probes) {
+  const uint64_t e = slot_rev(a, b, off, edges, rev, pi, probes);
+  return e != kNoSlot && ea_test(ea, e);
 }
 
 // Pair index positions: one wave per heavy row; each slot to a heavy
@@ -2775,6 +2788,41 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
         const uint64_t pa = r.next - 1 - sb;  // position of a in row(s)
         const bool block_a = may_block && ((cand[a] >> q2) & 1);
         const uint64_t ab = off[a], ae = off[a + 1];
+        if (se - sb < ae - ab) {
+          // Row(s) is the shorter: every u closing the cycle is in both rows,
+          // and both are sorted, so walking row(s) meets the u in the order
+          // of row(a); each u's slot a->u comes from slot_rev, its slot u->s
+          // and its position in row(s) from the walk itself.
+          for (uint64_t base = sb; base < se && !found; base += 64) {
+            const uint64_t e = base + lane;  // slot s->u
+            bool ok = false;
+            uint32_t u = 0;
+            if (e < se) {
+              u = edges[e];
+              const Word cu = cand[u];
+              if (((cu >> q2) & 1) && ea_test(ea, off[u] + rev[e])) {  // q2 in C(u), slot u->s active
+                const uint64_t au = slot_rev(a, u, off, edges, rev, pi, probes);
+                ok = au != kNoSlot && ea_test(ea, au);
+                // u an earlier first hop (before a in row(s), slot s->u active)
+                // whose second hops reached a
+                if (ok && block_a && ((cu >> q1) & 1) && e - sb < pa && ea_test(ea, e) &&
+                    ea_test(ea, off[u] + rev[au]))
+                  ok = false;
+              }
+            }
+            const uint64_t m = __ballot(ok);
+            slots += m ? __builtin_ctzll(m) + 1 : min<uint64_t>(64, se - base);
+            if (m) {
+              found = true;
+              u = rl32(u, __builtin_ctzll(m));
+              if (lane == 0) {
+                atomic_or_word<Word>(&already[a], Word(1) << q1);
+                atomic_or_word<Word>(&already[u], Word(1) << q2);
+              }
+            }
+          }
+          continue;
+        }
         for (uint64_t base = ab; base < ae && !found; base += 64) {
           const uint64_t e = base + lane;
           bool ok = false;
