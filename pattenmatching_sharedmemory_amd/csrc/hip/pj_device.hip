@@ -2732,6 +2732,125 @@ k_triangles_mid(const uint32_t* __restrict__ list, const uint32_t* __restrict__ 
   }
 }
 
+// 3-vertex cycle sources of 257 to kWideSlots slots (complete search; sorted
+// rows without duplicates, symmetric graph, reverse positions): one wave per
+// source.  The first first hop a (row order) is closed as the walk closes it
+// (close_cycle over the shorter of row(a) and row(s)); if that fails, the
+// closing vertices U = {u in row(s) : q2 in C(u), slot u->s
+// active} are compacted into the wave's LDS list; the later first hops are classified
+// 64 slots at a time (A_k = {a : slot s->a active, q1 in C(a)}) and the
+// |A_k| x |U| pairs of each chunk are tested 64 at a time by active_slot_rev.
+// The walk it replaces closes the cycle of every first hop a over the
+// shorter of row(a) and row(s): a source whose candidate fails read
+// |A| x min(deg) slots in |A| x min(deg) / 64 rounds instead of
+// |A| x |U| / 64.  Existence does not depend on the order of the pairs.
+constexpr uint32_t kWideSlots = 2048;
+static_assert(kWideSlots == kHubDegree, "bin 3 holds rows of 257 to kHubDegree slots");
+
+template <typename Word>
+__device__ bool triangle_wide(uint32_t s, int q1, int q2, uint64_t beg, uint32_t deg,
+                              const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+                              const uint32_t* __restrict__ ea, const uint32_t* __restrict__ rev,
+                              const PairIndex& pi, const Word* __restrict__ cand, uint32_t* U, uint64_t& slots) {
+  const int lane = lane_id();
+  const uint64_t below = (1ull << lane) - 1;
+  // The first first hop is closed as the walk closes it: most sources that
+  // succeed do so there, without the lists.
+  const RowScan r1 = scan_row<Word>(beg, beg + deg, edges, ea, cand, q1, 0, 0xFFFFFFFFu, slots);
+  if (!r1.found) return false;
+  if (close_cycle<Word>(r1.u, s, q2, off, edges, ea, rev, pi, cand, slots)) return true;
+  const uint32_t skip = static_cast<uint32_t>(r1.next - beg);  // first hops from here on
+  uint64_t probes = 0;
+  uint32_t nu = 0;
+  for (uint32_t c0 = 0; c0 < deg; c0 += 64) {
+    const uint32_t i = c0 + lane;
+    bool is_u = false;
+    uint32_t w = 0;
+    if (i < deg) {
+      w = edges[beg + i];
+      if ((cand[w] >> q2) & 1) is_u = ea_test(ea, off[w] + rev[beg + i]);  // slot w->s
+    }
+    const uint64_t m = __ballot(is_u);
+    if (is_u) U[nu + __popcll(m & below)] = w;
+    nu += __popcll(m);
+  }
+  // the list's stores by every lane before any lane reads it
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  bool found = false;
+  uint64_t read = deg;
+  for (uint32_t c0 = skip & ~63u; nu && c0 < deg && !found; c0 += 64) {
+    const uint32_t i = c0 + lane;
+    bool is_a = false;
+    uint32_t w = 0;
+    if (i >= skip && i < deg) {
+      w = edges[beg + i];
+      is_a = ((cand[w] >> q1) & 1) && ea_test(ea, beg + i);
+    }
+    read += min(64u, deg - c0);
+    const uint64_t ma = __ballot(is_a);
+    if (!ma) continue;
+    int a_from = 0, r = 0;  // lane r: slot of the r-th a of the chunk
+    for (uint64_t m = ma; m; m &= m - 1, ++r)
+      if (lane == r) a_from = __builtin_ctzll(m);
+    const uint32_t a_list = __shfl(w, a_from, 64);
+    const uint32_t pairs = static_cast<uint32_t>(__popcll(ma)) * nu;
+    for (uint32_t p0 = 0; p0 < pairs && !found; p0 += 64) {
+      const uint32_t p = p0 + lane;
+      const uint32_t ia = p / nu, iu = p - ia * nu;
+      const uint32_t a = __shfl(a_list, static_cast<int>(ia & 63), 64);
+      const bool ok = p < pairs && active_slot_rev(a, U[iu], off, edges, ea, rev, pi, probes);
+      found = __ballot(ok) != 0;
+    }
+  }
+  // every lane's reads of the list before the wave's next search rewrites it
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  probes = __ockl_wfred_add_u64(probes);
+  slots += read + probes;
+  return found;
+}
+
+template <typename Word>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
+k_triangles_wide(const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
+                 const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
+                 const uint32_t* __restrict__ ea, const uint32_t* __restrict__ rev, PairIndex pi,
+                 const Word* __restrict__ cand, Word* __restrict__ unmatch, uint8_t* __restrict__ flags,
+                 const DevTemplate* __restrict__ t, unsigned long long* counters) {
+  __shared__ uint32_t s_u[kBlock / 64][kWideSlots];
+  uint32_t* U = s_u[threadIdx.x >> 6];
+  const uint32_t n = *list_count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_ITEMS] = n;
+  const uint32_t waves = gridDim.x * (blockDim.x / 64);
+  const uint32_t wave_id = uniform(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6));
+  const uint32_t qv[3] = {t->nlcc_vertex[0], t->nlcc_vertex[1], t->nlcc_vertex[2]};
+  uint64_t slots = 0;
+  for (uint32_t w = wave_id; w < n; w += waves) {
+    const uint32_t s = list[w];
+    const Word cs = cand[s];
+    const uint64_t beg = off[s];
+    const uint32_t deg = static_cast<uint32_t>(off[s + 1] - beg);  // at most kWideSlots (bin 3)
+    Word fail = 0;
+    for (int j = 0; j < 3; ++j) {
+      const uint32_t q = qv[j];
+      if (!((cs >> q) & 1)) continue;
+      const int q1 = static_cast<int>(qv[(j + 1) % 3]), q2 = static_cast<int>(qv[(j + 2) % 3]);
+      if (!triangle_wide<Word>(s, q1, q2, beg, deg, off, edges, ea, rev, pi, cand, U, slots)) fail |= Word(1) << q;
+    }
+    if (fail && lane_id() == 0) {  // s is in one task list once: this wave is its only writer
+      unmatch[s] = static_cast<Word>(unmatch[s] | fail);
+      set_flag_atomic(flags, s, kModified);
+    }
+  }
+  if (lane_id() == 0 && slots) {
+    const int slot = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) & (kCounterSlots - 1);
+    atomicAdd(&counters[slot * kCounterStride + C_SLOTS], static_cast<unsigned long long>(slots));
+  }
+}
+
 // Memoised 3-vertex cycle constraint (cc_cpu.tpp:12-70 with L = 3) on a
 // graph without duplicate slots or self-loops (sorted rows, reverse
 // positions).  For start j the reference walks s -> a -> u -> s with
@@ -3038,6 +3157,7 @@ struct pj_ctx {
   // neighbour candidates only at 1.
   int fresh = 2;
   bool slot_labels_enabled = true;  // PJ_SLOT_LABELS=0: never
+  bool wide_triangles = true;        // PJ_WIDE_TRIANGLES=0: 257-2048-slot cycle sources walk
   // Live list: the vertices that may still be active (a superset of the
   // active ones; candidates only shrink until the state is replaced).
   // live == nullptr: every vertex.  Two buffers for the compaction.
@@ -3688,7 +3808,7 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
   for (int b = 0; b < kBins; ++b) {
     if (b == 2 && !triangles && !memo) continue;  // taken by the k_nlcc launch of bin 3
     unsigned long long* c =
-        begin_launch(ctx, triangles && b <= 2 ? K_TRIANGLES : memo ? K_MEMO_WALK : K_NLCC);
+        begin_launch(ctx, triangles && (b <= 2 || ctx->wide_triangles) ? K_TRIANGLES : memo ? K_MEMO_WALK : K_NLCC);
     if (triangles && b == 0)
       hipLaunchKernelGGL((k_triangles<Word, kWalkBin0>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
                          ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
@@ -3697,6 +3817,9 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
                          ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
     else if (triangles && b == 2)
       hipLaunchKernelGGL(k_triangles_mid<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
+                         ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
+    else if (triangles && b == 3 && ctx->wide_triangles)
+      hipLaunchKernelGGL(k_triangles_wide<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
                          ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
     else if (memo)  // first pass with LDS tables; overflows go to the list of d_ovf[0]
       hipLaunchKernelGGL(k_memo_lds<Word>, dim3(kMemoWaves), dim3(64), 0, ctx->stream, ctx->d_bins + b * V,
@@ -4277,6 +4400,9 @@ int pj_create(pj_ctx** out, int device) {
   // PJ_SLOT_LABELS=0 keeps the neighbour-candidate gathers in the first LCC
   // step (same results; the switch exists for comparisons).
   if (const char* sl = getenv("PJ_SLOT_LABELS")) ctx->slot_labels_enabled = strcmp(sl, "0") != 0;
+  // PJ_WIDE_TRIANGLES=0 walks the 3-vertex cycles of sources of 257-2048
+  // slots (first hop by first hop) instead of testing their pairs.
+  if (const char* wt = getenv("PJ_WIDE_TRIANGLES")) ctx->wide_triangles = strcmp(wt, "0") != 0;
   // PJ_PAIR_INDEX_MB bounds the pair index of heavy rows (0: none, every
   // pair search by binary search); PJ_PAIR_DEGREE is the smallest row length
   // above which a row is heavy (doubled until the index fits).  Same results

@@ -496,6 +496,37 @@ def test_pair_index_changes_nothing(pj, orc, case, monkeypatch):
     assert cc1 < cc0, case
 
 
+@pytest.mark.parametrize("case", ["C2-shape s16", "hub"])
+def test_wide_triangle_pairs_change_nothing(pj, orc, case, monkeypatch):
+    """3-vertex cycle sources of 257-2048 slots decided by pair tests against
+    the LDS list of closing vertices (k_triangles_wide) and by the walk
+    (PJ_WIDE_TRIANGLES=0) give identical traces and states, and the
+    reference's."""
+    if case == "hub":
+        g = pj.Graph.rmat(14, 32, seed=3).labels_uniform(4, seed=4)
+        p = pj.Graph.pattern(4, [(0, 1), (1, 2), (2, 0), (2, 3)], [0, 1, 2, 3])
+    else:
+        g = pj.Graph.rmat(16, 16, seed=20221).labels_zipf(32, 1.2, seed=7)
+        p = pj.Graph.pattern(6, TRIANGLE_TAIL, [0, 1, 2, 3, 4, 3])
+    deg = np.diff(g.offsets().astype(np.int64))
+    assert np.any((deg > 256) & (deg <= 2048))
+    c = pj.Constraints(p)
+    runs = []
+    for wide in ("0", "1"):
+        monkeypatch.setenv("PJ_WIDE_TRIANGLES", wide)
+        pr = pj.Pruner(g, c)
+        rows, _ = pr.run(enumerate_matches=False)
+        runs.append((rows, pr.state()))
+        pr.close()
+    (r0, s0), (r1, s1) = runs
+    assert [r.key() for r in r0] == [r.key() for r in r1], case
+    assert np.array_equal(s0.cand, s1.cand) and np.array_equal(s0.vflags, s1.vflags)
+    assert np.array_equal(s0.ea_words, s1.ea_words)
+    o = orc.Oracle(g, c, threads=8)
+    orows, _ = o.run(orc.FAITHFUL, enumerate_matches=False)
+    assert [r.key() for r in r1] == [r.key() for r in orows], case
+
+
 @pytest.mark.parametrize("pair_degree", ["512", "16"])
 @pytest.mark.parametrize("seed", [31, 32])
 def test_triangle_steps_from_random_states_in_every_degree_bin(pj, orc, seed, pair_degree, monkeypatch):
