@@ -65,16 +65,19 @@ constexpr int kWalkBlocksPerCu = 16;
 // Kernel kinds reported by pj_kernel_stats.
 enum KernelId {
   K_LCC0, K_BINS, K_LCC_P1, K_LCC_HUB, K_LCC_HUB_FINISH, K_P2, K_P2_HUB_FINISH, K_NLCC, K_NLCC_HUB, K_TRIANGLES,
-  K_TRIANGLES_MEMO, K_MEMO_WALK, K_HUB_PUSH, K_VIEW, K_LABEL_STAT, K_COUNT
+  K_TRIANGLES_MEMO, K_MEMO_WALK, K_HUB_PUSH, K_VIEW, K_LABEL_STAT, K_TRIANGLES_WIDE, K_COUNT
 };
 // nlcc_walk: k_nlcc (cycle / path / TDS / enumeration walks over the bins,
-// and the memo retry passes); triangles: k_triangles (3-vertex cycles from
-// sources of at most 8 / 32 slots); triangles_memo: k_triangles_memo;
-// memo_walk: the first pass of the memoised walks (k_memo_lds).
+// and the memo retry passes); triangles: k_triangles and k_triangles_mid
+// (3-vertex cycles from sources of at most 8 / 32 / 256 slots);
+// triangles_wide: k_triangles_wide (257-2048 slots); triangles_memo:
+// k_triangles_memo; memo_walk: the first pass of the memoised walks
+// (k_memo_lds).
 const char* const kKernelNames[K_COUNT] = {
     "lcc0",          "build_bins",        "lcc_phase1", "lcc_phase1_hub_chunks", "lcc_phase1_hub_finish",
     "phase2",        "phase2_hub_finish", "nlcc_walk",  "nlcc_walk_hubs",        "triangles",
-    "triangles_memo", "memo_walk",        "hub_rescan_push", "active_view",    "label_stat"};
+    "triangles_memo", "memo_walk",        "hub_rescan_push", "active_view",    "label_stat",
+    "triangles_wide"};
 
 // Template tables in device memory.
 struct DevTemplate {
@@ -3808,7 +3811,10 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
   for (int b = 0; b < kBins; ++b) {
     if (b == 2 && !triangles && !memo) continue;  // taken by the k_nlcc launch of bin 3
     unsigned long long* c =
-        begin_launch(ctx, triangles && (b <= 2 || ctx->wide_triangles) ? K_TRIANGLES : memo ? K_MEMO_WALK : K_NLCC);
+        begin_launch(ctx, triangles && b <= 2                         ? K_TRIANGLES
+                          : triangles && b == 3 && ctx->wide_triangles ? K_TRIANGLES_WIDE
+                          : memo                                       ? K_MEMO_WALK
+                                                                       : K_NLCC);
     if (triangles && b == 0)
       hipLaunchKernelGGL((k_triangles<Word, kWalkBin0>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
                          ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);

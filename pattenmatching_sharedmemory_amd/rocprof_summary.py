@@ -31,7 +31,7 @@ _TEMPLATE_NAMES = {
     "k_phase2_hub_finish": "phase2_hub_finish",
     "k_triangles": "triangles",
     "k_triangles_mid": "triangles",
-    "k_triangles_wide": "triangles",
+    "k_triangles_wide": "triangles_wide",
     "k_triangles_memo": "triangles_memo",
     "k_memo_lds": "memo_walk",
     "k_view_count": "active_view",
