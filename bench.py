@@ -95,11 +95,17 @@ def algorithmic_bytes(k: dict, vertices: int) -> float:
     no cache credit (W = candidate word bytes): offsets 8, ids 4, labels 1,
     flags 1, edge-active 1/8 per slot read or written.
       phase 1 (LCC): items*(9.25 + 2W) + slots*(5.25 + W) + 0.125*eliminated
+      phase 1 of the first step after LCC0 (*_labels: it reads a slot's
+                     label byte where the others read the neighbour id, 4,
+                     and candidate word, W): items*(9.25 + 2W) + slots*1.25
+                     + 0.125*eliminated
       phase 2:       items*(3W + 0.5) + 4.25*slots
       NLCC walks (nlcc_walk*, triangles*, memo_walk): 8*items + (4.25 + W)*slots
       bins:          vertices*(W + 9) + 4*listed"""
     W = k["word_bytes"]
     name = k["name"]
+    if name.startswith("lcc_phase1") and name.endswith("_labels"):
+        return k["items"] * (9.25 + 2 * W) + k["slots"] * 1.25 + 0.125 * k["edges_eliminated"]
     if name.startswith("lcc_phase1"):
         return k["items"] * (9.25 + 2 * W) + k["slots"] * (5.25 + W) + 0.125 * k["edges_eliminated"]
     if name.startswith("phase2"):

@@ -65,19 +65,22 @@ constexpr int kWalkBlocksPerCu = 16;
 // Kernel kinds reported by pj_kernel_stats.
 enum KernelId {
   K_LCC0, K_BINS, K_LCC_P1, K_LCC_HUB, K_LCC_HUB_FINISH, K_P2, K_P2_HUB_FINISH, K_NLCC, K_NLCC_HUB, K_TRIANGLES,
-  K_TRIANGLES_MEMO, K_MEMO_WALK, K_HUB_PUSH, K_VIEW, K_LABEL_STAT, K_TRIANGLES_WIDE, K_COUNT
+  K_TRIANGLES_MEMO, K_MEMO_WALK, K_HUB_PUSH, K_VIEW, K_LABEL_STAT, K_TRIANGLES_WIDE, K_TRIANGLES_MID,
+  K_LCC_P1_LABELS, K_LCC_HUB_LABELS, K_COUNT
 };
 // nlcc_walk: k_nlcc (cycle / path / TDS / enumeration walks over the bins,
-// and the memo retry passes); triangles: k_triangles and k_triangles_mid
-// (3-vertex cycles from sources of at most 8 / 32 / 256 slots);
-// triangles_wide: k_triangles_wide (257-2048 slots); triangles_memo:
+// and the memo retry passes); triangles: k_triangles (3-vertex cycles from
+// sources of at most 8 / 32 slots); triangles_mid: k_triangles_mid (33-256
+// slots); triangles_wide: k_triangles_wide (257-2048 slots); triangles_memo:
 // k_triangles_memo; memo_walk: the first pass of the memoised walks
-// (k_memo_lds).
+// (k_memo_lds); lcc_phase1_labels and lcc_phase1_hub_chunks_labels: the
+// builds of the first LCC step after LCC0, which read slot labels where the
+// others read neighbour ids and candidate words.
 const char* const kKernelNames[K_COUNT] = {
     "lcc0",          "build_bins",        "lcc_phase1", "lcc_phase1_hub_chunks", "lcc_phase1_hub_finish",
     "phase2",        "phase2_hub_finish", "nlcc_walk",  "nlcc_walk_hubs",        "triangles",
     "triangles_memo", "memo_walk",        "hub_rescan_push", "active_view",    "label_stat",
-    "triangles_wide"};
+    "triangles_wide", "triangles_mid",    "lcc_phase1_labels", "lcc_phase1_hub_chunks_labels"};
 
 // Template tables in device memory.
 struct DevTemplate {
@@ -856,7 +859,7 @@ __device__ void lcc_phase1_lanes(uint32_t block, uint32_t nblocks, const LccTabl
 // hub_elim[h]; k_lcc_hub_finish evaluates the requirements and clears them.
 
 // `derive`: neighbour codes from candidates, as in lcc_phase1_lanes.
-template <typename Word, int KW>
+template <typename Word, int KW, bool kSlot>
 __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* s_pcode, const uint8_t* s_scode,
                                const uint64_t* s_cv,
                                const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start,
@@ -917,7 +920,7 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
         uint32_t word[kPer], u[kPer];
         Word cu[kPer];
         uint8_t lb[kPer];
-        if (slot_lab) {  // first LCC step after LCC0: labels of the slots, as in lcc_phase1_lanes
+        if (kSlot) {  // first LCC step after LCC0: labels of the slots, as in lcc_phase1_lanes
 #pragma unroll
           for (int k = 0; k < kPer; ++k) {
             const uint64_t e = beg + threadIdx.x + k * kBlock;
@@ -947,7 +950,7 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
             lb[k] = act && !derive ? code[u[k]] : kNoCode;
           }
         }
-        if (derive && !slot_lab) {
+        if (derive && !kSlot) {
 #pragma unroll
           for (int k = 0; k < kPer; ++k)
             if (cu[k]) lb[k] = s_pcode[ctz<Word>(cu[k])];
@@ -1004,7 +1007,7 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
 
 // Hub rows of LCC phase 1 in a launch of their own (fused with the bins,
 // the unrolled chunk loop raises the register count of the whole kernel).
-template <typename Word, int KW>
+template <typename Word, int KW, bool kSlot>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW == 1 ? 7 : 6))) k_lcc_hub_chunks(
     const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
     const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
@@ -1021,7 +1024,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW 
   }
   for (int i = threadIdx.x; i < 256; i += blockDim.x) s_scode[i] = t->slot_code[i];
   __syncthreads();
-  lcc_hub_chunks<Word, KW>(blockIdx.x, gridDim.x, s_pcode, s_scode, s_cv, chunk_hub, chunk_start, nchunks, hubs, off, edges,
+  lcc_hub_chunks<Word, KW, kSlot>(blockIdx.x, gridDim.x, s_pcode, s_scode, s_cv, chunk_hub, chunk_start, nchunks, hubs, off, edges,
                        code, derive, slot_lab, cand, flags, dirty, ea, hub_count, hub_elim, t, counters);
 }
 
@@ -3587,7 +3590,7 @@ void launch_lcc_phase1(pj_ctx* ctx) {
   uint8_t* rescan = ctx->skip_clean && !ctx->rescan_all ? ctx->d_rescan : nullptr;  // push target
   const Word* cand = static_cast<const Word*>(ctx->d_cand);
   Word* unm = static_cast<Word*>(ctx->d_unmatch);
-  unsigned long long* c = begin_launch(ctx, K_LCC_P1);
+  unsigned long long* c = begin_launch(ctx, slot_lab ? K_LCC_P1_LABELS : K_LCC_P1);
   // the first step after LCC0 (slot labels) has a build of its own: one
   // body holding both ways of reading neighbours spills registers
   hipLaunchKernelGGL((slot_lab ? k_lcc_phase1_all<Word, KW, true> : k_lcc_phase1_all<Word, KW, false>),
@@ -3596,8 +3599,9 @@ void launch_lcc_phase1(pj_ctx* ctx) {
                      ctx->d_flags, dirty, rescan, ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
   if (ctx->nhubs) {
-    c = begin_launch(ctx, K_LCC_HUB);
-    hipLaunchKernelGGL((k_lcc_hub_chunks<Word, KW>), dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * kPhase1BlocksPerCu)),
+    c = begin_launch(ctx, slot_lab ? K_LCC_HUB_LABELS : K_LCC_HUB);
+    hipLaunchKernelGGL((slot_lab ? k_lcc_hub_chunks<Word, KW, true> : k_lcc_hub_chunks<Word, KW, false>),
+                       dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * kPhase1BlocksPerCu)),
                        dim3(kBlock), 0,
                        ctx->stream, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs, ctx->d_off,
                        ctx->d_edges, ctx->d_code, derive, slot_lab, cand, ctx->d_flags, dirty, ctx->d_ea,
@@ -3811,7 +3815,8 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
   for (int b = 0; b < kBins; ++b) {
     if (b == 2 && !triangles && !memo) continue;  // taken by the k_nlcc launch of bin 3
     unsigned long long* c =
-        begin_launch(ctx, triangles && b <= 2                         ? K_TRIANGLES
+        begin_launch(ctx, triangles && b <= 1                         ? K_TRIANGLES
+                          : triangles && b == 2                         ? K_TRIANGLES_MID
                           : triangles && b == 3 && ctx->wide_triangles ? K_TRIANGLES_WIDE
                           : memo                                       ? K_MEMO_WALK
                                                                        : K_NLCC);

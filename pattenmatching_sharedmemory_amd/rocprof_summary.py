@@ -30,7 +30,7 @@ _TEMPLATE_NAMES = {
     "k_phase2_all": "phase2",
     "k_phase2_hub_finish": "phase2_hub_finish",
     "k_triangles": "triangles",
-    "k_triangles_mid": "triangles",
+    "k_triangles_mid": "triangles_mid",
     "k_triangles_wide": "triangles_wide",
     "k_triangles_memo": "triangles_memo",
     "k_memo_lds": "memo_walk",
@@ -65,6 +65,8 @@ def stat_name(full: str, workgroup: int | None = None) -> str:
     base = s.split("<", 1)[0]
     args = s[len(base) + 1:-1].split(",") if "<" in s else []
     args = [a.strip() for a in args]
+    if base in ("k_lcc_phase1_all", "k_lcc_hub_chunks") and args and args[-1] == "true":
+        return _TEMPLATE_NAMES[base] + "_labels"  # the first LCC step after LCC0 (slot labels)
     if base in _TEMPLATE_NAMES:
         return _TEMPLATE_NAMES[base]
     if base == "k_nlcc":
