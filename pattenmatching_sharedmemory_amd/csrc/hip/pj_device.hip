@@ -1532,6 +1532,22 @@ __device__ uint32_t memo_probe(const MemoTable& m, uint64_t key, bool* found) {
   return 0;  // not reached: the table never fills past its limit
 }
 
+// The same lookup by one lane alone (linear probing from the same start, so
+// it stops at the slot the wave-cooperative probe returns): true with *slot
+// when `key` is present.  Lanes of a wave may probe different keys at once.
+__device__ bool memo_find_lane(const MemoTable& m, uint64_t key, uint32_t* slot) {
+  const uint32_t h = static_cast<uint32_t>(mix64(key)) & m.mask;
+  for (uint32_t i = 0; i <= m.mask; ++i) {
+    const uint32_t sl = (h + i) & m.mask;
+    if (wg_load(&m.epochs[sl]) != m.epoch) return false;
+    if (wg_load(&m.keys[sl]) == key) {
+      *slot = sl;
+      return true;
+    }
+  }
+  return false;
+}
+
 __device__ void memo_put(MemoTable& m, uint32_t slot, uint64_t key, uint64_t val) {
   if (lane_id() == 0) {
     wg_store(&m.vals[slot], val);
@@ -2175,7 +2191,16 @@ __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ m
           w = edges[e];
           ok = (cand[w] >> y) & 1;
         }
-        const uint64_t bm = __ballot(ok);
+        // every candidate of the 64 slots is tested against the memos at
+        // once (each lane probes on its own); the walk takes the first one in
+        // row order that no memo blocks, as testing them one by one does
+        bool blocked = false;
+        if (ok) {
+          uint32_t slot;
+          blocked = memo_find_lane(m, edge_key(cur, w), &slot) && ((wg_load(&m.vals[slot]) >> xt) & 1);
+          if (!blocked && hyp) blocked = memo_find_lane(m, hyp_key(w, y, node), &slot);
+        }
+        const uint64_t bm = __ballot(ok && !blocked);
         if (!bm) {
           base += 64;
           continue;
@@ -2183,10 +2208,6 @@ __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ m
         const int first = __builtin_ctzll(bm);
         u = rl32(w, first);
         next = base + first + 1;
-        if (memo_edge_has(m, cur, u, xt) || (hyp && memo_hyp_has(m, u, y, node))) {
-          base = next;
-          continue;
-        }
         found = true;
         break;
       }
