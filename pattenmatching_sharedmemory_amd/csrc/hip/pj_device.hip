@@ -1513,18 +1513,23 @@ __device__ __forceinline__ void wg_store(T* p, T v) {
 
 // Wave-cooperative lookup (64 consecutive probe slots per round).  Returns
 // the slot holding `key`, or the free slot where it belongs; *found says
-// which.  Every lane gets the same answer.
-__device__ uint32_t memo_probe(const MemoTable& m, uint64_t key, bool* found) {
+// which, and *val (when given) receives the value of a found key, read in
+// the same round as the keys.  Every lane gets the same answer.
+__device__ uint32_t memo_probe(const MemoTable& m, uint64_t key, bool* found, uint64_t* val = nullptr) {
   const int lane = lane_id();
   const uint32_t h = static_cast<uint32_t>(mix64(key)) & m.mask;
   for (uint32_t i = 0; i <= m.mask; i += 64) {
     const uint32_t slot = (h + i + lane) & m.mask;
-    const bool live = wg_load(&m.epochs[slot]) == m.epoch;
-    const bool hit = live && wg_load(&m.keys[slot]) == key;
+    const uint32_t ep = wg_load(&m.epochs[slot]);
+    const uint64_t k = wg_load(&m.keys[slot]);
+    const uint64_t v = val ? wg_load(&m.vals[slot]) : 0ull;
+    const bool live = ep == m.epoch;
+    const bool hit = live && k == key;
     const uint64_t stop = __ballot(hit || !live);
     if (stop) {
       const int l = __builtin_ctzll(stop);
       *found = (__ballot(hit) >> l) & 1;
+      if (val) *val = rl64(v, l);
       return rl32(slot, l);
     }
   }
@@ -1532,17 +1537,43 @@ __device__ uint32_t memo_probe(const MemoTable& m, uint64_t key, bool* found) {
   return 0;  // not reached: the table never fills past its limit
 }
 
-// The same lookup by one lane alone (linear probing from the same start, so
-// it stops at the slot the wave-cooperative probe returns): true with *slot
-// when `key` is present.  Lanes of a wave may probe different keys at once.
-__device__ bool memo_find_lane(const MemoTable& m, uint64_t key, uint32_t* slot) {
-  const uint32_t h = static_cast<uint32_t>(mix64(key)) & m.mask;
-  for (uint32_t i = 0; i <= m.mask; ++i) {
-    const uint32_t sl = (h + i) & m.mask;
-    if (wg_load(&m.epochs[sl]) != m.epoch) return false;
-    if (wg_load(&m.keys[sl]) == key) {
-      *slot = sl;
-      return true;
+// Whether one lane's candidate is blocked: edge memo of {a, u} holding `bit`,
+// or (hyp) the hypothesis memo holding hkey.  One lane probes alone, by
+// linear probing from the key's start, so it stops at the slot the
+// wave-cooperative probe returns; lanes of a wave probe their own keys at
+// once.  Both keys are probed in the same rounds (their loads, the edge
+// value included, issued together).
+__device__ bool memo_blocked_lane(const MemoTable& m, uint64_t ekey, uint32_t bit, bool hyp, uint64_t hkey) {
+  uint32_t he = static_cast<uint32_t>(mix64(ekey)) & m.mask;
+  uint32_t hh = static_cast<uint32_t>(mix64(hkey)) & m.mask;
+  bool edone = false, hdone = !hyp;
+  for (uint32_t i = 0; i <= m.mask && !(edone && hdone); ++i) {
+    const uint32_t se = (he + i) & m.mask, sh = (hh + i) & m.mask;
+    uint32_t epe = 0, eph = 0;
+    uint64_t ke = 0, ve = 0, kh = 0;
+    if (!edone) {
+      epe = wg_load(&m.epochs[se]);
+      ke = wg_load(&m.keys[se]);
+      ve = wg_load(&m.vals[se]);
+    }
+    if (!hdone) {
+      eph = wg_load(&m.epochs[sh]);
+      kh = wg_load(&m.keys[sh]);
+    }
+    if (!edone) {
+      if (epe != m.epoch) {
+        edone = true;
+      } else if (ke == ekey) {
+        if ((ve >> bit) & 1) return true;
+        edone = true;
+      }
+    }
+    if (!hdone) {
+      if (eph != m.epoch) {
+        hdone = true;
+      } else if (kh == hkey) {
+        return true;
+      }
     }
   }
   return false;
@@ -1568,16 +1599,18 @@ __device__ __forceinline__ uint64_t edge_key(uint32_t a, uint32_t b) {
 // Edge memo test: was template id / position `bit` inserted for edge {a, b}?
 __device__ bool memo_edge_has(const MemoTable& m, uint32_t a, uint32_t b, uint32_t bit) {
   bool found;
-  const uint32_t slot = memo_probe(m, edge_key(a, b), &found);
-  return found && ((wg_load(&m.vals[slot]) >> bit) & 1);
+  uint64_t v = 0;
+  memo_probe(m, edge_key(a, b), &found, &v);
+  return found && ((v >> bit) & 1);
 }
 
 __device__ void memo_edge_add(MemoTable& m, uint32_t a, uint32_t b, uint32_t bit) {
   bool found;
   const uint64_t key = edge_key(a, b);
-  const uint32_t slot = memo_probe(m, key, &found);
+  uint64_t old = 0;
+  const uint32_t slot = memo_probe(m, key, &found, &old);
   if (found) {
-    const uint64_t v = wg_load(&m.vals[slot]) | (1ull << bit);
+    const uint64_t v = old | (1ull << bit);
     if (lane_id() == 0) wg_store(&m.vals[slot], v);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -1608,12 +1641,107 @@ __device__ void memo_hyp_add(MemoTable& m, uint32_t u, uint32_t y, uint32_t node
 __device__ uint32_t memo_trie_child(MemoTable& m, uint32_t node, uint32_t u) {
   bool found;
   const uint64_t key = (2ull << 62) | (static_cast<uint64_t>(node) << 30) | u;
-  const uint32_t slot = memo_probe(m, key, &found);
-  if (found) return static_cast<uint32_t>(wg_load(&m.vals[slot]));
+  uint64_t v = 0;
+  const uint32_t slot = memo_probe(m, key, &found, &v);
+  if (found) return static_cast<uint32_t>(v);
   const uint32_t child = ++m.next_node;
   if (child >= (1u << 26) - 1) m.overflow = true;  // node ids are 26 bits in hypothesis keys
   memo_put(m, slot, key, child);
   return child;
+}
+
+// One lane's probe for `key` (linear probing from the key's start, so it
+// stops where the wave-cooperative probe stops): the slot holding the key,
+// with its value, or the first free slot.
+__device__ uint32_t memo_probe_one(const MemoTable& m, uint64_t key, bool* found, uint64_t* val) {
+  const uint32_t h = static_cast<uint32_t>(mix64(key)) & m.mask;
+  for (uint32_t i = 0; i <= m.mask; ++i) {
+    const uint32_t sl = (h + i) & m.mask;
+    const uint32_t ep = wg_load(&m.epochs[sl]);
+    const uint64_t k = wg_load(&m.keys[sl]);
+    const uint64_t v = wg_load(&m.vals[sl]);
+    if (ep != m.epoch) {
+      *found = false;
+      return sl;
+    }
+    if (k == key) {
+      *found = true;
+      *val = v;
+      return sl;
+    }
+  }
+  *found = false;
+  return 0;  // not reached: the table never fills past its limit
+}
+
+// The memo updates of a walk step that moved from `cur` to u (template
+// vertex y): with `store`, the hypothesis trie child of `node` by u; with
+// `hyp`, the hypothesis (u, y, child); always the edge memo bit y of {cur, u}
+// -- the same inserts memo_trie_child, memo_hyp_add and memo_edge_add make in
+// that order, in three rounds instead of up to six: the trie and edge keys
+// are probed together, then the hypothesis key, then every store is made
+// with one fence.  A new key lands in the first free slot of its probe
+// sequence; made one after the other, a later insert would probe past the
+// slot an earlier one filled, so the slots agree unless two new keys found
+// the same free slot: then nothing is stored and false is returned (the
+// caller makes the inserts one by one).  *node_out receives the child.
+__device__ bool memo_step_inserts(MemoTable& m, bool store, bool hyp, uint32_t node, uint32_t cur, uint32_t u,
+                                  uint32_t y, uint32_t* node_out) {
+  const int lane = lane_id();
+  const uint64_t tkey = (2ull << 62) | (static_cast<uint64_t>(node) << 30) | u;
+  const uint64_t ekey = edge_key(cur, u);
+  uint32_t sl = 0;
+  bool fnd = false;
+  uint64_t val = 0;
+  if ((lane == 0 && store) || lane == 1) sl = memo_probe_one(m, lane == 0 ? tkey : ekey, &fnd, &val);
+  const uint32_t t_slot = rl32(sl, 0), e_slot = rl32(sl, 1);
+  const bool t_found = rl32(fnd ? 1u : 0u, 0) != 0, e_found = rl32(fnd ? 1u : 0u, 1) != 0;
+  const uint64_t t_val = rl64(val, 0), e_val = rl64(val, 1);
+  const bool t_new = store && !t_found;
+  const uint32_t node2 = !store ? node : t_found ? static_cast<uint32_t>(t_val) : m.next_node + 1;
+  const uint64_t hkey = hyp_key(u, y, node2);
+  uint32_t h_slot = 0;
+  bool h_found = true;
+  if (hyp) {
+    uint32_t s2 = 0;
+    bool f2 = false;
+    uint64_t v2 = 0;
+    if (lane == 0) s2 = memo_probe_one(m, hkey, &f2, &v2);
+    h_slot = rl32(s2, 0);
+    h_found = rl32(f2 ? 1u : 0u, 0) != 0;
+  }
+  const bool h_new = hyp && !h_found, e_new = !e_found;
+  if ((t_new && e_new && t_slot == e_slot) || (t_new && h_new && t_slot == h_slot) ||
+      (e_new && h_new && e_slot == h_slot))
+    return false;
+  if (t_new) {
+    m.next_node = node2;
+    if (node2 >= (1u << 26) - 1) m.overflow = true;  // node ids are 26 bits in hypothesis keys
+  }
+  if (lane == 0) {
+    if (t_new) {
+      wg_store(&m.vals[t_slot], static_cast<uint64_t>(node2));
+      wg_store(&m.keys[t_slot], tkey);
+      wg_store(&m.epochs[t_slot], m.epoch);
+    }
+    if (h_new) {
+      wg_store(&m.vals[h_slot], uint64_t{0});
+      wg_store(&m.keys[h_slot], hkey);
+      wg_store(&m.epochs[h_slot], m.epoch);
+    }
+    wg_store(&m.vals[e_slot], static_cast<uint64_t>(e_val | (uint64_t{1} << y)));
+    if (e_new) {
+      wg_store(&m.keys[e_slot], ekey);
+      wg_store(&m.epochs[e_slot], m.epoch);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  m.used += (t_new ? 1u : 0u) + (h_new ? 1u : 0u) + (e_new ? 1u : 0u);
+  if (m.used > m.limit) m.overflow = true;
+  *node_out = node2;
+  return true;
 }
 
 // Failed-state memo of the complete search: a cycle or path walk has no
@@ -2194,12 +2322,7 @@ __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ m
         // every candidate of the 64 slots is tested against the memos at
         // once (each lane probes on its own); the walk takes the first one in
         // row order that no memo blocks, as testing them one by one does
-        bool blocked = false;
-        if (ok) {
-          uint32_t slot;
-          blocked = memo_find_lane(m, edge_key(cur, w), &slot) && ((wg_load(&m.vals[slot]) >> xt) & 1);
-          if (!blocked && hyp) blocked = memo_find_lane(m, hyp_key(w, y, node), &slot);
-        }
+        const bool blocked = ok && memo_blocked_lane(m, edge_key(cur, w), xt, hyp, hyp_key(w, y, node));
         const uint64_t bm = __ballot(ok && !blocked);
         if (!bm) {
           base += 64;
@@ -2213,9 +2336,13 @@ __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ m
       }
       slots += (found ? next : end) - from;
       if (found) {
-        const uint32_t node2 = (hyp && mo == PJ_MOVE_VERTEX_STORE) ? memo_trie_child(m, node, u) : node;
-        if (hyp) memo_hyp_add(m, u, y, node2);
-        memo_edge_add(m, cur, u, y);
+        const bool store = hyp && mo == PJ_MOVE_VERTEX_STORE;
+        uint32_t node2 = node;
+        if (!memo_step_inserts(m, store, hyp, node, cur, u, y, &node2)) {  // two new keys, one free slot
+          node2 = store ? memo_trie_child(m, node, u) : node;
+          if (hyp) memo_hyp_add(m, u, y, node2);
+          memo_edge_add(m, cur, u, y);
+        }
         if (m.overflow) return -1;
         if (lane == pos) st_c = next;
         if (lane == static_cast<int>(hidx[pos])) hist = u;
