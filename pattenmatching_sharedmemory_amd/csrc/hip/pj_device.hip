@@ -2285,7 +2285,17 @@ __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ m
   uint32_t cur = s, node = 0;
   bool resume = false;
   if (len <= 1) return 1;
-  auto searching = [&](int p) { return mv[p] == PJ_MOVE_VERTEX_STORE || mv[p] == PJ_MOVE_VERTEX_NO_STORE; };
+  // the plan in registers (lane p: move p), read by readlane at each step
+  const bool in_plan = lane < static_cast<int>(len);
+  const uint32_t r_mv = in_plan ? mv[lane] : 0u, r_vx = in_plan ? vx[lane] : 0u, r_tb = in_plan ? tb[lane] : 0u;
+  const uint32_t r_mb = in_plan ? mb[lane] : 0u, r_hidx = in_plan ? hidx[lane] : 0u;
+  // row bounds of cur when known (loaded with the step that moved to cur)
+  bool have_bounds = false;
+  uint64_t cur_beg = 0, cur_end = 0;
+  auto searching = [&](int p) {
+    const uint32_t mo = rl32(r_mv, p);
+    return mo == PJ_MOVE_VERTEX_STORE || mo == PJ_MOVE_VERTEX_NO_STORE;
+  };
   auto backtrack = [&]() -> bool {
     pos = uniform(pos - 1);
     while (pos >= 1 && !searching(pos)) pos = uniform(pos - 1);
@@ -2293,6 +2303,7 @@ __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ m
     cur = rl32(st_v, pos);
     node = rl32(st_n, pos);
     resume = true;
+    have_bounds = false;
     return true;
   };
   while (true) {
@@ -2302,12 +2313,12 @@ __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ m
         atomic_or_word<Word>(&already[lane + 1 < static_cast<int>(len) ? after : cur], Word(1) << vx[lane]);
       return 1;
     }
-    const uint8_t mo = mv[pos];
+    const uint32_t mo = rl32(r_mv, pos);
     if (mo == PJ_MOVE_VERTEX_STORE || mo == PJ_MOVE_VERTEX_NO_STORE) {
-      const uint64_t from = resume ? rl64(st_c, pos) : off[cur];
+      const uint64_t from = resume ? rl64(st_c, pos) : have_bounds ? cur_beg : off[cur];
       resume = false;
-      const uint32_t y = vx[pos], xt = tb[pos];
-      const uint64_t end = off[cur + 1];
+      const uint32_t y = rl32(r_vx, pos), xt = rl32(r_tb, pos);
+      const uint64_t end = have_bounds ? cur_end : off[cur + 1];
       bool found = false;
       uint32_t u = 0;
       uint64_t base = from, next = end;
@@ -2336,6 +2347,9 @@ __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ m
       }
       slots += (found ? next : end) - from;
       if (found) {
+        // u's row bounds, for the step after this one: their loads overlap
+        // the memo inserts
+        const uint64_t u_beg = off[u], u_end = off[u + 1];
         const bool store = hyp && mo == PJ_MOVE_VERTEX_STORE;
         uint32_t node2 = node;
         if (!memo_step_inserts(m, store, hyp, node, cur, u, y, &node2)) {  // two new keys, one free slot
@@ -2345,8 +2359,11 @@ __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ m
         }
         if (m.overflow) return -1;
         if (lane == pos) st_c = next;
-        if (lane == static_cast<int>(hidx[pos])) hist = u;
+        if (lane == static_cast<int>(rl32(r_hidx, pos))) hist = u;
         cur = u;
+        cur_beg = u_beg;
+        cur_end = u_end;
+        have_bounds = true;
         node = node2;
         pos = uniform(pos + 1);
         if (lane == pos) {
@@ -2358,7 +2375,7 @@ __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ m
     } else if (mo == PJ_MOVE_CHECK || mo == kMoveClose) {
       // an existence test (the walk goes on from cur either way): on sorted
       // rows a binary search, else a scan in row order
-      const uint32_t target = mo == kMoveClose ? s : rl32(hist, static_cast<int>(mb[pos]));
+      const uint32_t target = mo == kMoveClose ? s : rl32(hist, static_cast<int>(rl32(r_mb, pos)));
       bool hit;
       if (sorted) {
         uint64_t probes = 0;
@@ -2376,14 +2393,17 @@ __device__ int memo_walk(uint32_t s, uint32_t len, const uint8_t* __restrict__ m
         continue;
       }
     } else if (mo == kMoveLast) {
-      RowScan r = scan_row<Word>(off[cur], off[cur + 1], edges, ea, cand, static_cast<int>(vx[pos]), 0, s, slots);
+      RowScan r = scan_row<Word>(off[cur], off[cur + 1], edges, ea, cand, static_cast<int>(rl32(r_vx, pos)), 0, s,
+                                 slots);
       if (r.found) {
         cur = r.u;
+        have_bounds = false;
         pos = uniform(pos + 1);
         continue;
       }
     } else {  // MOVE_BACK
-      cur = rl32(hist, static_cast<int>(mb[pos]));
+      cur = rl32(hist, static_cast<int>(rl32(r_mb, pos)));
+      have_bounds = false;
       pos = uniform(pos + 1);
       if (lane == pos) {
         st_v = cur;
