@@ -1463,7 +1463,7 @@ __global__ void k_phase2_hub_finish(const uint32_t* __restrict__ hubs, uint32_t 
 constexpr uint8_t kMoveClose = 4;  // last hop of a cycle: an active slot back to the source
 constexpr uint8_t kMoveLast = 5;   // last hop of a path: a candidate other than the source
 constexpr int kMemoWaves = 2048;            // waves of a memoised launch (first pass)
-constexpr int kMemoLog2 = 14;               // entries per wave, first pass
+constexpr int kMemoLog2 = 16;               // entries per wave, first pass
 // Later passes, for sources whose walks outgrew the previous tables: fewer
 // waves with larger tables (the last one holds 2^29 entries, 10.7 GB).
 constexpr int kRetryPasses = 3;
@@ -3840,9 +3840,12 @@ int prepare_memo(pj_ctx* ctx, bool tables = true) {
     PJ_HIP(hipGetLastError());
     return 0;
   }
-  // Walks on larger graphs reach more states: 2^14 entries per wave up to
-  // 2^20 vertices, doubling per doubling of V up to 2^18 (2048 waves x 2^18
-  // x 20 B = 10.7 GB from 2^24 vertices).
+  // Walks on larger graphs reach more states: 2^16 entries per wave up to
+  // 2^22 vertices, doubling per doubling of V up to 2^18 (2048 waves x 2^18
+  // x 20 B = 10.7 GB from 2^24 vertices).  A source that outgrows its table
+  // reruns from the start with a larger one, so the floor is set by the
+  // heavy sources of small graphs: with 2^14 entries those of the scale-18
+  // TDS workload ran twice (135 ms per solve against 79 ms with 2^16).
   int log2 = ctx->memo_log2;
   if (!log2) {
     int bits = 0;

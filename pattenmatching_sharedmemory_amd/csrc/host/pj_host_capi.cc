@@ -299,7 +299,7 @@ uint64_t pjh_projected_bytes(const pjh_constraints* c, uint32_t vertex_count, ui
   if (tables) {  // first-pass tables: 2048 waves x 2^log2 entries x 20 bytes
     int bits = 0;
     while (bits < 63 && (1ull << bits) < V) ++bits;
-    const int log2 = std::min(18, std::max(14, bits - 6));
+    const int log2 = std::min(18, std::max(16, bits - 6));
     bytes += 2048ull * (1ull << log2) * 20;
   }
   if (memo) bytes += 2 * (V + 1) * 8 + (E + 1) * 4;  // active view: counts, offsets, at most every slot

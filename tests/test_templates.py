@@ -62,7 +62,7 @@ def test_projected_bytes_count_memo_tables_and_view(pj):
     assert h.pjh_query_uses_memo(memo.handle) == 1 and h.pjh_query_uses_memo(tds.handle) == 1
     base = h.pjh_projected_bytes(plain.handle, V, E)
     assert base == V * 57 + E // 8
-    tables = 2048 * (1 << 14) * 20  # 2^20 vertices: 2^14 entries per wave
+    tables = 2048 * (1 << 16) * 20  # 2^20 vertices: 2^16 entries per wave (the floor)
     view = 2 * (V + 1) * 8 + (E + 1) * 4
     extra = V * 8 + 8 + (2 * V + 2) * 4 + tables + view
     assert h.pjh_projected_bytes(memo.handle, V, E) == base + extra
