@@ -183,9 +183,16 @@ __global__ void __launch_bounds__(kThreads) k_label_stat(StatArgs a) {
     if (s.exact[i]) atomicAdd(&a.buffer[kExactAt + i], static_cast<unsigned long long>(s.exact[i]));
 }
 
-// Caps each active hub's summed tally and clears hub_count.
+// Caps each active hub's summed tally and clears hub_count.  The capped
+// counts of the block's hubs are summed in LDS (the exact table has a few
+// entries per label pair, which every hub would otherwise hit with its own
+// global atomic) and leave the block with one atomic per non-zero entry.
 template <typename Word>
 __global__ void __launch_bounds__(kThreads) k_stat_hub_finish(StatArgs a) {
+  __shared__ uint32_t s_exact[kExactMax];
+  const uint32_t X = a.exact_words;
+  for (uint32_t i = threadIdx.x; i < X; i += kThreads) s_exact[i] = 0;
+  __syncthreads();
   const Word* __restrict__ cand = static_cast<const Word*>(a.cand);
   const uint32_t* gtab = reinterpret_cast<const uint32_t*>(a.buffer + kCountWords);
   for (uint32_t h = blockIdx.x * kThreads + threadIdx.x; h < a.nhubs; h += gridDim.x * kThreads) {
@@ -200,9 +207,12 @@ __global__ void __launch_bounds__(kThreads) k_stat_hub_finish(StatArgs a) {
       if (!active) continue;
       const uint32_t tb = gtab[lf * kLabels + lt];
       const uint32_t m = tb & 0xFFu;
-      if (m) atomicAdd(&a.buffer[kExactAt + (tb >> 8) + min(c, m) - 1], 1ull);
+      if (m) atomicAdd(&s_exact[(tb >> 8) + min(c, m) - 1], 1u);
     }
   }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < X; i += kThreads)
+    if (s_exact[i]) atomicAdd(&a.buffer[kExactAt + i], static_cast<unsigned long long>(s_exact[i]));
 }
 
 __global__ void k_zero_counts(unsigned long long* p) {
