@@ -533,8 +533,9 @@ def test_triangle_steps_from_random_states_in_every_degree_bin(pj, orc, seed, pa
     """3-vertex cycle steps from random states (candidates, flags and
     edge-active bits) on a graph with sources in every degree bin, so that
     k_triangles (1-8 / 9-32 slots), k_triangles_mid (33-64 slots in one
-    chunk, 65-256 in up to four), the walk (257-2048) and the hub walk all
-    decide sources, many of them failing through cleared slots.  Rows above
+    chunk, 65-256 in up to four), k_triangles_wide (257-2048: the first hop
+    closed as the walk does, the rest by pairs) and the hub walk all decide
+    sources, many of them failing through cleared slots.  Rows above
     `pair_degree` slots are heavy: searches between two of them read the
     pair index."""
     from pattenmatching_sharedmemory_amd import STEP_CC
