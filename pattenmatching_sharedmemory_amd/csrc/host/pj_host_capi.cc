@@ -434,6 +434,10 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
   // batched: queries ready for a worker (query, parked lane or -1)
   std::deque<std::pair<int, int64_t>> ready;
   bool admission_done = false;
+  int idle = 0;  // batched: workers waiting for a ready query
+  // served (arrival times given): arrivals waiting before a batch starts
+  int batch_min = static_cast<int>(std::min<uint32_t>(lanes ? lanes : 1, 64));
+  if (const char* env = std::getenv("PJ_SERVED_BATCH_MIN")) batch_min = std::max(1, std::atoi(env));
   const auto t0 = std::chrono::steady_clock::now();
   auto now_ms = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
   auto fits = [&](int i) { return budget_bytes == 0 || reserved == 0 || reserved + projected[i] <= budget_bytes; };
@@ -462,9 +466,11 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
         std::unique_lock<std::mutex> lk(mu);
         if (batched) {
           // the next ready query, once its bytes fit the budget
+          ++idle;
           cv.wait(lk, [&] {
             return !first_error.empty() || (ready.empty() ? admission_done : fits(ready.front().first));
           });
+          --idle;
           if (!first_error.empty() || ready.empty()) return;
           i = ready.front().first;
           lane = ready.front().second;
@@ -510,6 +516,26 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
         while (next < n) {
           const double due = arrival_ms ? arrival_ms[next] : 0.0;
           if (now_ms() < due) break;
+          // With arrival times (served), a query that finds a worker idle and
+          // no query waiting for one is solved whole at once: the batch's
+          // sweeps cover the whole graph at every step, so at low load a
+          // template parks later than a worker finishes it; under load the
+          // workers are busy and arrivals take batch lanes.
+          if (arrival_ms && static_cast<int>(ready.size()) < idle) {
+            begin(next);
+            ready.emplace_back(next, -1);
+            ++next;
+            cv.notify_all();
+            continue;
+          }
+          // A batch step sweeps the whole graph whether few or many lanes
+          // take part: arrivals start a batch only once batch_min of them
+          // wait (and join one that runs); until then they wait for a worker.
+          if (arrival_ms && running == 0) {
+            int waiting = 0;
+            for (int k = next; k < n && waiting < batch_min && arrival_ms[k] <= now_ms(); ++k) ++waiting;
+            if (waiting < batch_min) break;
+          }
           if (running + parked >= lanes) break;
           uint32_t lane = 0;
           const int rc = pj_batch_admit(batch, templates[next]->c.desc(), &lane);
