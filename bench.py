@@ -567,7 +567,7 @@ def main() -> int:
         # scale-24 graph at the highest arrival rate
         result["batched_form"] = query_form("batch", g=g)
         del g
-        result["served_form"] = query_form("served", rates=(2000.0,))
+        result["served_form"] = query_form("served", rates=(50.0, 2000.0))
     print(json.dumps(result), flush=True)
     return 0
 
