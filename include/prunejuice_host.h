@@ -120,6 +120,8 @@ typedef struct pjh_query_result {
   double arrival_ms, start_ms, end_ms; /* on the batch clock */
   double device_ms;         /* summed step device time */
   double worker_ms;         /* when a worker context took it (start_ms: admission) */
+  int32_t batch_lane;       /* batch lane its LCC0 and first LCC fixed point ran in, or -1 */
+  int32_t reserved;
 } pjh_query_result;
 /* With PJH_BATCH_LCC in `flags` the LCC0 step and the LCC steps to the first
  * fixed point of every template the batch can hold (pj_batch_admit) run in a batch

@@ -437,7 +437,8 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
   int idle = 0;  // batched: workers waiting for a ready query
   // served (arrival times given): arrivals waiting before a batch starts
   int batch_min = static_cast<int>(std::min<uint32_t>(lanes ? lanes : 1, 64));
-  if (const char* env = std::getenv("PJ_SERVED_BATCH_MIN")) batch_min = std::max(1, std::atoi(env));
+  // PJ_SERVED_BATCH_MIN=0: every arrival takes a batch lane (no direct start)
+  if (const char* env = std::getenv("PJ_SERVED_BATCH_MIN")) batch_min = std::max(0, std::atoi(env));
   const auto t0 = std::chrono::steady_clock::now();
   auto now_ms = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
   auto fits = [&](int i) { return budget_bytes == 0 || reserved == 0 || reserved + projected[i] <= budget_bytes; };
@@ -447,6 +448,7 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
     r.projected_bytes = projected[i];
     r.arrival_ms = arrival_ms ? arrival_ms[i] : 0.0;
     r.start_ms = r.worker_ms = now_ms();
+    r.batch_lane = -1;
     if (!batched) reserved += projected[i];
   };
   auto finish = [&](int i, int w, const std::string& e) {
@@ -521,7 +523,7 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
           // sweeps cover the whole graph at every step, so at low load a
           // template parks later than a worker finishes it; under load the
           // workers are busy and arrivals take batch lanes.
-          if (arrival_ms && static_cast<int>(ready.size()) < idle) {
+          if (arrival_ms && batch_min > 0 && static_cast<int>(ready.size()) < idle) {
             begin(next);
             ready.emplace_back(next, -1);
             ++next;
@@ -531,7 +533,7 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
           // A batch step sweeps the whole graph whether few or many lanes
           // take part: arrivals start a batch only once batch_min of them
           // wait (and join one that runs); until then they wait for a worker.
-          if (arrival_ms && running == 0) {
+          if (arrival_ms && batch_min > 0 && running == 0) {
             int waiting = 0;
             for (int k = next; k < n && waiting < batch_min && arrival_ms[k] <= now_ms(); ++k) ++waiting;
             if (waiting < batch_min) break;
@@ -551,6 +553,7 @@ int pjh_run_queries(pj_ctx* owner, const pjh_constraints* const* templates, int 
             break;
           }
           begin(next);
+          out[next].batch_lane = static_cast<int32_t>(lane);
           lane_query[lane] = next;
           ++running;
           ++next;

@@ -91,7 +91,8 @@ class QueryResult(C.Structure):
     _fields_ = [("steps", C.c_int32), ("worker", C.c_int32), ("rows_hash", C.c_uint64),
                 ("state_checksum", C.c_uint64), ("active_vertices", C.c_uint64), ("active_edges", C.c_uint64),
                 ("projected_bytes", C.c_uint64), ("arrival_ms", C.c_double), ("start_ms", C.c_double),
-                ("end_ms", C.c_double), ("device_ms", C.c_double), ("worker_ms", C.c_double)]
+                ("end_ms", C.c_double), ("device_ms", C.c_double), ("worker_ms", C.c_double),
+                ("batch_lane", C.c_int32), ("reserved", C.c_int32)]
 
     def result(self) -> tuple:
         """What the solve decided (its rows and final state), times excluded."""

@@ -111,6 +111,7 @@ def test_batched_configuration_templates_against_the_oracle(pj, orc):
     assert len({r.worker for r in batch}) > 1
     # the LCC0 and first LCC fixed point of every template in shared sweeps
     shared, _ = owner.run_queries(cs, workers=4, run_tds=True, batch_lcc=True)
+    assert any(r.batch_lane >= 0 for r in shared)
     for i in determined:
         assert alone[i].result() == batch[i].result(), i
         assert alone[i].result() == shared[i].result(), i
@@ -135,9 +136,22 @@ def test_served_configuration_templates_against_the_oracle(pj, orc):
     budget = int(max(r.projected_bytes for r in alone) * 3)
     served, _ = owner.run_queries(cs, workers=6, run_tds=True, arrivals_ms=arrivals, budget_bytes=budget)
     # continuous batching: arrivals join the shared LCC sweeps as lanes free up
-    shared, _ = owner.run_queries(cs, workers=6, run_tds=True, arrivals_ms=arrivals, budget_bytes=budget,
-                                  batch_lcc=True)
-    for form in (served, shared):
+    # (every arrival takes a lane: this small a load would otherwise be sent
+    # whole to idle workers)
+    import os
+
+    os.environ["PJ_SERVED_BATCH_MIN"] = "0"
+    try:
+        shared, _ = owner.run_queries(cs, workers=6, run_tds=True, arrivals_ms=arrivals, budget_bytes=budget,
+                                      batch_lcc=True)
+    finally:
+        del os.environ["PJ_SERVED_BATCH_MIN"]
+    assert any(r.batch_lane >= 0 for r in shared)
+    # the default schedule: arrivals that find a worker idle run whole, a
+    # batch starts once enough of them wait
+    adaptive, _ = owner.run_queries(cs, workers=6, run_tds=True, arrivals_ms=arrivals, budget_bytes=budget,
+                                    batch_lcc=True)
+    for form in (served, shared, adaptive):
         for r, a in zip(form, arrivals):
             assert r.start_ms + 1e-6 >= a and r.worker_ms + 1e-6 >= r.start_ms
         for r in form:  # bytes of the queries on workers when r reached one
@@ -146,5 +160,6 @@ def test_served_configuration_templates_against_the_oracle(pj, orc):
     for i in determined:
         assert alone[i].result() == served[i].result(), i
         assert alone[i].result() == shared[i].result(), i
+        assert alone[i].result() == adaptive[i].result(), i
         assert served[i].state_checksum == oracle_checksum(pj, g, cs[i], finals[i]), i
     owner.close()

@@ -57,6 +57,7 @@ def test_shared_lcc_sweeps_give_every_template_its_own_result(pj):
     finally:
         del os.environ["PJ_BATCH_LANES"]
     assert [r.result() for r in seq] == [r.result() for r in shared]
+    assert all(r.batch_lane >= 0 for r in shared[:-1]) and shared[-1].batch_lane == -1
     owner.close()
 
 
