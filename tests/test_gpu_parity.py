@@ -453,16 +453,22 @@ def test_reverse_positions_change_nothing(pj, orc, case, monkeypatch):
     assert [r.key() for r in r1] == [r.key() for r in orows]
 
 
-@pytest.mark.parametrize("case", ["C2-shape s16", "hub", "four-cycle"])
+@pytest.mark.parametrize("case", ["C2-shape s16", "hub", "four-cycle", "bound by the budget"])
 def test_pair_index_changes_nothing(pj, orc, case, monkeypatch):
     """Cycle steps with the pair index of heavy rows (rows above 16 or 32
-    slots, so that most searches between a source's neighbours read it) and
+    slots, so that most searches between a source's neighbours read it; with
+    a 1 MB bound the threshold doubles from 4 slots until the index fits) and
     with binary searches only (PJ_PAIR_INDEX_MB=0) give identical traces and
     states, and the reference's; the index replaces searches, so the cycle
     steps read fewer slots with it."""
     from pattenmatching_sharedmemory_amd import STEP_CC
 
-    if case == "hub":
+    budget_on = "1024"
+    if case == "bound by the budget":
+        g = pj.Graph.rmat(14, 32, seed=3).labels_uniform(4, seed=4)
+        p = pj.Graph.pattern(4, [(0, 1), (1, 2), (2, 0), (2, 3)], [0, 1, 2, 3])
+        degree, budget_on = "4", "1"
+    elif case == "hub":
         g = pj.Graph.rmat(14, 32, seed=3).labels_uniform(4, seed=4)
         p = pj.Graph.pattern(4, [(0, 1), (1, 2), (2, 0), (2, 3)], [0, 1, 2, 3])
         degree = "16"
@@ -477,7 +483,7 @@ def test_pair_index_changes_nothing(pj, orc, case, monkeypatch):
     c = pj.Constraints(p)
     monkeypatch.setenv("PJ_PAIR_DEGREE", degree)
     runs = []
-    for budget in ("0", "1024"):
+    for budget in ("0", budget_on):
         monkeypatch.setenv("PJ_PAIR_INDEX_MB", budget)
         pr = pj.Pruner(g, c)
         rows, _ = pr.run(enumerate_matches=False)
