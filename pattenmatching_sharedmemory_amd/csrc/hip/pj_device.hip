@@ -3292,6 +3292,7 @@ struct pj_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_done = nullptr;  // no timing: polled by wait_stream
   std::string err;
   int cus = 256;
 
@@ -3406,6 +3407,8 @@ struct pj_ctx {
   float last_ms = 0.f;
   uint64_t last_slots = 0;
   unsigned long long* h_sums = nullptr;   // pinned: per counter block, kCounterStride sums
+  unsigned long long* h_sums_dev = nullptr;  // the device's address of h_sums (k_sum_counters writes it)
+  uint64_t* h_word = nullptr;  // pinned: one value read back between launches (counts, totals)
   unsigned long long* d_sums = nullptr;
   // Active-graph compaction: the counters of pj_compact_active and the
   // device outputs of pj_download_active (grown on demand).
@@ -3427,6 +3430,18 @@ namespace {
       return -1;                                                                 \
     }                                                                            \
   } while (0)
+
+// Waits until the work queued on the context's stream is done by polling a
+// marker event.  A blocking wait lets the thread sleep until an interrupt
+// wakes it; at the end of each pruning step, where the host reads the
+// step's counts before it queues the next step, that wake-up leaves the GPU
+// idle for tens of microseconds.
+hipError_t wait_stream(pj_ctx* ctx) {
+  hipError_t e = hipEventRecord(ctx->ev_done, ctx->stream);
+  if (e != hipSuccess) return e;
+  while ((e = hipEventQuery(ctx->ev_done)) == hipErrorNotReady) __builtin_ia32_pause();
+  return e;
+}
 
 int grid_for(const pj_ctx* ctx, uint64_t items, int per_block = kBlock) {
   uint64_t g = (items + per_block - 1) / per_block;
@@ -3613,7 +3628,7 @@ int state_reset_t(pj_ctx* ctx) {
   }
   mark_all_dirty(ctx);
   PJ_HIP(hipGetLastError());
-  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  PJ_HIP(wait_stream(ctx));
   ctx->active_vertices = ctx->V;
   ctx->active_edges = ctx->E;
   ctx->labels_consistent = false;  // every template vertex is a candidate again
@@ -3898,9 +3913,9 @@ int build_view(pj_ctx* ctx) {
   hipLaunchKernelGGL(k_scan_apply, dim3(spans), dim3(kBlock), 0, ctx->stream, ctx->d_view_count, V + 1,
                      ctx->d_scan_tmp, ctx->d_view_off);
   PJ_HIP(hipGetLastError());
-  uint64_t total = 0;
-  PJ_HIP(hipMemcpyAsync(&total, ctx->d_view_off + V, 8, hipMemcpyDeviceToHost, ctx->stream));
-  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  PJ_HIP(hipMemcpyAsync(ctx->h_word, ctx->d_view_off + V, 8, hipMemcpyDeviceToHost, ctx->stream));
+  PJ_HIP(wait_stream(ctx));
+  const uint64_t total = *ctx->h_word;
   if (total + 1 > ctx->view_cap) {
     if (ctx->d_view_edges) hipFree(ctx->d_view_edges);
     ctx->d_view_edges = nullptr;
@@ -4051,9 +4066,9 @@ int launch_nlcc_retry(pj_ctx* ctx, int kind) {
     uint32_t* out_count = ctx->d_ovf + ((r + 1) & 1);
     uint32_t* in_list = ctx->d_ovf + 2 + (r & 1) * V;
     uint32_t* out_list = ctx->d_ovf + 2 + ((r + 1) & 1) * V;
-    uint32_t n = 0;
-    PJ_HIP(hipMemcpyAsync(&n, in_count, 4, hipMemcpyDeviceToHost, ctx->stream));
-    PJ_HIP(hipStreamSynchronize(ctx->stream));
+    PJ_HIP(hipMemcpyAsync(ctx->h_word, in_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PJ_HIP(wait_stream(ctx));
+    const uint32_t n = *reinterpret_cast<const uint32_t*>(ctx->h_word);
     if (!n) return 0;
     MemoArgs m = ctx->memo1;
     int waves = kMemoWaves;
@@ -4076,9 +4091,9 @@ int launch_nlcc_retry(pj_ctx* ctx, int kind) {
                        static_cast<Word*>(ctx->d_unmatch), ctx->d_flags, ctx->d_t, m, c);
     end_launch(ctx);
   }
-  uint32_t n = 0;
-  PJ_HIP(hipMemcpyAsync(&n, ctx->d_ovf + ((kRetryPasses + 1) & 1), 4, hipMemcpyDeviceToHost, ctx->stream));
-  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  PJ_HIP(hipMemcpyAsync(ctx->h_word, ctx->d_ovf + ((kRetryPasses + 1) & 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+  PJ_HIP(wait_stream(ctx));
+  const uint32_t n = *reinterpret_cast<const uint32_t*>(ctx->h_word);
   if (n) {
     ctx->err = "memoised walk of a source exceeds 2^29 memo entries";  // after the last retry table
     return -1;
@@ -4346,11 +4361,10 @@ int compact_live_t(pj_ctx* ctx) {
   hipLaunchKernelGGL(k_live_compact<Word>, dim3(std::max<uint32_t>(blocks, 1)), dim3(kBlock), 0, ctx->stream,
                      ctx->live, ctx->n_live, static_cast<const Word*>(ctx->d_cand), out, n_out);
   PJ_HIP(hipGetLastError());
-  uint32_t n = 0;
-  PJ_HIP(hipMemcpyAsync(&n, n_out, 4, hipMemcpyDeviceToHost, ctx->stream));
-  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  PJ_HIP(hipMemcpyAsync(ctx->h_word, n_out, 4, hipMemcpyDeviceToHost, ctx->stream));
+  PJ_HIP(wait_stream(ctx));
   ctx->live = out;
-  ctx->n_live = n;
+  ctx->n_live = *reinterpret_cast<const uint32_t*>(ctx->h_word);
   return 0;
 }
 
@@ -4562,10 +4576,13 @@ int pj_create(pj_ctx** out, int device) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_done, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&ctx->d_t), sizeof(DevTemplate));
-  if (e == hipSuccess)
+  if (e == hipSuccess)  // coherent: the sums kernel's stores are visible once the stream is done
     e = hipHostMalloc(reinterpret_cast<void**>(&ctx->h_sums), (kMaxLaunches + 1) * kCounterStride * 8,
-                      hipHostMallocDefault);
+                      hipHostMallocCoherent);
+  if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->h_sums_dev), ctx->h_sums, 0);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&ctx->h_word), 64, hipHostMallocDefault);
   if (e == hipSuccess)
     e = hipMalloc(reinterpret_cast<void**>(&ctx->d_sums), (kMaxLaunches + 1) * kCounterStride * 8);
   if (e != hipSuccess) {
@@ -5122,10 +5139,9 @@ int pj_run_step(pj_ctx* ctx, int step, uint32_t index, pj_algo_results* out) {
   PJ_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   const size_t blocks = ctx->timing ? static_cast<size_t>(ctx->nlaunch) + 1 : 1;
   hipLaunchKernelGGL(k_sum_counters, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, ctx->stream,
-                     ctx->d_counters, ctx->d_sums);
+                     ctx->d_counters, ctx->h_sums_dev);
   PJ_HIP(hipGetLastError());
-  PJ_HIP(hipMemcpyAsync(ctx->h_sums, ctx->d_sums, blocks * kCounterStride * 8, hipMemcpyDeviceToHost, ctx->stream));
-  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  PJ_HIP(wait_stream(ctx));
   PJ_HIP(hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
   pj_algo_results r{0, 0, 0, 0};
   ctx->last_slots = 0;
@@ -5364,8 +5380,10 @@ void pj_destroy(pj_ctx* ctx) {
   if (ctx->d_counters) hipFree(ctx->d_counters);
   if (ctx->d_sums) hipFree(ctx->d_sums);
   if (ctx->h_sums) hipHostFree(ctx->h_sums);
+  if (ctx->h_word) hipHostFree(ctx->h_word);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  if (ctx->ev_done) hipEventDestroy(ctx->ev_done);
   for (int i = 0; i < 2 * kMaxLaunches; ++i)
     if (ctx->lev[i]) hipEventDestroy(ctx->lev[i]);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
