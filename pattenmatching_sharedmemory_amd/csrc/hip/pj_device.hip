@@ -1008,7 +1008,7 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
 // Hub rows of LCC phase 1 in a launch of their own (fused with the bins,
 // the unrolled chunk loop raises the register count of the whole kernel).
 template <typename Word, int KW, bool kSlot>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW == 1 ? 7 : 6))) k_lcc_hub_chunks(
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW == 1 ? 7 : KW == 8 && sizeof(Word) == 8 ? 5 : 6))) k_lcc_hub_chunks(
     const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
     const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
     const uint8_t* __restrict__ code, bool derive, const uint8_t* __restrict__ slot_lab,
@@ -1031,7 +1031,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW 
 // LCC phase 1 of the degree bins in one launch, each bin grid-strided over
 // the whole grid.
 template <typename Word, int KW, bool kSlot>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW == 1 && !kSlot ? 8 : KW <= 2 ? 7 : 1))) k_lcc_phase1_all(
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(Word) == 8 && !kSlot && KW <= 2 ? 6 : KW == 1 && !kSlot ? 8 : KW <= 2 ? 7 : 1))) k_lcc_phase1_all(
     const uint32_t* __restrict__ bins, const uint32_t* __restrict__ bin_count, uint32_t V,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code,
     bool derive, const uint8_t* __restrict__ slot_lab, const Word* __restrict__ cand, Word* __restrict__ unmatch,

@@ -8,6 +8,10 @@
         HBM bytes per launch from two --pmc passes (FETCH_SIZE, WRITE_SIZE;
         KiB per dispatch), each scaled by what the same counter reads for a
         plain device copy of N bytes (the copy passes)
+    python -m pattenmatching_sharedmemory_amd.rocprof_summary counters DIR NAME...
+        mean per dispatch of each named counter of one --pmc pass, per kernel
+        (e.g. TCC_REQ_sum, TCC_HIT_sum, TCC_MISS_sum: L2 requests, hits and
+        misses)
 
 Kernels are reported under the names pj_kernel_stats uses (lcc_phase1,
 nlcc_walk, ...), so the figures line up with the live per-kernel timing.
@@ -151,7 +155,21 @@ def main(argv=None) -> int:
     t.add_argument("--copy-fetch", required=True)
     t.add_argument("--copy-write", required=True)
     t.add_argument("--copy-bytes", type=int, required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("dir")
+    c.add_argument("names", nargs="+")
     a = ap.parse_args(argv)
+    if a.cmd == "counters":
+        got = {n: pmc(a.dir, n) for n in a.names}
+        kernels = sorted({k for g in got.values() for k in g},
+                         key=lambda k: -max(g.get(k, {"total": 0.0})["total"] for g in got.values()))
+        print("| kernel | dispatches | " + " | ".join(f"{n} per dispatch" for n in a.names) + " |")
+        print("|---|---:|" + "---:|" * len(a.names))
+        for k in kernels:
+            d = max(g.get(k, {"dispatches": 0})["dispatches"] for g in got.values())
+            vals = " | ".join(f"{got[n][k]['mean']:.4g}" if k in got[n] else "-" for n in a.names)
+            print(f"| {k} | {d} | {vals} |")
+        return 0
     if a.cmd == "stats":
         print("| kernel | launches | total ms | avg us | min us | max us | % |")
         print("|---|---:|---:|---:|---:|---:|---:|")
