@@ -729,7 +729,7 @@ __device__ __forceinline__ uint64_t sat_add_bytes(uint64_t a, uint64_t b) {
 // steps only remove candidates), so a neighbour's code is that of any of its
 // candidates, read from the LDS table instead of gathering code[u].
 template <typename Word, int G, int S, int KW, bool kSlot>
-__device__ void lcc_phase1_lanes(uint32_t block, uint32_t nblocks, const LccTables<KW>& tab,
+__device__ __forceinline__ void lcc_phase1_lanes(uint32_t block, uint32_t nblocks, const LccTables<KW>& tab,
                                  const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
                                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges,
                                  const uint8_t* __restrict__ code, bool derive,
@@ -860,7 +860,7 @@ __device__ void lcc_phase1_lanes(uint32_t block, uint32_t nblocks, const LccTabl
 
 // `derive`: neighbour codes from candidates, as in lcc_phase1_lanes.
 template <typename Word, int KW, bool kSlot>
-__device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* s_pcode, const uint8_t* s_scode,
+__device__ __forceinline__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* s_pcode, const uint8_t* s_scode,
                                const uint64_t* s_cv,
                                const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start,
                                uint32_t nchunks, const uint32_t* __restrict__ hubs, const uint64_t* __restrict__ off,
@@ -1005,8 +1005,10 @@ __device__ void lcc_hub_chunks(uint32_t block, uint32_t nblocks, const uint8_t* 
   wave_add(counters, C_ITEMS, items);
 }
 
-// Hub rows of LCC phase 1 in a launch of their own (fused with the bins,
-// the unrolled chunk loop raises the register count of the whole kernel).
+// Hub rows of LCC phase 1 in a launch of their own (PJ_PHASE1_FUSED=0;
+// by default k_lcc_phase1_hubs runs them in the launch of the bins).  Both
+// kernels inline the chunk and bin bodies: called from two kernels, the
+// compiler otherwise keeps them as functions with a stack in scratch.
 template <typename Word, int KW, bool kSlot>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KW == 1 ? 7 : KW == 8 && sizeof(Word) == 8 ? 5 : 6))) k_lcc_hub_chunks(
     const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
@@ -1051,6 +1053,41 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(siz
   // per scale-22 solve).  The slot-label and two-word builds need up to 72
   // (7 waves): held to 64 they spilled 40 bytes per lane to scratch, written
   // and read by every lane of every launch.
+  lcc_phase1_lanes<Word, 64, 4, KW, kSlot>(blockIdx.x, gridDim.x, tab, bins + 3 * V64, bin_count + 3, off, edges, code,
+                                    derive, slot_lab, cand, unmatch, flags, dirty, rescan, ea, t, counters);
+  lcc_phase1_lanes<Word, 32, 4, KW, kSlot>(blockIdx.x, gridDim.x, tab, bins + 2 * V64, bin_count + 2, off, edges, code,
+                                    derive, slot_lab, cand, unmatch, flags, dirty, rescan, ea, t, counters);
+  lcc_phase1_lanes<Word, 4, 4, KW, kSlot>(blockIdx.x, gridDim.x, tab, bins + V64, bin_count + 1, off, edges, code, derive,
+                                   slot_lab, cand, unmatch, flags, dirty, rescan, ea, t, counters);
+  lcc_phase1_lanes<Word, 1, 4, KW, kSlot>(blockIdx.x, gridDim.x, tab, bins, bin_count + 0, off, edges, code, derive,
+                                   slot_lab, cand, unmatch, flags, dirty, rescan, ea, t, counters);
+}
+
+// LCC phase 1 of the hub chunks and the degree bins in one launch: each
+// block first takes its share of the hub chunks, then its share of every
+// bin, so the blocks still on hub rows run beside the blocks already on the
+// bins (both parts wait on chains of dependent loads, neither saturates the
+// memory system alone) and no launch boundary separates the two.  Hubs are
+// never in a bin, so the two parts touch disjoint rows and vertices.
+template <typename Word, int KW, bool kSlot>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(Word) == 8 && !kSlot && KW <= 2 ? 6 : KW == 1 && !kSlot ? 8 : KW <= 2 ? 7 : 1))) k_lcc_phase1_hubs(
+    const uint32_t* __restrict__ bins, const uint32_t* __restrict__ bin_count, uint32_t V,
+    const uint32_t* __restrict__ chunk_hub, const uint64_t* __restrict__ chunk_start, uint32_t nchunks,
+    const uint32_t* __restrict__ hubs, uint32_t* __restrict__ hub_count, uint32_t* __restrict__ hub_elim,
+    const uint64_t* __restrict__ off, const uint32_t* __restrict__ edges, const uint8_t* __restrict__ code,
+    bool derive, const uint8_t* __restrict__ slot_lab, const Word* __restrict__ cand, Word* __restrict__ unmatch,
+    uint8_t* __restrict__ flags, uint8_t* __restrict__ dirty, uint8_t* __restrict__ rescan, uint32_t* __restrict__ ea,
+    const DevTemplate* __restrict__ t, unsigned long long* counters) {
+  __shared__ LccTables<KW> tab;
+  load_lcc_tables<KW>(tab, t);
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    atomicAdd(&counters[C_ITEMS],
+              static_cast<unsigned long long>(bin_count[0]) + bin_count[1] + bin_count[2] + bin_count[3]);
+  lcc_hub_chunks<Word, KW, kSlot>(blockIdx.x, gridDim.x, tab.pcode, tab.slot_code, tab.cv, chunk_hub, chunk_start,
+                                  nchunks, hubs, off, edges, code, derive, slot_lab, cand, flags, dirty, ea,
+                                  hub_count, hub_elim, t, counters);
+  const uint64_t V64 = V;
   lcc_phase1_lanes<Word, 64, 4, KW, kSlot>(blockIdx.x, gridDim.x, tab, bins + 3 * V64, bin_count + 3, off, edges, code,
                                     derive, slot_lab, cand, unmatch, flags, dirty, rescan, ea, t, counters);
   lcc_phase1_lanes<Word, 32, 4, KW, kSlot>(blockIdx.x, gridDim.x, tab, bins + 2 * V64, bin_count + 2, off, edges, code,
@@ -3332,6 +3369,7 @@ struct pj_ctx {
   // neighbour candidates only at 1.
   int fresh = 2;
   bool slot_labels_enabled = true;  // PJ_SLOT_LABELS=0: never
+  bool phase1_fused = true;          // PJ_PHASE1_FUSED=0: hub chunks of LCC phase 1 in a launch of their own
   bool wide_triangles = true;        // PJ_WIDE_TRIANGLES=0: 257-2048-slot cycle sources walk
   // Live list: the vertices that may still be active (a superset of the
   // active ones; candidates only shrink until the state is replaced).
@@ -3776,12 +3814,21 @@ void launch_lcc_phase1(pj_ctx* ctx) {
   unsigned long long* c = begin_launch(ctx, slot_lab ? K_LCC_P1_LABELS : K_LCC_P1);
   // the first step after LCC0 (slot labels) has a build of its own: one
   // body holding both ways of reading neighbours spills registers
-  hipLaunchKernelGGL((slot_lab ? k_lcc_phase1_all<Word, KW, true> : k_lcc_phase1_all<Word, KW, false>),
-                     dim3(ctx->cus * kPhase1BlocksPerCu), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
-                     ctx->d_bin_count, ctx->V, ctx->d_off, ctx->d_edges, ctx->d_code, derive, slot_lab, cand, unm,
-                     ctx->d_flags, dirty, rescan, ctx->d_ea, ctx->d_t, c);
+  const bool fused = ctx->nhubs && ctx->phase1_fused;
+  if (fused)
+    hipLaunchKernelGGL((slot_lab ? k_lcc_phase1_hubs<Word, KW, true> : k_lcc_phase1_hubs<Word, KW, false>),
+                       dim3(ctx->cus * kPhase1BlocksPerCu), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
+                       ctx->d_bin_count, ctx->V, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs,
+                       ctx->d_hub_count, ctx->d_hub_elim, ctx->d_off, ctx->d_edges, ctx->d_code, derive, slot_lab,
+                       cand, unm, ctx->d_flags, dirty, rescan, ctx->d_ea, ctx->d_t, c);
+  else
+    hipLaunchKernelGGL((slot_lab ? k_lcc_phase1_all<Word, KW, true> : k_lcc_phase1_all<Word, KW, false>),
+                       dim3(ctx->cus * kPhase1BlocksPerCu), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
+                       ctx->d_bin_count, ctx->V, ctx->d_off, ctx->d_edges, ctx->d_code, derive, slot_lab, cand, unm,
+                       ctx->d_flags, dirty, rescan, ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
   if (ctx->nhubs) {
+    if (!fused) {
     c = begin_launch(ctx, slot_lab ? K_LCC_HUB_LABELS : K_LCC_HUB);
     hipLaunchKernelGGL((slot_lab ? k_lcc_hub_chunks<Word, KW, true> : k_lcc_hub_chunks<Word, KW, false>),
                        dim3(std::min<uint32_t>(ctx->nchunks, ctx->cus * kPhase1BlocksPerCu)),
@@ -3790,6 +3837,7 @@ void launch_lcc_phase1(pj_ctx* ctx) {
                        ctx->d_edges, ctx->d_code, derive, slot_lab, cand, ctx->d_flags, dirty, ctx->d_ea,
                        ctx->d_hub_count, ctx->d_hub_elim, ctx->d_t, c);
     end_launch(ctx);
+    }
     begin_launch(ctx, K_LCC_HUB_FINISH);
     hipLaunchKernelGGL((k_lcc_hub_finish<Word, KW>), dim3(grid_for(ctx, ctx->nhubs)), dim3(kBlock), 0, ctx->stream,
                        ctx->d_hubs, ctx->nhubs, ctx->d_code, cand, unm, ctx->d_flags, dirty, ctx->d_hub_count,
@@ -4602,6 +4650,9 @@ int pj_create(pj_ctx** out, int device) {
   // PJ_WIDE_TRIANGLES=0 walks the 3-vertex cycles of sources of 257-2048
   // slots (first hop by first hop) instead of testing their pairs.
   if (const char* wt = getenv("PJ_WIDE_TRIANGLES")) ctx->wide_triangles = strcmp(wt, "0") != 0;
+  // PJ_PHASE1_FUSED=0 runs the hub chunks of LCC phase 1 in a launch after
+  // the degree bins instead of in the same launch (same results).
+  if (const char* pf = getenv("PJ_PHASE1_FUSED")) ctx->phase1_fused = strcmp(pf, "0") != 0;
   // PJ_PAIR_INDEX_MB bounds the pair index of heavy rows (0: none, every
   // pair search by binary search); PJ_PAIR_DEGREE is the smallest row length
   // above which a row is heavy (doubled until the index fits).  Same results
