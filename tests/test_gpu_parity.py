@@ -533,6 +533,37 @@ def test_wide_triangle_pairs_change_nothing(pj, orc, case, monkeypatch):
     assert [r.key() for r in r1] == [r.key() for r in orows], case
 
 
+@pytest.mark.parametrize("case", ["C2-shape s16", "P=20, 40 labels"])
+def test_phase1_hub_chunks_in_the_bins_launch_change_nothing(pj, orc, case, monkeypatch):
+    """LCC phase 1 with the hub chunks in the launch of the degree bins
+    (default) and in a launch of their own (PJ_PHASE1_FUSED=0) gives
+    identical traces and states, and the reference's."""
+    if case == "C2-shape s16":
+        g = pj.Graph.rmat(16, 16, seed=20221).labels_zipf(32, 1.2, seed=7)
+        p = pj.Graph.pattern(6, TRIANGLE_TAIL, [0, 1, 2, 3, 4, 3])
+    else:  # 32-bit candidate words, label counts over five words
+        n = 20
+        g = pj.Graph.rmat(15, 16, seed=9).labels_uniform(40, seed=10)
+        p = pj.Graph.pattern(n, [(i, i + 1) for i in range(n - 1)] + [(0, 2)], [i % 40 for i in range(n)])
+    deg = np.diff(g.offsets().astype(np.int64))
+    assert np.any(deg > 2048), case  # hub rows exist
+    c = pj.Constraints(p)
+    runs = []
+    for fused in ("0", "1"):
+        monkeypatch.setenv("PJ_PHASE1_FUSED", fused)
+        pr = pj.Pruner(g, c)
+        rows, _ = pr.run(enumerate_matches=False)
+        runs.append((rows, pr.state()))
+        pr.close()
+    (r0, s0), (r1, s1) = runs
+    assert [r.key() for r in r0] == [r.key() for r in r1], case
+    assert np.array_equal(s0.cand, s1.cand) and np.array_equal(s0.vflags, s1.vflags)
+    assert np.array_equal(s0.ea_words, s1.ea_words)
+    o = orc.Oracle(g, c, threads=8)
+    orows, _ = o.run(orc.FAITHFUL, enumerate_matches=False)
+    assert [r.key() for r in r1] == [r.key() for r in orows], case
+
+
 @pytest.mark.parametrize("pair_degree", ["512", "16"])
 @pytest.mark.parametrize("seed", [31, 32])
 def test_triangle_steps_from_random_states_in_every_degree_bin(pj, orc, seed, pair_degree, monkeypatch):
