@@ -335,13 +335,21 @@ __global__ void __launch_bounds__(kBlock) k_rows_symmetric_chunks(
   symmetric_flags(flag, bad, loop, before, after);
 }
 
-// Sums the kCounterSlots slots of counter block blockIdx.x.
-__global__ void k_sum_counters(const unsigned long long* __restrict__ counters, unsigned long long* __restrict__ sums) {
+// Sums the kCounterSlots slots of counter block blockIdx.x and clears
+// them; block 0 also clears the `nring` bin counts of the ring, so the next
+// step starts without a launch that clears them.
+__global__ void k_sum_counters(unsigned long long* __restrict__ counters, unsigned long long* __restrict__ sums,
+                               uint32_t* __restrict__ ring, uint32_t nring) {
   __shared__ unsigned long long part[kBlock / 64][kCounterStride];
-  const unsigned long long* blk = counters + static_cast<size_t>(blockIdx.x) * kCounterBlock;
+  unsigned long long* blk = counters + static_cast<size_t>(blockIdx.x) * kCounterBlock;
   unsigned long long acc[kCounterStride] = {};
   for (int s = threadIdx.x; s < kCounterSlots; s += blockDim.x)
-    for (int k = 0; k < kCounterStride; ++k) acc[k] += blk[s * kCounterStride + k];
+    for (int k = 0; k < kCounterStride; ++k) {
+      acc[k] += blk[s * kCounterStride + k];
+      blk[s * kCounterStride + k] = 0;
+    }
+  if (blockIdx.x == 0)
+    for (uint32_t i = threadIdx.x; i < nring; i += blockDim.x) ring[i] = 0;
   for (int k = 0; k < kCounterStride; ++k) {
     unsigned long long x = acc[k];
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
@@ -418,6 +426,7 @@ __global__ void k_lcc0(Word* __restrict__ cand, const uint8_t* __restrict__ code
 // phase 2 frontier (see bin_of).
 
 constexpr int kBins = 4;
+constexpr int kBinRing = 16;  // sets of bin counts a step takes before it clears them again
 __constant__ uint32_t c_bin_limit[kBins] = {4, 32, 256, kHubDegree};
 // Bin 0 of the walk task lists holds rows of up to 8 slots (k_triangles with
 // 8 lanes per source).
@@ -3405,7 +3414,10 @@ struct pj_ctx {
   uint64_t* d_scan_tmp = nullptr;  // per-span totals of the view-count scan
   uint32_t* d_ovf = nullptr;  // [0] first-pass overflows, [1] second-pass overflows, then the list
   uint32_t* d_bins = nullptr;        // kBins * V
-  uint32_t* d_bin_count = nullptr;   // kBins
+  uint32_t* d_bin_count = nullptr;   // kBins: the set of the ring the last frontier build filled
+  uint32_t* d_bin_ring = nullptr;    // kBinRing sets of kBins bin counts
+  uint32_t bin_slot = 1;             // sets of the ring from here on are zero
+  bool counters_clean = false;       // every counter block in use and every ring set are zero
   unsigned long long* d_counters = nullptr;  // kCounterSlots * kCounterStride
   // hubs
   uint32_t nhubs = 0, nchunks = 0;
@@ -3514,7 +3526,7 @@ void free_graph(pj_ctx* ctx) {
     ctx->graph_shared = false;
   }
   void* bufs[] = {ctx->d_off, ctx->d_edges, ctx->d_labels, ctx->d_code, ctx->d_cand, ctx->d_unmatch, ctx->d_u64,
-                  ctx->d_flags, ctx->d_vflags, ctx->d_dirty, ctx->d_rescan, ctx->d_ea, ctx->d_bins, ctx->d_bin_count,
+                  ctx->d_flags, ctx->d_vflags, ctx->d_dirty, ctx->d_rescan, ctx->d_ea, ctx->d_bins, ctx->d_bin_ring,
                   ctx->d_hubs,
                   ctx->d_chunk_hub, ctx->d_chunk_start, ctx->d_hub_count, ctx->d_hub_elim, ctx->d_nhubs, ctx->d_hub_hit,
                   ctx->d_last_scan, ctx->d_hub_new, ctx->memo1.keys, ctx->memo1.vals, ctx->memo1.epochs,
@@ -3545,7 +3557,8 @@ void free_graph(pj_ctx* ctx) {
   ctx->out_cap_v = ctx->out_cap_e = 0;
   ctx->live = nullptr;
   ctx->n_live = 0;
-  ctx->d_bins = ctx->d_bin_count = nullptr;
+  ctx->d_bins = ctx->d_bin_count = ctx->d_bin_ring = nullptr;
+  ctx->counters_clean = false;
   ctx->d_hubs = ctx->d_chunk_hub = nullptr;
   ctx->d_chunk_start = nullptr;
   ctx->d_hub_count = ctx->d_hub_elim = nullptr;
@@ -3628,10 +3641,13 @@ int alloc_state(pj_ctx* ctx) {
       dev_alloc(ctx, &ctx->d_vflags, V + 4) || dev_alloc(ctx, &ctx->d_dirty, V + 4) ||
       dev_alloc(ctx, &ctx->d_rescan, V + 4) || dev_alloc(ctx, &ctx->d_last_scan, V) ||
       dev_alloc(ctx, &ctx->d_ea, ctx->ea_words) ||
-      dev_alloc(ctx, &ctx->d_bins, V * kBins) || dev_alloc(ctx, &ctx->d_bin_count, kBins) ||
+      dev_alloc(ctx, &ctx->d_bins, V * kBins) || dev_alloc(ctx, &ctx->d_bin_ring, kBinRing * kBins) ||
       dev_alloc(ctx, &ctx->d_live[0], V + 1) || dev_alloc(ctx, &ctx->d_live[1], V + 1) ||
       dev_alloc(ctx, &ctx->d_counters, (kMaxLaunches + 1) * kCounterBlock))
     return -1;
+  ctx->d_bin_count = ctx->d_bin_ring;  // set 0 serves the upload's checks
+  ctx->bin_slot = 1;
+  ctx->counters_clean = false;
   return 0;
 }
 
@@ -3699,15 +3715,34 @@ void end_launch(pj_ctx* ctx) {
   ctx->cur_timed = false;
 }
 
+// Counters and bin counts start each step at zero.  The sums kernel at the
+// end of a step clears them (counters_clean), so a step that follows one
+// queues no clearing launch; anything else that may have written them
+// leaves counters_clean false and the clearing is queued here.
 int zero_counters(pj_ctx* ctx) {
-  const uint64_t n = ctx->timing ? static_cast<uint64_t>(kMaxLaunches + 1) * kCounterBlock : kCounterBlock;
-  hipLaunchKernelGGL(k_fill_u64, dim3(ctx->timing ? 64 : 4), dim3(kBlock), 0, ctx->stream, ctx->d_counters, n, 0ull);
+  if (!ctx->counters_clean) {
+    const uint64_t n = ctx->timing ? static_cast<uint64_t>(kMaxLaunches + 1) * kCounterBlock : kCounterBlock;
+    hipLaunchKernelGGL(k_fill_u64, dim3(ctx->timing ? 64 : 4), dim3(kBlock), 0, ctx->stream, ctx->d_counters, n,
+                       0ull);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bin_ring,
+                       static_cast<uint64_t>(kBinRing * kBins), 0u);
+  }
+  ctx->counters_clean = false;
+  ctx->bin_slot = 0;
   ctx->nlaunch = 0;
   return 0;
 }
 
+// Each frontier build takes the next zero set of bin counts of the ring;
+// a step that uses the whole ring clears it (after every launch queued
+// before, which may still read it) and starts over.
 int zero_bins(pj_ctx* ctx) {
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bin_count, static_cast<uint64_t>(kBins), 0u);
+  if (ctx->bin_slot >= kBinRing) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bin_ring,
+                       static_cast<uint64_t>(kBinRing * kBins), 0u);
+    ctx->bin_slot = 0;
+  }
+  ctx->d_bin_count = ctx->d_bin_ring + static_cast<size_t>(ctx->bin_slot++) * kBins;
   return 0;
 }
 
@@ -4830,7 +4865,9 @@ static int upload_finish(pj_ctx* ctx, const uint64_t* host_off) {
     if (host_off[v + 1] < host_off[v]) return refuse("offsets decrease at vertex " + std::to_string(v));
   if (setup_hubs(ctx, host_off)) return -1;
   {
-    // The edge-active bitset is free until a template is installed.
+    // The edge-active bitset is free until a template is installed; the
+    // check's flags go to a set of bin counts.
+    ctx->counters_clean = false;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bin_count, static_cast<uint64_t>(4), 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ctx, ctx->ea_words)), dim3(kBlock), 0, ctx->stream, ctx->d_ea,
                        ctx->ea_words, 0u);
@@ -5190,8 +5227,9 @@ int pj_run_step(pj_ctx* ctx, int step, uint32_t index, pj_algo_results* out) {
   PJ_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   const size_t blocks = ctx->timing ? static_cast<size_t>(ctx->nlaunch) + 1 : 1;
   hipLaunchKernelGGL(k_sum_counters, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, ctx->stream,
-                     ctx->d_counters, ctx->h_sums_dev);
+                     ctx->d_counters, ctx->h_sums_dev, ctx->d_bin_ring, static_cast<uint32_t>(kBinRing * kBins));
   PJ_HIP(hipGetLastError());
+  ctx->counters_clean = true;
   PJ_HIP(wait_stream(ctx));
   PJ_HIP(hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
   pj_algo_results r{0, 0, 0, 0};
@@ -5229,6 +5267,7 @@ int pj_set_kernel_timing(pj_ctx* ctx, int enable) {
   if (enable && !ctx->lev[0])
     for (int i = 0; i < 2 * kMaxLaunches; ++i) PJ_HIP(hipEventCreate(&ctx->lev[i]));
   ctx->timing = enable != 0;
+  ctx->counters_clean = false;  // the steps use a different number of counter blocks
   ctx->timing_kid = -1;
   for (int k = 0; k < K_COUNT; ++k) {
     ctx->k_launches[k] = ctx->k_items[k] = ctx->k_slots[k] = ctx->k_elim[k] = 0;
