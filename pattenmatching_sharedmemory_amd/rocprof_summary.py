@@ -29,6 +29,7 @@ _TEMPLATE_NAMES = {
     "k_lcc0": "lcc0",
     "k_build_bins": "build_bins",
     "k_lcc_phase1_all": "lcc_phase1",
+    "k_lcc_phase1_hubs": "lcc_phase1",  # the bins and the hub chunks in one launch
     "k_lcc_hub_chunks": "lcc_phase1_hub_chunks",
     "k_lcc_hub_finish": "lcc_phase1_hub_finish",
     "k_phase2_all": "phase2",
@@ -69,7 +70,7 @@ def stat_name(full: str, workgroup: int | None = None) -> str:
     base = s.split("<", 1)[0]
     args = s[len(base) + 1:-1].split(",") if "<" in s else []
     args = [a.strip() for a in args]
-    if base in ("k_lcc_phase1_all", "k_lcc_hub_chunks") and args and args[-1] == "true":
+    if base in ("k_lcc_phase1_all", "k_lcc_phase1_hubs", "k_lcc_hub_chunks") and args and args[-1] == "true":
         return _TEMPLATE_NAMES[base] + "_labels"  # the first LCC step after LCC0 (slot labels)
     if base in _TEMPLATE_NAMES:
         return _TEMPLATE_NAMES[base]
