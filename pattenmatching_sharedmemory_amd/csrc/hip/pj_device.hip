@@ -3339,6 +3339,12 @@ struct pj_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t ev_done = nullptr;  // no timing: polled by wait_stream
+  // A second stream for walks of one degree bin that run beside the walks
+  // of the other bins on `stream` (fork: the side stream waits for ev_fork;
+  // join: `stream` waits for ev_join).
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool side_pending = false;
   std::string err;
   int cus = 256;
 
@@ -3379,6 +3385,7 @@ struct pj_ctx {
   int fresh = 2;
   bool slot_labels_enabled = true;  // PJ_SLOT_LABELS=0: never
   bool phase1_fused = true;          // PJ_PHASE1_FUSED=0: hub chunks of LCC phase 1 in a launch of their own
+  bool side_walks = true;            // PJ_SIDE_WALKS=0: every walk launch on the context stream
   bool wide_triangles = true;        // PJ_WIDE_TRIANGLES=0: 257-2048-slot cycle sources walk
   // Live list: the vertices that may still be active (a superset of the
   // active ones; candidates only shrink until the state is replaced).
@@ -4080,9 +4087,27 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
   const uint32_t* rev = reverse_positions(ctx);
   const PairIndex pi = memo ? PairIndex{nullptr, nullptr, 0} : pair_index(ctx);  // positions refer to the CSR
   const bool triangles = kind == 0 && !tables && rev && ctx->h_t.nlcc_len == 3;
+  // Sources of 257-2048 slots (k_triangles_wide) run on the side stream,
+  // beside the other bins' walks: each walk writes only its own source's
+  // unmatched candidates and reads state that no walk of the step writes,
+  // and the walks wait on chains of dependent loads, so the chip has room
+  // for both.  A launch timed with events stays on the context stream.
+  const bool side = triangles && ctx->wide_triangles && ctx->side_walks &&
+                    !(ctx->timing && (ctx->timing_kid < 0 || ctx->timing_kid == K_TRIANGLES_WIDE));
+  if (side) {
+    PJ_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+    PJ_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+    unsigned long long* c = begin_launch(ctx, K_TRIANGLES_WIDE);
+    hipLaunchKernelGGL(k_triangles_wide<Word>, dim3(g), dim3(kBlock), 0, ctx->side, ctx->d_bins + 3 * V,
+                       ctx->d_bin_count + 3, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
+    end_launch(ctx);
+    PJ_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+    ctx->side_pending = true;
+  }
   // Every bin holds active vertices; hubs are run from their own list.
   for (int b = 0; b < kBins; ++b) {
     if (b == 2 && !triangles && !memo) continue;  // taken by the k_nlcc launch of bin 3
+    if (b == 3 && side) continue;
     unsigned long long* c =
         begin_launch(ctx, triangles && b <= 1                         ? K_TRIANGLES
                           : triangles && b == 2                         ? K_TRIANGLES_MID
@@ -4412,6 +4437,10 @@ int run_step_t(pj_ctx* ctx, int step, uint32_t index) {
       ctx->fresh = 2;
       if (launch_nlcc<Word>(ctx, kind)) return -1;
       launch_nlcc_hubs<Word>(ctx, kind, ctx->d_nhubs);
+      if (ctx->side_pending) {  // the side stream's walks are done before the step goes on
+        PJ_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+        ctx->side_pending = false;
+      }
       if (ctx->h_t.nlcc_memo && kind != 3 && !triangle_memo(ctx, kind) && launch_nlcc_retry<Word>(ctx, kind))
         return -1;
       if (step != PJ_STEP_ENUMERATION) {
@@ -4660,6 +4689,9 @@ int pj_create(pj_ctx** out, int device) {
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_done, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&ctx->d_t), sizeof(DevTemplate));
   if (e == hipSuccess)  // coherent: the sums kernel's stores are visible once the stream is done
     e = hipHostMalloc(reinterpret_cast<void**>(&ctx->h_sums), (kMaxLaunches + 1) * kCounterStride * 8,
@@ -4688,6 +4720,9 @@ int pj_create(pj_ctx** out, int device) {
   // PJ_PHASE1_FUSED=0 runs the hub chunks of LCC phase 1 in a launch after
   // the degree bins instead of in the same launch (same results).
   if (const char* pf = getenv("PJ_PHASE1_FUSED")) ctx->phase1_fused = strcmp(pf, "0") != 0;
+  // PJ_SIDE_WALKS=0 runs the 3-vertex cycle walks of 257-2048-slot sources
+  // on the context stream after the other bins instead of beside them.
+  if (const char* sw = getenv("PJ_SIDE_WALKS")) ctx->side_walks = strcmp(sw, "0") != 0;
   // PJ_PAIR_INDEX_MB bounds the pair index of heavy rows (0: none, every
   // pair search by binary search); PJ_PAIR_DEGREE is the smallest row length
   // above which a row is heavy (doubled until the index fits).  Same results
@@ -5464,6 +5499,7 @@ int pj_graph_stat(pj_ctx* ctx, pj_label_stat* out) {
 void pj_destroy(pj_ctx* ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
+  if (ctx->side) hipStreamSynchronize(ctx->side);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   free_graph(ctx);
   if (ctx->d_t) hipFree(ctx->d_t);
@@ -5476,6 +5512,9 @@ void pj_destroy(pj_ctx* ctx) {
   if (ctx->ev_done) hipEventDestroy(ctx->ev_done);
   for (int i = 0; i < 2 * kMaxLaunches; ++i)
     if (ctx->lev[i]) hipEventDestroy(ctx->lev[i]);
+  if (ctx->side) hipStreamDestroy(ctx->side);
+  if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
 }

@@ -505,8 +505,9 @@ def test_pair_index_changes_nothing(pj, orc, case, monkeypatch):
 @pytest.mark.parametrize("case", ["C2-shape s16", "hub"])
 def test_wide_triangle_pairs_change_nothing(pj, orc, case, monkeypatch):
     """3-vertex cycle sources of 257-2048 slots decided by pair tests against
-    the LDS list of closing vertices (k_triangles_wide) and by the walk
-    (PJ_WIDE_TRIANGLES=0) give identical traces and states, and the
+    the LDS list of closing vertices (k_triangles_wide, on the side stream
+    beside the other bins' walks, or after them with PJ_SIDE_WALKS=0) and by
+    the walk (PJ_WIDE_TRIANGLES=0) give identical traces and states, and the
     reference's."""
     if case == "hub":
         g = pj.Graph.rmat(14, 32, seed=3).labels_uniform(4, seed=4)
@@ -518,16 +519,19 @@ def test_wide_triangle_pairs_change_nothing(pj, orc, case, monkeypatch):
     assert np.any((deg > 256) & (deg <= 2048))
     c = pj.Constraints(p)
     runs = []
-    for wide in ("0", "1"):
+    for wide, side in (("0", "1"), ("1", "0"), ("1", "1")):
         monkeypatch.setenv("PJ_WIDE_TRIANGLES", wide)
+        monkeypatch.setenv("PJ_SIDE_WALKS", side)
         pr = pj.Pruner(g, c)
         rows, _ = pr.run(enumerate_matches=False)
         runs.append((rows, pr.state()))
         pr.close()
-    (r0, s0), (r1, s1) = runs
-    assert [r.key() for r in r0] == [r.key() for r in r1], case
-    assert np.array_equal(s0.cand, s1.cand) and np.array_equal(s0.vflags, s1.vflags)
-    assert np.array_equal(s0.ea_words, s1.ea_words)
+    (r0, s0) = runs[0]
+    for r1, s1 in runs[1:]:
+        assert [r.key() for r in r0] == [r.key() for r in r1], case
+        assert np.array_equal(s0.cand, s1.cand) and np.array_equal(s0.vflags, s1.vflags)
+        assert np.array_equal(s0.ea_words, s1.ea_words)
+    r1 = runs[-1][0]
     o = orc.Oracle(g, c, threads=8)
     orows, _ = o.run(orc.FAITHFUL, enumerate_matches=False)
     assert [r.key() for r in r1] == [r.key() for r in orows], case
