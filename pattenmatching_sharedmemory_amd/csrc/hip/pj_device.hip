@@ -3385,7 +3385,7 @@ struct pj_ctx {
   int fresh = 2;
   bool slot_labels_enabled = true;  // PJ_SLOT_LABELS=0: never
   bool phase1_fused = true;          // PJ_PHASE1_FUSED=0: hub chunks of LCC phase 1 in a launch of their own
-  bool side_walks = true;            // PJ_SIDE_WALKS=0: every walk launch on the context stream
+  uint32_t side_walks = 0xC;         // PJ_SIDE_WALKS: bins whose 3-vertex cycle walks run on the side stream
   bool wide_triangles = true;        // PJ_WIDE_TRIANGLES=0: 257-2048-slot cycle sources walk
   // Live list: the vertices that may still be active (a superset of the
   // active ones; candidates only shrink until the state is replaced).
@@ -4087,55 +4087,67 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
   const uint32_t* rev = reverse_positions(ctx);
   const PairIndex pi = memo ? PairIndex{nullptr, nullptr, 0} : pair_index(ctx);  // positions refer to the CSR
   const bool triangles = kind == 0 && !tables && rev && ctx->h_t.nlcc_len == 3;
-  // Sources of 257-2048 slots (k_triangles_wide) run on the side stream,
+  // The 3-vertex cycle walks of the bins in side_walks (a bit per bin;
+  // default bins 2 and 3, sources of 33-2048 slots) run on the side stream,
   // beside the other bins' walks: each walk writes only its own source's
   // unmatched candidates and reads state that no walk of the step writes,
   // and the walks wait on chains of dependent loads, so the chip has room
   // for both.  A launch timed with events stays on the context stream.
-  const bool side = triangles && ctx->wide_triangles && ctx->side_walks &&
-                    !(ctx->timing && (ctx->timing_kid < 0 || ctx->timing_kid == K_TRIANGLES_WIDE));
-  if (side) {
-    PJ_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
-    PJ_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-    unsigned long long* c = begin_launch(ctx, K_TRIANGLES_WIDE);
-    hipLaunchKernelGGL(k_triangles_wide<Word>, dim3(g), dim3(kBlock), 0, ctx->side, ctx->d_bins + 3 * V,
-                       ctx->d_bin_count + 3, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
+  auto kid_of = [&](int b) {
+    return triangles && b <= 1                         ? K_TRIANGLES
+           : triangles && b == 2                         ? K_TRIANGLES_MID
+           : triangles && b == 3 && ctx->wide_triangles ? K_TRIANGLES_WIDE
+           : memo                                       ? K_MEMO_WALK
+                                                        : K_NLCC;
+  };
+  auto on_side = [&](int b) {
+    return triangles && ((ctx->side_walks >> b) & 1u) && (b != 3 || ctx->wide_triangles) &&
+           !(ctx->timing && (ctx->timing_kid < 0 || ctx->timing_kid == kid_of(b)));
+  };
+  auto launch_bin = [&](int b, hipStream_t st) {
+    unsigned long long* c = begin_launch(ctx, kid_of(b));
+    if (triangles && b == 0)
+      hipLaunchKernelGGL((k_triangles<Word, kWalkBin0>), dim3(g), dim3(kBlock), 0, st, ctx->d_bins + b * V,
+                         ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
+    else if (triangles && b == 1)
+      hipLaunchKernelGGL((k_triangles<Word, 32>), dim3(g), dim3(kBlock), 0, st, ctx->d_bins + b * V,
+                         ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
+    else if (triangles && b == 2)
+      hipLaunchKernelGGL(k_triangles_mid<Word>, dim3(g), dim3(kBlock), 0, st, ctx->d_bins + b * V,
+                         ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
+    else if (triangles && b == 3 && ctx->wide_triangles)
+      hipLaunchKernelGGL(k_triangles_wide<Word>, dim3(g), dim3(kBlock), 0, st, ctx->d_bins + b * V,
+                         ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
+    else if (memo)  // first pass with LDS tables; overflows go to the list of d_ovf[0]
+      hipLaunchKernelGGL(k_memo_lds<Word>, dim3(kMemoWaves), dim3(64), 0, st, ctx->d_bins + b * V,
+                         ctx->d_bin_count + b, off, edges, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
+    else  // bin 3 (with bin 2 unless k_triangles_mid took it)
+      hipLaunchKernelGGL((tables ? k_nlcc<Word, false> : k_nlcc<Word, true>), dim3(g), dim3(kBlock), 0, st,
+                         kind, ctx->d_bins + b * V,
+                         ctx->d_bin_count + b, b == 3 && !triangles ? ctx->d_bins + 2 * V : nullptr,
+                         b == 3 && !triangles ? ctx->d_bin_count + 2 : nullptr, off, edges, ea,
+                         reverse_positions(ctx), pi, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
     end_launch(ctx);
+  };
+  bool forked = false;
+  for (int b = kBins - 1; b >= 0; --b) {  // the widest sources first
+    if (!on_side(b)) continue;
+    if (!forked) {  // the side stream starts after the bins were built
+      PJ_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+      PJ_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+      forked = true;
+    }
+    launch_bin(b, ctx->side);
+  }
+  if (forked) {
     PJ_HIP(hipEventRecord(ctx->ev_join, ctx->side));
     ctx->side_pending = true;
   }
   // Every bin holds active vertices; hubs are run from their own list.
   for (int b = 0; b < kBins; ++b) {
     if (b == 2 && !triangles && !memo) continue;  // taken by the k_nlcc launch of bin 3
-    if (b == 3 && side) continue;
-    unsigned long long* c =
-        begin_launch(ctx, triangles && b <= 1                         ? K_TRIANGLES
-                          : triangles && b == 2                         ? K_TRIANGLES_MID
-                          : triangles && b == 3 && ctx->wide_triangles ? K_TRIANGLES_WIDE
-                          : memo                                       ? K_MEMO_WALK
-                                                                       : K_NLCC);
-    if (triangles && b == 0)
-      hipLaunchKernelGGL((k_triangles<Word, kWalkBin0>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
-                         ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
-    else if (triangles && b == 1)
-      hipLaunchKernelGGL((k_triangles<Word, 32>), dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
-                         ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
-    else if (triangles && b == 2)
-      hipLaunchKernelGGL(k_triangles_mid<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
-                         ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
-    else if (triangles && b == 3 && ctx->wide_triangles)
-      hipLaunchKernelGGL(k_triangles_wide<Word>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->d_bins + b * V,
-                         ctx->d_bin_count + b, off, edges, ea, rev, pi, cand, unm, ctx->d_flags, ctx->d_t, c);
-    else if (memo)  // first pass with LDS tables; overflows go to the list of d_ovf[0]
-      hipLaunchKernelGGL(k_memo_lds<Word>, dim3(kMemoWaves), dim3(64), 0, ctx->stream, ctx->d_bins + b * V,
-                         ctx->d_bin_count + b, off, edges, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
-    else  // bin 3 (with bin 2 unless k_triangles_mid took it)
-      hipLaunchKernelGGL((tables ? k_nlcc<Word, false> : k_nlcc<Word, true>), dim3(g), dim3(kBlock), 0, ctx->stream,
-                         kind, ctx->d_bins + b * V,
-                         ctx->d_bin_count + b, b == 3 && !triangles ? ctx->d_bins + 2 * V : nullptr,
-                         b == 3 && !triangles ? ctx->d_bin_count + 2 : nullptr, off, edges, ea,
-                         reverse_positions(ctx), pi, cand, unm, ctx->d_flags, ctx->d_t, ma, c);
-    end_launch(ctx);
+    if (on_side(b)) continue;
+    launch_bin(b, ctx->stream);
   }
   return 0;
 }
@@ -4720,9 +4732,10 @@ int pj_create(pj_ctx** out, int device) {
   // PJ_PHASE1_FUSED=0 runs the hub chunks of LCC phase 1 in a launch after
   // the degree bins instead of in the same launch (same results).
   if (const char* pf = getenv("PJ_PHASE1_FUSED")) ctx->phase1_fused = strcmp(pf, "0") != 0;
-  // PJ_SIDE_WALKS=0 runs the 3-vertex cycle walks of 257-2048-slot sources
-  // on the context stream after the other bins instead of beside them.
-  if (const char* sw = getenv("PJ_SIDE_WALKS")) ctx->side_walks = strcmp(sw, "0") != 0;
+  // PJ_SIDE_WALKS: a bit per degree bin whose 3-vertex cycle walks run on
+  // the side stream beside the others (default 12: the bins of 33-256 and
+  // 257-2048 slots; 0: every walk on the context stream, one after another).
+  if (const char* sw = getenv("PJ_SIDE_WALKS")) ctx->side_walks = static_cast<uint32_t>(strtoul(sw, nullptr, 0)) & 0xFu;
   // PJ_PAIR_INDEX_MB bounds the pair index of heavy rows (0: none, every
   // pair search by binary search); PJ_PAIR_DEGREE is the smallest row length
   // above which a row is heavy (doubled until the index fits).  Same results
