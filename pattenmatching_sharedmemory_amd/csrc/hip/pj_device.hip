@@ -4100,8 +4100,18 @@ int launch_nlcc(pj_ctx* ctx, int kind) {
            : memo                                       ? K_MEMO_WALK
                                                         : K_NLCC;
   };
+  // The side stream is created on first use: streams are spread over the
+  // hardware queues in the order they are created, so contexts that never
+  // use one (the query forms' workers) must not take a queue's share.
+  if (triangles && ctx->side_walks && !ctx->side &&
+      (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+       hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+       hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)) {
+    ctx->err = "launch_nlcc: creating the side stream failed";
+    return -1;
+  }
   auto on_side = [&](int b) {
-    return triangles && ((ctx->side_walks >> b) & 1u) && (b != 3 || ctx->wide_triangles) &&
+    return triangles && ctx->side && ((ctx->side_walks >> b) & 1u) && (b != 3 || ctx->wide_triangles) &&
            !(ctx->timing && (ctx->timing_kid < 0 || ctx->timing_kid == kid_of(b)));
   };
   auto launch_bin = [&](int b, hipStream_t st) {
@@ -4701,9 +4711,6 @@ int pj_create(pj_ctx** out, int device) {
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_done, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&ctx->d_t), sizeof(DevTemplate));
   if (e == hipSuccess)  // coherent: the sums kernel's stores are visible once the stream is done
     e = hipHostMalloc(reinterpret_cast<void**>(&ctx->h_sums), (kMaxLaunches + 1) * kCounterStride * 8,
@@ -5099,6 +5106,11 @@ int pj_share_graph(pj_ctx* ctx, const pj_ctx* owner) {
   PJ_HIP(hipSetDevice(ctx->device));
   free_graph(ctx);
   ctx->graph_shared = true;
+  // Contexts that share a graph (the workers of the query forms) keep their
+  // walks on one stream unless PJ_SIDE_WALKS asks otherwise: several
+  // contexts already run side by side, and every extra stream takes a share
+  // of the hardware queues that the workers' streams are spread over.
+  if (!getenv("PJ_SIDE_WALKS")) ctx->side_walks = 0;
   ctx->V = owner->V;
   ctx->E = owner->E;
   ctx->ea_words = owner->ea_words;
