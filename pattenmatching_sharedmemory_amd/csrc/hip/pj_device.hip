@@ -3385,6 +3385,7 @@ struct pj_ctx {
   int fresh = 2;
   bool slot_labels_enabled = true;  // PJ_SLOT_LABELS=0: never
   bool phase1_fused = true;          // PJ_PHASE1_FUSED=0: hub chunks of LCC phase 1 in a launch of their own
+  int phase1_blocks = kPhase1BlocksPerCu;  // PJ_PHASE1_BLOCKS: blocks per compute unit of the phase-1 launch
   uint32_t side_walks = 0xC;         // PJ_SIDE_WALKS: bins whose 3-vertex cycle walks run on the side stream
   bool wide_triangles = true;        // PJ_WIDE_TRIANGLES=0: 257-2048-slot cycle sources walk
   // Live list: the vertices that may still be active (a superset of the
@@ -3859,13 +3860,13 @@ void launch_lcc_phase1(pj_ctx* ctx) {
   const bool fused = ctx->nhubs && ctx->phase1_fused;
   if (fused)
     hipLaunchKernelGGL((slot_lab ? k_lcc_phase1_hubs<Word, KW, true> : k_lcc_phase1_hubs<Word, KW, false>),
-                       dim3(ctx->cus * kPhase1BlocksPerCu), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
+                       dim3(ctx->cus * ctx->phase1_blocks), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
                        ctx->d_bin_count, ctx->V, ctx->d_chunk_hub, ctx->d_chunk_start, ctx->nchunks, ctx->d_hubs,
                        ctx->d_hub_count, ctx->d_hub_elim, ctx->d_off, ctx->d_edges, ctx->d_code, derive, slot_lab,
                        cand, unm, ctx->d_flags, dirty, rescan, ctx->d_ea, ctx->d_t, c);
   else
     hipLaunchKernelGGL((slot_lab ? k_lcc_phase1_all<Word, KW, true> : k_lcc_phase1_all<Word, KW, false>),
-                       dim3(ctx->cus * kPhase1BlocksPerCu), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
+                       dim3(ctx->cus * ctx->phase1_blocks), dim3(kBlock), 0, ctx->stream, ctx->d_bins,
                        ctx->d_bin_count, ctx->V, ctx->d_off, ctx->d_edges, ctx->d_code, derive, slot_lab, cand, unm,
                        ctx->d_flags, dirty, rescan, ctx->d_ea, ctx->d_t, c);
   end_launch(ctx);
@@ -4739,6 +4740,8 @@ int pj_create(pj_ctx** out, int device) {
   // PJ_PHASE1_FUSED=0 runs the hub chunks of LCC phase 1 in a launch after
   // the degree bins instead of in the same launch (same results).
   if (const char* pf = getenv("PJ_PHASE1_FUSED")) ctx->phase1_fused = strcmp(pf, "0") != 0;
+  // PJ_PHASE1_BLOCKS: blocks per compute unit of the LCC phase-1 launch (1-64)
+  if (const char* pb = getenv("PJ_PHASE1_BLOCKS")) ctx->phase1_blocks = std::min(64, std::max(1, atoi(pb)));
   // PJ_SIDE_WALKS: a bit per degree bin whose 3-vertex cycle walks run on
   // the side stream beside the others (default 12: the bins of 33-256 and
   // 257-2048 slots; 0: every walk on the context stream, one after another).
